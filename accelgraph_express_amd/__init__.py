@@ -1,0 +1,13 @@
+"""MI355X-native CSR builder + pull PageRank (the hot path of AccelGraph_Express).
+
+The compute lives in ``libaccelgraph_amd.so`` (hand-written HIP kernels for
+gfx950 behind the C ABI in ``include/accelgraph_amd.h``).  This package is the
+Python host-side mirror used by tests and ``bench.py``.
+"""
+from . import _lib
+from ._lib import AGError, load
+from .graph import (Context, Graph, PageRankStats, gen_rmat, gen_uniform, get_tuning,
+                    set_tuning)
+
+__all__ = ["AGError", "Context", "Graph", "PageRankStats", "gen_rmat", "gen_uniform",
+           "get_tuning", "set_tuning", "load", "_lib"]

@@ -1,0 +1,158 @@
+// Internal declarations shared by the library's translation units.
+// Not part of the C ABI (include/accelgraph_amd.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "accelgraph_amd.h"
+
+namespace ag {
+
+// ---- errors -----------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(int code, const char *msg);
+[[noreturn]] void fail(int code, const char *fmt, ...);
+
+#define AG_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            ::ag::fail(AG_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                       __FILE__, __LINE__);                                            \
+    } while (0)
+
+// Every extern "C" entry point runs its body through guard(): exceptions become
+// status codes plus ag_last_error() text.
+template <typename F>
+int guard(F &&f)
+{
+    try {
+        f();
+        set_last_error(AG_OK, "");
+        return AG_OK;
+    } catch (const Error &e) {
+        set_last_error(e.code, e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error(AG_ERR_OOM, "host allocation failed");
+        return AG_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_last_error(AG_ERR_INVALID, e.what());
+        return AG_ERR_INVALID;
+    }
+}
+
+// ---- device memory ----------------------------------------------------------
+// Owning device buffer; frees on destruction.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n)
+    {
+        release();
+        if (n == 0) n = 4;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            fail(AG_ERR_OOM, "hipMalloc(%zu bytes): %s", n, hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ---- context ----------------------------------------------------------------
+} // namespace ag
+
+struct ag_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 0;
+    std::string arch;
+};
+
+namespace ag {
+
+// Gather layout of a graph for the pull kernels (built lazily, cached).
+struct Plan;
+struct PprState;
+void plan_destroy(Plan *p);
+
+} // namespace ag
+
+struct ag_graph {
+    uint32_t N = 0, E = 0;
+    bool has_full_fwd = false; // sorted src/dst arrays + forward idx present
+    bool has_inverse_full = false;
+    ag::DevBuf fwd_src, fwd_dst, fwd_outdeg, fwd_idx, fwd_off;
+    ag::DevBuf inv_src, inv_dst, inv_outdeg, inv_idx, inv_off;
+    ag::Plan *plan = nullptr;
+    uint32_t ppr_cpw[3] = {0, 0, 0}; // autotuned columns-per-wave per precision
+    ~ag_graph() { ag::plan_destroy(plan); }
+    uint64_t bytes() const;
+};
+
+namespace ag {
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+inline uint32_t bits_for(uint64_t limit) // bits needed for keys in [0, limit)
+{
+    uint32_t b = 0;
+    while (b < 32 && (1ull << b) < limit) b++;
+    return b;
+}
+
+ag_ctx *default_ctx();
+void check_ctx(ag_ctx *ctx);
+
+// ---- sort / scan primitives (ag_sort.hip) ------------------------------------
+// Exclusive scan of n u32 values into out[0..n]; out[n] = total.  in == out allowed.
+void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n);
+// Stable LSD radix sort of (key, val) pairs by the low `key_bits` bits of key.
+// Inputs are not modified; outputs may not alias inputs.
+void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                      uint32_t *vout, uint32_t n, uint32_t key_bits);
+// Degree / edges_idx / offsets of a key-sorted array with keys < N.
+void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N,
+                uint32_t *degree, uint32_t *edges_idx, uint32_t *offsets /* N+1 or null */);
+// max over two arrays (returns host value); 0 for n == 0.
+uint32_t device_max2(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n);
+uint32_t device_max1(ag_ctx *ctx, const uint32_t *a, uint32_t n);
+
+// ---- tuning ----------------------------------------------------------------
+struct Tuning {
+    uint32_t low_max = 24;
+    uint32_t hub_min = 4096;
+    uint32_t hub_chunk = 2048;
+};
+Tuning &tuning();
+
+} // namespace ag

@@ -1,0 +1,884 @@
+// Pull PageRank on the device: the gather of the reference's PageRank CSR PULL
+// compute unit (03_capi_integration/accelerator_rtl/cu_control/cu_PageRank/
+// CSR/PULL: next[v] = sum of auxiliary1[inverse_dest[j]] over v's inverse
+// edge range, cu_edge_job_control.sv:373-449, cu_sum_kernel_control.sv:188-228)
+// fused with the host loops of pageRankPullGraphCSR (pageRank.c:1024-1085).
+//
+// Layout ("plan"), built once per graph on the device:
+//   * vertices are relabelled by descending in-degree (stable by id), so the
+//     in-degree bins are contiguous label ranges and the hottest sources sit
+//     together in the contribution vector;
+//   * hub rows   (in-degree >= hub_min): CSR, split in fixed chunks of
+//                  hub_chunk edges, one workgroup per chunk; the last chunk
+//                  to finish (device-scope ticket) sums the chunk partials in
+//                  chunk order and updates the vertex;
+//   * mid rows   (low_max < in-degree < hub_min): CSR, one wave per row,
+//                  per-lane partial sums and a shuffle reduction;
+//   * low rows   (1 <= in-degree <= low_max): SELL-64 slices, 64 rows per
+//                  wave, one lane per row, column-major so each step of the
+//                  wave is one coalesced 256-byte load; each lane sums its row
+//                  sequentially in the reference's edge order;
+//   * zero rows: no gather.
+// One sweep is one kernel launch: gather + update + next contribution + FP64
+// residual partials; the last workgroup to finish reduces the partials in a
+// fixed order (deterministic) and decides convergence on the device.
+#include <hip/hip_fp16.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "ag_internal.h"
+
+namespace ag {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMidRowsPerWave = 4;
+constexpr int kMidRowsPerBlock = kMidRowsPerWave * 4;
+constexpr int kZeroRowsPerBlock = 1024;
+constexpr uint32_t kFmtF32 = 0, kFmtBF16 = 1; // 2 = FP16
+
+} // namespace
+
+// BF16 storage word (round-to-nearest-even from FP32, widened by a shift).
+struct bf16_t {
+    uint16_t x;
+};
+
+struct Work {
+    uint32_t max_iter = 0;
+    uint32_t grid = 0;
+    DevBuf pr, c0, c1, blk_err, blk_act, blk_dang, state, err_iter, act_iter, fmt, sum_part, out;
+};
+
+struct Plan {
+    uint32_t N = 0, E = 0;
+    uint32_t n_hub = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
+    uint32_t low_slices = 0, hub_chunks = 0;
+    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0;
+    uint64_t csr_edges = 0, sell_slots = 0;
+    DevBuf order, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial, hub_ticket;
+    uint32_t nb_hub = 0, nb_mid = 0, nb_low = 0, nb_zero = 0, nvblocks = 0;
+    Work work;
+    std::mutex mu;
+    uint64_t bytes() const
+    {
+        return order.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
+               chunk_row.bytes + chunk_first.bytes + hub_partial.bytes + hub_ticket.bytes;
+    }
+};
+
+void plan_destroy(Plan *p) { delete p; }
+
+// Device-side sweep state.
+struct SweepState {
+    uint32_t done;
+    uint32_t ticket;
+    uint32_t iterations;
+    uint32_t sweeps;
+    uint32_t last_active;
+    uint32_t fp32_switch;
+    uint32_t pad0, pad1;
+    double last_err;
+    double prev_err;
+    double dangling;
+    float tele;
+    float pad2;
+};
+
+struct SweepArgs {
+    uint32_t N;
+    uint32_t n_hub, n_nonlow, n_nonzero;
+    uint32_t nb_hub, nb_mid, nb_low, nb_zero, nvblocks;
+    uint32_t hub_chunk;
+    uint32_t mode;
+    uint32_t max_iter;
+    uint32_t finish_fp32;
+    float damp, base;
+    double epsilon;
+    double inv_n;
+    const uint32_t *__restrict__ roff;
+    const uint32_t *__restrict__ cols;
+    const uint32_t *__restrict__ slice_ptr;
+    const uint32_t *__restrict__ outdeg;
+    const uint32_t *__restrict__ chunk_row;
+    const uint32_t *__restrict__ chunk_first;
+    float *hub_partial;
+    uint32_t *hub_ticket;
+    float *pr;
+    void *c0, *c1; // contribution ping-pong buffers (sized for FP32)
+    double *blk_err, *blk_dang;
+    uint32_t *blk_act;
+    SweepState *st;
+    double *err_iter;
+    uint32_t *act_iter;
+    uint32_t *fmt; // fmt[j] = storage format of the contributions read by sweep j
+};
+
+namespace {
+
+__device__ __forceinline__ float load_c(const float *p, uint32_t i) { return p[i]; }
+__device__ __forceinline__ float load_c(const bf16_t *p, uint32_t i)
+{
+    return __uint_as_float((uint32_t)p[i].x << 16);
+}
+__device__ __forceinline__ float load_c(const __half *p, uint32_t i) { return __half2float(p[i]); }
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
+{
+    const uint32_t b = __float_as_uint(f);
+    return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ void store_c(void *buf, uint32_t fmt, uint32_t i, float c)
+{
+    if (fmt == kFmtF32)
+        static_cast<float *>(buf)[i] = c;
+    else if (fmt == kFmtBF16)
+        static_cast<uint16_t *>(buf)[i] = f32_to_bf16_rne(c);
+    else
+        static_cast<__half *>(buf)[i] = __float2half_rn(c);
+}
+
+__device__ __forceinline__ uint32_t ldnt(const uint32_t *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+__device__ __forceinline__ float wave_sum(float x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Vertex update for label i with gathered sum s (pageRank.c:1055-1070 in
+// REFERENCE mode; x' = tele + d*s in STANDARD mode), then the contribution the
+// next sweep reads (pageRank.c:1027-1032).  Written without contraction so
+// that base + damp*next rounds twice, as the reference's C does.
+__device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, float s,
+                                              float tele, void *cout, uint32_t wfmt,
+                                              double &err, uint32_t &act, double &dang)
+{
+#pragma clang fp contract(off)
+    const float old = a.pr[i];
+    float nw;
+    if (a.mode == AG_PR_REFERENCE)
+        nw = a.base + a.damp * s;
+    else
+        nw = tele + a.damp * s;
+    a.pr[i] = nw;
+    const uint32_t od = a.outdeg[i];
+    if (a.mode == AG_PR_REFERENCE) {
+        const double e = fabs((double)(nw - old));
+        err += e / (double)a.N;
+        act += (e >= a.epsilon) ? 1u : 0u;
+    } else {
+        err += fabs((double)nw - (double)old);
+        if (od == 0) dang += (double)nw;
+    }
+    const float c = od ? nw / (float)od : 0.0f;
+    store_c(cout, wfmt, i, c);
+}
+
+template <typename CT>
+__device__ void sweep_body(const SweepArgs &a, uint32_t it, uint32_t wfmt, const CT *__restrict__ cin,
+                           void *cout, float tele, double &err, uint32_t &act, double &dang)
+{
+    __shared__ float red[kBlock / 64];
+    __shared__ uint32_t last_flag;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+
+    for (uint32_t vb = blockIdx.x; vb < a.nvblocks; vb += gridDim.x) {
+        if (vb < a.nb_hub) {
+            // ---- hub chunk: workgroup sums hub_chunk edges ----
+            const uint32_t c = vb;
+            const uint32_t row = a.chunk_row[c];
+            const uint32_t k = c - a.chunk_first[row];
+            const uint32_t nch = a.chunk_first[row + 1] - a.chunk_first[row];
+            const uint32_t rb = a.roff[row], re = a.roff[row + 1];
+            const uint32_t b0 = rb + k * a.hub_chunk;
+            const uint32_t b1 = min(re, b0 + a.hub_chunk);
+            float s = 0.0f;
+            for (uint32_t j = b0 + threadIdx.x; j < b1; j += kBlock) s += load_c(cin, ldnt(a.cols + j));
+            s = wave_sum(s);
+            if (lane == 0) red[w] = s;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const float tot = (red[0] + red[1]) + (red[2] + red[3]);
+                a.hub_partial[c] = tot;
+                __threadfence();
+                const uint32_t t = atomicAdd(&a.hub_ticket[row], 1u);
+                last_flag = (t == nch - 1) ? 1u : 0u;
+            }
+            __syncthreads();
+            if (last_flag && threadIdx.x == 0) {
+                __threadfence();
+                const uint32_t f0 = a.chunk_first[row];
+                float sum = 0.0f;
+                for (uint32_t q = 0; q < nch; q++)
+                    sum += __hip_atomic_load(&a.hub_partial[f0 + q], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                a.hub_ticket[row] = 0;
+                update_vertex(a, row, sum, tele, cout, wfmt, err, act, dang);
+            }
+            __syncthreads();
+        } else if (vb < a.nb_hub + a.nb_mid) {
+            // ---- mid rows: one wave per row ----
+            const uint32_t r0 = a.n_hub + (vb - a.nb_hub) * kMidRowsPerBlock + w * kMidRowsPerWave;
+#pragma unroll 1
+            for (int q = 0; q < kMidRowsPerWave; q++) {
+                const uint32_t row = r0 + q;
+                if (row >= a.n_nonlow) break;
+                const uint32_t rb = a.roff[row], re = a.roff[row + 1];
+                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+                uint32_t j = rb + lane;
+                for (; j + 192 < re; j += 256) {
+                    const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + 64);
+                    const uint32_t u2 = ldnt(a.cols + j + 128), u3 = ldnt(a.cols + j + 192);
+                    s0 += load_c(cin, u0);
+                    s1 += load_c(cin, u1);
+                    s2 += load_c(cin, u2);
+                    s3 += load_c(cin, u3);
+                }
+                for (; j < re; j += 64) s0 += load_c(cin, ldnt(a.cols + j));
+                float s = wave_sum((s0 + s1) + (s2 + s3));
+                if (lane == 0) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
+            }
+        } else if (vb < a.nb_hub + a.nb_mid + a.nb_low) {
+            // ---- low rows: SELL-64 slice per wave, lane per row ----
+            const uint32_t sl = (vb - a.nb_hub - a.nb_mid) * 4 + w;
+            const uint32_t nlow = a.n_nonzero - a.n_nonlow;
+            if (sl * 64 < nlow) {
+                const uint32_t row = a.n_nonlow + sl * 64 + lane;
+                const bool valid = row < a.n_nonzero;
+                const uint32_t deg = valid ? a.roff[row + 1] - a.roff[row] : 0u;
+                const uint32_t width = __shfl(deg, 0, 64); // rows sorted: lane 0 is the widest
+                const uint32_t *p = a.cols + a.slice_ptr[sl] + lane;
+                float s = 0.0f;
+                uint32_t k = 0;
+                for (; k + 4 <= width; k += 4) {
+                    uint32_t u0 = 0, u1 = 0, u2 = 0, u3 = 0;
+                    if (k < deg) u0 = ldnt(p + (k + 0) * 64);
+                    if (k + 1 < deg) u1 = ldnt(p + (k + 1) * 64);
+                    if (k + 2 < deg) u2 = ldnt(p + (k + 2) * 64);
+                    if (k + 3 < deg) u3 = ldnt(p + (k + 3) * 64);
+                    if (k < deg) s += load_c(cin, u0);
+                    if (k + 1 < deg) s += load_c(cin, u1);
+                    if (k + 2 < deg) s += load_c(cin, u2);
+                    if (k + 3 < deg) s += load_c(cin, u3);
+                }
+                for (; k < width; k++)
+                    if (k < deg) s += load_c(cin, ldnt(p + k * 64));
+                if (valid) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
+            }
+        } else {
+            // ---- zero in-degree rows ----
+            const uint32_t r0 = a.n_nonzero + (vb - a.nb_hub - a.nb_mid - a.nb_low) * kZeroRowsPerBlock;
+#pragma unroll
+            for (int q = 0; q < kZeroRowsPerBlock / kBlock; q++) {
+                const uint32_t row = r0 + q * kBlock + threadIdx.x;
+                if (row < a.N) update_vertex(a, row, 0.0f, tele, cout, wfmt, err, act, dang);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
+{
+    __shared__ double s_err[kBlock / 64], s_dang[kBlock / 64];
+    __shared__ uint32_t s_act[kBlock / 64];
+    __shared__ uint32_t s_last;
+    SweepState *st = a.st;
+    if (st->done) return; // converged in an earlier sweep: uniform early exit
+    const uint32_t rfmt = a.fmt[it];
+    const uint32_t wfmt = a.fmt[it + 1];
+    const void *cin = (it & 1) ? a.c1 : a.c0;
+    void *cout = (it & 1) ? a.c0 : a.c1;
+    const float tele = st->tele;
+    double err = 0.0, dang = 0.0;
+    uint32_t act = 0;
+    if (rfmt == kFmtF32)
+        sweep_body<float>(a, it, wfmt, static_cast<const float *>(cin), cout, tele, err, act, dang);
+    else if (rfmt == kFmtBF16)
+        sweep_body<bf16_t>(a, it, wfmt, static_cast<const bf16_t *>(cin), cout,
+                                   tele, err, act, dang);
+    else
+        sweep_body<__half>(a, it, wfmt, static_cast<const __half *>(cin), cout, tele, err, act,
+                           dang);
+
+    // ---- block partials, then the last block reduces them in fixed order ----
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    err = wave_sum(err);
+    dang = wave_sum(dang);
+    act = wave_sum(act);
+    if (lane == 0) { s_err[w] = err; s_dang[w] = dang; s_act[w] = act; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.blk_err[blockIdx.x] = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
+        a.blk_dang[blockIdx.x] = (s_dang[0] + s_dang[1]) + (s_dang[2] + s_dang[3]);
+        a.blk_act[blockIdx.x] = s_act[0] + s_act[1] + s_act[2] + s_act[3];
+        __threadfence();
+        const uint32_t t = atomicAdd(&st->ticket, 1u);
+        s_last = (t == gridDim.x - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    double e = 0.0, d = 0.0;
+    uint32_t c = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        e += __hip_atomic_load(&a.blk_err[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        d += __hip_atomic_load(&a.blk_dang[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c += __hip_atomic_load(&a.blk_act[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    e = wave_sum(e);
+    d = wave_sum(d);
+    c = wave_sum(c);
+    __syncthreads();
+    if (lane == 0) { s_err[w] = e; s_dang[w] = d; s_act[w] = c; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double E_ = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
+    const double D_ = (s_dang[0] + s_dang[1]) + (s_dang[2] + s_dang[3]);
+    const uint32_t A_ = s_act[0] + s_act[1] + s_act[2] + s_act[3];
+    a.err_iter[it] = E_;
+    a.act_iter[it] = A_;
+    st->prev_err = st->last_err;
+    st->last_err = E_;
+    st->last_active = A_;
+    st->sweeps = it + 1;
+    st->ticket = 0;
+    // Storage format of the contributions written by the next sweep.
+    uint32_t nf = a.fmt[it + 1];
+    if (nf != kFmtF32 && a.finish_fp32 && it >= 2 && E_ > 0.97 * st->prev_err) {
+        nf = kFmtF32;
+        st->fp32_switch = it + 2;
+    }
+    if (it + 2 < a.max_iter + 2) a.fmt[it + 2] = nf;
+    if (a.mode == AG_PR_REFERENCE) {
+        if (A_ == 0) { st->done = 1; st->iterations = it; }
+        else st->iterations = it + 1;
+    } else {
+        st->dangling = D_;
+        st->tele = (float)((1.0 - (double)a.damp) * a.inv_n + (double)a.damp * D_ * a.inv_n);
+        st->iterations = it + 1;
+        if (a.epsilon > 0.0 && E_ <= a.epsilon) st->done = 1;
+    }
+}
+
+__global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        SweepState *st = a.st;
+        st->done = 0; st->ticket = 0; st->iterations = 0; st->sweeps = 0;
+        st->last_active = 0; st->fp32_switch = 0;
+        st->last_err = 0.0; st->prev_err = 0.0; st->dangling = 0.0; st->tele = tele0;
+    }
+    if (i <= a.max_iter + 1) a.fmt[i] = fmt0;
+    if (i < a.n_hub) a.hub_ticket[i] = 0;
+    if (i >= a.N) return;
+    a.pr[i] = x0;
+    const uint32_t od = a.outdeg[i];
+    store_c(a.c0, fmt0, i, od ? x0 / (float)od : 0.0f);
+}
+
+// Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
+// the caller's order; per-block FP64 partial sums of the returned ranks.
+__global__ __launch_bounds__(kBlock) void k_pr_finish(const float *__restrict__ pr,
+                                                      const uint32_t *__restrict__ order,
+                                                      uint32_t N, uint32_t divide,
+                                                      float *__restrict__ out,
+                                                      double *__restrict__ part)
+{
+    __shared__ double s[kBlock / 64];
+    double acc = 0.0;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < N) {
+        float x = pr[i];
+        if (divide) x = x / (float)N;
+        out[order[i]] = x;
+        acc = (double)x;
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+__global__ void k_sum_parts(const double *part, uint32_t n, double *out)
+{
+    __shared__ double s[kBlock / 64];
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < n; b += kBlock) acc += part[b];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) *out = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// ---- plan construction kernels ----
+__global__ void k_degree_keys(const uint32_t *__restrict__ indeg, uint32_t N, uint32_t maxdeg,
+                              uint32_t *__restrict__ keys, uint32_t *__restrict__ ids)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    keys[v] = maxdeg - indeg[v];
+    ids[v] = v;
+}
+
+__global__ void k_relabel_vertex(const uint32_t *__restrict__ order,
+                                 const uint32_t *__restrict__ skeys, uint32_t maxdeg,
+                                 const uint32_t *__restrict__ fwd_outdeg, uint32_t N,
+                                 uint32_t *__restrict__ rank, uint32_t *__restrict__ rdeg,
+                                 uint32_t *__restrict__ outdeg_r, uint32_t low_max,
+                                 uint32_t hub_min, uint32_t *__restrict__ counts)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t d = 0, od = 1;
+    if (i < N) {
+        const uint32_t v = order[i];
+        rank[v] = i;
+        d = maxdeg - skeys[i];
+        rdeg[i] = d;
+        od = fwd_outdeg[v];
+        outdeg_r[i] = od;
+    }
+    const uint64_t hub = __ballot(i < N && d >= hub_min);
+    const uint64_t nonlow = __ballot(i < N && d > low_max);
+    const uint64_t nonzero = __ballot(i < N && d > 0);
+    const uint64_t dangl = __ballot(i < N && od == 0);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counts[0], (uint32_t)__popcll(hub));
+        atomicAdd(&counts[1], (uint32_t)__popcll(nonlow));
+        atomicAdd(&counts[2], (uint32_t)__popcll(nonzero));
+        atomicAdd(&counts[3], (uint32_t)__popcll(dangl));
+    }
+}
+
+__global__ void k_slice_width(const uint32_t *__restrict__ rdeg, uint32_t low_begin,
+                              uint32_t nslices, uint32_t *__restrict__ wid)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nslices) wid[s] = 64u * rdeg[low_begin + s * 64];
+}
+
+__global__ void k_add_base(uint32_t *__restrict__ x, uint32_t n, uint32_t base)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += base;
+}
+
+// CSR rows (hub + mid): one wave per row, relabelled sources in original order.
+__global__ void k_fill_csr_rows(const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
+                                const uint32_t *__restrict__ inv_idx,
+                                const uint32_t *__restrict__ inv_dst,
+                                const uint32_t *__restrict__ rank, uint32_t nrows,
+                                uint32_t *__restrict__ cols)
+{
+    const uint32_t row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= nrows) return;
+    const uint32_t v = order[row];
+    const uint32_t b = roff[row], n = roff[row + 1] - b;
+    const uint32_t s = inv_idx[v];
+    for (uint32_t k = lane; k < n; k += 64) cols[b + k] = rank[inv_dst[s + k]];
+}
+
+// SELL rows: one thread per row.
+__global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint32_t *__restrict__ rdeg,
+                                 const uint32_t *__restrict__ inv_idx,
+                                 const uint32_t *__restrict__ inv_dst,
+                                 const uint32_t *__restrict__ rank,
+                                 const uint32_t *__restrict__ slice_ptr, uint32_t low_begin,
+                                 uint32_t nlow, uint32_t *__restrict__ cols)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nlow) return;
+    const uint32_t row = low_begin + t;
+    const uint32_t v = order[row];
+    const uint32_t n = rdeg[row];
+    const uint32_t s = inv_idx[v];
+    uint32_t *p = cols + slice_ptr[t / 64] + (t % 64);
+    for (uint32_t k = 0; k < n; k++) p[k * 64] = rank[inv_dst[s + k]];
+}
+
+} // namespace
+
+static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t N = g->N, E = g->E;
+    const Tuning tn = tuning();
+    std::unique_ptr<Plan> p(new Plan());
+    p->N = N;
+    p->E = E;
+    p->low_max = tn.low_max;
+    p->hub_min = tn.hub_min;
+    p->hub_chunk = tn.hub_chunk;
+    if (!g->inv_outdeg.p || !g->inv_idx.p || !g->inv_dst.p || !g->fwd_outdeg.p)
+        fail(AG_ERR_INVALID, "graph has no inverse CSR (built with AG_BUILD_NO_INVERSE?)");
+
+    p->order.alloc((size_t)N * 4);
+    p->outdeg.alloc((size_t)N * 4);
+    p->roff.alloc((size_t)(N + 1) * 4);
+    if (N == 0) {
+        AG_HIP(hipMemsetAsync(p->roff.p, 0, 4, s));
+        AG_HIP(hipStreamSynchronize(s));
+        return p.release();
+    }
+    const uint32_t maxdeg = device_max1(ctx, g->inv_outdeg.as<uint32_t>(), N);
+    const uint32_t kbits = bits_for((uint64_t)maxdeg + 1);
+    DevBuf keys((size_t)N * 4), ids((size_t)N * 4), skeys((size_t)N * 4);
+    DevBuf rank((size_t)N * 4), rdeg((size_t)N * 4), counts(16);
+    const uint32_t nb = ceil_div(N, 256);
+    hipLaunchKernelGGL(k_degree_keys, dim3(nb), dim3(256), 0, s, g->inv_outdeg.as<uint32_t>(), N,
+                       maxdeg, keys.as<uint32_t>(), ids.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    radix_sort_pairs(ctx, keys.as<uint32_t>(), ids.as<uint32_t>(), skeys.as<uint32_t>(),
+                     p->order.as<uint32_t>(), N, kbits);
+    AG_HIP(hipMemsetAsync(counts.p, 0, 16, s));
+    hipLaunchKernelGGL(k_relabel_vertex, dim3(nb), dim3(256), 0, s, p->order.as<uint32_t>(),
+                       skeys.as<uint32_t>(), maxdeg, g->fwd_outdeg.as<uint32_t>(), N,
+                       rank.as<uint32_t>(), rdeg.as<uint32_t>(), p->outdeg.as<uint32_t>(),
+                       tn.low_max, tn.hub_min, counts.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    scan_exclusive_u32(ctx, rdeg.as<uint32_t>(), p->roff.as<uint32_t>(), N);
+    uint32_t cnt[4];
+    AG_HIP(hipMemcpyAsync(cnt, counts.p, 16, hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    p->n_hub = cnt[0];
+    p->n_nonlow = cnt[1];
+    p->n_nonzero = cnt[2];
+    p->n_dangling = cnt[3];
+    uint32_t csr_edges = 0;
+    AG_HIP(hipMemcpyAsync(&csr_edges, p->roff.as<uint32_t>() + p->n_nonlow, 4,
+                          hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    p->csr_edges = csr_edges;
+    const uint32_t nlow = p->n_nonzero - p->n_nonlow;
+    p->low_slices = ceil_div(nlow, 64);
+    p->slice_ptr.alloc((size_t)(p->low_slices + 1) * 4);
+    uint32_t sell_slots = 0;
+    if (p->low_slices) {
+        hipLaunchKernelGGL(k_slice_width, dim3(ceil_div(p->low_slices, 256)), dim3(256), 0, s,
+                           rdeg.as<uint32_t>(), p->n_nonlow, p->low_slices,
+                           p->slice_ptr.as<uint32_t>());
+        scan_exclusive_u32(ctx, p->slice_ptr.as<uint32_t>(), p->slice_ptr.as<uint32_t>(),
+                           p->low_slices);
+        AG_HIP(hipMemcpyAsync(&sell_slots, p->slice_ptr.as<uint32_t>() + p->low_slices, 4,
+                              hipMemcpyDeviceToHost, s));
+        AG_HIP(hipStreamSynchronize(s));
+        if ((uint64_t)csr_edges + sell_slots > 0xffffffffull)
+            fail(AG_ERR_INVALID, "gather layout exceeds 2^32 slots");
+        hipLaunchKernelGGL(k_add_base, dim3(ceil_div(p->low_slices + 1, 256)), dim3(256), 0, s,
+                           p->slice_ptr.as<uint32_t>(), p->low_slices + 1, csr_edges);
+    } else {
+        AG_HIP(hipMemsetAsync(p->slice_ptr.p, 0, 4, s));
+    }
+    p->sell_slots = sell_slots;
+    const uint64_t ncols = (uint64_t)csr_edges + sell_slots;
+    p->cols.alloc((size_t)ncols * 4);
+    AG_HIP(hipMemsetAsync(p->cols.p, 0, (size_t)ncols * 4, s));
+    if (p->n_nonlow)
+        hipLaunchKernelGGL(k_fill_csr_rows, dim3(ceil_div(p->n_nonlow, 4)), dim3(256), 0, s,
+                           p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
+                           g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
+                           rank.as<uint32_t>(), p->n_nonlow, p->cols.as<uint32_t>());
+    if (nlow)
+        hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
+                           p->order.as<uint32_t>(), rdeg.as<uint32_t>(),
+                           g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
+                           rank.as<uint32_t>(), p->slice_ptr.as<uint32_t>(), p->n_nonlow, nlow,
+                           p->cols.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+
+    // Hub chunk table (host: hub rows are few).
+    std::vector<uint32_t> hdeg(p->n_hub);
+    if (p->n_hub)
+        AG_HIP(hipMemcpyAsync(hdeg.data(), rdeg.p, (size_t)p->n_hub * 4, hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> first(p->n_hub + 1), crow;
+    uint32_t tot = 0;
+    for (uint32_t i = 0; i < p->n_hub; i++) {
+        first[i] = tot;
+        const uint32_t nch = ceil_div(hdeg[i], tn.hub_chunk);
+        for (uint32_t q = 0; q < nch; q++) crow.push_back(i);
+        tot += nch;
+    }
+    first[p->n_hub] = tot;
+    p->hub_chunks = tot;
+    p->chunk_first.alloc((size_t)(p->n_hub + 1) * 4);
+    p->chunk_row.alloc((size_t)std::max<uint32_t>(tot, 1) * 4);
+    p->hub_partial.alloc((size_t)std::max<uint32_t>(tot, 1) * 4);
+    p->hub_ticket.alloc((size_t)std::max<uint32_t>(p->n_hub, 1) * 4);
+    AG_HIP(hipMemcpyAsync(p->chunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
+    if (tot) AG_HIP(hipMemcpyAsync(p->chunk_row.p, crow.data(), crow.size() * 4, hipMemcpyHostToDevice, s));
+    AG_HIP(hipMemsetAsync(p->hub_ticket.p, 0, p->hub_ticket.bytes, s));
+
+    p->nb_hub = tot;
+    p->nb_mid = ceil_div(p->n_nonlow - p->n_hub, kMidRowsPerBlock);
+    p->nb_low = ceil_div(p->low_slices, 4);
+    p->nb_zero = ceil_div(N - p->n_nonzero, kZeroRowsPerBlock);
+    p->nvblocks = p->nb_hub + p->nb_mid + p->nb_low + p->nb_zero;
+    AG_HIP(hipStreamSynchronize(s));
+    return p.release();
+}
+
+static Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms)
+{
+    if (ms) *ms = 0.0;
+    if (g->plan) return g->plan;
+    auto t0 = std::chrono::steady_clock::now();
+    g->plan = build_plan(ctx, g);
+    auto t1 = std::chrono::steady_clock::now();
+    if (ms) *ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return g->plan;
+}
+
+static void ensure_work(ag_ctx *ctx, Plan *p, uint32_t max_iter)
+{
+    Work &w = p->work;
+    const uint32_t N = p->N;
+    if (!w.pr.p) {
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(p->nvblocks, (uint32_t)ctx->num_cus * 8));
+        w.grid = grid;
+        w.pr.alloc((size_t)N * 4);
+        w.c0.alloc((size_t)N * 4);
+        w.c1.alloc((size_t)N * 4);
+        w.blk_err.alloc((size_t)grid * 8);
+        w.blk_dang.alloc((size_t)grid * 8);
+        w.blk_act.alloc((size_t)grid * 4);
+        w.state.alloc(sizeof(SweepState));
+        w.sum_part.alloc((size_t)(ceil_div(N, kBlock) + 1) * 8 + 8);
+        w.out.alloc((size_t)N * 4);
+    }
+    if (w.max_iter < max_iter || !w.err_iter.p) {
+        w.err_iter.alloc((size_t)(max_iter + 1) * 8);
+        w.act_iter.alloc((size_t)(max_iter + 1) * 4);
+        w.fmt.alloc((size_t)(max_iter + 3) * 4);
+        w.max_iter = max_iter;
+    }
+}
+
+static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_iter)
+{
+    const Work &w = p->work;
+    SweepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.N = p->N;
+    a.n_hub = p->n_hub;
+    a.n_nonlow = p->n_nonlow;
+    a.n_nonzero = p->n_nonzero;
+    a.nb_hub = p->nb_hub;
+    a.nb_mid = p->nb_mid;
+    a.nb_low = p->nb_low;
+    a.nb_zero = p->nb_zero;
+    a.nvblocks = p->nvblocks;
+    a.hub_chunk = p->hub_chunk;
+    a.mode = o.mode;
+    a.max_iter = max_iter;
+    a.finish_fp32 = (o.flags & AG_PR_FLAG_FINISH_FP32) ? 1u : 0u;
+    a.damp = o.damp;
+    {
+#pragma clang fp contract(off)
+        a.base = 1.0f - o.damp;
+    }
+    a.epsilon = o.epsilon;
+    a.inv_n = p->N ? 1.0 / (double)p->N : 0.0;
+    a.roff = p->roff.as<uint32_t>();
+    a.cols = p->cols.as<uint32_t>();
+    a.slice_ptr = p->slice_ptr.as<uint32_t>();
+    a.outdeg = p->outdeg.as<uint32_t>();
+    a.chunk_row = p->chunk_row.as<uint32_t>();
+    a.chunk_first = p->chunk_first.as<uint32_t>();
+    a.hub_partial = p->hub_partial.as<float>();
+    a.hub_ticket = p->hub_ticket.as<uint32_t>();
+    a.pr = w.pr.as<float>();
+    a.c0 = w.c0.p;
+    a.c1 = w.c1.p;
+    a.blk_err = w.blk_err.as<double>();
+    a.blk_dang = w.blk_dang.as<double>();
+    a.blk_act = w.blk_act.as<uint32_t>();
+    a.st = w.state.as<SweepState>();
+    a.err_iter = w.err_iter.as<double>();
+    a.act_iter = w.act_iter.as<uint32_t>();
+    a.fmt = w.fmt.as<uint32_t>();
+    return a;
+}
+
+// Runs the sweep loop; ranks land in d_out (device, caller order).
+static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float *d_out,
+                         uint32_t *active_iter, double *err_iter, ag_pr_result *res)
+{
+    hipStream_t s = ctx->stream;
+    ag_pr_result r;
+    std::memset(&r, 0, sizeof(r));
+    Plan *p = ensure_plan(ctx, g, &r.plan_ms);
+    std::lock_guard<std::mutex> lk(p->mu);
+    const uint32_t N = p->N;
+    const uint32_t max_iter = o.iterations;
+    if (o.mode != AG_PR_REFERENCE && o.mode != AG_PR_STANDARD)
+        fail(AG_ERR_INVALID, "unknown PageRank mode %u", o.mode);
+    if (o.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", o.precision);
+    if (!(o.damp > 0.0f && o.damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
+    if (N == 0) {
+        if (res) *res = r;
+        return;
+    }
+    ensure_work(ctx, p, max_iter);
+    SweepArgs a = make_args(p, o, max_iter);
+    const float x0 = (o.mode == AG_PR_REFERENCE) ? a.base : (float)(1.0 / (double)N);
+    const double dang0 = (double)p->n_dangling * (1.0 / (double)N);
+    const float tele0 = (float)((1.0 - (double)o.damp) / N + (double)o.damp * dang0 / N);
+    hipEvent_t ev[4];
+    for (auto &e : ev) AG_HIP(hipEventCreate(&e));
+    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } eg{ev};
+
+    AG_HIP(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_pr_init, dim3(ceil_div((uint64_t)std::max(N, max_iter + 2), 256)),
+                       dim3(256), 0, s, a, x0, tele0, o.precision);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipEventRecord(ev[1], s));
+    const uint32_t grid = p->work.grid;
+    const bool can_stop = o.epsilon > 0.0;
+    const uint32_t group = can_stop ? 16u : max_iter;
+    for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
+        const uint32_t it1 = std::min(max_iter, it0 + group);
+        for (uint32_t it = it0; it < it1; it++)
+            hipLaunchKernelGGL(k_pr_sweep, dim3(grid), dim3(kBlock), 0, s, a, it);
+        AG_HIP(hipGetLastError());
+        if (can_stop && it1 < max_iter) {
+            uint32_t done = 0;
+            AG_HIP(hipMemcpyAsync(&done, &a.st->done, 4, hipMemcpyDeviceToHost, s));
+            AG_HIP(hipStreamSynchronize(s));
+            if (done) break;
+        }
+    }
+    AG_HIP(hipEventRecord(ev[2], s));
+    const uint32_t nbf = ceil_div(N, kBlock);
+    double *parts = p->work.sum_part.as<double>();
+    hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, p->work.pr.as<float>(),
+                       p->order.as<uint32_t>(), N, o.mode == AG_PR_REFERENCE ? 1u : 0u, d_out,
+                       parts);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipEventRecord(ev[3], s));
+    SweepState st;
+    AG_HIP(hipMemcpyAsync(&st, a.st, sizeof(st), hipMemcpyDeviceToHost, s));
+    AG_HIP(hipMemcpyAsync(&r.pr_sum, parts + nbf, 8, hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    if (active_iter && st.sweeps)
+        AG_HIP(hipMemcpyAsync(active_iter, a.act_iter, (size_t)st.sweeps * 4,
+                              hipMemcpyDeviceToHost, s));
+    if (err_iter && st.sweeps)
+        AG_HIP(hipMemcpyAsync(err_iter, a.err_iter, (size_t)st.sweeps * 8,
+                              hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    AG_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    r.loop_ms = ms;
+    AG_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
+    r.total_ms = ms;
+    r.iterations = st.iterations;
+    r.sweeps = st.sweeps;
+    r.active_vertices = st.last_active;
+    r.error_total = st.last_err;
+    r.fp32_switch_sweep = st.fp32_switch;
+    if (res) *res = r;
+}
+
+} // namespace ag
+
+using namespace ag;
+
+extern "C" {
+
+void ag_pr_options_default(ag_pr_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->iterations = 20;
+    o->mode = AG_PR_REFERENCE;
+    o->precision = AG_PREC_FP32;
+    o->epsilon = 1e-4;
+    o->damp = 0.85f;
+}
+
+int ag_pagerank_prepare(ag_ctx *ctx, ag_graph *g, double *ms)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g) fail(AG_ERR_INVALID, "null graph");
+        ensure_plan(ctx, g, ms);
+    });
+}
+
+int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info)
+{
+    return guard([&] {
+        if (!g || !info) fail(AG_ERR_INVALID, "null argument");
+        std::memset(info, 0, sizeof(*info));
+        const Plan *p = g->plan;
+        if (!p) fail(AG_ERR_INVALID, "no plan built yet (call ag_pagerank_prepare)");
+        info->n_hub = p->n_hub;
+        info->n_mid = p->n_nonlow - p->n_hub;
+        info->n_low = p->n_nonzero - p->n_nonlow;
+        info->n_zero = p->N - p->n_nonzero;
+        info->hub_chunks = p->hub_chunks;
+        info->low_max = p->low_max;
+        info->hub_min = p->hub_min;
+        info->sell_slots = p->sell_slots;
+        info->plan_bytes = p->bytes();
+    });
+}
+
+int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *ranks,
+                     uint32_t *active_iter, double *err_iter, ag_pr_result *res)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
+        Plan *p = ensure_plan(ctx, g, nullptr);
+        if (g->N == 0) {
+            run_pagerank(ctx, g, *opt, nullptr, active_iter, err_iter, res);
+            return;
+        }
+        ensure_work(ctx, p, opt->iterations);
+        run_pagerank(ctx, g, *opt, p->work.out.as<float>(), active_iter, err_iter, res);
+        if (ranks) {
+            AG_HIP(hipMemcpyAsync(ranks, p->work.out.p, (size_t)g->N * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            AG_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    });
+}
+
+int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *d_ranks,
+                            ag_pr_result *res)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
+        if (g->N && !d_ranks) fail(AG_ERR_INVALID, "null output");
+        run_pagerank(ctx, g, *opt, d_ranks, nullptr, nullptr, res);
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,396 @@
+// Device sort / scan / vertex-mapping primitives for the GraphCSR builder.
+//
+// The reference builds its CSR with OpenGraph's stable counting / LSD radix
+// sort by source (sortRunAlgorithms, modes at README.md:434-439, called at
+// SSSP.c:548-553) followed by a vertex mapping that yields out_degree and
+// edges_idx (fields read at capienv.c:409-426).  Here the same permutation is
+// produced by stable LSD radix passes of 8-bit digits, each pass being:
+//
+//   k_radix_hist     per-tile digit histogram in LDS (tile = 4096 keys)
+//   k_hist_*         digit-major exclusive scan of the tile histograms
+//                    (chunk sums -> one-block chunk scan -> chunk apply)
+//   k_radix_scatter  per-wave stable ranking with ballot-based digit
+//                    matching, cross-wave prefix in LDS, scatter
+//
+// Every step is deterministic and atomic-free in global memory, so the sorted
+// arrays are bit-identical run to run and identical to a serial stable
+// counting sort.
+#include "ag_internal.h"
+
+namespace ag {
+
+namespace {
+
+constexpr int kBlock = 256;       // threads per workgroup (4 wave64)
+constexpr int kItems = 16;        // keys per thread per tile
+constexpr int kTile = kBlock * kItems;
+constexpr int kRadix = 256;       // 8-bit digits
+constexpr int kWaves = kBlock / 64;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// ---- generic exclusive scan ---------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t *__restrict__ in,
+                                                        uint32_t n,
+                                                        uint32_t *__restrict__ block_sums)
+{
+    __shared__ uint32_t ws[kWaves];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) ws[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kWaves; i++) t += ws[i];
+        block_sums[blockIdx.x] = t;
+    }
+}
+
+// Block-exclusive scan of one tile (blocked: thread t owns [t*16, t*16+16)),
+// plus a carry-in per block.  The last block writes out[n] = total.
+__global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32_t *out,
+                                                      uint32_t n,
+                                                      const uint32_t *__restrict__ block_pref,
+                                                      uint32_t nblocks)
+{
+    __shared__ uint32_t ws[kWaves];
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        uint64_t idx = base + i;
+        v[i] = idx < n ? in[idx] : 0u;
+        s += v[i];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = wave_incl_scan(s, lane);
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wpref = 0, btot = 0;
+    for (int i = 0; i < kWaves; i++) {
+        if (i < w) wpref += ws[i];
+        btot += ws[i];
+    }
+    uint32_t run = block_pref[blockIdx.x] + wpref + incl - s;
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        uint64_t idx = base + i;
+        if (idx < n) out[idx] = run;
+        run += v[i];
+    }
+    if (blockIdx.x == nblocks - 1 && threadIdx.x == 0) out[n] = block_pref[blockIdx.x] + btot;
+}
+
+// ---- radix sort -----------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restrict__ keys,
+                                                       uint32_t n, uint32_t shift,
+                                                       uint32_t mask,
+                                                       uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * kTile;
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = h[threadIdx.x];
+}
+
+// Phase 1: per chunk of `tpc` tiles, per digit sums.
+__global__ __launch_bounds__(kRadix) void k_hist_chunk_sums(const uint32_t *__restrict__ hist,
+                                                            uint32_t ntiles, uint32_t tpc,
+                                                            uint32_t *__restrict__ csum)
+{
+    const uint32_t d = threadIdx.x;
+    const uint32_t t0 = blockIdx.x * tpc;
+    const uint32_t t1 = min(ntiles, t0 + tpc);
+    uint32_t s = 0;
+#pragma unroll 8
+    for (uint32_t t = t0; t < t1; t++) s += hist[(uint64_t)t * kRadix + d];
+    csum[(uint64_t)blockIdx.x * kRadix + d] = s;
+}
+
+// Phase 2 (one block): exclusive prefix over chunks per digit, then over digits.
+__global__ __launch_bounds__(kRadix) void k_hist_chunk_scan(uint32_t *__restrict__ csum,
+                                                            uint32_t nchunks)
+{
+    __shared__ uint32_t tot[kRadix];
+    __shared__ uint32_t ws[kRadix / 64];
+    const uint32_t d = threadIdx.x;
+    uint32_t run = 0;
+    for (uint32_t c = 0; c < nchunks; c++) {
+        uint32_t x = csum[(uint64_t)c * kRadix + d];
+        csum[(uint64_t)c * kRadix + d] = run;
+        run += x;
+    }
+    tot[d] = run;
+    __syncthreads();
+    const int lane = d & 63, w = d >> 6;
+    uint32_t incl = wave_incl_scan(run, lane);
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t wpref = 0;
+    for (int i = 0; i < w; i++) wpref += ws[i];
+    const uint32_t digit_base = wpref + incl - run;
+    for (uint32_t c = 0; c < nchunks; c++) csum[(uint64_t)c * kRadix + d] += digit_base;
+}
+
+// Phase 3: rewrite each tile histogram entry as its global exclusive offset.
+__global__ __launch_bounds__(kRadix) void k_hist_chunk_apply(uint32_t *__restrict__ hist,
+                                                             uint32_t ntiles, uint32_t tpc,
+                                                             const uint32_t *__restrict__ csum)
+{
+    const uint32_t d = threadIdx.x;
+    const uint32_t t0 = blockIdx.x * tpc;
+    const uint32_t t1 = min(ntiles, t0 + tpc);
+    uint32_t run = csum[(uint64_t)blockIdx.x * kRadix + d];
+    for (uint32_t t = t0; t < t1; t++) {
+        uint64_t i = (uint64_t)t * kRadix + d;
+        uint32_t x = hist[i];
+        hist[i] = run;
+        run += x;
+    }
+}
+
+// Stable scatter.  Wave w of a tile owns the 1024 consecutive keys
+// [tile*4096 + w*1024, +1024) and walks them in 16 steps of 64.  In each step
+// the lanes holding equal digits are found with `nbits` ballots; the rank of a
+// lane is the wave's running count for its digit plus the number of equal-digit
+// lanes below it.  The lowest lane of each equal-digit group then advances the
+// running count.  After all steps the per-wave counts are turned into a
+// cross-wave exclusive prefix, which keeps the order wave 0 < 1 < 2 < 3, i.e.
+// the input order.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
+    uint32_t nbits, const uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t wcnt[kWaves][kRadix];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t mask = (1u << nbits) - 1u;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) wcnt[i][threadIdx.x] = 0;
+    __syncthreads();
+
+    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)w * (kTile / kWaves);
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t k[kItems], v[kItems], r[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        const uint64_t idx = base + (uint64_t)i * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t val = valid ? vin[idx] : 0u;
+        const uint32_t d = (key >> shift) & mask;
+        uint64_t peers = __ballot(valid);
+        for (uint32_t b = 0; b < nbits; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = 0;
+        if (valid) {
+            const uint32_t before = wcnt[w][d];
+            rank = before + (uint32_t)__popcll(peers & lt_mask);
+            const int leader = __ffsll((unsigned long long)peers) - 1;
+            if (lane == leader) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+        }
+        k[i] = key;
+        v[i] = val;
+        r[i] = rank;
+    }
+    __syncthreads();
+    {
+        const uint32_t d = threadIdx.x;
+        uint32_t run = hist[(uint64_t)blockIdx.x * kRadix + d];
+#pragma unroll
+        for (int i = 0; i < kWaves; i++) {
+            uint32_t c = wcnt[i][d];
+            wcnt[i][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        const uint64_t idx = base + (uint64_t)i * 64 + lane;
+        if (idx < n) {
+            const uint32_t d = (k[i] >> shift) & mask;
+            const uint32_t pos = wcnt[w][d] + r[i];
+            kout[pos] = k[i];
+            vout[pos] = v[i];
+        }
+    }
+}
+
+// ---- vertex mapping -------------------------------------------------------------
+// Segment starts -> edges_idx[v]; segment ends -> deg[v] = end (fixed up below).
+__global__ void k_segment_bounds(const uint32_t *__restrict__ keys, uint32_t n,
+                                 uint32_t *__restrict__ idx, uint32_t *__restrict__ end)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = keys[i];
+    if (i == 0 || keys[i - 1] != v) idx[v] = (uint32_t)i;
+    if (i == (uint64_t)n - 1 || keys[i + 1] != v) end[v] = (uint32_t)(i + 1);
+}
+
+__global__ void k_segment_degree(const uint32_t *__restrict__ idx, uint32_t *__restrict__ deg,
+                                 uint32_t N)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const uint32_t e = deg[v];
+    deg[v] = e ? e - idx[v] : 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_max2(const uint32_t *__restrict__ a,
+                                                 const uint32_t *__restrict__ b, uint32_t n,
+                                                 uint32_t *__restrict__ out)
+{
+    uint32_t m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kBlock) {
+        m = max(m, a[i]);
+        if (b) m = max(m, b[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+} // namespace
+
+void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n)
+{
+    hipStream_t s = ctx->stream;
+    if (n == 0) {
+        AG_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
+        return;
+    }
+    const uint32_t nb = ceil_div(n, kTile);
+    DevBuf sums((size_t)(nb + 1) * sizeof(uint32_t));
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, sums.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    if (nb == 1) {
+        AG_HIP(hipMemsetAsync(sums.p, 0, sizeof(uint32_t), s));
+    } else {
+        scan_exclusive_u32(ctx, sums.as<uint32_t>(), sums.as<uint32_t>(), nb);
+    }
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, out, n,
+                       sums.as<uint32_t>(), nb);
+    AG_HIP(hipGetLastError());
+    // `sums` is freed at scope exit: hipFree synchronises with pending work.
+}
+
+void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                      uint32_t *vout, uint32_t n, uint32_t key_bits)
+{
+    hipStream_t s = ctx->stream;
+    if (n == 0) return;
+    const uint32_t passes = (key_bits + 7) / 8;
+    if (passes == 0) {
+        AG_HIP(hipMemcpyAsync(kout, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        AG_HIP(hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    const uint32_t ntiles = ceil_div(n, kTile);
+    const uint32_t tpc = ceil_div(ntiles, 256);
+    const uint32_t nchunks = ceil_div(ntiles, tpc);
+    DevBuf hist((size_t)ntiles * kRadix * 4);
+    DevBuf csum((size_t)nchunks * kRadix * 4);
+    DevBuf tk, tv;
+    if (passes > 1) {
+        tk.alloc((size_t)n * 4);
+        tv.alloc((size_t)n * 4);
+    }
+    // Ping-pong so that the last pass lands in (kout, vout).
+    const uint32_t *ck = kin, *cv = vin;
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t shift = 8 * p;
+        const uint32_t nbits = std::min<uint32_t>(8, key_bits - shift);
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t *ok = to_out ? kout : tk.as<uint32_t>();
+        uint32_t *ov = to_out ? vout : tv.as<uint32_t>();
+        hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ck, n, shift,
+                           (1u << nbits) - 1u, hist.as<uint32_t>());
+        hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nchunks), dim3(kRadix), 0, s,
+                           hist.as<uint32_t>(), ntiles, tpc, csum.as<uint32_t>());
+        hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum.as<uint32_t>(),
+                           nchunks);
+        hipLaunchKernelGGL(k_hist_chunk_apply, dim3(nchunks), dim3(kRadix), 0, s,
+                           hist.as<uint32_t>(), ntiles, tpc, csum.as<uint32_t>());
+        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, ck, cv, ok, ov, n,
+                           shift, nbits, hist.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        ck = ok;
+        cv = ov;
+    }
+}
+
+void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N,
+                uint32_t *degree, uint32_t *edges_idx, uint32_t *offsets)
+{
+    hipStream_t s = ctx->stream;
+    if (N == 0) {
+        if (offsets) AG_HIP(hipMemsetAsync(offsets, 0, 4, s));
+        return;
+    }
+    AG_HIP(hipMemsetAsync(degree, 0, (size_t)N * 4, s));
+    AG_HIP(hipMemsetAsync(edges_idx, 0, (size_t)N * 4, s));
+    if (n) {
+        hipLaunchKernelGGL(k_segment_bounds, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                           sorted_keys, n, edges_idx, degree);
+        hipLaunchKernelGGL(k_segment_degree, dim3(ceil_div(N, 256)), dim3(256), 0, s, edges_idx,
+                           degree, N);
+        AG_HIP(hipGetLastError());
+    }
+    if (offsets) scan_exclusive_u32(ctx, degree, offsets, N);
+}
+
+static uint32_t device_max_impl(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n)
+{
+    if (n == 0) return 0;
+    DevBuf out(4);
+    AG_HIP(hipMemsetAsync(out.p, 0, 4, ctx->stream));
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock * 8), 4096);
+    hipLaunchKernelGGL(k_max2, dim3(blocks), dim3(kBlock), 0, ctx->stream, a, b, n,
+                       out.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    uint32_t m = 0;
+    AG_HIP(hipMemcpyAsync(&m, out.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    AG_HIP(hipStreamSynchronize(ctx->stream));
+    return m;
+}
+
+uint32_t device_max2(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n)
+{
+    return device_max_impl(ctx, a, b, n);
+}
+uint32_t device_max1(ag_ctx *ctx, const uint32_t *a, uint32_t n)
+{
+    return device_max_impl(ctx, a, nullptr, n);
+}
+
+} // namespace ag

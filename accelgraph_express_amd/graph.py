@@ -1,0 +1,255 @@
+"""Python host-side mirror of the CSR builder / pull-PageRank path.
+
+Thin wrappers over the C ABI (``_lib``) with numpy arrays at the boundary.  The
+names follow the reference's C API where one exists:
+
+* ``Graph.build``          GraphCSR builder: edge list -> stable sort by src ->
+                           vertex mapping, plus the inverse CSR
+                           (SSSP.c:485-488,518-520,548-563 call sequence)
+* ``Graph.download``       the GraphCSR arrays (``vertices->out_degree`` ...,
+                           ``sorted_edges_array->edges_array_src`` ...)
+* ``Graph.pagerank_pull``  pageRankPullGraphCSR (pageRank.c:942-1108) and the
+                           PageRankStats fields it fills (pageRank.h:19-30)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a: np.ndarray, ct=C.c_uint32):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Context:
+    """One HIP device + one stream (``ag_ctx``)."""
+
+    def __init__(self, device: int = -1):
+        lib = L.load()
+        h = C.c_void_p()
+        L.check(lib.ag_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def synchronize(self) -> None:
+        L.check(L.load().ag_ctx_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return L.load().ag_ctx_stream(self.h) or 0
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.load().ag_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class PageRankStats:
+    """Mirror of struct PageRankStats (pageRank.h:19-30) plus device timings."""
+    damp: float
+    base_pr: float
+    iterations: int
+    num_vertices: int
+    activeVertices: int
+    pageRanks: np.ndarray
+    error_total: float
+    pr_sum: float
+    sweeps: int
+    loop_ms: float
+    total_ms: float
+    plan_ms: float
+    fp32_switch_sweep: int = 0
+    active_per_iter: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    error_per_iter: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+    @property
+    def realRanks(self) -> np.ndarray:
+        return np.arange(self.num_vertices, dtype=np.uint32)
+
+
+_MODES = {"reference": L.AG_PR_REFERENCE, "standard": L.AG_PR_STANDARD}
+_PRECS = {"fp32": L.AG_PREC_FP32, "bf16": L.AG_PREC_BF16, "fp16": L.AG_PREC_FP16}
+_SORTS = {"radix_src": 0, "radix_src_dst": 1, "count_src": 2, "count_src_dst": 3}
+
+
+class Graph:
+    """A device-resident graph (``ag_graph``)."""
+
+    def __init__(self, ctx: Context, handle: C.c_void_p):
+        self.ctx = ctx
+        self.h = handle
+
+    # ---- construction ----
+    @classmethod
+    def build(cls, ctx: Context, src, dst, num_vertices: int = 0, sort=0,
+              inverse_from_input: bool = False, device_ptrs: tuple | None = None,
+              num_edges: int | None = None) -> "Graph":
+        """Builds forward + inverse CSR on the GPU.
+
+        ``sort`` is the reference's sort mode (0 radix-src, 1 radix-src-dst,
+        2 count-src, 3 count-src-dst) or its name.  ``device_ptrs`` =
+        (src_ptr, dst_ptr) builds from device memory instead of host arrays.
+        """
+        lib = L.load()
+        mode = _SORTS[sort] if isinstance(sort, str) else int(sort)
+        flags = mode | (L.AG_BUILD_INVERSE_FROM_INPUT if inverse_from_input else 0)
+        h = C.c_void_p()
+        if device_ptrs is not None:
+            flags |= L.AG_BUILD_DEVICE_INPUT
+            L.check(lib.ag_csr_build(ctx.h, C.c_void_p(device_ptrs[0]), C.c_void_p(device_ptrs[1]),
+                                     int(num_edges), num_vertices, flags, C.byref(h)))
+        else:
+            s, d = _u32(src), _u32(dst)
+            if s.shape != d.shape:
+                raise ValueError("src and dst differ in length")
+            L.check(lib.ag_csr_build(ctx.h, s.ctypes.data_as(C.c_void_p),
+                                     d.ctypes.data_as(C.c_void_p), s.size, num_vertices, flags,
+                                     C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_csr(cls, ctx: Context, num_vertices: int, fwd_out_degree, inv_out_degree,
+                 inv_edges_idx, inv_dest) -> "Graph":
+        """Uploads an existing (host) CSR: what pull PageRank reads."""
+        lib = L.load()
+        a, b, c, d = (_u32(x) for x in (fwd_out_degree, inv_out_degree, inv_edges_idx, inv_dest))
+        h = C.c_void_p()
+        L.check(lib.ag_graph_from_csr(ctx.h, num_vertices, d.size, _ptr(a), _ptr(b), _ptr(c),
+                                      _ptr(d), C.byref(h)))
+        return cls(ctx, h)
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            L.load().ag_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def num_vertices(self) -> int:
+        return int(L.load().ag_graph_num_vertices(self.h))
+
+    @property
+    def num_edges(self) -> int:
+        return int(L.load().ag_graph_num_edges(self.h))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(L.load().ag_graph_device_bytes(self.h))
+
+    def view(self) -> L.CsrView:
+        v = L.CsrView()
+        L.check(L.load().ag_graph_view(self.h, C.byref(v)))
+        return v
+
+    def download(self) -> dict:
+        """GraphCSR arrays on the host, keyed by the reference's field paths."""
+        n, m = self.num_vertices, self.num_edges
+        names = ["vertices.out_degree", "vertices.in_degree", "vertices.edges_idx",
+                 "sorted_edges_array.edges_array_src", "sorted_edges_array.edges_array_dest",
+                 "inverse_vertices.out_degree", "inverse_vertices.in_degree",
+                 "inverse_vertices.edges_idx", "inverse_sorted_edges_array.edges_array_src",
+                 "inverse_sorted_edges_array.edges_array_dest"]
+        sizes = [n, n, n, m, m, n, n, n, m, m]
+        out = {k: np.zeros(s, np.uint32) for k, s in zip(names, sizes)}
+        L.check(L.load().ag_graph_download(self.ctx.h, self.h, *[_ptr(out[k]) for k in names]))
+        return out
+
+    # ---- PageRank ----
+    def prepare(self) -> float:
+        ms = C.c_double(0.0)
+        L.check(L.load().ag_pagerank_prepare(self.ctx.h, self.h, C.byref(ms)))
+        return ms.value
+
+    def plan_info(self) -> dict:
+        info = L.PlanInfo()
+        L.check(L.load().ag_pagerank_plan_info(self.h, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in L.PlanInfo._fields_}
+
+    def pagerank_pull(self, iterations: int = 20, epsilon: float = 1e-4, mode="reference",
+                      precision="fp32", finish_fp32: bool = False,
+                      damp: float = 0.85) -> PageRankStats:
+        lib = L.load()
+        o = L.PrOptions()
+        lib.ag_pr_options_default(C.byref(o))
+        o.iterations = iterations
+        o.epsilon = epsilon
+        o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
+        o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
+        o.flags = L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0
+        o.damp = damp
+        n = self.num_vertices
+        ranks = np.zeros(n, np.float32)
+        act = np.zeros(max(iterations, 1), np.uint32)
+        err = np.zeros(max(iterations, 1), np.float64)
+        r = L.PrResult()
+        L.check(lib.ag_pagerank_pull(self.ctx.h, self.h, C.byref(o), _ptr(ranks, C.c_float),
+                                     _ptr(act), _ptr(err, C.c_double), C.byref(r)))
+        damp32 = np.float32(damp)
+        return PageRankStats(
+            damp=float(damp32), base_pr=float(np.float32(1.0) - damp32), iterations=r.iterations,
+            num_vertices=n, activeVertices=r.active_vertices, pageRanks=ranks,
+            error_total=r.error_total, pr_sum=r.pr_sum, sweeps=r.sweeps, loop_ms=r.loop_ms,
+            total_ms=r.total_ms, plan_ms=r.plan_ms, fp32_switch_sweep=r.fp32_switch_sweep,
+            active_per_iter=act[:r.sweeps].copy(), error_per_iter=err[:r.sweeps].copy())
+
+    def pagerank_pull_device(self, d_ranks_ptr: int, iterations: int = 20, epsilon: float = 0.0,
+                             mode="reference", precision="fp32", finish_fp32=False) -> L.PrResult:
+        lib = L.load()
+        o = L.PrOptions()
+        lib.ag_pr_options_default(C.byref(o))
+        o.iterations, o.epsilon = iterations, epsilon
+        o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
+        o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
+        o.flags = L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0
+        r = L.PrResult()
+        L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
+                                            C.c_void_p(d_ranks_ptr), C.byref(r)))
+        return r
+
+
+def set_tuning(low_max: int = 0, hub_min: int = 0, hub_chunk: int = 0) -> None:
+    t = L.Tuning(low_max, hub_min, hub_chunk, 0)
+    L.check(L.load().ag_set_tuning(C.byref(t)))
+
+
+def get_tuning() -> dict:
+    t = L.Tuning()
+    L.check(L.load().ag_get_tuning(C.byref(t)))
+    return {"low_max": t.low_max, "hub_min": t.hub_min, "hub_chunk": t.hub_chunk}
+
+
+# ---- synthetic inputs ----
+def gen_uniform(seed: int, num_vertices: int, num_edges: int):
+    src = np.empty(num_edges, np.uint32)
+    dst = np.empty(num_edges, np.uint32)
+    L.check(L.load().ag_gen_uniform(seed, num_vertices, num_edges, _ptr(src), _ptr(dst)))
+    return src, dst
+
+
+def gen_rmat(seed: int, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19,
+             permute: bool = True, threads: int = 0):
+    m = (1 << scale) * edge_factor
+    src = np.empty(m, np.uint32)
+    dst = np.empty(m, np.uint32)
+    L.check(L.load().ag_gen_rmat(seed, scale, edge_factor, a, b, c, 1 if permute else 0,
+                                 _ptr(src), _ptr(dst), threads))
+    return src, dst

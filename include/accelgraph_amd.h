@@ -1,0 +1,273 @@
+/*
+ * accelgraph_amd.h - C ABI of the MI355X (gfx950) CSR-builder / pull-PageRank
+ * engine.  Plain C types and pointers only; every call returns an AG_* status
+ * and leaves a message for ag_last_error() (thread-local) on failure.
+ *
+ * This layer replaces the reference's CAPI runtime (capienv.h:146-166:
+ * mapGraphCSRToWED / setupAFUGraphCSR / startAFU / startCU / waitAFU /
+ * releaseAFU), which packs CSR pointers into a WED descriptor and drives the
+ * FPGA gather (pageRank.c:993-1046).  Here a context owns one HIP stream on
+ * one device, a graph owns its device-resident CSR, and one call runs the
+ * whole PageRank loop on the device.  The reference-named functions in
+ * graphCSR.h / sortRun.h / pageRank.h are built on top of this API.
+ */
+#ifndef ACCELGRAPH_AMD_H
+#define ACCELGRAPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define AG_OK              0
+#define AG_ERR_INVALID     1  /* bad argument, shape or state */
+#define AG_ERR_HIP         2  /* HIP runtime error (message names the call) */
+#define AG_ERR_OOM         3  /* device or host allocation failed */
+#define AG_ERR_ADMISSION   4  /* serving: HBM budget or slot limit refused */
+#define AG_ERR_NOT_FOUND   5  /* unknown ticket / graph id */
+#define AG_ERR_CANCELLED   6  /* serving: query cancelled */
+#define AG_ERR_IO          7  /* file I/O */
+#define AG_ERR_INJECTED    8  /* serving: injected fault (test hook) */
+#define AG_ERR_PENDING     9  /* serving: query not finished yet */
+#define AG_ERR_NO_DEVICE  10  /* no HIP device visible */
+
+const char *ag_last_error(void);
+const char *ag_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct ag_ctx ag_ctx;
+
+/* device < 0 selects the current HIP device. */
+int ag_ctx_create(int device, ag_ctx **out);
+int ag_ctx_destroy(ag_ctx *ctx);
+int ag_ctx_synchronize(ag_ctx *ctx);
+/* hipStream_t of the context, as an opaque pointer. */
+void *ag_ctx_stream(ag_ctx *ctx);
+/* Process-wide default context on the current device (created on first use;
+ * the reference-named drop-in functions use it). */
+int ag_default_ctx(ag_ctx **out);
+
+/* ---- device-resident graph ---------------------------------------------- */
+typedef struct ag_graph ag_graph;
+
+/* sort modes (sortRun.h); flags may add: */
+#define AG_SORT_RADIX_SRC        0u
+#define AG_SORT_RADIX_SRC_DST    1u
+#define AG_SORT_COUNT_SRC        2u
+#define AG_SORT_COUNT_SRC_DST    3u
+#define AG_BUILD_INVERSE_FROM_INPUT 0x100u /* inverse from the input order, not the sorted one */
+#define AG_BUILD_NO_INVERSE         0x200u /* forward CSR only */
+#define AG_BUILD_DEVICE_INPUT       0x400u /* src/dst are device pointers */
+
+/* Builds forward + inverse CSR on the GPU from an edge list (host pointers
+ * unless AG_BUILD_DEVICE_INPUT).  num_vertices 0 means max id + 1. */
+int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t num_edges,
+                 uint32_t num_vertices, uint32_t flags, ag_graph **out);
+
+/* Uploads an existing CSR (host arrays in the reference layout) holding only
+ * what pull PageRank reads: forward out-degree, inverse degree / edges_idx /
+ * dest (capienv.c:409,421-426). */
+int ag_graph_from_csr(ag_ctx *ctx, uint32_t num_vertices, uint32_t num_edges,
+                      const uint32_t *fwd_out_degree, const uint32_t *inv_out_degree,
+                      const uint32_t *inv_edges_idx, const uint32_t *inv_dest,
+                      ag_graph **out);
+
+int ag_graph_free(ag_graph *g);
+uint32_t ag_graph_num_vertices(const ag_graph *g);
+uint32_t ag_graph_num_edges(const ag_graph *g);
+/* Device bytes held by the graph (CSR + gather layout once built). */
+uint64_t ag_graph_device_bytes(const ag_graph *g);
+
+/* Device pointers of the CSR arrays (NULL where not built). */
+typedef struct ag_csr_view
+{
+    uint32_t num_vertices, num_edges;
+    const uint32_t *out_degree, *in_degree, *edges_idx, *sorted_src, *sorted_dest;
+    const uint32_t *inv_out_degree, *inv_in_degree, *inv_edges_idx, *inv_sorted_src,
+        *inv_sorted_dest;
+    const uint32_t *offsets, *inv_offsets; /* exclusive scans, N + 1 entries */
+} ag_csr_view;
+int ag_graph_view(const ag_graph *g, ag_csr_view *out);
+
+/* Copies CSR arrays to host memory; any pointer may be NULL (skipped).
+ * Vertex arrays hold N entries, edge arrays E. */
+int ag_graph_download(ag_ctx *ctx, const ag_graph *g,
+                      uint32_t *out_degree, uint32_t *in_degree, uint32_t *edges_idx,
+                      uint32_t *sorted_src, uint32_t *sorted_dest,
+                      uint32_t *inv_out_degree, uint32_t *inv_in_degree,
+                      uint32_t *inv_edges_idx, uint32_t *inv_sorted_src,
+                      uint32_t *inv_sorted_dest);
+
+/* Stable sort of (key, val) pairs by key (keys < key_limit) on the device:
+ * the radix passes used by the builder, exposed for sortRunAlgorithms.  All
+ * four pointers are host memory; in-place is allowed. */
+int ag_sort_pairs_host(ag_ctx *ctx, const uint32_t *keys, const uint32_t *vals, uint32_t n,
+                       uint32_t key_limit, uint32_t *keys_out, uint32_t *vals_out);
+
+/* Vertex mapping of a key-sorted array (host in/out): degree and edges_idx. */
+int ag_vertex_map_host(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t num_vertices,
+                       uint32_t *degree, uint32_t *edges_idx);
+
+/* ---- pull PageRank ------------------------------------------------------ */
+#define AG_PR_REFERENCE 0u /* pageRank.c:942-1108 formulation */
+#define AG_PR_STANDARD  1u /* x0 = 1/N, dangling mass spread uniformly, FP64 dangling sum */
+
+#define AG_PREC_FP32 0u    /* contributions stored FP32 */
+#define AG_PREC_BF16 1u    /* stored BF16 (RNE), FP32 accumulate */
+#define AG_PREC_FP16 2u    /* stored FP16 (RNE), FP32 accumulate */
+
+#define AG_PR_FLAG_FINISH_FP32 0x1u /* low precision: switch to FP32 storage when the residual stalls */
+
+typedef struct ag_pr_options
+{
+    uint32_t iterations; /* sweep cap */
+    uint32_t mode;       /* AG_PR_* */
+    uint32_t precision;  /* AG_PREC_* */
+    uint32_t flags;      /* AG_PR_FLAG_* */
+    double epsilon;      /* REFERENCE: per-vertex active threshold (0: fixed sweeps);
+                            STANDARD: stop when the L1 residual <= epsilon (0: fixed) */
+    float damp;          /* 0.85f */
+    float reserved;
+} ag_pr_options;
+
+void ag_pr_options_default(ag_pr_options *opt);
+
+typedef struct ag_pr_result
+{
+    uint32_t iterations;      /* reported count (reference semantics in REFERENCE mode) */
+    uint32_t sweeps;          /* sweeps executed on the device */
+    uint32_t active_vertices; /* active count of the last sweep */
+    uint32_t fp32_switch_sweep; /* first FP32-stored sweep (precision modes), else 0 */
+    double error_total;       /* REFERENCE: mean |delta|; STANDARD: L1 residual */
+    double pr_sum;            /* sum of returned ranks (FP64) */
+    double loop_ms;           /* device time of the sweep loop (HIP events) */
+    double total_ms;          /* device time init + loop + normalise */
+    double plan_ms;           /* gather-layout build time (0 if cached) */
+} ag_pr_result;
+
+/* Runs PageRank on a resident graph.  ranks (host, N floats) may be NULL;
+ * active_iter / err_iter (host, `iterations` entries) may be NULL. */
+int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *ranks,
+                     uint32_t *active_iter, double *err_iter, ag_pr_result *res);
+
+/* Same, writing ranks to a device buffer (N floats). */
+int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
+                            float *d_ranks, ag_pr_result *res);
+
+/* Builds (or rebuilds) the degree-binned gather layout of g now, so that a
+ * timed call does not include it.  Returns its build time in *ms. */
+int ag_pagerank_prepare(ag_ctx *ctx, ag_graph *g, double *ms);
+
+/* Gather-layout statistics (for tuning and reports). */
+typedef struct ag_plan_info
+{
+    uint32_t n_hub, n_mid, n_low, n_zero; /* vertices per in-degree bin */
+    uint32_t hub_chunks;                  /* workgroup chunks over hub rows */
+    uint32_t low_max, hub_min;            /* bin thresholds (in-degree) */
+    uint64_t sell_slots;                  /* padded slots in the low-degree slices */
+    uint64_t plan_bytes;
+} ag_plan_info;
+int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info);
+
+/* Kernel tuning knobs (0 keeps the default).  Applies to plans built later. */
+typedef struct ag_tuning
+{
+    uint32_t low_max;     /* in-degree <= low_max: thread per vertex */
+    uint32_t hub_min;     /* in-degree >= hub_min: workgroup chunks */
+    uint32_t hub_chunk;   /* edges per hub workgroup chunk */
+    uint32_t relabel;     /* 1: internal degree-sorted labels (default), 2: identity */
+} ag_tuning;
+int ag_set_tuning(const ag_tuning *t);
+int ag_get_tuning(ag_tuning *t);
+
+/* ---- batched personalized PageRank -------------------------------------- */
+typedef struct ag_ppr_query
+{
+    const uint32_t *seeds; /* teleport set, mass split evenly (duplicates add) */
+    uint32_t nseeds;
+    uint32_t max_iter;     /* 0: 1000 */
+    double tol;            /* stop at L1 residual <= tol */
+} ag_ppr_query;
+
+typedef struct ag_ppr_stat
+{
+    uint32_t iterations;
+    uint32_t status;       /* AG_OK or AG_ERR_* */
+    double residual;
+    double latency_ms;     /* serving: submit -> done */
+} ag_ppr_stat;
+
+typedef struct ag_ppr_options
+{
+    uint32_t precision;       /* AG_PREC_* for stored columns */
+    uint32_t cols_per_wave;   /* 0: autotuned from {4, 8, 16, 32} */
+    float damp;               /* 0.85f */
+    uint32_t use_graph_replay;/* 1: capture sweeps in a hipGraph and replay */
+} ag_ppr_options;
+void ag_ppr_options_default(ag_ppr_options *opt);
+
+/* Runs nq queries as multi-right-hand-side columns with per-column retirement.
+ * x (host, nq * N floats, row q = query q) may be NULL; stats (nq) may be NULL. */
+int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
+                 const ag_ppr_options *opt, float *x, ag_ppr_stat *stats, double *elapsed_ms);
+
+/* Chooses cols_per_wave for g by timing {4, 8, 16, 32}; result cached per graph. */
+int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cols_per_wave);
+
+/* ---- serving loop -------------------------------------------------------- */
+typedef struct ag_server ag_server;
+
+typedef struct ag_serve_config
+{
+    uint64_t hbm_budget_bytes; /* admission limit for column state (0: 8 GiB) */
+    uint32_t max_columns;      /* per-graph column slots in flight (0: 64) */
+    uint32_t precision;        /* AG_PREC_* */
+    uint32_t cols_per_wave;    /* 0: autotune */
+    uint32_t inject_fail_alloc;/* test hook: refuse the n-th admission (0: off) */
+    uint32_t inject_fail_query;/* test hook: fail ticket n (0: off) */
+    float damp;
+    uint32_t reserved;
+} ag_serve_config;
+void ag_serve_config_default(ag_serve_config *cfg);
+
+int ag_serve_create(ag_ctx *ctx, const ag_serve_config *cfg, ag_server **out);
+int ag_serve_destroy(ag_server *s);
+/* Registers a resident graph (not owned). */
+int ag_serve_add_graph(ag_server *s, ag_graph *g, uint32_t *graph_id);
+/* Queues a query; it is admitted into a column slot when memory and slots free. */
+int ag_serve_submit(ag_server *s, uint32_t graph_id, const uint32_t *seeds, uint32_t nseeds,
+                    double tol, uint32_t max_iter, uint64_t *ticket);
+int ag_serve_cancel(ag_server *s, uint64_t ticket);
+/* Advances the scheduler: admits, runs one sweep for every graph with active
+ * columns, retires converged columns.  *active receives the number of queries
+ * still queued or running. */
+int ag_serve_step(ag_server *s, uint32_t *active);
+/* Result of a finished query (x: N floats, may be NULL).  AG_ERR_PENDING if
+ * not finished.  The result is released after a successful poll. */
+int ag_serve_poll(ag_server *s, uint64_t ticket, float *x, uint32_t n, ag_ppr_stat *st);
+/* Bytes of HBM currently reserved for column state. */
+uint64_t ag_serve_reserved_bytes(const ag_server *s);
+
+/* ---- synthetic inputs (host, deterministic) ------------------------------ */
+/* Uniform random directed edges: MT19937(seed); per edge src then dst, each
+ * floor(r * N / 2^32).  Self-loops and duplicates kept. */
+int ag_gen_uniform(uint32_t seed, uint32_t num_vertices, uint32_t num_edges, uint32_t *src,
+                   uint32_t *dst);
+/* R-MAT (a, b, c, d = 1 - a - b - c), 2^scale vertices; edge e draws one
+ * 32-bit MT19937 word per level (most significant bit first) from the
+ * stream of its 65,536-edge block, seeded with seed ^ (block * 0x9E3779B9),
+ * and picks quadrant a/b/c/d as (src,dst) bit (0,0)/(0,1)/(1,0)/(1,1);
+ * permute != 0
+ * relabels vertices by a Fisher-Yates shuffle from MT19937(seed + 1).
+ * threads <= 0: all hardware threads.  Output is independent of threads. */
+int ag_gen_rmat(uint32_t seed, uint32_t scale, uint32_t edge_factor, double a, double b,
+                double c, int permute, uint32_t *src, uint32_t *dst, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,425 @@
+/*
+ * CPU oracle (serial restatement + OpenMP baselines) for the GraphCSR builder
+ * and pageRankPullGraphCSR.  TEST INFRASTRUCTURE ONLY: see oracle.h.
+ *
+ * Compiled with -ffp-contract=off so that `base + damp * next` is a separate
+ * multiply and add, as written in pageRank.c:1058.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static void *xcalloc(size_t n, size_t sz)
+{
+    void *p = calloc(n ? n : 1, sz ? sz : 1);
+    if (!p) abort();
+    return p;
+}
+
+uint32_t oracle_num_vertices(uint32_t E, const uint32_t *src, const uint32_t *dst)
+{
+    uint32_t m = 0;
+    if (E == 0) return 0;
+    for (uint32_t e = 0; e < E; e++) {
+        if (src[e] > m) m = src[e];
+        if (dst[e] > m) m = dst[e];
+    }
+    return m + 1;
+}
+
+/* Textbook stable counting sort: histogram, exclusive scan, scatter in input
+ * order.  This is sortRunAlgorithms(el, 2) (count sort by src; modes listed in
+ * README.md:434-439, call site SSSP.c:548-553). */
+void oracle_count_sort(uint32_t N, uint32_t E, const uint32_t *key, const uint32_t *val,
+                       uint32_t *key_out, uint32_t *val_out)
+{
+    uint64_t *pos = (uint64_t *)xcalloc((size_t)N + 1, sizeof(uint64_t));
+    for (uint32_t e = 0; e < E; e++) pos[key[e] + 1]++;
+    for (uint32_t v = 0; v < N; v++) pos[v + 1] += pos[v];
+    for (uint32_t e = 0; e < E; e++) {
+        uint64_t p = pos[key[e]]++;
+        key_out[p] = key[e];
+        val_out[p] = val[e];
+    }
+    free(pos);
+}
+
+/* graphCSRAssignEdgeList's vertex mapping (OpenGraph, absent; fields read at
+ * capienv.c:409-414): degree = segment length, edges_idx = segment start, and
+ * 0 for an empty segment (convention inferred, inert for every consumer since
+ * they read [edges_idx, edges_idx + degree)). */
+void oracle_vertex_map(uint32_t N, uint32_t E, const uint32_t *sorted_key,
+                       uint32_t *degree, uint32_t *edges_idx)
+{
+    memset(degree, 0, (size_t)N * sizeof(uint32_t));
+    memset(edges_idx, 0, (size_t)N * sizeof(uint32_t));
+    for (uint32_t e = 0; e < E; e++) {
+        uint32_t v = sorted_key[e];
+        if (degree[v] == 0) edges_idx[v] = e;
+        degree[v]++;
+    }
+}
+
+void oracle_build_csr(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
+                      uint32_t sort_mode,
+                      uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,
+                      uint32_t *fwd_indeg, uint32_t *fwd_idx,
+                      uint32_t *inv_src, uint32_t *inv_dst, uint32_t *inv_outdeg,
+                      uint32_t *inv_indeg, uint32_t *inv_idx)
+{
+    uint32_t mode = sort_mode & 0xff;
+    int inverse_from_input = (sort_mode & 0x100) != 0;
+    int by_src_dst = (mode == 1 || mode == 3);
+    uint32_t *ta = (uint32_t *)xcalloc(E, sizeof(uint32_t));
+    uint32_t *tb = (uint32_t *)xcalloc(E, sizeof(uint32_t));
+
+    /* forward: stable by src, or lexicographic (src, dst) via two stable passes */
+    if (by_src_dst) {
+        oracle_count_sort(N, E, dst, src, tb, ta);      /* tb = dst sorted, ta = src */
+        oracle_count_sort(N, E, ta, tb, fwd_src, fwd_dst);
+    } else {
+        oracle_count_sort(N, E, src, dst, fwd_src, fwd_dst);
+    }
+
+    /* inverse: swap endpoints (SSSP.c:518-520) of the forward-sorted list (or of
+     * the input list), then sort by the new src with the same mode. */
+    const uint32_t *isrc = inverse_from_input ? dst : fwd_dst;
+    const uint32_t *idst = inverse_from_input ? src : fwd_src;
+    if (by_src_dst) {
+        oracle_count_sort(N, E, idst, isrc, tb, ta);
+        oracle_count_sort(N, E, ta, tb, inv_src, inv_dst);
+    } else {
+        oracle_count_sort(N, E, isrc, idst, inv_src, inv_dst);
+    }
+
+    oracle_vertex_map(N, E, fwd_src, fwd_outdeg, fwd_idx);
+    oracle_vertex_map(N, E, inv_src, inv_outdeg, inv_idx);
+    memcpy(fwd_indeg, inv_outdeg, (size_t)N * sizeof(uint32_t));
+    memcpy(inv_indeg, fwd_outdeg, (size_t)N * sizeof(uint32_t));
+    free(ta);
+    free(tb);
+}
+
+/* pageRankPullGraphCSR, FP32 (pageRank.c:942-1108), with the OpenMP gather
+ * form of pageRank.c:2890-2907 run serially:
+ *   init        pr[v] = base_pr (pageRank.c:57-58,71), next[v] = 0 (:1014-1019)
+ *   contrib     outdeg ? pr/outdeg : 0 with the FORWARD out-degree (:1024-1033)
+ *   gather      sequential FP32 += over the inverse edge range (:2900-2907)
+ *   update      new = base + damp*next; err = fabs(new-old) (float diff,
+ *               promoted); error_total += err/N; active += err >= epsilon
+ *               (:1055-1070); break if active == 0 without ++iterations (:1073)
+ *   normalise   pr /= N; sum += pr in double (:1079-1085)                   */
+uint32_t oracle_pagerank_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
+                                  const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                  const uint32_t *inv_dest, uint32_t iterations,
+                                  double epsilon, float *pr, double *error_total,
+                                  double *pr_sum, uint32_t *active_iter, double *err_iter)
+{
+    const float damp = 0.85f;
+    const float base = 1.0f - damp;
+    float *contrib = (float *)xcalloc(N, sizeof(float));
+    float *next = (float *)xcalloc(N, sizeof(float));
+    double err_tot = 0.0;
+    uint32_t it;
+
+    for (uint32_t v = 0; v < N; v++) { pr[v] = base; next[v] = 0.0f; }
+
+    for (it = 0; it < iterations; it++) {
+        uint32_t active = 0;
+        err_tot = 0.0;
+        for (uint32_t v = 0; v < N; v++)
+            contrib[v] = fwd_outdeg[v] ? pr[v] / (float)fwd_outdeg[v] : 0.0f;
+        for (uint32_t v = 0; v < N; v++) {
+            float s = 0.0f;
+            uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+            for (uint32_t j = j0; j < j1; j++) s += contrib[inv_dest[j]];
+            next[v] = s;
+        }
+        for (uint32_t v = 0; v < N; v++) {
+            float old = pr[v];
+            float nw = base + damp * next[v];
+            pr[v] = nw;
+            next[v] = 0.0f;
+            double e = fabs(nw - old);
+            err_tot += e / N;
+            if (e >= epsilon) active++;
+        }
+        if (active_iter) active_iter[it] = active;
+        if (err_iter) err_iter[it] = err_tot;
+        if (active == 0) break;
+    }
+
+    double sum = 0.0;
+    for (uint32_t v = 0; v < N; v++) {
+        pr[v] = pr[v] / N;
+        sum += pr[v];
+    }
+    if (error_total) *error_total = err_tot;
+    if (pr_sum) *pr_sum = sum;
+    free(contrib);
+    free(next);
+    return it;
+}
+
+uint32_t oracle_pagerank_pull_f64(uint32_t N, const uint32_t *fwd_outdeg,
+                                  const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                  const uint32_t *inv_dest, uint32_t iterations,
+                                  double epsilon, double *pr, double *error_total)
+{
+    const double damp = (double)0.85f;
+    const double base = (double)(1.0f - 0.85f);
+    double *contrib = (double *)xcalloc(N, sizeof(double));
+    double err_tot = 0.0;
+    uint32_t it;
+
+    for (uint32_t v = 0; v < N; v++) pr[v] = base;
+    for (it = 0; it < iterations; it++) {
+        uint32_t active = 0;
+        err_tot = 0.0;
+        for (uint32_t v = 0; v < N; v++)
+            contrib[v] = fwd_outdeg[v] ? pr[v] / (double)fwd_outdeg[v] : 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            double s = 0.0;
+            uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+            for (uint32_t j = j0; j < j1; j++) s += contrib[inv_dest[j]];
+            double nw = base + damp * s;
+            double e = fabs(nw - pr[v]);
+            pr[v] = nw;
+            err_tot += e / N;
+            if (e >= epsilon) active++;
+        }
+        if (active == 0) break;
+    }
+    for (uint32_t v = 0; v < N; v++) pr[v] /= N;
+    if (error_total) *error_total = err_tot;
+    free(contrib);
+    return it;
+}
+
+uint32_t oracle_pagerank_standard_f64(uint32_t N, const uint32_t *fwd_outdeg,
+                                      const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                      const uint32_t *inv_dest, double damp,
+                                      uint32_t iterations, double tol, double *x,
+                                      double *residual)
+{
+    double *contrib = (double *)xcalloc(N, sizeof(double));
+    double *nx = (double *)xcalloc(N, sizeof(double));
+    double res = 0.0;
+    uint32_t it;
+    if (N == 0) { if (residual) *residual = 0.0; free(contrib); free(nx); return 0; }
+    for (uint32_t v = 0; v < N; v++) x[v] = 1.0 / N;
+    for (it = 0; it < iterations; it++) {
+        double dangling = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            if (fwd_outdeg[v]) contrib[v] = x[v] / (double)fwd_outdeg[v];
+            else { contrib[v] = 0.0; dangling += x[v]; }
+        }
+        double tele = (1.0 - damp) / N + damp * dangling / N;
+        res = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            double s = 0.0;
+            uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+            for (uint32_t j = j0; j < j1; j++) s += contrib[inv_dest[j]];
+            nx[v] = tele + damp * s;
+            res += fabs(nx[v] - x[v]);
+        }
+        memcpy(x, nx, (size_t)N * sizeof(double));
+        if (tol > 0.0 && res <= tol) { it++; break; }
+    }
+    if (residual) *residual = res;
+    free(contrib);
+    free(nx);
+    return it;
+}
+
+uint32_t oracle_ppr_f64(uint32_t N, const uint32_t *fwd_outdeg, const uint32_t *inv_deg,
+                        const uint32_t *inv_idx, const uint32_t *inv_dest, double damp,
+                        const uint32_t *seeds, uint32_t nseeds, double tol,
+                        uint32_t max_iter, double *x, double *residual)
+{
+    double *s = (double *)xcalloc(N, sizeof(double));
+    double *contrib = (double *)xcalloc(N, sizeof(double));
+    double *nx = (double *)xcalloc(N, sizeof(double));
+    double res = 0.0;
+    uint32_t it;
+    for (uint32_t k = 0; k < nseeds; k++) s[seeds[k]] += 1.0 / nseeds;
+    memcpy(x, s, (size_t)N * sizeof(double));
+    for (it = 0; it < max_iter; it++) {
+        double dangling = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            if (fwd_outdeg[v]) contrib[v] = x[v] / (double)fwd_outdeg[v];
+            else { contrib[v] = 0.0; dangling += x[v]; }
+        }
+        res = 0.0;
+        for (uint32_t v = 0; v < N; v++) {
+            double acc = 0.0;
+            uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+            for (uint32_t j = j0; j < j1; j++) acc += contrib[inv_dest[j]];
+            nx[v] = (1.0 - damp) * s[v] + damp * (acc + dangling * s[v]);
+            res += fabs(nx[v] - x[v]);
+        }
+        memcpy(x, nx, (size_t)N * sizeof(double));
+        if (res <= tol) { it++; break; }
+    }
+    if (residual) *residual = res;
+    free(s);
+    free(contrib);
+    free(nx);
+    return it;
+}
+
+/* ------------------------------------------------------------------------- */
+/* OpenMP baselines                                                          */
+/* ------------------------------------------------------------------------- */
+
+int oracle_omp_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static double now_s(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+/* Parallel stable counting sort: per-thread histograms over contiguous input
+ * chunks, offsets ordered (vertex, thread), scatter in chunk order. */
+static void omp_count_sort(uint32_t N, uint32_t E, const uint32_t *key, const uint32_t *val,
+                           uint32_t *kout, uint32_t *vout, int T)
+{
+    uint32_t *cnt = (uint32_t *)xcalloc((size_t)T * N, sizeof(uint32_t));
+    uint64_t *off = (uint64_t *)xcalloc((size_t)N + 1, sizeof(uint64_t));
+    #pragma omp parallel num_threads(T)
+    {
+        int t = 0, nt = 1;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+        nt = omp_get_num_threads();
+#endif
+        uint64_t lo = (uint64_t)E * t / nt, hi = (uint64_t)E * (t + 1) / nt;
+        uint32_t *c = cnt + (size_t)t * N;
+        for (uint64_t e = lo; e < hi; e++) c[key[e]]++;
+        #pragma omp barrier
+        #pragma omp for schedule(static)
+        for (uint32_t v = 0; v < N; v++) {
+            uint32_t run = 0;
+            for (int u = 0; u < nt; u++) {
+                uint32_t x = cnt[(size_t)u * N + v];
+                cnt[(size_t)u * N + v] = run;
+                run += x;
+            }
+            off[v + 1] = run;
+        }
+        #pragma omp single
+        for (uint32_t v = 0; v < N; v++) off[v + 1] += off[v];
+        for (uint64_t e = lo; e < hi; e++) {
+            uint32_t k = key[e];
+            uint64_t p = off[k] + c[k]++;
+            kout[p] = k;
+            vout[p] = val[e];
+        }
+    }
+    free(cnt);
+    free(off);
+}
+
+static void omp_vertex_map(uint32_t N, uint32_t E, const uint32_t *sk, uint32_t *deg,
+                           uint32_t *idx, int T)
+{
+    #pragma omp parallel for num_threads(T) schedule(static)
+    for (uint32_t v = 0; v < N; v++) { deg[v] = 0; idx[v] = 0; }
+    #pragma omp parallel for num_threads(T) schedule(static)
+    for (uint32_t e = 0; e < E; e++) {
+        if (e == 0 || sk[e] != sk[e - 1]) {
+            uint32_t v = sk[e];
+            uint32_t end = e + 1;
+            while (end < E && sk[end] == v) end++;
+            idx[v] = e;
+            deg[v] = end - e;
+        }
+    }
+}
+
+double oracle_csr_build_omp(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
+                            uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,
+                            uint32_t *fwd_idx, uint32_t *inv_src, uint32_t *inv_dst,
+                            uint32_t *inv_outdeg, uint32_t *inv_idx, int threads)
+{
+    int T = threads > 0 ? threads : oracle_omp_max_threads();
+    double t0 = now_s();
+    omp_count_sort(N, E, src, dst, fwd_src, fwd_dst, T);
+    omp_count_sort(N, E, fwd_dst, fwd_src, inv_src, inv_dst, T);
+    omp_vertex_map(N, E, fwd_src, fwd_outdeg, fwd_idx, T);
+    omp_vertex_map(N, E, inv_src, inv_outdeg, inv_idx, T);
+    return now_s() - t0;
+}
+
+/* pageRankPullGraphCSR with the reference's OpenMP loop structure:
+ * contrib `parallel for` (pageRank.c:1024), gather `schedule(dynamic,1024)`
+ * (pageRank.c:2890), update `reduction(+:error_total,activeVertices)`
+ * (pageRank.c:1055), normalise `reduction(+:sum)` (pageRank.c:1079). */
+double oracle_pagerank_pull_omp(uint32_t N, const uint32_t *fwd_outdeg,
+                                const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                const uint32_t *inv_dest, uint32_t iterations,
+                                double epsilon, float *pr, double *error_total,
+                                uint32_t *iters_done, int threads)
+{
+    int T = threads > 0 ? threads : oracle_omp_max_threads();
+    const float damp = 0.85f;
+    const float base = 1.0f - damp;
+    float *contrib = (float *)xcalloc(N, sizeof(float));
+    float *next = (float *)xcalloc(N, sizeof(float));
+    double err_tot = 0.0;
+    uint32_t it;
+    #pragma omp parallel for num_threads(T)
+    for (uint32_t v = 0; v < N; v++) { pr[v] = base; next[v] = 0.0f; }
+
+    double t0 = now_s();
+    for (it = 0; it < iterations; it++) {
+        uint32_t active = 0;
+        double e_sum = 0.0;
+        #pragma omp parallel for num_threads(T)
+        for (uint32_t v = 0; v < N; v++)
+            contrib[v] = fwd_outdeg[v] ? pr[v] / (float)fwd_outdeg[v] : 0.0f;
+        #pragma omp parallel for num_threads(T) schedule(dynamic, 1024)
+        for (uint32_t v = 0; v < N; v++) {
+            float s = 0.0f;
+            uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+            for (uint32_t j = j0; j < j1; j++) s += contrib[inv_dest[j]];
+            next[v] = s;
+        }
+        #pragma omp parallel for num_threads(T) reduction(+ : e_sum, active)
+        for (uint32_t v = 0; v < N; v++) {
+            float old = pr[v];
+            float nw = base + damp * next[v];
+            pr[v] = nw;
+            next[v] = 0.0f;
+            double e = fabs(nw - old);
+            e_sum += e / N;
+            if (e >= epsilon) active++;
+        }
+        err_tot = e_sum;
+        if (active == 0) break;
+    }
+    double secs = now_s() - t0;
+    #pragma omp parallel for num_threads(T)
+    for (uint32_t v = 0; v < N; v++) pr[v] = pr[v] / N;
+    if (error_total) *error_total = err_tot;
+    if (iters_done) *iters_done = it;
+    free(contrib);
+    free(next);
+    return secs;
+}

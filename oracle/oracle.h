@@ -1,0 +1,101 @@
+/*
+ * CPU oracle for the CSR builder and pull PageRank hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by the
+ * tests, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * never by the product library (accelgraph_express_amd), which must run on the
+ * GPU or fail.
+ *
+ * Every function restates the reference's algorithm serially (or, for the
+ * *_omp baselines, with the reference's OpenMP loop structure).  The reference
+ * C files are cited per function.  The OpenGraph submodule that holds the
+ * reference's CSR builder (graphCSR.c, edgeList.c, sortRun.c) is absent from
+ * the reference tree, so the builder restatement follows the textbook stable
+ * counting sort and the in-tree call sites (SSSP.c:443-565).
+ *
+ * Pinning (which check fixes which function) is listed in tests/test_oracle_*.py.
+ */
+#ifndef AG_ORACLE_H
+#define AG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* num_vertices = max(src, dst) + 1 (0 for an empty list). */
+uint32_t oracle_num_vertices(uint32_t E, const uint32_t *src, const uint32_t *dst);
+
+/* Stable counting sort of (key, val) pairs by key (keys < N). */
+void oracle_count_sort(uint32_t N, uint32_t E, const uint32_t *key, const uint32_t *val,
+                       uint32_t *key_out, uint32_t *val_out);
+
+/* Vertex mapping of a sorted edge list: out_degree[v] = #edges with key v,
+ * edges_idx[v] = first index of v's segment, 0 when the degree is 0. */
+void oracle_vertex_map(uint32_t N, uint32_t E, const uint32_t *sorted_key,
+                       uint32_t *degree, uint32_t *edges_idx);
+
+/* Full GraphCSR build: forward (stable by src; sort_mode 1/3 = by (src,dst))
+ * and inverse (from the forward-sorted list with endpoints swapped, stable by
+ * the new src).  in_degree of each side is the other side's out_degree. */
+void oracle_build_csr(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
+                      uint32_t sort_mode,
+                      uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,
+                      uint32_t *fwd_indeg, uint32_t *fwd_idx,
+                      uint32_t *inv_src, uint32_t *inv_dst, uint32_t *inv_outdeg,
+                      uint32_t *inv_indeg, uint32_t *inv_idx);
+
+/* Reference pull PageRank (pageRank.c:942-1108), FP32, serial.  Returns the
+ * reported iteration count.  ranks[] receives the normalised ranks (pr/N).
+ * active_iter/err_iter (may be NULL) receive per-iteration values, length
+ * `iterations`. */
+uint32_t oracle_pagerank_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
+                                  const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                  const uint32_t *inv_dest, uint32_t iterations,
+                                  double epsilon, float *ranks, double *error_total,
+                                  double *pr_sum, uint32_t *active_iter, double *err_iter);
+
+/* Same operator in FP64 (the damping constants are the FP32 ones, promoted). */
+uint32_t oracle_pagerank_pull_f64(uint32_t N, const uint32_t *fwd_outdeg,
+                                  const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                  const uint32_t *inv_dest, uint32_t iterations,
+                                  double epsilon, double *ranks, double *error_total);
+
+/* STANDARD formulation, FP64: x0 = 1/N; dangling mass redistributed uniformly;
+ * runs `iterations` sweeps or until the L1 residual sum|x'-x| <= tol (tol<=0:
+ * fixed count).  Returns sweeps done; *residual = last L1 residual. */
+uint32_t oracle_pagerank_standard_f64(uint32_t N, const uint32_t *fwd_outdeg,
+                                      const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                      const uint32_t *inv_dest, double damp,
+                                      uint32_t iterations, double tol, double *x,
+                                      double *residual);
+
+/* Personalized PageRank, FP64, one query: teleport vector s uniform over the
+ * seed list (duplicates add mass); dangling mass returned to s.
+ * x' = (1-d) s + d (A^T D^-1 x + dangling(x) s); x0 = s.  Stops when the L1
+ * residual <= tol or after max_iter sweeps. */
+uint32_t oracle_ppr_f64(uint32_t N, const uint32_t *fwd_outdeg, const uint32_t *inv_deg,
+                        const uint32_t *inv_idx, const uint32_t *inv_dest, double damp,
+                        const uint32_t *seeds, uint32_t nseeds, double tol,
+                        uint32_t max_iter, double *x, double *residual);
+
+/* OpenMP CPU baselines (reported beside the GPU, never the product path). */
+double oracle_csr_build_omp(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
+                            uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,
+                            uint32_t *fwd_idx, uint32_t *inv_src, uint32_t *inv_dst,
+                            uint32_t *inv_outdeg, uint32_t *inv_idx, int threads);
+
+/* Returns seconds spent in the iteration loop; semantics as the f32 oracle. */
+double oracle_pagerank_pull_omp(uint32_t N, const uint32_t *fwd_outdeg,
+                                const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                const uint32_t *inv_dest, uint32_t iterations,
+                                double epsilon, float *ranks, double *error_total,
+                                uint32_t *iters_done, int threads);
+
+int oracle_omp_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,149 @@
+"""GPU pull PageRank parity against the oracle.
+
+Bar (north star): after a fixed sweep count the FP32 ranks are within 1e-5
+relative per vertex of the oracle's FP32 restatement of pageRankPullGraphCSR
+(pageRank.c:942-1108); iteration counts and active counts match exactly; the
+error against the FP64 oracle is reported.  REL_TOL is that 1e-5 bound."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import accelgraph_express_amd as ag
+import oracle
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+REL_TOL = 1e-5
+
+
+def _rel(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+def _run(ctx, src, dst, n, iterations=20, epsilon=0.0):
+    print(f"case: N={n} E={len(src)} iterations={iterations} epsilon={epsilon}", flush=True)
+    g = ag.Graph.build(ctx, src, dst, num_vertices=n)
+    st = g.pagerank_pull(iterations=iterations, epsilon=epsilon)
+    csr = oracle.build_csr(src, dst, g.num_vertices)
+    ref, it, err, psum, act, errs = oracle.pagerank_pull_f32(csr, iterations, epsilon)
+    assert st.iterations == it
+    if epsilon == 0.0:
+        np.testing.assert_array_equal(st.active_per_iter, act)
+    else:
+        # A vertex whose |delta| lies within FP32 reordering error of epsilon
+        # may count on one side and not the other: allow a handful per sweep.
+        assert st.active_per_iter.shape == act.shape
+        diff = np.abs(st.active_per_iter.astype(np.int64) - act.astype(np.int64))
+        assert diff.max() <= max(4, int(1e-4 * n)), diff.max()
+    assert _rel(st.pageRanks, ref).max() <= REL_TOL
+    assert abs(st.error_total - err) <= 1e-6 * abs(err) + 1e-15
+    assert abs(st.pr_sum - psum) <= 1e-6 * abs(psum)
+    return g, st, csr, ref
+
+
+def test_c1_uniform_20_sweeps(ctx):
+    src, dst = ag.gen_uniform(27491095, 65536, 1048576)
+    g, st, csr, ref = _run(ctx, src, dst, 65536)
+    golden = json.load(open(os.path.join(GOLDEN, "c1_uniform.json")))
+    assert st.iterations == golden["iterations"]
+    idx = np.array(golden["rank_sample_index"])
+    want = np.array([float.fromhex(h) for h in golden["rank_sample_f32_hex"]], np.float32)
+    assert _rel(st.pageRanks[idx], want).max() <= REL_TOL
+    f64, _, _ = oracle.pagerank_pull_f64(csr, 20, 0.0)
+    e = np.abs(st.pageRanks - f64)
+    print(f"C1 vs FP64 oracle: max abs {e.max():.3e}, L1 {e.sum():.3e}, "
+          f"max rel {_rel(st.pageRanks, f64).max():.3e}")
+
+
+def test_c1_epsilon_default_break(ctx):
+    src, dst = ag.gen_uniform(27491095, 65536, 1048576)
+    _run(ctx, src, dst, 65536, iterations=100, epsilon=1e-4)
+
+
+def test_low_degree_rows_bit_exact(ctx):
+    """Rows of in-degree <= low_max sum sequentially in the reference's edge
+    order, so with every row in that bin the ranks equal the FP32 oracle."""
+    src, dst = ag.gen_uniform(99, 4096, 20000)  # in-degree ~ Poisson(4.9)
+    g = ag.Graph.build(ctx, src, dst, 4096)
+    g.prepare()
+    info = g.plan_info()
+    assert info["n_mid"] == 0 and info["n_hub"] == 0
+    st = g.pagerank_pull(20, 0.0)
+    ref, *_ = oracle.pagerank_pull_f32(oracle.build_csr(src, dst, 4096), 20, 0.0)
+    np.testing.assert_array_equal(st.pageRanks, ref)
+
+
+def test_closed_form_graphs(ctx):
+    n = 1000
+    s = np.arange(n, dtype=np.uint32)
+    g = ag.Graph.build(ctx, s, (s + 1) % n, n)  # directed cycle
+    st = g.pagerank_pull(7, 0.0)
+    closed = (1.0 - 0.85 ** 8) / n
+    assert np.allclose(st.pageRanks, closed, rtol=2e-6)
+    # star into a hub that crosses the hub threshold (chunked workgroups)
+    n = 20000
+    g = ag.Graph.build(ctx, np.arange(1, n, dtype=np.uint32), np.zeros(n - 1, np.uint32), n)
+    g.prepare()
+    assert g.plan_info()["n_hub"] == 1
+    st = g.pagerank_pull(3, 0.0)
+    assert abs(st.pageRanks[0] * n - (0.15 + 0.85 * 0.15 * (n - 1))) < 1e-3 * 0.85 * 0.15 * n
+    assert np.allclose(st.pageRanks[1:] * n, 0.15, rtol=1e-6)
+
+
+def test_edge_cases(ctx):
+    _run(ctx, np.zeros(0, np.uint32), np.zeros(0, np.uint32), 10)
+    _run(ctx, np.array([0], np.uint32), np.array([1], np.uint32), 2, iterations=5)
+    _run(ctx, np.array([4, 4, 4], np.uint32), np.array([4, 4, 4], np.uint32), 5, iterations=5)
+
+
+def test_skewed_bins_all_paths(ctx):
+    """R-MAT scale 16: hub, mid, low and zero bins all populated."""
+    ag.set_tuning(low_max=24, hub_min=512, hub_chunk=256)
+    try:
+        src, dst = ag.gen_rmat(77, 16, 16)
+        g, st, csr, ref = _run(ctx, src, dst, 1 << 16)
+        info = g.plan_info()
+        assert min(info["n_hub"], info["n_mid"], info["n_low"], info["n_zero"]) > 0
+        assert info["hub_chunks"] > info["n_hub"]
+    finally:
+        ag.set_tuning(low_max=24, hub_min=4096, hub_chunk=2048)
+
+
+def test_rmat_s22_20_sweeps(ctx):
+    src, dst = ag.gen_rmat(2022, 22, 16)
+    n = 1 << 22
+    g, st, csr, ref = _run(ctx, src, dst, n)
+    f64, _, _ = oracle.pagerank_pull_f64(csr, 20, 0.0)
+    e = np.abs(st.pageRanks - f64)
+    print(f"s22 vs FP64 oracle: max abs {e.max():.3e}, L1 {e.sum():.3e}, "
+          f"max rel {_rel(st.pageRanks, f64).max():.3e}; loop {st.loop_ms:.3f} ms")
+
+
+def test_repeat_calls_identical(ctx):
+    src, dst = ag.gen_rmat(8, 18, 16)
+    g = ag.Graph.build(ctx, src, dst, 1 << 18)
+    a = g.pagerank_pull(20, 0.0).pageRanks
+    b = g.pagerank_pull(20, 0.0).pageRanks
+    np.testing.assert_array_equal(a, b)
+
+
+def test_graph_from_host_csr(ctx):
+    src, dst = ag.gen_rmat(9, 14, 8)
+    csr = oracle.build_csr(src, dst, 1 << 14)
+    g = ag.Graph.from_csr(ctx, 1 << 14, csr["vertices.out_degree"],
+                          csr["inverse_vertices.out_degree"], csr["inverse_vertices.edges_idx"],
+                          csr["inverse_sorted_edges_array.edges_array_dest"])
+    st = g.pagerank_pull(20, 0.0)
+    ref, *_ = oracle.pagerank_pull_f32(csr, 20, 0.0)
+    assert _rel(st.pageRanks, ref).max() <= REL_TOL
+    bad = csr["inverse_vertices.edges_idx"].copy()
+    bad[5] = 1 << 30
+    with pytest.raises(ag.AGError):
+        ag.Graph.from_csr(ctx, 1 << 14, csr["vertices.out_degree"],
+                          csr["inverse_vertices.out_degree"], bad,
+                          csr["inverse_sorted_edges_array.edges_array_dest"])
