@@ -134,6 +134,8 @@ _SIGS = {
     "ag_ppr_batch": (C.c_int, [vp, vp, C.POINTER(PprQuery), C.c_uint32, C.POINTER(PprOptions),
                                f32p, C.POINTER(PprStat), f64p]),
     "ag_ppr_autotune": (C.c_int, [vp, vp, C.c_uint32, u32p]),
+    "ag_ppr_single": (C.c_int, [vp, vp, C.POINTER(PprQuery), C.c_uint32, f32p,
+                                C.POINTER(PprStat)]),
     "ag_serve_config_default": (None, [C.POINTER(ServeConfig)]),
     "ag_serve_create": (C.c_int, [vp, C.POINTER(ServeConfig), C.POINTER(vp)]),
     "ag_serve_destroy": (C.c_int, [vp]),

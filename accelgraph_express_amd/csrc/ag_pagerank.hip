@@ -9,9 +9,9 @@
 //     in-degree bins are contiguous label ranges and the hottest sources sit
 //     together in the contribution vector;
 //   * hub rows   (in-degree >= hub_min): CSR, split in fixed chunks of
-//                  hub_chunk edges, one workgroup per chunk; the last chunk
-//                  to finish (device-scope ticket) sums the chunk partials in
-//                  chunk order and updates the vertex;
+//                  hub_chunk edges, one workgroup per chunk; a one-workgroup
+//                  finalize kernel launched after the sweep sums each row's
+//                  chunk partials in chunk order and updates the vertex;
 //   * mid rows   (low_max < in-degree < hub_min): CSR, one wave per row,
 //                  per-lane partial sums and a shuffle reduction;
 //   * low rows   (1 <= in-degree <= low_max): SELL-64 slices, 64 rows per
@@ -19,9 +19,12 @@
 //                  wave is one coalesced 256-byte load; each lane sums its row
 //                  sequentially in the reference's edge order;
 //   * zero rows: no gather.
-// One sweep is one kernel launch: gather + update + next contribution + FP64
-// residual partials; the last workgroup to finish reduces the partials in a
-// fixed order (deterministic) and decides convergence on the device.
+// One sweep is two launches: the sweep kernel (gather + update + next
+// contribution + FP64 residual partials per workgroup) and the finalize kernel
+// (hub rows, fixed-order reduction, convergence decided on the device).  No
+// device-scope fence is used: on gfx950 one compiles to an L2 write-back plus
+// invalidate (buffer_wbl2 / buffer_inv), which evicts the cached contribution
+// vector; the kernel boundary orders the two launches instead.
 #include <hip/hip_fp16.h>
 
 #include <chrono>
@@ -29,7 +32,7 @@
 #include <cstring>
 #include <mutex>
 
-#include "ag_internal.h"
+#include "ag_plan.h"
 
 namespace ag {
 
@@ -42,34 +45,6 @@ constexpr int kZeroRowsPerBlock = 1024;
 constexpr uint32_t kFmtF32 = 0, kFmtBF16 = 1; // 2 = FP16
 
 } // namespace
-
-// BF16 storage word (round-to-nearest-even from FP32, widened by a shift).
-struct bf16_t {
-    uint16_t x;
-};
-
-struct Work {
-    uint32_t max_iter = 0;
-    uint32_t grid = 0;
-    DevBuf pr, c0, c1, blk_err, blk_act, blk_dang, state, err_iter, act_iter, fmt, sum_part, out;
-};
-
-struct Plan {
-    uint32_t N = 0, E = 0;
-    uint32_t n_hub = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
-    uint32_t low_slices = 0, hub_chunks = 0;
-    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0;
-    uint64_t csr_edges = 0, sell_slots = 0;
-    DevBuf order, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial, hub_ticket;
-    uint32_t nb_hub = 0, nb_mid = 0, nb_low = 0, nb_zero = 0, nvblocks = 0;
-    Work work;
-    std::mutex mu;
-    uint64_t bytes() const
-    {
-        return order.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
-               chunk_row.bytes + chunk_first.bytes + hub_partial.bytes + hub_ticket.bytes;
-    }
-};
 
 void plan_destroy(Plan *p) { delete p; }
 
@@ -107,7 +82,7 @@ struct SweepArgs {
     const uint32_t *__restrict__ chunk_row;
     const uint32_t *__restrict__ chunk_first;
     float *hub_partial;
-    uint32_t *hub_ticket;
+    uint32_t nparts; // sweep workgroups (partials written per sweep)
     float *pr;
     void *c0, *c1; // contribution ping-pong buffers (sized for FP32)
     double *blk_err, *blk_dang;
@@ -116,6 +91,10 @@ struct SweepArgs {
     double *err_iter;
     uint32_t *act_iter;
     uint32_t *fmt; // fmt[j] = storage format of the contributions read by sweep j
+    const uint32_t *seed_bits; // kModePPR: seed bitmap / labels / weights
+    const uint32_t *seed_v;
+    const float *seed_w;
+    uint32_t nseeds;
 };
 
 namespace {
@@ -178,10 +157,20 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
 #pragma clang fp contract(off)
     const float old = a.pr[i];
     float nw;
-    if (a.mode == AG_PR_REFERENCE)
+    if (a.mode == AG_PR_REFERENCE) {
         nw = a.base + a.damp * s;
-    else
+    } else if (a.mode == kModePPR) {
+        // x' = d * gathered + s_v * ((1 - d) + d * dangling); tele holds the
+        // parenthesised factor, s_v is 0 off the seed set.
+        float sv = 0.0f;
+        if ((a.seed_bits[i >> 5] >> (i & 31)) & 1u)
+            for (uint32_t q = 0; q < a.nseeds; q++)
+                if (a.seed_v[q] == i) sv += a.seed_w[q];
+        const float gathered = a.damp * s;
+        nw = gathered + sv * tele;
+    } else {
         nw = tele + a.damp * s;
+    }
     a.pr[i] = nw;
     const uint32_t od = a.outdeg[i];
     if (a.mode == AG_PR_REFERENCE) {
@@ -201,7 +190,6 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t it, uint32_t wfmt, const
                            void *cout, float tele, double &err, uint32_t &act, double &dang)
 {
     __shared__ float red[kBlock / 64];
-    __shared__ uint32_t last_flag;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 
     for (uint32_t vb = blockIdx.x; vb < a.nvblocks; vb += gridDim.x) {
@@ -210,33 +198,22 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t it, uint32_t wfmt, const
             const uint32_t c = vb;
             const uint32_t row = a.chunk_row[c];
             const uint32_t k = c - a.chunk_first[row];
-            const uint32_t nch = a.chunk_first[row + 1] - a.chunk_first[row];
             const uint32_t rb = a.roff[row], re = a.roff[row + 1];
             const uint32_t b0 = rb + k * a.hub_chunk;
             const uint32_t b1 = min(re, b0 + a.hub_chunk);
-            float s = 0.0f;
-            for (uint32_t j = b0 + threadIdx.x; j < b1; j += kBlock) s += load_c(cin, ldnt(a.cols + j));
-            s = wave_sum(s);
+            float s0 = 0.0f, s1 = 0.0f;
+            uint32_t j = b0 + threadIdx.x;
+            for (; j + kBlock < b1; j += 2 * kBlock) {
+                const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + kBlock);
+                s0 += load_c(cin, u0);
+                s1 += load_c(cin, u1);
+            }
+            if (j < b1) s0 += load_c(cin, ldnt(a.cols + j));
+            const float s = wave_sum(s0 + s1);
             if (lane == 0) red[w] = s;
             __syncthreads();
-            if (threadIdx.x == 0) {
-                const float tot = (red[0] + red[1]) + (red[2] + red[3]);
-                a.hub_partial[c] = tot;
-                __threadfence();
-                const uint32_t t = atomicAdd(&a.hub_ticket[row], 1u);
-                last_flag = (t == nch - 1) ? 1u : 0u;
-            }
-            __syncthreads();
-            if (last_flag && threadIdx.x == 0) {
-                __threadfence();
-                const uint32_t f0 = a.chunk_first[row];
-                float sum = 0.0f;
-                for (uint32_t q = 0; q < nch; q++)
-                    sum += __hip_atomic_load(&a.hub_partial[f0 + q], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-                a.hub_ticket[row] = 0;
-                update_vertex(a, row, sum, tele, cout, wfmt, err, act, dang);
-            }
+            // chunk partial; k_pr_finalize sums a row's chunks in chunk order
+            if (threadIdx.x == 0) a.hub_partial[c] = (red[0] + red[1]) + (red[2] + red[3]);
             __syncthreads();
         } else if (vb < a.nb_hub + a.nb_mid) {
             // ---- mid rows: one wave per row ----
@@ -303,7 +280,6 @@ __global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
 {
     __shared__ double s_err[kBlock / 64], s_dang[kBlock / 64];
     __shared__ uint32_t s_act[kBlock / 64];
-    __shared__ uint32_t s_last;
     SweepState *st = a.st;
     if (st->done) return; // converged in an earlier sweep: uniform early exit
     const uint32_t rfmt = a.fmt[it];
@@ -322,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
         sweep_body<__half>(a, it, wfmt, static_cast<const __half *>(cin), cout, tele, err, act,
                            dang);
 
-    // ---- block partials, then the last block reduces them in fixed order ----
+    // ---- block partials; k_pr_finalize reduces them in a fixed order ----
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     err = wave_sum(err);
     dang = wave_sum(dang);
@@ -333,37 +309,53 @@ __global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
         a.blk_err[blockIdx.x] = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
         a.blk_dang[blockIdx.x] = (s_dang[0] + s_dang[1]) + (s_dang[2] + s_dang[3]);
         a.blk_act[blockIdx.x] = s_act[0] + s_act[1] + s_act[2] + s_act[3];
-        __threadfence();
-        const uint32_t t = atomicAdd(&st->ticket, 1u);
-        s_last = (t == gridDim.x - 1) ? 1u : 0u;
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    double e = 0.0, d = 0.0;
-    uint32_t c = 0;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        e += __hip_atomic_load(&a.blk_err[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        d += __hip_atomic_load(&a.blk_dang[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        c += __hip_atomic_load(&a.blk_act[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One workgroup, launched after each sweep (the kernel boundary orders it
+// after every sweep workgroup, so no device-scope fence is needed): sums each
+// hub row's chunk partials in chunk order and updates the hub vertices, then
+// reduces all residual / active / dangling partials in a fixed order and
+// decides convergence.
+constexpr int kFinBlock = 1024;
+__global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t it)
+{
+    __shared__ double s_err[kFinBlock / 64], s_dang[kFinBlock / 64];
+    __shared__ uint32_t s_act[kFinBlock / 64];
+    SweepState *st = a.st;
+    if (st->done) return;
+    const uint32_t wfmt = a.fmt[it + 1];
+    void *cout = (it & 1) ? a.c0 : a.c1;
+    const float tele = st->tele;
+    double err = 0.0, dang = 0.0;
+    uint32_t act = 0;
+    for (uint32_t row = threadIdx.x; row < a.n_hub; row += kFinBlock) {
+        const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
+        float sum = 0.0f;
+        for (uint32_t q = f0; q < f1; q++) sum += a.hub_partial[q];
+        update_vertex(a, row, sum, tele, cout, wfmt, err, act, dang);
     }
-    e = wave_sum(e);
-    d = wave_sum(d);
-    c = wave_sum(c);
-    __syncthreads();
-    if (lane == 0) { s_err[w] = e; s_dang[w] = d; s_act[w] = c; }
+    for (uint32_t b = threadIdx.x; b < a.nparts; b += kFinBlock) {
+        err += a.blk_err[b];
+        dang += a.blk_dang[b];
+        act += a.blk_act[b];
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    err = wave_sum(err);
+    dang = wave_sum(dang);
+    act = wave_sum(act);
+    if (lane == 0) { s_err[w] = err; s_dang[w] = dang; s_act[w] = act; }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    const double E_ = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
-    const double D_ = (s_dang[0] + s_dang[1]) + (s_dang[2] + s_dang[3]);
-    const uint32_t A_ = s_act[0] + s_act[1] + s_act[2] + s_act[3];
+    double E_ = 0.0, D_ = 0.0;
+    uint32_t A_ = 0;
+    for (int i = 0; i < kFinBlock / 64; i++) { E_ += s_err[i]; D_ += s_dang[i]; A_ += s_act[i]; }
     a.err_iter[it] = E_;
     a.act_iter[it] = A_;
     st->prev_err = st->last_err;
     st->last_err = E_;
     st->last_active = A_;
     st->sweeps = it + 1;
-    st->ticket = 0;
     // Storage format of the contributions written by the next sweep.
     uint32_t nf = a.fmt[it + 1];
     if (nf != kFmtF32 && a.finish_fp32 && it >= 2 && E_ > 0.97 * st->prev_err) {
@@ -374,6 +366,11 @@ __global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
     if (a.mode == AG_PR_REFERENCE) {
         if (A_ == 0) { st->done = 1; st->iterations = it; }
         else st->iterations = it + 1;
+    } else if (a.mode == kModePPR) {
+        st->dangling = D_;
+        st->tele = (float)((1.0 - (double)a.damp) + (double)a.damp * D_);
+        st->iterations = it + 1;
+        if (E_ <= a.epsilon) st->done = 1;
     } else {
         st->dangling = D_;
         st->tele = (float)((1.0 - (double)a.damp) * a.inv_n + (double)a.damp * D_ * a.inv_n);
@@ -390,13 +387,27 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
         st->done = 0; st->ticket = 0; st->iterations = 0; st->sweeps = 0;
         st->last_active = 0; st->fp32_switch = 0;
         st->last_err = 0.0; st->prev_err = 0.0; st->dangling = 0.0; st->tele = tele0;
+        if (a.mode == kModePPR) {
+            // dangling mass of x0 = s: seed weight on vertices without out-edges
+            double d0 = 0.0;
+            for (uint32_t q = 0; q < a.nseeds; q++)
+                if (a.outdeg[a.seed_v[q]] == 0) d0 += (double)a.seed_w[q];
+            st->dangling = d0;
+            st->tele = (float)((1.0 - (double)a.damp) + (double)a.damp * d0);
+        }
     }
     if (i <= a.max_iter + 1) a.fmt[i] = fmt0;
-    if (i < a.n_hub) a.hub_ticket[i] = 0;
     if (i >= a.N) return;
-    a.pr[i] = x0;
+    float x = x0;
+    if (a.mode == kModePPR) {
+        x = 0.0f;
+        if ((a.seed_bits[i >> 5] >> (i & 31)) & 1u)
+            for (uint32_t q = 0; q < a.nseeds; q++)
+                if (a.seed_v[q] == i) x += a.seed_w[q];
+    }
+    a.pr[i] = x;
     const uint32_t od = a.outdeg[i];
-    store_c(a.c0, fmt0, i, od ? x0 / (float)od : 0.0f);
+    store_c(a.c0, fmt0, i, od ? x / (float)od : 0.0f);
 }
 
 // Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
@@ -546,7 +557,8 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     const uint32_t maxdeg = device_max1(ctx, g->inv_outdeg.as<uint32_t>(), N);
     const uint32_t kbits = bits_for((uint64_t)maxdeg + 1);
     DevBuf keys((size_t)N * 4), ids((size_t)N * 4), skeys((size_t)N * 4);
-    DevBuf rank((size_t)N * 4), rdeg((size_t)N * 4), counts(16);
+    p->rank.alloc((size_t)N * 4);
+    DevBuf rdeg((size_t)N * 4), counts(16);
     const uint32_t nb = ceil_div(N, 256);
     hipLaunchKernelGGL(k_degree_keys, dim3(nb), dim3(256), 0, s, g->inv_outdeg.as<uint32_t>(), N,
                        maxdeg, keys.as<uint32_t>(), ids.as<uint32_t>());
@@ -556,7 +568,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     AG_HIP(hipMemsetAsync(counts.p, 0, 16, s));
     hipLaunchKernelGGL(k_relabel_vertex, dim3(nb), dim3(256), 0, s, p->order.as<uint32_t>(),
                        skeys.as<uint32_t>(), maxdeg, g->fwd_outdeg.as<uint32_t>(), N,
-                       rank.as<uint32_t>(), rdeg.as<uint32_t>(), p->outdeg.as<uint32_t>(),
+                       p->rank.as<uint32_t>(), rdeg.as<uint32_t>(), p->outdeg.as<uint32_t>(),
                        tn.low_max, tn.hub_min, counts.as<uint32_t>());
     AG_HIP(hipGetLastError());
     scan_exclusive_u32(ctx, rdeg.as<uint32_t>(), p->roff.as<uint32_t>(), N);
@@ -600,12 +612,12 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
         hipLaunchKernelGGL(k_fill_csr_rows, dim3(ceil_div(p->n_nonlow, 4)), dim3(256), 0, s,
                            p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
-                           rank.as<uint32_t>(), p->n_nonlow, p->cols.as<uint32_t>());
+                           p->rank.as<uint32_t>(), p->n_nonlow, p->cols.as<uint32_t>());
     if (nlow)
         hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
                            p->order.as<uint32_t>(), rdeg.as<uint32_t>(),
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
-                           rank.as<uint32_t>(), p->slice_ptr.as<uint32_t>(), p->n_nonlow, nlow,
+                           p->rank.as<uint32_t>(), p->slice_ptr.as<uint32_t>(), p->n_nonlow, nlow,
                            p->cols.as<uint32_t>());
     AG_HIP(hipGetLastError());
 
@@ -627,10 +639,8 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->chunk_first.alloc((size_t)(p->n_hub + 1) * 4);
     p->chunk_row.alloc((size_t)std::max<uint32_t>(tot, 1) * 4);
     p->hub_partial.alloc((size_t)std::max<uint32_t>(tot, 1) * 4);
-    p->hub_ticket.alloc((size_t)std::max<uint32_t>(p->n_hub, 1) * 4);
     AG_HIP(hipMemcpyAsync(p->chunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
     if (tot) AG_HIP(hipMemcpyAsync(p->chunk_row.p, crow.data(), crow.size() * 4, hipMemcpyHostToDevice, s));
-    AG_HIP(hipMemsetAsync(p->hub_ticket.p, 0, p->hub_ticket.bytes, s));
 
     p->nb_hub = tot;
     p->nb_mid = ceil_div(p->n_nonlow - p->n_hub, kMidRowsPerBlock);
@@ -641,7 +651,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     return p.release();
 }
 
-static Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms)
+Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms)
 {
     if (ms) *ms = 0.0;
     if (g->plan) return g->plan;
@@ -709,7 +719,7 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.chunk_row = p->chunk_row.as<uint32_t>();
     a.chunk_first = p->chunk_first.as<uint32_t>();
     a.hub_partial = p->hub_partial.as<float>();
-    a.hub_ticket = p->hub_ticket.as<uint32_t>();
+    a.nparts = w.grid;
     a.pr = w.pr.as<float>();
     a.c0 = w.c0.p;
     a.c1 = w.c1.p;
@@ -724,8 +734,9 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
 }
 
 // Runs the sweep loop; ranks land in d_out (device, caller order).
-static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float *d_out,
-                         uint32_t *active_iter, double *err_iter, ag_pr_result *res)
+static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
+                              const SeedTable *seeds, float *d_out, uint32_t *active_iter,
+                              double *err_iter, ag_pr_result *res)
 {
     hipStream_t s = ctx->stream;
     ag_pr_result r;
@@ -734,8 +745,10 @@ static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float
     std::lock_guard<std::mutex> lk(p->mu);
     const uint32_t N = p->N;
     const uint32_t max_iter = o.iterations;
-    if (o.mode != AG_PR_REFERENCE && o.mode != AG_PR_STANDARD)
+    if (o.mode != AG_PR_REFERENCE && o.mode != AG_PR_STANDARD && o.mode != kModePPR)
         fail(AG_ERR_INVALID, "unknown PageRank mode %u", o.mode);
+    if ((o.mode == kModePPR) != (seeds != nullptr))
+        fail(AG_ERR_INVALID, "personalized mode needs a seed table");
     if (o.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", o.precision);
     if (!(o.damp > 0.0f && o.damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
     if (N == 0) {
@@ -744,6 +757,12 @@ static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float
     }
     ensure_work(ctx, p, max_iter);
     SweepArgs a = make_args(p, o, max_iter);
+    if (seeds) {
+        a.seed_bits = seeds->bits;
+        a.seed_v = seeds->v;
+        a.seed_w = seeds->w;
+        a.nseeds = seeds->n;
+    }
     const float x0 = (o.mode == AG_PR_REFERENCE) ? a.base : (float)(1.0 / (double)N);
     const double dang0 = (double)p->n_dangling * (1.0 / (double)N);
     const float tele0 = (float)((1.0 - (double)o.damp) / N + (double)o.damp * dang0 / N);
@@ -761,8 +780,10 @@ static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float
     const uint32_t group = can_stop ? 16u : max_iter;
     for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
         const uint32_t it1 = std::min(max_iter, it0 + group);
-        for (uint32_t it = it0; it < it1; it++)
+        for (uint32_t it = it0; it < it1; it++) {
             hipLaunchKernelGGL(k_pr_sweep, dim3(grid), dim3(kBlock), 0, s, a, it);
+            hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
+        }
         AG_HIP(hipGetLastError());
         if (can_stop && it1 < max_iter) {
             uint32_t done = 0;
@@ -802,6 +823,18 @@ static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float
     r.error_total = st.last_err;
     r.fp32_switch_sweep = st.fp32_switch;
     if (res) *res = r;
+}
+
+static void run_pagerank(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, float *d_out,
+                         uint32_t *active_iter, double *err_iter, ag_pr_result *res)
+{
+    run_pagerank_impl(ctx, g, o, nullptr, d_out, active_iter, err_iter, res);
+}
+
+void run_pagerank_seeded(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, const SeedTable &st,
+                         float *d_out, ag_pr_result *res)
+{
+    run_pagerank_impl(ctx, g, o, &st, d_out, nullptr, nullptr, res);
 }
 
 } // namespace ag

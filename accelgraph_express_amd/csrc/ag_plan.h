@@ -1,0 +1,57 @@
+// Gather layout ("plan") shared by the PageRank and personalized-PageRank
+// kernels.  Internal; not part of the C ABI.
+#pragma once
+
+#include <mutex>
+
+#include "ag_internal.h"
+
+namespace ag {
+
+// BF16 storage word (round-to-nearest-even from FP32, widened by a shift).
+struct bf16_t {
+    uint16_t x;
+};
+
+struct Work {
+    uint32_t max_iter = 0;
+    uint32_t grid = 0;
+    DevBuf pr, c0, c1, blk_err, blk_act, blk_dang, state, err_iter, act_iter, fmt, sum_part, out;
+};
+
+struct Plan {
+    uint32_t N = 0, E = 0;
+    uint32_t n_hub = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
+    uint32_t low_slices = 0, hub_chunks = 0;
+    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0;
+    uint64_t csr_edges = 0, sell_slots = 0;
+    // order[i] = vertex with label i; rank[v] = label of vertex v
+    DevBuf order, rank, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial;
+    uint32_t nb_hub = 0, nb_mid = 0, nb_low = 0, nb_zero = 0, nvblocks = 0;
+    Work work;
+    std::mutex mu;
+    uint64_t bytes() const
+    {
+        return order.bytes + rank.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
+               chunk_row.bytes + chunk_first.bytes + hub_partial.bytes;
+    }
+};
+
+
+// Internal PageRank mode: single-query personalized PageRank (teleport and
+// dangling mass to the seed set), used by ag_ppr_single.
+constexpr uint32_t kModePPR = 2;
+
+// Seed table of one personalized query, in label space (device arrays).
+struct SeedTable {
+    const uint32_t *bits = nullptr; // N-bit membership bitmap
+    const uint32_t *v = nullptr;    // labels, unique
+    const float *w = nullptr;       // teleport weights
+    uint32_t n = 0;
+};
+
+Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms);
+void run_pagerank_seeded(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, const SeedTable &st,
+                         float *d_out, ag_pr_result *res);
+
+} // namespace ag

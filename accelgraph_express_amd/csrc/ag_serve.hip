@@ -1,0 +1,297 @@
+// Serving loop for personalized-PageRank queries over several resident graphs.
+//
+// Continuous batching: every graph with work owns one PprBatch (column slots
+// in HBM).  ag_serve_step admits queued queries into free slots (FIFO, by HBM
+// budget: a batch's column state is reserved when the batch is created and
+// released when it drains), runs sweeps for every graph with active columns,
+// and retires converged columns immediately, so queries that arrive while
+// others are running join the next sweep.  Host code only; the sweeps are the
+// kernels of ag_ppr.hip.  The reference has no serving path.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <unordered_map>
+
+#include "ag_ppr.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum class QState { Queued, Running, Done, Failed, Cancelled };
+
+struct Query {
+    uint64_t ticket = 0;
+    uint32_t graph = 0;
+    std::vector<uint32_t> seeds;
+    double tol = 1e-4;
+    uint32_t max_iter = 1000;
+    QState state = QState::Queued;
+    int code = AG_OK;
+    std::vector<float> result;
+    ag_ppr_stat stat{};
+    Clock::time_point t_submit;
+};
+
+struct GraphEntry {
+    ag_graph *g = nullptr;
+    std::unique_ptr<ag::PprBatch> batch;
+    uint64_t reserved = 0;
+    uint32_t cpw = 0;
+};
+
+} // namespace
+
+struct ag_server {
+    ag_ctx *ctx = nullptr;
+    ag_serve_config cfg{};
+    std::vector<GraphEntry> graphs;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Query> queries;
+    uint64_t next_ticket = 1;
+    uint64_t reserved = 0;
+    uint32_t alloc_attempts = 0;
+    std::mutex mu;
+};
+
+using namespace ag;
+
+namespace {
+
+// Creates the graph's batch with the largest capacity (multiple of 4, at most
+// max_columns) whose column state fits the remaining budget.  False: no room.
+bool ensure_batch(ag_server *s, GraphEntry &ge)
+{
+    if (ge.batch) return true;
+    const uint32_t N = ge.g->N;
+    uint32_t cap = std::min<uint32_t>(64, std::max<uint32_t>(4, s->cfg.max_columns / 4 * 4));
+    while (cap >= 4 && s->reserved + PprBatch::bytes_for(N, cap, s->cfg.precision) >
+                           s->cfg.hbm_budget_bytes)
+        cap -= 4;
+    if (cap < 4) return false;
+    s->alloc_attempts++;
+    if (s->cfg.inject_fail_alloc && s->alloc_attempts == s->cfg.inject_fail_alloc)
+        return false; // injected allocation failure: the query stays queued
+    if (!ge.cpw) ge.cpw = s->cfg.cols_per_wave ? s->cfg.cols_per_wave
+                                               : ppr_autotune(s->ctx, ge.g, s->cfg.precision);
+    ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp));
+    ge.reserved = PprBatch::bytes_for(N, cap, s->cfg.precision);
+    s->reserved += ge.reserved;
+    return true;
+}
+
+void release_batch(ag_server *s, GraphEntry &ge)
+{
+    ge.batch.reset();
+    s->reserved -= ge.reserved;
+    ge.reserved = 0;
+}
+
+void finish(Query &q, QState st, int code)
+{
+    q.state = st;
+    q.code = code;
+    q.stat.status = (uint32_t)code;
+    q.stat.latency_ms =
+        std::chrono::duration<double, std::milli>(Clock::now() - q.t_submit).count();
+}
+
+void admit(ag_server *s)
+{
+    for (auto it = s->queue.begin(); it != s->queue.end();) {
+        Query &q = s->queries.at(*it);
+        if (q.state != QState::Queued) { it = s->queue.erase(it); continue; }
+        if (s->cfg.inject_fail_query && q.ticket == s->cfg.inject_fail_query) {
+            finish(q, QState::Failed, AG_ERR_INJECTED);
+            it = s->queue.erase(it);
+            continue;
+        }
+        GraphEntry &ge = s->graphs[q.graph];
+        if (!ensure_batch(s, ge) || ge.batch->free_slots() == 0) { ++it; continue; }
+        PprSeedSet set = ge.batch->make_seeds(q.seeds.data(), (uint32_t)q.seeds.size(), q.tol,
+                                              q.max_iter);
+        ge.batch->admit(set, q.ticket);
+        q.state = QState::Running;
+        it = s->queue.erase(it);
+    }
+}
+
+void retire(ag_server *s, uint32_t gi)
+{
+    GraphEntry &ge = s->graphs[gi];
+    if (!ge.batch) return;
+    auto done = ge.batch->poll();
+    std::sort(done.begin(), done.end(),
+              [](const PprBatch::Done &a, const PprBatch::Done &b) { return a.slot > b.slot; });
+    for (auto &d : done) {
+        auto qi = s->queries.find(d.tag);
+        if (qi != s->queries.end() && qi->second.state == QState::Running) {
+            Query &q = qi->second;
+            q.result.resize(ge.g->N);
+            ge.batch->extract(d.slot, q.result.data());
+            q.stat.iterations = d.iters;
+            q.stat.residual = d.residual;
+            finish(q, QState::Done, AG_OK);
+        }
+        ge.batch->remove(d.slot);
+    }
+}
+
+bool graph_has_queued(ag_server *s, uint32_t gi)
+{
+    for (uint64_t t : s->queue) {
+        auto it = s->queries.find(t);
+        if (it != s->queries.end() && it->second.graph == gi && it->second.state == QState::Queued)
+            return true;
+    }
+    return false;
+}
+
+} // namespace
+
+extern "C" {
+
+void ag_serve_config_default(ag_serve_config *c)
+{
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->hbm_budget_bytes = 8ull << 30;
+    c->max_columns = 64;
+    c->precision = AG_PREC_BF16;
+    c->damp = 0.85f;
+}
+
+int ag_serve_create(ag_ctx *ctx, const ag_serve_config *cfg, ag_server **out)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!out) fail(AG_ERR_INVALID, "null out");
+        std::unique_ptr<ag_server> s(new ag_server());
+        s->ctx = ctx;
+        if (cfg) s->cfg = *cfg; else ag_serve_config_default(&s->cfg);
+        if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = 8ull << 30;
+        if (!s->cfg.max_columns) s->cfg.max_columns = 64;
+        if (s->cfg.damp == 0.0f) s->cfg.damp = 0.85f;
+        if (s->cfg.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision");
+        *out = s.release();
+    });
+}
+
+int ag_serve_destroy(ag_server *s)
+{
+    return guard([&] { delete s; });
+}
+
+int ag_serve_add_graph(ag_server *s, ag_graph *g, uint32_t *id)
+{
+    return guard([&] {
+        if (!s || !g) fail(AG_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (g->N == 0) fail(AG_ERR_INVALID, "graph has no vertices");
+        ensure_plan(s->ctx, g, nullptr);
+        GraphEntry ge;
+        ge.g = g;
+        s->graphs.push_back(std::move(ge));
+        if (id) *id = (uint32_t)(s->graphs.size() - 1);
+    });
+}
+
+int ag_serve_submit(ag_server *s, uint32_t gid, const uint32_t *seeds, uint32_t nseeds, double tol,
+                    uint32_t max_iter, uint64_t *ticket)
+{
+    return guard([&] {
+        if (!s || !ticket) fail(AG_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (gid >= s->graphs.size()) fail(AG_ERR_NOT_FOUND, "unknown graph id %u", gid);
+        if (!seeds || nseeds == 0) fail(AG_ERR_INVALID, "a query needs at least one seed");
+        const uint32_t N = s->graphs[gid].g->N;
+        for (uint32_t i = 0; i < nseeds; i++)
+            if (seeds[i] >= N) fail(AG_ERR_INVALID, "seed %u >= num_vertices %u", seeds[i], N);
+        if (!(tol > 0.0)) fail(AG_ERR_INVALID, "tolerance must be > 0");
+        Query q;
+        q.ticket = s->next_ticket++;
+        q.graph = gid;
+        q.seeds.assign(seeds, seeds + nseeds);
+        q.tol = tol;
+        q.max_iter = max_iter ? max_iter : 1000;
+        q.t_submit = Clock::now();
+        s->queue.push_back(q.ticket);
+        *ticket = q.ticket;
+        s->queries.emplace(q.ticket, std::move(q));
+    });
+}
+
+int ag_serve_cancel(ag_server *s, uint64_t ticket)
+{
+    return guard([&] {
+        if (!s) fail(AG_ERR_INVALID, "null server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->queries.find(ticket);
+        if (it == s->queries.end()) fail(AG_ERR_NOT_FOUND, "unknown ticket");
+        Query &q = it->second;
+        if (q.state == QState::Running) {
+            GraphEntry &ge = s->graphs[q.graph];
+            for (uint32_t k = 0; k < ge.batch->active(); k++)
+                if (ge.batch->tag(k) == ticket) { ge.batch->remove(k); break; }
+        }
+        if (q.state == QState::Queued || q.state == QState::Running)
+            finish(q, QState::Cancelled, AG_ERR_CANCELLED);
+    });
+}
+
+int ag_serve_step(ag_server *s, uint32_t *active)
+{
+    return guard([&] {
+        if (!s) fail(AG_ERR_INVALID, "null server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        check_ctx(s->ctx);
+        admit(s);
+        for (uint32_t gi = 0; gi < s->graphs.size(); gi++) {
+            GraphEntry &ge = s->graphs[gi];
+            if (ge.batch && ge.batch->active()) ge.batch->sweep(2, true);
+        }
+        for (uint32_t gi = 0; gi < s->graphs.size(); gi++) {
+            GraphEntry &ge = s->graphs[gi];
+            retire(s, gi);
+            if (ge.batch && ge.batch->active() == 0 && !graph_has_queued(s, gi))
+                release_batch(s, ge);
+        }
+        admit(s);
+        if (active) {
+            uint32_t n = 0;
+            for (auto &kv : s->queries)
+                if (kv.second.state == QState::Queued || kv.second.state == QState::Running) n++;
+            *active = n;
+        }
+    });
+}
+
+int ag_serve_poll(ag_server *s, uint64_t ticket, float *x, uint32_t n, ag_ppr_stat *st)
+{
+    if (!s) return AG_ERR_INVALID;
+    int code = AG_OK;
+    int rc = guard([&] {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->queries.find(ticket);
+        if (it == s->queries.end()) fail(AG_ERR_NOT_FOUND, "unknown ticket");
+        Query &q = it->second;
+        if (q.state == QState::Queued || q.state == QState::Running) {
+            code = AG_ERR_PENDING;
+            return;
+        }
+        if (st) *st = q.stat;
+        if (q.state == QState::Done && x) {
+            if (n < q.result.size()) fail(AG_ERR_INVALID, "output holds %u < %zu values", n, q.result.size());
+            std::memcpy(x, q.result.data(), q.result.size() * 4);
+        }
+        code = q.code;
+        s->queries.erase(it);
+    });
+    return rc != AG_OK ? rc : code;
+}
+
+uint64_t ag_serve_reserved_bytes(const ag_server *s) { return s ? s->reserved : 0; }
+
+} // extern "C"

@@ -253,3 +253,122 @@ def gen_rmat(seed: int, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.1
     L.check(L.load().ag_gen_rmat(seed, scale, edge_factor, a, b, c, 1 if permute else 0,
                                  _ptr(src), _ptr(dst), threads))
     return src, dst
+
+
+# ---- personalized PageRank ----
+def _query(seeds, tol, max_iter):
+    s = _u32(seeds)
+    q = L.PprQuery(_ptr(s), s.size, max_iter, tol)
+    return q, s
+
+
+def ppr_single(graph: Graph, seeds, tol: float = 1e-4, max_iter: int = 1000,
+               precision="fp32"):
+    """One query with the single-vector sweep kernel.  Returns (x, stat)."""
+    q, keep = _query(seeds, tol, max_iter)
+    x = np.zeros(graph.num_vertices, np.float32)
+    st = L.PprStat()
+    prec = _PRECS[precision] if isinstance(precision, str) else int(precision)
+    L.check(L.load().ag_ppr_single(graph.ctx.h, graph.h, C.byref(q), prec,
+                                   _ptr(x, C.c_float), C.byref(st)))
+    del keep
+    return x, st
+
+
+def ppr_batch(graph: Graph, queries, precision="bf16", cols_per_wave: int = 0,
+              replay: bool = True, damp: float = 0.85):
+    """queries: list of (seeds, tol, max_iter).  Returns (X [nq, N], stats, elapsed_ms)."""
+    lib = L.load()
+    nq = len(queries)
+    arr = (L.PprQuery * nq)()
+    keep = []
+    for i, (seeds, tol, mi) in enumerate(queries):
+        q, s = _query(seeds, tol, mi)
+        arr[i] = q
+        keep.append(s)
+    o = L.PprOptions()
+    lib.ag_ppr_options_default(C.byref(o))
+    o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
+    o.cols_per_wave = cols_per_wave
+    o.use_graph_replay = 1 if replay else 0
+    o.damp = damp
+    X = np.zeros((nq, graph.num_vertices), np.float32)
+    stats = (L.PprStat * nq)()
+    ms = C.c_double()
+    L.check(lib.ag_ppr_batch(graph.ctx.h, graph.h, arr, nq, C.byref(o), _ptr(X, C.c_float),
+                             stats, C.byref(ms)))
+    return X, list(stats), ms.value
+
+
+def ppr_autotune(graph: Graph, precision="bf16") -> int:
+    c = C.c_uint32()
+    prec = _PRECS[precision] if isinstance(precision, str) else int(precision)
+    L.check(L.load().ag_ppr_autotune(graph.ctx.h, graph.h, prec, C.byref(c)))
+    return c.value
+
+
+class Server:
+    """Continuous-batching PPR server over resident graphs (``ag_server``)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, max_columns: int = 0,
+                 precision="bf16", cols_per_wave: int = 0, inject_fail_alloc: int = 0,
+                 inject_fail_query: int = 0):
+        lib = L.load()
+        cfg = L.ServeConfig()
+        lib.ag_serve_config_default(C.byref(cfg))
+        if hbm_budget_bytes:
+            cfg.hbm_budget_bytes = hbm_budget_bytes
+        if max_columns:
+            cfg.max_columns = max_columns
+        cfg.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
+        cfg.cols_per_wave = cols_per_wave
+        cfg.inject_fail_alloc = inject_fail_alloc
+        cfg.inject_fail_query = inject_fail_query
+        h = C.c_void_p()
+        L.check(lib.ag_serve_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.graphs = []
+
+    def add_graph(self, g: Graph) -> int:
+        gid = C.c_uint32()
+        L.check(L.load().ag_serve_add_graph(self.h, g.h, C.byref(gid)))
+        self.graphs.append(g)
+        return gid.value
+
+    def submit(self, gid: int, seeds, tol: float = 1e-4, max_iter: int = 1000) -> int:
+        s = _u32(seeds)
+        t = C.c_uint64()
+        L.check(L.load().ag_serve_submit(self.h, gid, _ptr(s), s.size, tol, max_iter,
+                                         C.byref(t)))
+        return t.value
+
+    def cancel(self, ticket: int) -> None:
+        L.check(L.load().ag_serve_cancel(self.h, ticket))
+
+    def step(self) -> int:
+        n = C.c_uint32()
+        L.check(L.load().ag_serve_step(self.h, C.byref(n)))
+        return n.value
+
+    def poll(self, ticket: int, n: int):
+        """Returns (code, x or None, stat).  code AG_ERR_PENDING while running."""
+        x = np.zeros(n, np.float32)
+        st = L.PprStat()
+        code = L.load().ag_serve_poll(self.h, ticket, _ptr(x, C.c_float), n, C.byref(st))
+        return code, (x if code == L.AG_OK else None), st
+
+    @property
+    def reserved_bytes(self) -> int:
+        return int(L.load().ag_serve_reserved_bytes(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            L.load().ag_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -214,6 +214,12 @@ void ag_ppr_options_default(ag_ppr_options *opt);
 int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
                  const ag_ppr_options *opt, float *x, ag_ppr_stat *stats, double *elapsed_ms);
 
+/* One query with the single-vector sweep kernel (the one-query-at-a-time
+ * path): x (host, N floats) may be NULL.  precision AG_PREC_BF16/FP16 stores
+ * contributions in 16 bits and finishes in FP32 once the residual stalls. */
+int ag_ppr_single(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t precision, float *x,
+                  ag_ppr_stat *stat);
+
 /* Chooses cols_per_wave for g by timing {4, 8, 16, 32}; result cached per graph. */
 int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cols_per_wave);
 

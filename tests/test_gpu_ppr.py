@@ -1,0 +1,86 @@
+"""Personalized PageRank on the GPU against the FP64 oracle (oracle.ppr_f64).
+
+The reference has no PPR; these results are pinned by the FP64 oracle, whose
+fixed point is itself pinned by a dense linear solve (test_oracle_pagerank).
+Bars:
+  * FP32 storage: L1 distance to the FP64 oracle run with the same stop rule
+    <= FP32_L1 (rounding only), iteration count within one sweep;
+  * batch == one query alone, bit for bit, for the same columns-per-wave;
+  * graph replay == eager launches, bit for bit.
+"""
+import numpy as np
+import pytest
+
+import accelgraph_express_amd as ag
+import oracle
+
+pytestmark = pytest.mark.gpu
+FP32_L1 = 2e-5
+
+
+def _graph(ctx, scale=14, seed=31):
+    src, dst = ag.gen_rmat(seed, scale, 16)
+    n = 1 << scale
+    return ag.Graph.build(ctx, src, dst, n), oracle.build_csr(src, dst, n), n
+
+
+def _queries(n, k, seed, tol=1e-6):
+    rng = np.random.RandomState(seed)
+    return [(rng.randint(0, n, rng.randint(1, 17)).astype(np.uint32), tol, 1000) for _ in range(k)]
+
+
+def test_single_fp32_vs_fp64(ctx):
+    g, csr, n = _graph(ctx)
+    for seeds in ([3], [3, 17, 17, 900], list(range(0, n, n // 16))):
+        print(f"case: single seeds={len(seeds)}", flush=True)
+        x, st = ag.ppr_single(g, seeds, tol=1e-6)
+        ref, it, res = oracle.ppr_f64(csr, np.array(seeds, np.uint32), 1e-6)
+        assert abs(int(st.iterations) - it) <= 1
+        assert np.abs(x - ref).sum() <= FP32_L1
+        assert abs(x.sum() - 1.0) < 1e-4
+
+
+@pytest.mark.parametrize("cpw", [4, 8, 16, 32])
+def test_batch_fp32_vs_fp64(ctx, cpw):
+    g, csr, n = _graph(ctx)
+    qs = _queries(n, 20, 5)
+    print(f"case: batch fp32 cpw={cpw} nq=20", flush=True)
+    X, stats, ms = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=cpw)
+    for i, (seeds, tol, mi) in enumerate(qs):
+        ref, it, res = oracle.ppr_f64(csr, seeds, tol, mi)
+        assert abs(int(stats[i].iterations) - it) <= 1, i
+        assert stats[i].residual <= tol
+        assert np.abs(X[i] - ref).sum() <= FP32_L1, i
+
+
+def test_batch_alone_equals_mixed(ctx):
+    g, csr, n = _graph(ctx)
+    qs = _queries(n, 40, 9)
+    X, st, _ = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=16)
+    for i in (0, 7, 39):
+        Xa, sa, _ = ag.ppr_batch(g, [qs[i]], precision="fp32", cols_per_wave=16)
+        assert sa[0].iterations == st[i].iterations
+        np.testing.assert_array_equal(Xa[0], X[i])
+
+
+def test_graph_replay_matches_eager(ctx):
+    g, csr, n = _graph(ctx)
+    qs = _queries(n, 12, 11)
+    Xr, sr, _ = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=8, replay=True)
+    Xe, se, _ = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=8, replay=False)
+    np.testing.assert_array_equal(Xr, Xe)
+    assert [s.iterations for s in sr] == [s.iterations for s in se]
+
+
+def test_more_queries_than_slots(ctx):
+    g, csr, n = _graph(ctx, scale=12)
+    qs = _queries(n, 150, 13, tol=1e-5)  # > 64 slots: admission as slots free up
+    X, st, _ = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=32)
+    for i in (0, 64, 149):
+        ref, it, res = oracle.ppr_f64(csr, qs[i][0], qs[i][1])
+        assert np.abs(X[i] - ref).sum() <= FP32_L1
+
+
+def test_autotune_picks_a_supported_width(ctx):
+    g, csr, n = _graph(ctx)
+    assert ag.ppr_autotune(g, "bf16") in (4, 8, 16, 32)

@@ -1,0 +1,128 @@
+"""Serving loop: continuous batching over resident graphs, HBM-budget
+admission, injected faults and cancellation.  Each query's result must be the
+same alone and in the mix (bit for bit at a fixed columns-per-wave) and match
+the FP64 oracle to the FP32 bound."""
+import numpy as np
+import pytest
+
+import accelgraph_express_amd as ag
+from accelgraph_express_amd import _lib as L
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _drain(srv, tickets, n_of, limit=5000):
+    out = {}
+    for _ in range(limit):
+        active = srv.step()
+        for t in list(tickets):
+            code, x, st = srv.poll(t, n_of[t])
+            if code != L.AG_ERR_PENDING:
+                out[t] = (code, x, st)
+                tickets.remove(t)
+        if not tickets and active == 0:
+            break
+    assert not tickets, "queries did not finish"
+    return out
+
+
+def _setup(ctx):
+    graphs = []
+    for scale, seed in ((12, 1), (13, 2)):
+        src, dst = ag.gen_rmat(seed, scale, 16)
+        graphs.append((ag.Graph.build(ctx, src, dst, 1 << scale),
+                       oracle.build_csr(src, dst, 1 << scale), 1 << scale))
+    return graphs
+
+
+def test_alone_equals_in_mix(ctx):
+    graphs = _setup(ctx)
+    rng = np.random.RandomState(3)
+    srv = ag.Server(ctx, precision="fp32", cols_per_wave=16)
+    gids = [srv.add_graph(g) for g, _, _ in graphs]
+    tickets, meta, n_of = [], {}, {}
+    for i in range(30):
+        k = rng.randint(0, 2)
+        n = graphs[k][2]
+        seeds = rng.randint(0, n, rng.randint(1, 17)).astype(np.uint32)
+        tol = float(10 ** rng.uniform(-6, -4))
+        t = srv.submit(gids[k], seeds, tol)
+        tickets.append(t)
+        meta[t] = (k, seeds, tol)
+        n_of[t] = n
+        if i % 5 == 4:  # arrivals while others run
+            srv.step()
+    res = _drain(srv, list(tickets), n_of)
+    for t in tickets[:6]:
+        code, x, st = res[t]
+        assert code == L.AG_OK
+        k, seeds, tol = meta[t]
+        Xa, sa, _ = ag.ppr_batch(graphs[k][0], [(seeds, tol, 1000)], precision="fp32",
+                                 cols_per_wave=16)
+        np.testing.assert_array_equal(x, Xa[0])
+        ref, it, _ = oracle.ppr_f64(graphs[k][1], seeds, tol)
+        assert np.abs(x - ref).sum() <= 2e-5
+        assert st.latency_ms > 0
+    srv.close()
+
+
+def test_hbm_budget_admission(ctx):
+    graphs = _setup(ctx)
+    n = graphs[0][2]
+    # room for one 8-column batch only
+    budget = 8 * n * 12 + n * 4 + (1 << 20) + 100
+    srv = ag.Server(ctx, hbm_budget_bytes=budget, max_columns=8, precision="fp32",
+                    cols_per_wave=8)
+    gid = srv.add_graph(graphs[0][0])
+    rng = np.random.RandomState(4)
+    tickets = [srv.submit(gid, rng.randint(0, n, 3).astype(np.uint32), 1e-5) for _ in range(20)]
+    srv.step()
+    assert 0 < srv.reserved_bytes <= budget
+    res = _drain(srv, list(tickets), {t: n for t in tickets})
+    assert all(c == L.AG_OK for c, _, _ in res.values())
+    assert srv.reserved_bytes == 0  # batch released once drained
+    # a graph whose column state never fits stays queued, and nothing breaks
+    tiny = ag.Server(ctx, hbm_budget_bytes=1024, precision="fp32", cols_per_wave=8)
+    g2 = tiny.add_graph(graphs[0][0])
+    t = tiny.submit(g2, [1, 2], 1e-4)
+    for _ in range(3):
+        tiny.step()
+    assert tiny.poll(t, n)[0] == L.AG_ERR_PENDING
+    tiny.close()
+    srv.close()
+
+
+def test_injected_faults_and_cancel(ctx):
+    graphs = _setup(ctx)
+    g, csr, n = graphs[0]
+    srv = ag.Server(ctx, precision="fp32", cols_per_wave=8, inject_fail_alloc=1,
+                    inject_fail_query=3)
+    gid = srv.add_graph(g)
+    rng = np.random.RandomState(5)
+    seeds = [rng.randint(0, n, 4).astype(np.uint32) for _ in range(6)]
+    tickets = [srv.submit(gid, s, 1e-6) for s in seeds]
+    srv.step()  # first allocation attempt fails by injection: nothing admitted yet
+    srv.step()
+    srv.cancel(tickets[4])  # cancel one mid-flight
+    res = _drain(srv, [t for t in tickets if t != tickets[4]], {t: n for t in tickets})
+    assert srv.poll(tickets[4], n)[0] == L.AG_ERR_CANCELLED
+    assert res[tickets[2]][0] == L.AG_ERR_INJECTED  # ticket 3
+    for i in (0, 1, 3, 5):
+        code, x, st = res[tickets[i]]
+        assert code == L.AG_OK
+        ref, it, _ = oracle.ppr_f64(csr, seeds[i], 1e-6)
+        assert np.abs(x - ref).sum() <= 2e-5
+    srv.close()
+
+
+def test_bad_requests(ctx):
+    graphs = _setup(ctx)
+    srv = ag.Server(ctx, precision="fp32", cols_per_wave=8)
+    gid = srv.add_graph(graphs[0][0])
+    with pytest.raises(ag.AGError):
+        srv.submit(gid, [graphs[0][2] + 5], 1e-4)
+    with pytest.raises(ag.AGError):
+        srv.submit(gid + 7, [1], 1e-4)
+    assert srv.poll(12345, 10)[0] == L.AG_ERR_NOT_FOUND
+    srv.close()
