@@ -69,7 +69,7 @@ class PlanInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("low_max", C.c_uint32), ("hub_min", C.c_uint32), ("hub_chunk", C.c_uint32),
-                ("relabel", C.c_uint32)]
+                ("hot_bytes", C.c_uint32)]
 
 
 class PprQuery(C.Structure):

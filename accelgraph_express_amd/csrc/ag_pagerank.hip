@@ -39,9 +39,13 @@ namespace ag {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMidRowsPerWave = 4;
-constexpr int kMidRowsPerBlock = kMidRowsPerWave * 4;
-constexpr int kZeroRowsPerBlock = 1024;
+constexpr int kSweepBlock = 1024; // persistent sweep workgroup (16 waves)
+constexpr int kSweepWaves = kSweepBlock / 64;
+constexpr uint32_t kWaveRowMin = 1024; // in-degree >= this: 64 lanes per row
+constexpr uint32_t kHalfRowMin = 256;  // >= this: 32 lanes per row; else 16
+constexpr int kZeroRowsPerUnit = kSweepBlock * 4;
+constexpr uint32_t kLdsBytes = 160 * 1024;   // LDS per CU on gfx950
+constexpr uint32_t kStaticLds = 2048;        // static shared arrays of the sweep
 constexpr uint32_t kFmtF32 = 0, kFmtBF16 = 1; // 2 = FP16
 
 } // namespace
@@ -66,9 +70,9 @@ struct SweepState {
 
 struct SweepArgs {
     uint32_t N;
-    uint32_t n_hub, n_nonlow, n_nonzero;
-    uint32_t nb_hub, nb_mid, nb_low, nb_zero, nvblocks;
-    uint32_t hub_chunk;
+    uint32_t n_hub, n_w, n_h, n_nonlow, n_nonzero; // bin boundaries (labels)
+    uint32_t nb_hub, nb_w, nb_h, nb_q, nb_low, nb_zero, nvblocks; // work units per bin
+    uint32_t hub_chunk, hub_chunks;
     uint32_t mode;
     uint32_t max_iter;
     uint32_t finish_fp32;
@@ -82,7 +86,8 @@ struct SweepArgs {
     const uint32_t *__restrict__ chunk_row;
     const uint32_t *__restrict__ chunk_first;
     float *hub_partial;
-    uint32_t nparts; // sweep workgroups (partials written per sweep)
+    uint32_t nparts;    // sweep workgroups (partials written per sweep)
+    uint32_t hot_bytes; // LDS bytes of the contribution head per workgroup
     float *pr;
     void *c0, *c1; // contribution ping-pong buffers (sized for FP32)
     double *blk_err, *blk_dang;
@@ -185,61 +190,92 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
     store_c(cout, wfmt, i, c);
 }
 
+// Gather source: labels below `hot` read the workgroup's LDS copy of the
+// contribution vector's head, the rest read global memory.
 template <typename CT>
-__device__ void sweep_body(const SweepArgs &a, uint32_t it, uint32_t wfmt, const CT *__restrict__ cin,
-                           void *cout, float tele, double &err, uint32_t &act, double &dang)
+__device__ __forceinline__ float gather(const CT *__restrict__ g, uint32_t u, uint32_t hot)
 {
-    __shared__ float red[kBlock / 64];
+    extern __shared__ __align__(16) unsigned char hot_raw[];
+    const CT *lt = reinterpret_cast<const CT *>(hot_raw);
+    return u < hot ? load_c(lt, u) : load_c(g, u);
+}
+
+// Rows [.., row_end) with G lanes per row (G = 16, 32 or 64): lanes of a
+// group stride over the row's edges, four loads in flight each, then a
+// shuffle reduction inside the group and one lane updates the vertex.
+template <int G, typename CT>
+__device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uint32_t row_end,
+                                           const CT *__restrict__ cin, void *cout, uint32_t wfmt,
+                                           float tele, uint32_t hot, double &err, uint32_t &act,
+                                           double &dang, int lane)
+{
+    const int gl = lane & (G - 1);
+    const bool valid = row < row_end;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (valid) {
+        const uint32_t rb = a.roff[row], re = a.roff[row + 1];
+        uint32_t j = rb + gl;
+        for (; j + 3 * G < re; j += 4 * G) {
+            const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + G);
+            const uint32_t u2 = ldnt(a.cols + j + 2 * G), u3 = ldnt(a.cols + j + 3 * G);
+            s0 += gather(cin, u0, hot);
+            s1 += gather(cin, u1, hot);
+            s2 += gather(cin, u2, hot);
+            s3 += gather(cin, u3, hot);
+        }
+        for (; j < re; j += G) s0 += gather(cin, ldnt(a.cols + j), hot);
+    }
+    float s = (s0 + s1) + (s2 + s3);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (valid && gl == 0) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
+}
+
+template <typename CT>
+__device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restrict__ cin,
+                           void *cout, float tele, uint32_t hot, double &err, uint32_t &act,
+                           double &dang)
+{
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t u_w = a.nb_hub, u_h = u_w + a.nb_w, u_q = u_h + a.nb_h, u_l = u_q + a.nb_q,
+                   u_z = u_l + a.nb_low;
 
     for (uint32_t vb = blockIdx.x; vb < a.nvblocks; vb += gridDim.x) {
-        if (vb < a.nb_hub) {
-            // ---- hub chunk: workgroup sums hub_chunk edges ----
-            const uint32_t c = vb;
-            const uint32_t row = a.chunk_row[c];
-            const uint32_t k = c - a.chunk_first[row];
-            const uint32_t rb = a.roff[row], re = a.roff[row + 1];
-            const uint32_t b0 = rb + k * a.hub_chunk;
-            const uint32_t b1 = min(re, b0 + a.hub_chunk);
-            float s0 = 0.0f, s1 = 0.0f;
-            uint32_t j = b0 + threadIdx.x;
-            for (; j + kBlock < b1; j += 2 * kBlock) {
-                const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + kBlock);
-                s0 += load_c(cin, u0);
-                s1 += load_c(cin, u1);
-            }
-            if (j < b1) s0 += load_c(cin, ldnt(a.cols + j));
-            const float s = wave_sum(s0 + s1);
-            if (lane == 0) red[w] = s;
-            __syncthreads();
-            // chunk partial; k_pr_finalize sums a row's chunks in chunk order
-            if (threadIdx.x == 0) a.hub_partial[c] = (red[0] + red[1]) + (red[2] + red[3]);
-            __syncthreads();
-        } else if (vb < a.nb_hub + a.nb_mid) {
-            // ---- mid rows: one wave per row ----
-            const uint32_t r0 = a.n_hub + (vb - a.nb_hub) * kMidRowsPerBlock + w * kMidRowsPerWave;
-#pragma unroll 1
-            for (int q = 0; q < kMidRowsPerWave; q++) {
-                const uint32_t row = r0 + q;
-                if (row >= a.n_nonlow) break;
-                const uint32_t rb = a.roff[row], re = a.roff[row + 1];
+        if (vb < u_w) {
+            // ---- hub rows: one wave per chunk of hub_chunk edges ----
+            const uint32_t c = vb * kSweepWaves + w;
+            if (c < a.hub_chunks) {
+                const uint32_t row = a.chunk_row[c];
+                const uint32_t k = c - a.chunk_first[row];
+                const uint32_t b0 = a.roff[row] + k * a.hub_chunk;
+                const uint32_t b1 = min(a.roff[row + 1], b0 + a.hub_chunk);
                 float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-                uint32_t j = rb + lane;
-                for (; j + 192 < re; j += 256) {
+                uint32_t j = b0 + lane;
+                for (; j + 192 < b1; j += 256) {
                     const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + 64);
                     const uint32_t u2 = ldnt(a.cols + j + 128), u3 = ldnt(a.cols + j + 192);
-                    s0 += load_c(cin, u0);
-                    s1 += load_c(cin, u1);
-                    s2 += load_c(cin, u2);
-                    s3 += load_c(cin, u3);
+                    s0 += gather(cin, u0, hot);
+                    s1 += gather(cin, u1, hot);
+                    s2 += gather(cin, u2, hot);
+                    s3 += gather(cin, u3, hot);
                 }
-                for (; j < re; j += 64) s0 += load_c(cin, ldnt(a.cols + j));
-                float s = wave_sum((s0 + s1) + (s2 + s3));
-                if (lane == 0) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
+                for (; j < b1; j += 64) s0 += gather(cin, ldnt(a.cols + j), hot);
+                const float s = wave_sum((s0 + s1) + (s2 + s3));
+                // chunk partial; k_pr_finalize sums a row's chunks in chunk order
+                if (lane == 0) a.hub_partial[c] = s;
             }
-        } else if (vb < a.nb_hub + a.nb_mid + a.nb_low) {
-            // ---- low rows: SELL-64 slice per wave, lane per row ----
-            const uint32_t sl = (vb - a.nb_hub - a.nb_mid) * 4 + w;
+        } else if (vb < u_h) {
+            const uint32_t row = a.n_hub + (vb - u_w) * kSweepWaves + w;
+            group_rows<64>(a, row, a.n_w, cin, cout, wfmt, tele, hot, err, act, dang, lane);
+        } else if (vb < u_q) {
+            const uint32_t row = a.n_w + (vb - u_h) * (2 * kSweepWaves) + w * 2 + (lane >> 5);
+            group_rows<32>(a, row, a.n_h, cin, cout, wfmt, tele, hot, err, act, dang, lane);
+        } else if (vb < u_l) {
+            const uint32_t row = a.n_h + (vb - u_q) * (4 * kSweepWaves) + w * 4 + (lane >> 4);
+            group_rows<16>(a, row, a.n_nonlow, cin, cout, wfmt, tele, hot, err, act, dang, lane);
+        } else if (vb < u_z) {
+            // ---- low rows: SELL-64 slice per wave, lane per row, edges in order ----
+            const uint32_t sl = (vb - u_l) * kSweepWaves + w;
             const uint32_t nlow = a.n_nonzero - a.n_nonlow;
             if (sl * 64 < nlow) {
                 const uint32_t row = a.n_nonlow + sl * 64 + lane;
@@ -249,37 +285,54 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t it, uint32_t wfmt, const
                 const uint32_t *p = a.cols + a.slice_ptr[sl] + lane;
                 float s = 0.0f;
                 uint32_t k = 0;
-                for (; k + 4 <= width; k += 4) {
-                    uint32_t u0 = 0, u1 = 0, u2 = 0, u3 = 0;
-                    if (k < deg) u0 = ldnt(p + (k + 0) * 64);
-                    if (k + 1 < deg) u1 = ldnt(p + (k + 1) * 64);
-                    if (k + 2 < deg) u2 = ldnt(p + (k + 2) * 64);
-                    if (k + 3 < deg) u3 = ldnt(p + (k + 3) * 64);
-                    if (k < deg) s += load_c(cin, u0);
-                    if (k + 1 < deg) s += load_c(cin, u1);
-                    if (k + 2 < deg) s += load_c(cin, u2);
-                    if (k + 3 < deg) s += load_c(cin, u3);
+                for (; k + 8 <= width; k += 8) {
+                    uint32_t u[8];
+                    float x[8];
+#pragma unroll
+                    for (int t = 0; t < 8; t++) u[t] = (k + t < deg) ? ldnt(p + (k + t) * 64) : 0u;
+#pragma unroll
+                    for (int t = 0; t < 8; t++) x[t] = (k + t < deg) ? gather(cin, u[t], hot) : 0.f;
+#pragma unroll
+                    for (int t = 0; t < 8; t++)
+                        if (k + t < deg) s += x[t]; // sequential: the reference's order
                 }
                 for (; k < width; k++)
-                    if (k < deg) s += load_c(cin, ldnt(p + k * 64));
+                    if (k < deg) s += gather(cin, ldnt(p + k * 64), hot);
                 if (valid) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
             }
         } else {
             // ---- zero in-degree rows ----
-            const uint32_t r0 = a.n_nonzero + (vb - a.nb_hub - a.nb_mid - a.nb_low) * kZeroRowsPerBlock;
+            const uint32_t r0 = a.n_nonzero + (vb - u_z) * kZeroRowsPerUnit;
 #pragma unroll
-            for (int q = 0; q < kZeroRowsPerBlock / kBlock; q++) {
-                const uint32_t row = r0 + q * kBlock + threadIdx.x;
+            for (int q = 0; q < kZeroRowsPerUnit / kSweepBlock; q++) {
+                const uint32_t row = r0 + q * kSweepBlock + threadIdx.x;
                 if (row < a.N) update_vertex(a, row, 0.0f, tele, cout, wfmt, err, act, dang);
             }
         }
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
+// Copies the first `hot` contributions into LDS (16-byte loads).
+template <typename CT>
+__device__ __forceinline__ void load_hot(const CT *__restrict__ cin, uint32_t hot)
 {
-    __shared__ double s_err[kBlock / 64], s_dang[kBlock / 64];
-    __shared__ uint32_t s_act[kBlock / 64];
+    extern __shared__ __align__(16) unsigned char hot_raw[];
+    const uint32_t words = (uint32_t)(((uint64_t)hot * sizeof(CT) + 3) / 4);
+    const uint4 *src = reinterpret_cast<const uint4 *>(cin);
+    uint4 *dst = reinterpret_cast<uint4 *>(hot_raw);
+    for (uint32_t i = threadIdx.x; i < words / 4; i += kSweepBlock) dst[i] = src[i];
+    const uint32_t *sw = reinterpret_cast<const uint32_t *>(cin);
+    uint32_t *dw = reinterpret_cast<uint32_t *>(hot_raw);
+    for (uint32_t i = (words / 4) * 4 + threadIdx.x; i < words; i += kSweepBlock) dw[i] = sw[i];
+    __syncthreads();
+}
+
+// Persistent sweep: one 1024-thread workgroup per CU slot walks the work units
+// (hub chunks, mid rows, low slices, zero rows) with a grid stride.
+__global__ __launch_bounds__(kSweepBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
+{
+    __shared__ double s_err[kSweepWaves], s_dang[kSweepWaves];
+    __shared__ uint32_t s_act[kSweepWaves];
     SweepState *st = a.st;
     if (st->done) return; // converged in an earlier sweep: uniform early exit
     const uint32_t rfmt = a.fmt[it];
@@ -289,14 +342,21 @@ __global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
     const float tele = st->tele;
     double err = 0.0, dang = 0.0;
     uint32_t act = 0;
-    if (rfmt == kFmtF32)
-        sweep_body<float>(a, it, wfmt, static_cast<const float *>(cin), cout, tele, err, act, dang);
-    else if (rfmt == kFmtBF16)
-        sweep_body<bf16_t>(a, it, wfmt, static_cast<const bf16_t *>(cin), cout,
-                                   tele, err, act, dang);
-    else
-        sweep_body<__half>(a, it, wfmt, static_cast<const __half *>(cin), cout, tele, err, act,
+    const uint32_t csize = rfmt == kFmtF32 ? 4u : 2u;
+    const uint32_t hot = min(a.N, a.hot_bytes / csize);
+    if (rfmt == kFmtF32) {
+        load_hot(static_cast<const float *>(cin), hot);
+        sweep_body<float>(a, wfmt, static_cast<const float *>(cin), cout, tele, hot, err, act,
+                          dang);
+    } else if (rfmt == kFmtBF16) {
+        load_hot(static_cast<const bf16_t *>(cin), hot);
+        sweep_body<bf16_t>(a, wfmt, static_cast<const bf16_t *>(cin), cout, tele, hot, err, act,
                            dang);
+    } else {
+        load_hot(static_cast<const __half *>(cin), hot);
+        sweep_body<__half>(a, wfmt, static_cast<const __half *>(cin), cout, tele, hot, err, act,
+                           dang);
+    }
 
     // ---- block partials; k_pr_finalize reduces them in a fixed order ----
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -306,9 +366,12 @@ __global__ __launch_bounds__(kBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
     if (lane == 0) { s_err[w] = err; s_dang[w] = dang; s_act[w] = act; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        a.blk_err[blockIdx.x] = (s_err[0] + s_err[1]) + (s_err[2] + s_err[3]);
-        a.blk_dang[blockIdx.x] = (s_dang[0] + s_dang[1]) + (s_dang[2] + s_dang[3]);
-        a.blk_act[blockIdx.x] = s_act[0] + s_act[1] + s_act[2] + s_act[3];
+        double e = 0.0, d = 0.0;
+        uint32_t c = 0;
+        for (int i = 0; i < kSweepWaves; i++) { e += s_err[i]; d += s_dang[i]; c += s_act[i]; }
+        a.blk_err[blockIdx.x] = e;
+        a.blk_dang[blockIdx.x] = d;
+        a.blk_act[blockIdx.x] = c;
     }
 }
 
@@ -475,7 +538,11 @@ __global__ void k_relabel_vertex(const uint32_t *__restrict__ order,
     const uint64_t nonlow = __ballot(i < N && d > low_max);
     const uint64_t nonzero = __ballot(i < N && d > 0);
     const uint64_t dangl = __ballot(i < N && od == 0);
+    const uint64_t wide = __ballot(i < N && d >= kWaveRowMin);
+    const uint64_t half = __ballot(i < N && d >= kHalfRowMin);
     if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counts[4], (uint32_t)__popcll(wide));
+        atomicAdd(&counts[5], (uint32_t)__popcll(half));
         atomicAdd(&counts[0], (uint32_t)__popcll(hub));
         atomicAdd(&counts[1], (uint32_t)__popcll(nonlow));
         atomicAdd(&counts[2], (uint32_t)__popcll(nonzero));
@@ -543,6 +610,16 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->low_max = tn.low_max;
     p->hub_min = tn.hub_min;
     p->hub_chunk = tn.hub_chunk;
+    p->hot_bytes = tn.hot_bytes;
+    if (p->hot_bytes) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            AG_HIP(hipFuncSetAttribute((const void *)k_pr_sweep,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)(kLdsBytes - kStaticLds)));
+            attr_set = true;
+        }
+    }
     if (!g->inv_outdeg.p || !g->inv_idx.p || !g->inv_dst.p || !g->fwd_outdeg.p)
         fail(AG_ERR_INVALID, "graph has no inverse CSR (built with AG_BUILD_NO_INVERSE?)");
 
@@ -558,27 +635,31 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     const uint32_t kbits = bits_for((uint64_t)maxdeg + 1);
     DevBuf keys((size_t)N * 4), ids((size_t)N * 4), skeys((size_t)N * 4);
     p->rank.alloc((size_t)N * 4);
-    DevBuf rdeg((size_t)N * 4), counts(16);
+    DevBuf rdeg((size_t)N * 4), counts(32);
     const uint32_t nb = ceil_div(N, 256);
     hipLaunchKernelGGL(k_degree_keys, dim3(nb), dim3(256), 0, s, g->inv_outdeg.as<uint32_t>(), N,
                        maxdeg, keys.as<uint32_t>(), ids.as<uint32_t>());
     AG_HIP(hipGetLastError());
     radix_sort_pairs(ctx, keys.as<uint32_t>(), ids.as<uint32_t>(), skeys.as<uint32_t>(),
                      p->order.as<uint32_t>(), N, kbits);
-    AG_HIP(hipMemsetAsync(counts.p, 0, 16, s));
+    AG_HIP(hipMemsetAsync(counts.p, 0, 32, s));
     hipLaunchKernelGGL(k_relabel_vertex, dim3(nb), dim3(256), 0, s, p->order.as<uint32_t>(),
                        skeys.as<uint32_t>(), maxdeg, g->fwd_outdeg.as<uint32_t>(), N,
                        p->rank.as<uint32_t>(), rdeg.as<uint32_t>(), p->outdeg.as<uint32_t>(),
                        tn.low_max, tn.hub_min, counts.as<uint32_t>());
     AG_HIP(hipGetLastError());
     scan_exclusive_u32(ctx, rdeg.as<uint32_t>(), p->roff.as<uint32_t>(), N);
-    uint32_t cnt[4];
-    AG_HIP(hipMemcpyAsync(cnt, counts.p, 16, hipMemcpyDeviceToHost, s));
+    uint32_t cnt[8];
+    AG_HIP(hipMemcpyAsync(cnt, counts.p, 32, hipMemcpyDeviceToHost, s));
     AG_HIP(hipStreamSynchronize(s));
     p->n_hub = cnt[0];
     p->n_nonlow = cnt[1];
     p->n_nonzero = cnt[2];
     p->n_dangling = cnt[3];
+    // lane-group bins inside the CSR rows: [n_hub, n_w) 64 lanes per row,
+    // [n_w, n_h) 32, [n_h, n_nonlow) 16 (labels are sorted by degree)
+    p->n_w = std::min(std::max(cnt[4], p->n_hub), p->n_nonlow);
+    p->n_h = std::min(std::max(cnt[5], p->n_w), p->n_nonlow);
     uint32_t csr_edges = 0;
     AG_HIP(hipMemcpyAsync(&csr_edges, p->roff.as<uint32_t>() + p->n_nonlow, 4,
                           hipMemcpyDeviceToHost, s));
@@ -642,11 +723,13 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     AG_HIP(hipMemcpyAsync(p->chunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
     if (tot) AG_HIP(hipMemcpyAsync(p->chunk_row.p, crow.data(), crow.size() * 4, hipMemcpyHostToDevice, s));
 
-    p->nb_hub = tot;
-    p->nb_mid = ceil_div(p->n_nonlow - p->n_hub, kMidRowsPerBlock);
-    p->nb_low = ceil_div(p->low_slices, 4);
-    p->nb_zero = ceil_div(N - p->n_nonzero, kZeroRowsPerBlock);
-    p->nvblocks = p->nb_hub + p->nb_mid + p->nb_low + p->nb_zero;
+    p->nb_hub = ceil_div(tot, kSweepWaves);
+    p->nb_w = ceil_div(p->n_w - p->n_hub, kSweepWaves);
+    p->nb_h = ceil_div(p->n_h - p->n_w, 2 * kSweepWaves);
+    p->nb_q = ceil_div(p->n_nonlow - p->n_h, 4 * kSweepWaves);
+    p->nb_low = ceil_div(p->low_slices, kSweepWaves);
+    p->nb_zero = ceil_div(N - p->n_nonzero, kZeroRowsPerUnit);
+    p->nvblocks = p->nb_hub + p->nb_w + p->nb_h + p->nb_q + p->nb_low + p->nb_zero;
     AG_HIP(hipStreamSynchronize(s));
     return p.release();
 }
@@ -667,7 +750,11 @@ static void ensure_work(ag_ctx *ctx, Plan *p, uint32_t max_iter)
     Work &w = p->work;
     const uint32_t N = p->N;
     if (!w.pr.p) {
-        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(p->nvblocks, (uint32_t)ctx->num_cus * 8));
+        // persistent workgroups: as many per CU as the LDS hot table allows (<= 2)
+        const uint32_t per_cu =
+            std::min<uint32_t>(2, std::max<uint32_t>(1, kLdsBytes / (p->hot_bytes + kStaticLds)));
+        const uint32_t grid = std::max<uint32_t>(
+            1, std::min<uint32_t>(p->nvblocks, (uint32_t)ctx->num_cus * per_cu));
         w.grid = grid;
         w.pr.alloc((size_t)N * 4);
         w.c0.alloc((size_t)N * 4);
@@ -697,7 +784,12 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.n_nonlow = p->n_nonlow;
     a.n_nonzero = p->n_nonzero;
     a.nb_hub = p->nb_hub;
-    a.nb_mid = p->nb_mid;
+    a.nb_w = p->nb_w;
+    a.nb_h = p->nb_h;
+    a.nb_q = p->nb_q;
+    a.n_w = p->n_w;
+    a.n_h = p->n_h;
+    a.hub_chunks = p->hub_chunks;
     a.nb_low = p->nb_low;
     a.nb_zero = p->nb_zero;
     a.nvblocks = p->nvblocks;
@@ -720,6 +812,7 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.chunk_first = p->chunk_first.as<uint32_t>();
     a.hub_partial = p->hub_partial.as<float>();
     a.nparts = w.grid;
+    a.hot_bytes = p->hot_bytes;
     a.pr = w.pr.as<float>();
     a.c0 = w.c0.p;
     a.c1 = w.c1.p;
@@ -781,7 +874,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
         const uint32_t it1 = std::min(max_iter, it0 + group);
         for (uint32_t it = it0; it < it1; it++) {
-            hipLaunchKernelGGL(k_pr_sweep, dim3(grid), dim3(kBlock), 0, s, a, it);
+            hipLaunchKernelGGL(k_pr_sweep, dim3(grid), dim3(kSweepBlock), p->hot_bytes, s, a, it);
             hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
         }
         AG_HIP(hipGetLastError());

@@ -21,13 +21,13 @@ struct Work {
 
 struct Plan {
     uint32_t N = 0, E = 0;
-    uint32_t n_hub = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
+    uint32_t n_hub = 0, n_w = 0, n_h = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
     uint32_t low_slices = 0, hub_chunks = 0;
-    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0;
+    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0, hot_bytes = 0;
     uint64_t csr_edges = 0, sell_slots = 0;
     // order[i] = vertex with label i; rank[v] = label of vertex v
     DevBuf order, rank, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial;
-    uint32_t nb_hub = 0, nb_mid = 0, nb_low = 0, nb_zero = 0, nvblocks = 0;
+    uint32_t nb_hub = 0, nb_w = 0, nb_h = 0, nb_q = 0, nb_low = 0, nb_zero = 0, nvblocks = 0;
     Work work;
     std::mutex mu;
     uint64_t bytes() const

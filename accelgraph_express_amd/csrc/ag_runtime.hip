@@ -140,6 +140,9 @@ int ag_set_tuning(const ag_tuning *t)
         if (t->low_max) n.low_max = t->low_max;
         if (t->hub_min) n.hub_min = t->hub_min;
         if (t->hub_chunk) n.hub_chunk = t->hub_chunk;
+        if (t->hot_bytes) n.hot_bytes = t->hot_bytes == 1 ? 0u : t->hot_bytes;
+        if (n.hot_bytes % 16 || n.hot_bytes > 160 * 1024 - 2048)
+            fail(AG_ERR_INVALID, "hot_bytes must be a multiple of 16 and <= 161792");
         if (n.low_max >= n.hub_min) fail(AG_ERR_INVALID, "low_max must be < hub_min");
         if (n.hub_chunk < 256) fail(AG_ERR_INVALID, "hub_chunk must be >= 256");
         g_tuning = n;
@@ -153,7 +156,7 @@ int ag_get_tuning(ag_tuning *t)
         t->low_max = g_tuning.low_max;
         t->hub_min = g_tuning.hub_min;
         t->hub_chunk = g_tuning.hub_chunk;
-        t->relabel = 1;
+        t->hot_bytes = g_tuning.hot_bytes;
     });
 }
 

@@ -226,15 +226,18 @@ class Graph:
         return r
 
 
-def set_tuning(low_max: int = 0, hub_min: int = 0, hub_chunk: int = 0) -> None:
-    t = L.Tuning(low_max, hub_min, hub_chunk, 0)
+def set_tuning(low_max: int = 0, hub_min: int = 0, hub_chunk: int = 0,
+               hot_bytes: int = 0) -> None:
+    """0 keeps a knob; hot_bytes=1 disables the LDS table of hot contributions."""
+    t = L.Tuning(low_max, hub_min, hub_chunk, hot_bytes)
     L.check(L.load().ag_set_tuning(C.byref(t)))
 
 
 def get_tuning() -> dict:
     t = L.Tuning()
     L.check(L.load().ag_get_tuning(C.byref(t)))
-    return {"low_max": t.low_max, "hub_min": t.hub_min, "hub_chunk": t.hub_chunk}
+    return {"low_max": t.low_max, "hub_min": t.hub_min, "hub_chunk": t.hub_chunk,
+            "hot_bytes": t.hot_bytes}
 
 
 # ---- synthetic inputs ----

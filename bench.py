@@ -44,6 +44,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2022)
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 256 MiB device copy first (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); every box here has one "
@@ -62,6 +64,13 @@ def main() -> int:
     log(f"R-MAT s{scale} ef{ef}: N={N} E={E} generated in {time.time() - t:.1f}s")
 
     ctx = ag.Context(0)
+    if args.copy_probe:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 268435456 bytes read + written by one device copy")
+        del a, b
     d_src = torch.from_numpy(src.view(np.int32)).to("cuda")
     d_dst = torch.from_numpy(dst.view(np.int32)).to("cuda")
     torch.cuda.synchronize()

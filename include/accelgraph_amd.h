@@ -178,7 +178,8 @@ typedef struct ag_tuning
     uint32_t low_max;     /* in-degree <= low_max: thread per vertex */
     uint32_t hub_min;     /* in-degree >= hub_min: workgroup chunks */
     uint32_t hub_chunk;   /* edges per hub workgroup chunk */
-    uint32_t relabel;     /* 1: internal degree-sorted labels (default), 2: identity */
+    uint32_t hot_bytes;   /* LDS bytes per sweep workgroup caching the hottest
+                             contributions (multiple of 16; 1 disables the table) */
 } ag_tuning;
 int ag_set_tuning(const ag_tuning *t);
 int ag_get_tuning(ag_tuning *t);

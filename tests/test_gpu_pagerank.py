@@ -134,7 +134,7 @@ def test_skewed_bins_all_paths(ctx):
         assert min(info["n_hub"], info["n_mid"], info["n_low"], info["n_zero"]) > 0
         assert info["hub_chunks"] > info["n_hub"]
     finally:
-        ag.set_tuning(low_max=24, hub_min=4096, hub_chunk=2048)
+        ag.set_tuning(low_max=64, hub_min=4096, hub_chunk=4096)
 
 
 def test_rmat_s22_20_sweeps(ctx):
@@ -148,6 +148,20 @@ def test_rmat_s22_20_sweeps(ctx):
           f"max rel {_rel(ref, f64).max():.3e}")
     print(f"s22 vs FP64 oracle: max abs {e.max():.3e}, L1 {e.sum():.3e}, "
           f"max rel {_rel(st.pageRanks, f64).max():.3e}; loop {st.loop_ms:.3f} ms")
+
+
+@pytest.mark.parametrize("hot_bytes", [1, 4096, 65536])
+def test_hot_table_sizes(ctx, hot_bytes):
+    """The LDS table of hot contributions changes where values are read from,
+    not the arithmetic: ranks are identical for every table size."""
+    src, dst = ag.gen_rmat(21, 16, 16)
+    ref = ag.Graph.build(ctx, src, dst, 1 << 16).pagerank_pull(20, 0.0).pageRanks
+    ag.set_tuning(hot_bytes=hot_bytes)
+    try:
+        got = ag.Graph.build(ctx, src, dst, 1 << 16).pagerank_pull(20, 0.0).pageRanks
+    finally:
+        ag.set_tuning(hot_bytes=159744)
+    np.testing.assert_array_equal(got, ref)
 
 
 def test_repeat_calls_identical(ctx):
