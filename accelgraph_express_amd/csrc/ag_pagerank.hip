@@ -73,6 +73,10 @@ struct SweepArgs {
     uint32_t n_hub, n_w, n_h, n_nonlow, n_nonzero; // bin boundaries (labels)
     uint32_t nb_hub, nb_w, nb_h, nb_q, nb_low, nb_zero, nvblocks; // work units per bin
     uint32_t hub_chunk, hub_chunks;
+    // FP32 REFERENCE mode: zero in-degree rows keep pr = base and contribution
+    // base/outdeg in both buffers (written at init); their |delta| is 0, so
+    // they count as active exactly when epsilon <= 0.
+    uint32_t skip_zero;
     uint32_t mode;
     uint32_t max_iter;
     uint32_t finish_fp32;
@@ -214,16 +218,24 @@ __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uin
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     if (valid) {
         const uint32_t rb = a.roff[row], re = a.roff[row + 1];
+        // indices of the next batch are in flight while the current batch gathers
         uint32_t j = rb + gl;
-        for (; j + 3 * G < re; j += 4 * G) {
-            const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + G);
-            const uint32_t u2 = ldnt(a.cols + j + 2 * G), u3 = ldnt(a.cols + j + 3 * G);
-            s0 += gather(cin, u0, hot);
-            s1 += gather(cin, u1, hot);
-            s2 += gather(cin, u2, hot);
-            s3 += gather(cin, u3, hot);
+        uint32_t c0 = j < re ? ldnt(a.cols + j) : 0u;
+        uint32_t c1 = j + G < re ? ldnt(a.cols + j + G) : 0u;
+        uint32_t c2 = j + 2 * G < re ? ldnt(a.cols + j + 2 * G) : 0u;
+        uint32_t c3 = j + 3 * G < re ? ldnt(a.cols + j + 3 * G) : 0u;
+        for (; j < re; j += 4 * G) {
+            const uint32_t jn = j + 4 * G;
+            const uint32_t n0 = jn < re ? ldnt(a.cols + jn) : 0u;
+            const uint32_t n1 = jn + G < re ? ldnt(a.cols + jn + G) : 0u;
+            const uint32_t n2 = jn + 2 * G < re ? ldnt(a.cols + jn + 2 * G) : 0u;
+            const uint32_t n3 = jn + 3 * G < re ? ldnt(a.cols + jn + 3 * G) : 0u;
+            s0 += gather(cin, c0, hot);
+            if (j + G < re) s1 += gather(cin, c1, hot);
+            if (j + 2 * G < re) s2 += gather(cin, c2, hot);
+            if (j + 3 * G < re) s3 += gather(cin, c3, hot);
+            c0 = n0; c1 = n1; c2 = n2; c3 = n3;
         }
-        for (; j < re; j += G) s0 += gather(cin, ldnt(a.cols + j), hot);
     }
     float s = (s0 + s1) + (s2 + s3);
 #pragma unroll
@@ -251,15 +263,22 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                 const uint32_t b1 = min(a.roff[row + 1], b0 + a.hub_chunk);
                 float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
                 uint32_t j = b0 + lane;
-                for (; j + 192 < b1; j += 256) {
-                    const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + 64);
-                    const uint32_t u2 = ldnt(a.cols + j + 128), u3 = ldnt(a.cols + j + 192);
-                    s0 += gather(cin, u0, hot);
-                    s1 += gather(cin, u1, hot);
-                    s2 += gather(cin, u2, hot);
-                    s3 += gather(cin, u3, hot);
+                uint32_t c0 = j < b1 ? ldnt(a.cols + j) : 0u;
+                uint32_t c1 = j + 64 < b1 ? ldnt(a.cols + j + 64) : 0u;
+                uint32_t c2 = j + 128 < b1 ? ldnt(a.cols + j + 128) : 0u;
+                uint32_t c3 = j + 192 < b1 ? ldnt(a.cols + j + 192) : 0u;
+                for (; j < b1; j += 256) {
+                    const uint32_t jn = j + 256;
+                    const uint32_t n0 = jn < b1 ? ldnt(a.cols + jn) : 0u;
+                    const uint32_t n1 = jn + 64 < b1 ? ldnt(a.cols + jn + 64) : 0u;
+                    const uint32_t n2 = jn + 128 < b1 ? ldnt(a.cols + jn + 128) : 0u;
+                    const uint32_t n3 = jn + 192 < b1 ? ldnt(a.cols + jn + 192) : 0u;
+                    s0 += gather(cin, c0, hot);
+                    if (j + 64 < b1) s1 += gather(cin, c1, hot);
+                    if (j + 128 < b1) s2 += gather(cin, c2, hot);
+                    if (j + 192 < b1) s3 += gather(cin, c3, hot);
+                    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
                 }
-                for (; j < b1; j += 64) s0 += gather(cin, ldnt(a.cols + j), hot);
                 const float s = wave_sum((s0 + s1) + (s2 + s3));
                 // chunk partial; k_pr_finalize sums a row's chunks in chunk order
                 if (lane == 0) a.hub_partial[c] = s;
@@ -284,23 +303,28 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                 const uint32_t width = __shfl(deg, 0, 64); // rows sorted: lane 0 is the widest
                 const uint32_t *p = a.cols + a.slice_ptr[sl] + lane;
                 float s = 0.0f;
-                uint32_t k = 0;
-                for (; k + 8 <= width; k += 8) {
-                    uint32_t u[8];
-                    float x[8];
+                // batch of 8 indices in flight while the previous batch gathers;
+                // each lane adds in the reference's edge order
+                uint32_t u[8];
 #pragma unroll
-                    for (int t = 0; t < 8; t++) u[t] = (k + t < deg) ? ldnt(p + (k + t) * 64) : 0u;
-#pragma unroll
-                    for (int t = 0; t < 8; t++) x[t] = (k + t < deg) ? gather(cin, u[t], hot) : 0.f;
+                for (int t = 0; t < 8; t++) u[t] = (uint32_t)t < deg ? ldnt(p + t * 64) : 0u;
+                for (uint32_t k = 0; k < width; k += 8) {
+                    uint32_t nx[8];
 #pragma unroll
                     for (int t = 0; t < 8; t++)
-                        if (k + t < deg) s += x[t]; // sequential: the reference's order
+                        nx[t] = k + 8 + t < deg ? ldnt(p + (k + 8 + t) * 64) : 0u;
+                    float x[8];
+#pragma unroll
+                    for (int t = 0; t < 8; t++) x[t] = k + t < deg ? gather(cin, u[t], hot) : 0.f;
+#pragma unroll
+                    for (int t = 0; t < 8; t++)
+                        if (k + t < deg) s += x[t];
+#pragma unroll
+                    for (int t = 0; t < 8; t++) u[t] = nx[t];
                 }
-                for (; k < width; k++)
-                    if (k < deg) s += gather(cin, ldnt(p + k * 64), hot);
                 if (valid) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
             }
-        } else {
+        } else if (!a.skip_zero) {
             // ---- zero in-degree rows ----
             const uint32_t r0 = a.n_nonzero + (vb - u_z) * kZeroRowsPerUnit;
 #pragma unroll
@@ -329,34 +353,32 @@ __device__ __forceinline__ void load_hot(const CT *__restrict__ cin, uint32_t ho
 
 // Persistent sweep: one 1024-thread workgroup per CU slot walks the work units
 // (hub chunks, mid rows, low slices, zero rows) with a grid stride.
+template <typename CT> struct FmtOf;
+template <> struct FmtOf<float> { static constexpr uint32_t v = kFmtF32; };
+template <> struct FmtOf<bf16_t> { static constexpr uint32_t v = kFmtBF16; };
+template <> struct FmtOf<__half> { static constexpr uint32_t v = 2u; };
+
+// One kernel per storage format of the contributions it reads (keeps each
+// kernel's register footprint to one instantiation).  A mixed-precision run
+// launches the low-precision and the FP32 kernel every sweep; the one whose
+// format does not match the sweep's exits at once.
+template <typename CT>
 __global__ __launch_bounds__(kSweepBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
 {
     __shared__ double s_err[kSweepWaves], s_dang[kSweepWaves];
     __shared__ uint32_t s_act[kSweepWaves];
     SweepState *st = a.st;
     if (st->done) return; // converged in an earlier sweep: uniform early exit
-    const uint32_t rfmt = a.fmt[it];
+    if (a.fmt[it] != FmtOf<CT>::v) return;
     const uint32_t wfmt = a.fmt[it + 1];
-    const void *cin = (it & 1) ? a.c1 : a.c0;
+    const CT *cin = static_cast<const CT *>((it & 1) ? a.c1 : a.c0);
     void *cout = (it & 1) ? a.c0 : a.c1;
     const float tele = st->tele;
     double err = 0.0, dang = 0.0;
     uint32_t act = 0;
-    const uint32_t csize = rfmt == kFmtF32 ? 4u : 2u;
-    const uint32_t hot = min(a.N, a.hot_bytes / csize);
-    if (rfmt == kFmtF32) {
-        load_hot(static_cast<const float *>(cin), hot);
-        sweep_body<float>(a, wfmt, static_cast<const float *>(cin), cout, tele, hot, err, act,
-                          dang);
-    } else if (rfmt == kFmtBF16) {
-        load_hot(static_cast<const bf16_t *>(cin), hot);
-        sweep_body<bf16_t>(a, wfmt, static_cast<const bf16_t *>(cin), cout, tele, hot, err, act,
-                           dang);
-    } else {
-        load_hot(static_cast<const __half *>(cin), hot);
-        sweep_body<__half>(a, wfmt, static_cast<const __half *>(cin), cout, tele, hot, err, act,
-                           dang);
-    }
+    const uint32_t hot = min(a.N, a.hot_bytes / (uint32_t)sizeof(CT));
+    load_hot(cin, hot);
+    sweep_body<CT>(a, wfmt, cin, cout, tele, hot, err, act, dang);
 
     // ---- block partials; k_pr_finalize reduces them in a fixed order ----
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -413,6 +435,7 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     double E_ = 0.0, D_ = 0.0;
     uint32_t A_ = 0;
     for (int i = 0; i < kFinBlock / 64; i++) { E_ += s_err[i]; D_ += s_dang[i]; A_ += s_act[i]; }
+    if (a.skip_zero && a.epsilon <= 0.0) A_ += a.N - a.n_nonzero;
     a.err_iter[it] = E_;
     a.act_iter[it] = A_;
     st->prev_err = st->last_err;
@@ -470,7 +493,9 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
     }
     a.pr[i] = x;
     const uint32_t od = a.outdeg[i];
-    store_c(a.c0, fmt0, i, od ? x / (float)od : 0.0f);
+    const float c = od ? x / (float)od : 0.0f;
+    store_c(a.c0, fmt0, i, c);
+    if (a.skip_zero && i >= a.n_nonzero) store_c(a.c1, fmt0, i, c);
 }
 
 // Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
@@ -614,9 +639,12 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     if (p->hot_bytes) {
         static bool attr_set = false;
         if (!attr_set) {
-            AG_HIP(hipFuncSetAttribute((const void *)k_pr_sweep,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)(kLdsBytes - kStaticLds)));
+            const void *ks[3] = {(const void *)k_pr_sweep<float>,
+                                 (const void *)k_pr_sweep<bf16_t>,
+                                 (const void *)k_pr_sweep<__half>};
+            for (const void *k : ks)
+                AG_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(kLdsBytes - kStaticLds)));
             attr_set = true;
         }
     }
@@ -797,6 +825,7 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.mode = o.mode;
     a.max_iter = max_iter;
     a.finish_fp32 = (o.flags & AG_PR_FLAG_FINISH_FP32) ? 1u : 0u;
+    a.skip_zero = (o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ? 1u : 0u;
     a.damp = o.damp;
     {
 #pragma clang fp contract(off)
@@ -874,7 +903,15 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
         const uint32_t it1 = std::min(max_iter, it0 + group);
         for (uint32_t it = it0; it < it1; it++) {
-            hipLaunchKernelGGL(k_pr_sweep, dim3(grid), dim3(kSweepBlock), p->hot_bytes, s, a, it);
+            if (o.precision == AG_PREC_BF16)
+                hipLaunchKernelGGL(k_pr_sweep<bf16_t>, dim3(grid), dim3(kSweepBlock),
+                                   p->hot_bytes, s, a, it);
+            else if (o.precision == AG_PREC_FP16)
+                hipLaunchKernelGGL(k_pr_sweep<__half>, dim3(grid), dim3(kSweepBlock),
+                                   p->hot_bytes, s, a, it);
+            if (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32))
+                hipLaunchKernelGGL(k_pr_sweep<float>, dim3(grid), dim3(kSweepBlock),
+                                   p->hot_bytes, s, a, it);
             hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
         }
         AG_HIP(hipGetLastError());
