@@ -2,7 +2,9 @@
 // one graph, advanced sweep by sweep (internal; not part of the C ABI).
 #pragma once
 
+#include <algorithm>
 #include <deque>
+#include <memory>
 #include <vector>
 
 #include "ag_plan.h"
@@ -18,6 +20,8 @@ struct PprCol {
     double tol;
     double residual;
     double dangling;
+    uint32_t stalled;  // low-precision column frozen for its FP32 finish
+    uint32_t finish;   // 1: freeze at the stall / tolerance instead of finishing
 };
 
 struct PprBatchState {
@@ -66,8 +70,12 @@ class PprBatch {
         uint64_t tag;
         uint32_t iters;
         double residual;
+        bool stalled; // low precision: ready to move to an FP32 batch
     };
     std::vector<Done> poll();
+    // Moves a stalled column into `dst` (an FP32 batch), iterate and state
+    // included; returns the new slot in dst.  The slot here is not removed.
+    uint32_t migrate(uint32_t slot, PprBatch &dst);
     // Copies a finished column (caller's vertex order) to host memory.
     void extract(uint32_t slot, float *host_out);
     // Drops a slot; the last active slot moves into it.  Returns the tag that
@@ -94,6 +102,64 @@ class PprBatch {
     hipGraph_t graph_ = nullptr;
     double last_ms_ = 0.0;
 };
+
+// A graph's column slots in two stages: columns run in the requested storage
+// precision; with BF16/FP16 storage a column that stalls (or meets its
+// tolerance) is frozen and finished in an FP32 batch.  FP32 runs use one stage.
+class PprPipeline {
+  public:
+    PprPipeline(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt, float damp);
+    static uint64_t bytes_for(uint32_t N, uint32_t cap, uint32_t fmt);
+    uint32_t free_slots() const { return first_->free_slots(); }
+    uint32_t active() const { return first_->active() + (fp32_ ? fp32_->active() : 0u); }
+    PprSeedSet make_seeds(const uint32_t *seeds, uint32_t n, double tol, uint32_t max_iter)
+    {
+        return first_->make_seeds(seeds, n, tol, max_iter);
+    }
+    void admit(const PprSeedSet &q, uint64_t tag) { first_->admit(q, tag); }
+    void sweep(uint32_t n, bool replay);
+    struct Finished {
+        uint64_t tag;
+        uint32_t iters;
+        double residual;
+    };
+    // Migrates stalled columns, retires finished ones; `out(tag)` names the
+    // host buffer for a finished column's result (nullptr: drop it).
+    template <typename F> std::vector<Finished> retire(F out);
+    bool cancel(uint64_t tag);
+
+  private:
+    std::unique_ptr<PprBatch> first_, fp32_;
+};
+
+template <typename F>
+std::vector<PprPipeline::Finished> PprPipeline::retire(F out)
+{
+    std::vector<Finished> fin;
+    auto by_slot = [](const PprBatch::Done &a, const PprBatch::Done &b) { return a.slot > b.slot; };
+    auto done = first_->poll();
+    std::sort(done.begin(), done.end(), by_slot);
+    for (auto &d : done) {
+        if (d.stalled) {
+            if (!fp32_ || fp32_->free_slots() == 0) continue; // stays frozen until room
+            first_->migrate(d.slot, *fp32_);
+        } else {
+            first_->extract(d.slot, out(d.tag));
+            fin.push_back({d.tag, d.iters, d.residual});
+        }
+        first_->remove(d.slot);
+    }
+    if (fp32_) {
+        auto d2 = fp32_->poll();
+        std::sort(d2.begin(), d2.end(), by_slot);
+        for (auto &d : d2) {
+            fp32_->extract(d.slot, out(d.tag));
+            fin.push_back({d.tag, d.iters, d.residual});
+            fp32_->remove(d.slot);
+        }
+    }
+    return fin;
+}
 
 uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt);
 

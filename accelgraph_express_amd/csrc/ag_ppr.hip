@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
             cur = (int)cg;
         }
         const uint32_t c = cg * CPW + cl;
-        const bool act = c < K && !a.cs[c].done;
+        const bool act = c < K && !a.cs[c].done && !a.cs[c].stalled;
         const float t = act ? a.cs[c].t : 0.0f;
         if (u < a.nb_hub) {
             // hub chunk: the workgroup's 4*G lane groups stride over the chunk
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kMaxCap) void k_ppr_hub(PprArgs a)
     if (K == 0) return;
     const uint32_t row = blockIdx.x, c = threadIdx.x;
     double res = 0.0, dang = 0.0;
-    if (c < K && !a.cs[c].done) {
+    if (c < K && !a.cs[c].done && !a.cs[c].stalled) {
         const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
         float sum = 0.0f;
         for (uint32_t q = f0; q < f1; q++) sum += a.hub_part[(uint64_t)q * a.cap + c];
@@ -285,14 +285,20 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a)
     __syncthreads();
     if (threadIdx.x < K) {
         PprCol &col = a.cs[c];
-        if (!col.done) {
+        if (!col.done && !col.stalled) {
             double R = 0.0, D = 0.0;
             for (uint32_t i = 0; i < P; i++) { R += q_res[i][c]; D += q_dang[i][c]; }
+            const double prev = col.residual;
             col.iters += 1;
             col.residual = R;
             col.dangling = D;
             col.t = (float)((1.0 - (double)a.damp) + (double)a.damp * D);
-            if (R <= col.tol || col.iters >= col.max_iter) col.done = 1;
+            if (col.iters >= col.max_iter)
+                col.done = 1;
+            else if (col.finish && (R <= col.tol || (col.iters >= 3 && R > 0.97 * prev)))
+                col.stalled = 1; // low-precision floor reached: finish in FP32
+            else if (R <= col.tol)
+                col.done = 1;
         }
     }
     if (threadIdx.x == 0) a.bs->sweeps += 1;
@@ -333,6 +339,8 @@ __global__ void k_ppr_seed_col(float *X, void *Cin, const uint32_t *outdeg, uint
         col.tol = tol;
         col.residual = INFINITY;
         col.dangling = d0;
+        col.stalled = 0;
+        col.finish = fmt != AG_PREC_FP32 ? 1u : 0u;
         cs[c] = col;
     }
 }
@@ -357,6 +365,30 @@ __global__ void k_ppr_move_x(float *X, uint32_t N, uint32_t cap, uint32_t from, 
     if (i >= N) return;
     const uint64_t b = (uint64_t)i * cap;
     X[b + to] = X[b + from];
+}
+
+// Migration of one column into an FP32 batch: iterate copied, both
+// contribution buffers recomputed from it (x / outdeg), state carried over.
+__global__ void k_ppr_migrate_col(const float *Xs, uint32_t scap, uint32_t sc, float *Xd, float *C0,
+                                  float *C1, uint32_t dcap, uint32_t dc, const uint32_t *outdeg,
+                                  uint32_t N, const PprCol *cs_src, PprCol *cs_dst)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        PprCol col = cs_src[sc];
+        col.stalled = 0;
+        col.done = 0;
+        col.finish = 0;
+        cs_dst[dc] = col;
+    }
+    if (i >= N) return;
+    const float x = Xs[(uint64_t)i * scap + sc];
+    const uint32_t od = outdeg[i];
+    const float c = od ? x / (float)od : 0.0f;
+    const uint64_t k = (uint64_t)i * dcap + dc;
+    Xd[k] = x;
+    C0[k] = c;
+    C1[k] = c;
 }
 
 __global__ void k_ppr_extract(const float *X, const uint32_t *order, uint32_t N, uint32_t cap,
@@ -649,21 +681,40 @@ std::vector<PprBatch::Done> PprBatch::poll()
                           ctx_->stream));
     AG_HIP(hipStreamSynchronize(ctx_->stream));
     for (uint32_t k = 0; k < K_; k++)
-        if (cs[k].done) out.push_back({k, tags_[k], cs[k].iters, cs[k].residual});
+        if (cs[k].done || cs[k].stalled)
+            out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].stalled != 0});
     return out;
+}
+
+uint32_t PprBatch::migrate(uint32_t slot, PprBatch &dst)
+{
+    if (slot >= K_) fail(AG_ERR_INVALID, "slot %u not active", slot);
+    if (dst.fmt_ != AG_PREC_FP32) fail(AG_ERR_INVALID, "migration target must store FP32");
+    if (dst.K_ >= dst.cap_) fail(AG_ERR_ADMISSION, "no free FP32 slot");
+    hipStream_t s = ctx_->stream;
+    const uint32_t dc = dst.K_;
+    hipLaunchKernelGGL(k_ppr_migrate_col, dim3(ceil_div(N_, 256)), dim3(256), 0, s,
+                       X_.as<float>(), cap_, slot, dst.X_.as<float>(), dst.C_[0].as<float>(),
+                       dst.C_[1].as<float>(), dst.cap_, dc, p_->outdeg.as<uint32_t>(), N_,
+                       cols_.as<PprCol>(), dst.cols_.as<PprCol>());
+    AG_HIP(hipGetLastError());
+    dst.q_[dc] = q_[slot];
+    dst.tags_[dc] = tags_[slot];
+    dst.K_++;
+    dst.upload_seed_table();
+    return dc;
 }
 
 void PprBatch::extract(uint32_t slot, float *host_out)
 {
     if (slot >= K_) fail(AG_ERR_INVALID, "slot %u not active", slot);
+    if (!host_out) return;
     hipStream_t s = ctx_->stream;
     hipLaunchKernelGGL(k_ppr_extract, dim3(ceil_div(N_, 256)), dim3(256), 0, s, X_.as<float>(),
                        p_->order.as<uint32_t>(), N_, cap_, slot, out_.as<float>());
     AG_HIP(hipGetLastError());
-    if (host_out) {
-        AG_HIP(hipMemcpyAsync(host_out, out_.p, (size_t)N_ * 4, hipMemcpyDeviceToHost, s));
-        AG_HIP(hipStreamSynchronize(s));
-    }
+    AG_HIP(hipMemcpyAsync(host_out, out_.p, (size_t)N_ * 4, hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
 }
 
 uint64_t PprBatch::remove(uint32_t slot)
@@ -693,6 +744,39 @@ uint64_t PprBatch::remove(uint32_t slot)
     K_--;
     upload_seed_table();
     return moved;
+}
+
+PprPipeline::PprPipeline(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt,
+                         float damp)
+{
+    first_.reset(new PprBatch(ctx, g, cap, cpw, fmt, damp));
+    if (fmt != AG_PREC_FP32) fp32_.reset(new PprBatch(ctx, g, cap, cpw, AG_PREC_FP32, damp));
+}
+
+uint64_t PprPipeline::bytes_for(uint32_t N, uint32_t cap, uint32_t fmt)
+{
+    uint64_t b = PprBatch::bytes_for(N, cap, fmt);
+    if (fmt != AG_PREC_FP32) b += PprBatch::bytes_for(N, cap, AG_PREC_FP32);
+    return b;
+}
+
+void PprPipeline::sweep(uint32_t n, bool replay)
+{
+    if (first_->active()) first_->sweep(n, replay);
+    if (fp32_ && fp32_->active()) fp32_->sweep(n, replay);
+}
+
+bool PprPipeline::cancel(uint64_t tag)
+{
+    for (PprBatch *b : {first_.get(), fp32_.get()}) {
+        if (!b) continue;
+        for (uint32_t k = 0; k < b->active(); k++)
+            if (b->tag(k) == tag) {
+                b->remove(k);
+                return true;
+            }
+    }
+    return false;
 }
 
 uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt)
@@ -757,34 +841,29 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
         if (nq == 0) return;
         const uint32_t cpw = o.cols_per_wave ? o.cols_per_wave : ppr_autotune(ctx, g, o.precision);
         const uint32_t cap = std::min<uint32_t>(kMaxCap, ((nq + 3) / 4) * 4);
-        PprBatch b(ctx, g, cap, cpw, o.precision, o.damp);
+        PprPipeline pl(ctx, g, cap, cpw, o.precision, o.damp);
         const uint32_t N = g->N;
         std::vector<PprSeedSet> sets(nq);
         for (uint32_t i = 0; i < nq; i++)
-            sets[i] = b.make_seeds(q[i].seeds, q[i].nseeds, q[i].tol, q[i].max_iter);
+            sets[i] = pl.make_seeds(q[i].seeds, q[i].nseeds, q[i].tol, q[i].max_iter);
         AG_HIP(hipStreamSynchronize(ctx->stream));
         const auto t0 = std::chrono::steady_clock::now();
         uint32_t next = 0, finished = 0;
-        while (next < nq && b.free_slots()) { b.admit(sets[next], next); next++; }
+        while (next < nq && pl.free_slots()) { pl.admit(sets[next], next); next++; }
         while (finished < nq) {
-            b.sweep(4, o.use_graph_replay != 0);
-            auto done = b.poll();
-            // retire from the highest slot down so that moved slots stay valid
-            std::sort(done.begin(), done.end(),
-                      [](const PprBatch::Done &a, const PprBatch::Done &c) { return a.slot > c.slot; });
-            for (auto &d : done) {
-                if (x) b.extract(d.slot, x + (size_t)d.tag * N);
+            pl.sweep(4, o.use_graph_replay != 0);
+            auto fin = pl.retire([&](uint64_t tag) { return x ? x + (size_t)tag * N : nullptr; });
+            for (auto &f : fin) {
                 if (stats) {
-                    stats[d.tag].iterations = d.iters;
-                    stats[d.tag].residual = d.residual;
-                    stats[d.tag].status = AG_OK;
-                    stats[d.tag].latency_ms = std::chrono::duration<double, std::milli>(
+                    stats[f.tag].iterations = f.iters;
+                    stats[f.tag].residual = f.residual;
+                    stats[f.tag].status = AG_OK;
+                    stats[f.tag].latency_ms = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - t0).count();
                 }
-                b.remove(d.slot);
                 finished++;
             }
-            while (next < nq && b.free_slots()) { b.admit(sets[next], next); next++; }
+            while (next < nq && pl.free_slots()) { pl.admit(sets[next], next); next++; }
         }
         AG_HIP(hipStreamSynchronize(ctx->stream));
         if (elapsed_ms)

@@ -37,7 +37,7 @@ struct Query {
 
 struct GraphEntry {
     ag_graph *g = nullptr;
-    std::unique_ptr<ag::PprBatch> batch;
+    std::unique_ptr<ag::PprPipeline> batch;
     uint64_t reserved = 0;
     uint32_t cpw = 0;
 };
@@ -67,7 +67,7 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
     if (ge.batch) return true;
     const uint32_t N = ge.g->N;
     uint32_t cap = std::min<uint32_t>(64, std::max<uint32_t>(4, s->cfg.max_columns / 4 * 4));
-    while (cap >= 4 && s->reserved + PprBatch::bytes_for(N, cap, s->cfg.precision) >
+    while (cap >= 4 && s->reserved + PprPipeline::bytes_for(N, cap, s->cfg.precision) >
                            s->cfg.hbm_budget_bytes)
         cap -= 4;
     if (cap < 4) return false;
@@ -76,8 +76,8 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
         return false; // injected allocation failure: the query stays queued
     if (!ge.cpw) ge.cpw = s->cfg.cols_per_wave ? s->cfg.cols_per_wave
                                                : ppr_autotune(s->ctx, ge.g, s->cfg.precision);
-    ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp));
-    ge.reserved = PprBatch::bytes_for(N, cap, s->cfg.precision);
+    ge.batch.reset(new PprPipeline(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp));
+    ge.reserved = PprPipeline::bytes_for(N, cap, s->cfg.precision);
     s->reserved += ge.reserved;
     return true;
 }
@@ -122,20 +122,18 @@ void retire(ag_server *s, uint32_t gi)
 {
     GraphEntry &ge = s->graphs[gi];
     if (!ge.batch) return;
-    auto done = ge.batch->poll();
-    std::sort(done.begin(), done.end(),
-              [](const PprBatch::Done &a, const PprBatch::Done &b) { return a.slot > b.slot; });
-    for (auto &d : done) {
-        auto qi = s->queries.find(d.tag);
-        if (qi != s->queries.end() && qi->second.state == QState::Running) {
-            Query &q = qi->second;
-            q.result.resize(ge.g->N);
-            ge.batch->extract(d.slot, q.result.data());
-            q.stat.iterations = d.iters;
-            q.stat.residual = d.residual;
-            finish(q, QState::Done, AG_OK);
-        }
-        ge.batch->remove(d.slot);
+    auto fin = ge.batch->retire([&](uint64_t tag) -> float * {
+        auto qi = s->queries.find(tag);
+        if (qi == s->queries.end() || qi->second.state != QState::Running) return nullptr;
+        qi->second.result.resize(ge.g->N);
+        return qi->second.result.data();
+    });
+    for (auto &f : fin) {
+        auto qi = s->queries.find(f.tag);
+        if (qi == s->queries.end() || qi->second.state != QState::Running) continue;
+        qi->second.stat.iterations = f.iters;
+        qi->second.stat.residual = f.residual;
+        finish(qi->second, QState::Done, AG_OK);
     }
 }
 
@@ -231,11 +229,8 @@ int ag_serve_cancel(ag_server *s, uint64_t ticket)
         auto it = s->queries.find(ticket);
         if (it == s->queries.end()) fail(AG_ERR_NOT_FOUND, "unknown ticket");
         Query &q = it->second;
-        if (q.state == QState::Running) {
-            GraphEntry &ge = s->graphs[q.graph];
-            for (uint32_t k = 0; k < ge.batch->active(); k++)
-                if (ge.batch->tag(k) == ticket) { ge.batch->remove(k); break; }
-        }
+        if (q.state == QState::Running && s->graphs[q.graph].batch)
+            s->graphs[q.graph].batch->cancel(ticket);
         if (q.state == QState::Queued || q.state == QState::Running)
             finish(q, QState::Cancelled, AG_ERR_CANCELLED);
     });

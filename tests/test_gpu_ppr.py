@@ -84,3 +84,38 @@ def test_more_queries_than_slots(ctx):
 def test_autotune_picks_a_supported_width(ctx):
     g, csr, n = _graph(ctx)
     assert ag.ppr_autotune(g, "bf16") in (4, 8, 16, 32)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_batch_low_precision_finishes_in_fp32(ctx, precision):
+    """Columns stored in 16 bits stall near the storage noise floor, are frozen
+    and finished in FP32: the result obeys the residual bound against the
+    converged FP64 fixed point, ||x - x*||_1 <= d/(1-d) * residual."""
+    g, csr, n = _graph(ctx)
+    qs = _queries(n, 16, 21, tol=1e-6)
+    print(f"case: batch {precision} nq=16 tol=1e-6", flush=True)
+    X, stats, ms = ag.ppr_batch(g, qs, precision=precision, cols_per_wave=16)
+    for i, (seeds, tol, mi) in enumerate(qs):
+        assert stats[i].residual <= tol
+        star, _, _ = oracle.ppr_f64(csr, seeds, 1e-13, 5000)
+        err = np.abs(X[i] - star).sum()
+        assert err <= 0.85 / 0.15 * tol + FP32_L1, (i, err)
+
+
+def test_bf16_batch_alone_equals_mixed(ctx):
+    g, csr, n = _graph(ctx)
+    qs = _queries(n, 24, 23, tol=1e-5)
+    X, st, _ = ag.ppr_batch(g, qs, precision="bf16", cols_per_wave=8)
+    for i in (0, 11, 23):
+        Xa, sa, _ = ag.ppr_batch(g, [qs[i]], precision="bf16", cols_per_wave=8)
+        assert sa[0].iterations == st[i].iterations
+        np.testing.assert_array_equal(Xa[0], X[i])
+
+
+def test_single_bf16_finishes_in_fp32(ctx):
+    g, csr, n = _graph(ctx)
+    seeds = np.array([5, 77, 4000], np.uint32)
+    x, st = ag.ppr_single(g, seeds, tol=1e-6, precision="bf16")
+    star, _, _ = oracle.ppr_f64(csr, seeds, 1e-13, 5000)
+    assert st.residual <= 1e-6
+    assert np.abs(x - star).sum() <= 0.85 / 0.15 * 1e-6 + FP32_L1

@@ -67,10 +67,35 @@ def test_alone_equals_in_mix(ctx):
     srv.close()
 
 
+def test_bf16_serving_alone_equals_in_mix(ctx):
+    graphs = _setup(ctx)
+    g, csr, n = graphs[1]
+    rng = np.random.RandomState(8)
+    srv = ag.Server(ctx, precision="bf16", cols_per_wave=8)
+    gid = srv.add_graph(g)
+    qs = [(rng.randint(0, n, rng.randint(1, 17)).astype(np.uint32),
+           float(10 ** rng.uniform(-6, -4))) for _ in range(20)]
+    tickets = []
+    for i, (s, tol) in enumerate(qs):
+        tickets.append(srv.submit(gid, s, tol))
+        if i % 3 == 2:
+            srv.step()
+    res = _drain(srv, list(tickets), {t: n for t in tickets})
+    for i in (0, 9, 19):
+        code, x, st = res[tickets[i]]
+        assert code == L.AG_OK
+        Xa, sa, _ = ag.ppr_batch(g, [(qs[i][0], qs[i][1], 1000)], precision="bf16",
+                                 cols_per_wave=8)
+        np.testing.assert_array_equal(x, Xa[0])
+        star, _, _ = oracle.ppr_f64(csr, qs[i][0], 1e-13, 5000)
+        assert np.abs(x - star).sum() <= 0.85 / 0.15 * qs[i][1] + 2e-5
+    srv.close()
+
+
 def test_hbm_budget_admission(ctx):
     graphs = _setup(ctx)
     n = graphs[0][2]
-    # room for one 8-column batch only
+    # room for one 8-column batch only (FP32: iterate + two contribution buffers)
     budget = 8 * n * 12 + n * 4 + (1 << 20) + 100
     srv = ag.Server(ctx, hbm_budget_bytes=budget, max_columns=8, precision="fp32",
                     cols_per_wave=8)
