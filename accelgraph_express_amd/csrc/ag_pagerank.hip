@@ -77,6 +77,7 @@ struct SweepArgs {
     // base/outdeg in both buffers (written at init); their |delta| is 0, so
     // they count as active exactly when epsilon <= 0.
     uint32_t skip_zero;
+    float f16_scale, f16_inv; // FP16 storage scale (power of two) and its inverse
     uint32_t mode;
     uint32_t max_iter;
     uint32_t finish_fp32;
@@ -121,14 +122,17 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
     return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
 }
 
-__device__ __forceinline__ void store_c(void *buf, uint32_t fmt, uint32_t i, float c)
+// FP16 words hold c * scale (a power of two, so scaling is exact), which moves
+// contributions of size ~1/N into FP16's normal range.
+__device__ __forceinline__ void store_c(void *buf, uint32_t fmt, uint32_t i, float c,
+                                        float f16_scale)
 {
     if (fmt == kFmtF32)
         static_cast<float *>(buf)[i] = c;
     else if (fmt == kFmtBF16)
         static_cast<uint16_t *>(buf)[i] = f32_to_bf16_rne(c);
     else
-        static_cast<__half *>(buf)[i] = __float2half_rn(c);
+        static_cast<__half *>(buf)[i] = __float2half_rn(c * f16_scale);
 }
 
 __device__ __forceinline__ uint32_t ldnt(const uint32_t *p)
@@ -191,8 +195,13 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
         if (od == 0) dang += (double)nw;
     }
     const float c = od ? nw / (float)od : 0.0f;
-    store_c(cout, wfmt, i, c);
+    store_c(cout, wfmt, i, c, a.f16_scale);
 }
+
+template <typename CT> struct FmtOf;
+template <> struct FmtOf<float> { static constexpr uint32_t v = kFmtF32; };
+template <> struct FmtOf<bf16_t> { static constexpr uint32_t v = kFmtBF16; };
+template <> struct FmtOf<__half> { static constexpr uint32_t v = 2u; };
 
 // Gather source: labels below `hot` read the workgroup's LDS copy of the
 // contribution vector's head, the rest read global memory.
@@ -210,8 +219,8 @@ __device__ __forceinline__ float gather(const CT *__restrict__ g, uint32_t u, ui
 template <int G, typename CT>
 __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uint32_t row_end,
                                            const CT *__restrict__ cin, void *cout, uint32_t wfmt,
-                                           float tele, uint32_t hot, double &err, uint32_t &act,
-                                           double &dang, int lane)
+                                           float tele, uint32_t hot, float rscale, double &err,
+                                           uint32_t &act, double &dang, int lane)
 {
     const int gl = lane & (G - 1);
     const bool valid = row < row_end;
@@ -240,7 +249,7 @@ __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uin
     float s = (s0 + s1) + (s2 + s3);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (valid && gl == 0) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
+    if (valid && gl == 0) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang);
 }
 
 template <typename CT>
@@ -248,6 +257,8 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                            void *cout, float tele, uint32_t hot, double &err, uint32_t &act,
                            double &dang)
 {
+    // FP16 words hold contributions times f16_scale: gathered sums scale back
+    const float rscale = FmtOf<CT>::v == 2u ? a.f16_inv : 1.0f;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t u_w = a.nb_hub, u_h = u_w + a.nb_w, u_q = u_h + a.nb_h, u_l = u_q + a.nb_q,
                    u_z = u_l + a.nb_low;
@@ -285,13 +296,13 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
             }
         } else if (vb < u_h) {
             const uint32_t row = a.n_hub + (vb - u_w) * kSweepWaves + w;
-            group_rows<64>(a, row, a.n_w, cin, cout, wfmt, tele, hot, err, act, dang, lane);
+            group_rows<64>(a, row, a.n_w, cin, cout, wfmt, tele, hot, rscale, err, act, dang, lane);
         } else if (vb < u_q) {
             const uint32_t row = a.n_w + (vb - u_h) * (2 * kSweepWaves) + w * 2 + (lane >> 5);
-            group_rows<32>(a, row, a.n_h, cin, cout, wfmt, tele, hot, err, act, dang, lane);
+            group_rows<32>(a, row, a.n_h, cin, cout, wfmt, tele, hot, rscale, err, act, dang, lane);
         } else if (vb < u_l) {
             const uint32_t row = a.n_h + (vb - u_q) * (4 * kSweepWaves) + w * 4 + (lane >> 4);
-            group_rows<16>(a, row, a.n_nonlow, cin, cout, wfmt, tele, hot, err, act, dang, lane);
+            group_rows<16>(a, row, a.n_nonlow, cin, cout, wfmt, tele, hot, rscale, err, act, dang, lane);
         } else if (vb < u_z) {
             // ---- low rows: SELL-64 slice per wave, lane per row, edges in order ----
             const uint32_t sl = (vb - u_l) * kSweepWaves + w;
@@ -322,7 +333,7 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
 #pragma unroll
                     for (int t = 0; t < 8; t++) u[t] = nx[t];
                 }
-                if (valid) update_vertex(a, row, s, tele, cout, wfmt, err, act, dang);
+                if (valid) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang);
             }
         } else if (!a.skip_zero) {
             // ---- zero in-degree rows ----
@@ -353,11 +364,6 @@ __device__ __forceinline__ void load_hot(const CT *__restrict__ cin, uint32_t ho
 
 // Persistent sweep: one 1024-thread workgroup per CU slot walks the work units
 // (hub chunks, mid rows, low slices, zero rows) with a grid stride.
-template <typename CT> struct FmtOf;
-template <> struct FmtOf<float> { static constexpr uint32_t v = kFmtF32; };
-template <> struct FmtOf<bf16_t> { static constexpr uint32_t v = kFmtBF16; };
-template <> struct FmtOf<__half> { static constexpr uint32_t v = 2u; };
-
 // One kernel per storage format of the contributions it reads (keeps each
 // kernel's register footprint to one instantiation).  A mixed-precision run
 // launches the low-precision and the FP32 kernel every sweep; the one whose
@@ -410,6 +416,7 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     SweepState *st = a.st;
     if (st->done) return;
     const uint32_t wfmt = a.fmt[it + 1];
+    const float rscale = a.fmt[it] == 2u ? a.f16_inv : 1.0f;
     void *cout = (it & 1) ? a.c0 : a.c1;
     const float tele = st->tele;
     double err = 0.0, dang = 0.0;
@@ -418,7 +425,7 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
         const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
         float sum = 0.0f;
         for (uint32_t q = f0; q < f1; q++) sum += a.hub_partial[q];
-        update_vertex(a, row, sum, tele, cout, wfmt, err, act, dang);
+        update_vertex(a, row, sum * rscale, tele, cout, wfmt, err, act, dang);
     }
     for (uint32_t b = threadIdx.x; b < a.nparts; b += kFinBlock) {
         err += a.blk_err[b];
@@ -494,8 +501,8 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
     a.pr[i] = x;
     const uint32_t od = a.outdeg[i];
     const float c = od ? x / (float)od : 0.0f;
-    store_c(a.c0, fmt0, i, c);
-    if (a.skip_zero && i >= a.n_nonzero) store_c(a.c1, fmt0, i, c);
+    store_c(a.c0, fmt0, i, c, a.f16_scale);
+    if (a.skip_zero && i >= a.n_nonzero) store_c(a.c1, fmt0, i, c, a.f16_scale);
 }
 
 // Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
@@ -826,6 +833,9 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.max_iter = max_iter;
     a.finish_fp32 = (o.flags & AG_PR_FLAG_FINISH_FP32) ? 1u : 0u;
     a.skip_zero = (o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ? 1u : 0u;
+    // STANDARD ranks are ~1/N: scale FP16 words by 2^ceil(log2 N)
+    a.f16_scale = o.mode == AG_PR_STANDARD ? std::ldexp(1.0f, (int)bits_for(p->N)) : 1.0f;
+    a.f16_inv = 1.0f / a.f16_scale;
     a.damp = o.damp;
     {
 #pragma clang fp contract(off)

@@ -29,6 +29,8 @@ struct Plan {
     DevBuf order, rank, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial;
     uint32_t nb_hub = 0, nb_w = 0, nb_h = 0, nb_q = 0, nb_low = 0, nb_zero = 0, nvblocks = 0;
     Work work;
+    // single-query personalized PageRank scratch (reused across calls)
+    DevBuf ppr_bits, ppr_ids, ppr_labels, ppr_w, ppr_out;
     std::mutex mu;
     uint64_t bytes() const
     {

@@ -41,8 +41,8 @@ struct PprSeedSet {
 
 class PprBatch {
   public:
-    // capacity: column slots (multiple of 32, <= 64); cpw: columns per wave
-    // group in {4, 8, 16, 32}; fmt: AG_PREC_* storage of contributions.
+    // capacity: column slots (multiple of 4, <= 64); cpw: columns per lane
+    // group in {4, 8, 16, 32, 64}; fmt: AG_PREC_* storage of contributions.
     PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t cpw, uint32_t fmt,
              float damp);
     ~PprBatch();
@@ -51,20 +51,20 @@ class PprBatch {
 
     static uint64_t bytes_for(uint32_t N, uint32_t capacity, uint32_t fmt);
     uint32_t capacity() const { return cap_; }
-    uint32_t active() const { return K_; }
-    uint32_t free_slots() const { return cap_ - K_; }
+    uint32_t active() const { return used_count_; }
+    uint32_t free_slots() const { return cap_ - used_count_; }
     uint64_t bytes() const;
     uint32_t cpw() const { return cpw_; }
     void set_cpw(uint32_t cpw);
 
     // Maps vertex ids to labels and merges duplicates (weights 1/n each).
     PprSeedSet make_seeds(const uint32_t *seeds, uint32_t n, double tol, uint32_t max_iter);
-    // Admits a query into the next slot; returns the slot.  Enqueued on the
-    // context stream (no host synchronisation).
+    // Takes a free slot for a query; its columns are initialised on the next
+    // sweep (all admissions of a step in one launch).  Returns the slot.
     uint32_t admit(const PprSeedSet &q, uint64_t tag);
     // Runs `n` sweeps over all active columns (graph replay when enabled).
     void sweep(uint32_t n, bool replay);
-    // Synchronises and returns the slots whose columns finished.
+    // Synchronises and returns the slots whose columns finished or stalled.
     struct Done {
         uint32_t slot;
         uint64_t tag;
@@ -73,31 +73,36 @@ class PprBatch {
         bool stalled; // low precision: ready to move to an FP32 batch
     };
     std::vector<Done> poll();
-    // Moves a stalled column into `dst` (an FP32 batch), iterate and state
-    // included; returns the new slot in dst.  The slot here is not removed.
-    uint32_t migrate(uint32_t slot, PprBatch &dst);
+    // Moves stalled columns into `dst` (an FP32 batch) in one launch, iterate
+    // and state included; the source slots are released.
+    void migrate(const std::vector<uint32_t> &slots, PprBatch &dst);
     // Copies a finished column (caller's vertex order) to host memory.
     void extract(uint32_t slot, float *host_out);
-    // Drops a slot; the last active slot moves into it.  Returns the tag that
-    // moved (or ~0 when none moved) so callers can update their maps.
-    uint64_t remove(uint32_t slot);
+    // Frees a slot (a running column is stopped first).
+    void release(uint32_t slot);
+    bool used(uint32_t slot) const { return slot < cap_ && used_[slot]; }
     uint64_t tag(uint32_t slot) const { return tags_[slot]; }
     double last_sweep_ms() const { return last_ms_; }
 
   private:
+    void flush();
     void upload_seed_table();
     void launch_sweep(bool even);
     ag_ctx *ctx_;
-    ag_graph *g_;
     Plan *p_;
-    uint32_t N_, cap_, cpw_, fmt_, K_ = 0;
+    uint32_t N_, cap_, cpw_, fmt_;
+    uint32_t used_count_ = 0, hi_ = 0; // hi_: one past the highest used slot
     float damp_;
     int cur_ = 0; // contribution buffer read by the next sweep
     uint32_t grid_;
+    bool dirty_ = true;
     DevBuf X_, C_[2], cols_, bstate_, blk_res_, blk_dang_, hub_part_, hub_res_, hub_dang_;
-    DevBuf seed_bits_, seed_v_, seed_c_, seed_w_, out_;
+    DevBuf seed_bits_, seed_v_, seed_c_, seed_w_, out_, desc_;
     std::vector<PprSeedSet> q_;    // per slot
     std::vector<uint64_t> tags_;   // per slot
+    std::vector<uint8_t> used_;    // per slot
+    std::vector<uint32_t> pending_; // admitted, columns not initialised yet
+    std::vector<uint32_t> stop_;    // released while running: mark done on device
     hipGraphExec_t gexec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     double last_ms_ = 0.0;
@@ -136,26 +141,22 @@ template <typename F>
 std::vector<PprPipeline::Finished> PprPipeline::retire(F out)
 {
     std::vector<Finished> fin;
-    auto by_slot = [](const PprBatch::Done &a, const PprBatch::Done &b) { return a.slot > b.slot; };
-    auto done = first_->poll();
-    std::sort(done.begin(), done.end(), by_slot);
-    for (auto &d : done) {
+    std::vector<uint32_t> moving;
+    for (auto &d : first_->poll()) {
         if (d.stalled) {
-            if (!fp32_ || fp32_->free_slots() == 0) continue; // stays frozen until room
-            first_->migrate(d.slot, *fp32_);
-        } else {
-            first_->extract(d.slot, out(d.tag));
-            fin.push_back({d.tag, d.iters, d.residual});
+            if (fp32_ && moving.size() < fp32_->free_slots()) moving.push_back(d.slot);
+            continue; // without room it stays frozen until a later step
         }
-        first_->remove(d.slot);
+        first_->extract(d.slot, out(d.tag));
+        fin.push_back({d.tag, d.iters, d.residual});
+        first_->release(d.slot);
     }
+    if (!moving.empty()) first_->migrate(moving, *fp32_);
     if (fp32_) {
-        auto d2 = fp32_->poll();
-        std::sort(d2.begin(), d2.end(), by_slot);
-        for (auto &d : d2) {
+        for (auto &d : fp32_->poll()) {
             fp32_->extract(d.slot, out(d.tag));
             fin.push_back({d.tag, d.iters, d.residual});
-            fp32_->remove(d.slot);
+            fp32_->release(d.slot);
         }
     }
     return fin;

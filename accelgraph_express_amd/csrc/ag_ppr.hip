@@ -61,6 +61,12 @@ struct PprArgs {
     uint32_t nparts;
 };
 
+// Per-slot descriptor of one admission (or migration: e0 = destination slot).
+struct PprAdmit {
+    uint32_t slot, e0, e1, max_iter;
+    double tol;
+};
+
 __device__ __forceinline__ float ld(const float *p, uint64_t i) { return p[i]; }
 __device__ __forceinline__ float ld(const bf16_t *p, uint64_t i)
 {
@@ -174,9 +180,21 @@ __global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
             const uint32_t b0 = rb + k * a.hub_chunk;
             const uint32_t b1 = min(re, b0 + a.hub_chunk);
             float acc = 0.0f;
-            if (act)
-                for (uint32_t j = b0 + w * G + g; j < b1; j += 4 * G)
-                    acc += ld(cin, (uint64_t)ldnt(a.cols + j) * a.cap + c);
+            if (act) {
+                float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+                constexpr uint32_t S = 4 * G; // lane groups of the workgroup
+                uint32_t j = b0 + w * G + g;
+                for (; j + 3 * S < b1; j += 4 * S) {
+                    const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + S);
+                    const uint32_t u2 = ldnt(a.cols + j + 2 * S), u3 = ldnt(a.cols + j + 3 * S);
+                    a0 += ld(cin, (uint64_t)u0 * a.cap + c);
+                    a1 += ld(cin, (uint64_t)u1 * a.cap + c);
+                    a2 += ld(cin, (uint64_t)u2 * a.cap + c);
+                    a3 += ld(cin, (uint64_t)u3 * a.cap + c);
+                }
+                for (; j < b1; j += S) a0 += ld(cin, (uint64_t)ldnt(a.cols + j) * a.cap + c);
+                acc = (a0 + a1) + (a2 + a3);
+            }
             acc = group_sum<CPW>(acc);
             if (g == 0) hub_red[w][cl] = acc;
             __syncthreads();
@@ -191,17 +209,20 @@ __global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
             const uint32_t row = a.n_hub + (u - a.nb_hub) * 4 + w;
             if (row < a.n_nonlow) {
                 const uint32_t rb = a.roff[row], re = a.roff[row + 1];
-                float a0 = 0.0f, a1 = 0.0f;
+                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
                 if (act) {
                     uint32_t j = rb + g;
-                    for (; j + G < re; j += 2 * G) {
+                    for (; j + 3 * G < re; j += 4 * G) {
                         const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + G);
+                        const uint32_t u2 = ldnt(a.cols + j + 2 * G), u3 = ldnt(a.cols + j + 3 * G);
                         a0 += ld(cin, (uint64_t)u0 * a.cap + c);
                         a1 += ld(cin, (uint64_t)u1 * a.cap + c);
+                        a2 += ld(cin, (uint64_t)u2 * a.cap + c);
+                        a3 += ld(cin, (uint64_t)u3 * a.cap + c);
                     }
-                    if (j < re) a0 += ld(cin, (uint64_t)ldnt(a.cols + j) * a.cap + c);
+                    for (; j < re; j += G) a0 += ld(cin, (uint64_t)ldnt(a.cols + j) * a.cap + c);
                 }
-                const float acc = group_sum<CPW>(a0 + a1);
+                const float acc = group_sum<CPW>((a0 + a1) + (a2 + a3));
                 if (g == 0 && act) ppr_update(a, row, c, acc, t, res, dang);
             }
         } else if (u < a.nb_hub + a.nmid_units + a.nlow_units) {
@@ -213,14 +234,19 @@ __global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
                 const uint32_t *p = a.cols + a.slice_ptr[rl >> 6] + (rl & 63);
                 float acc = 0.0f;
                 uint32_t k = 0;
-                for (; k + 2 <= deg; k += 2) {
+                for (; k + 4 <= deg; k += 4) {
                     const uint32_t u0 = ldnt(p + k * 64), u1 = ldnt(p + (k + 1) * 64);
+                    const uint32_t u2 = ldnt(p + (k + 2) * 64), u3 = ldnt(p + (k + 3) * 64);
                     const float x0 = ld(cin, (uint64_t)u0 * a.cap + c);
                     const float x1 = ld(cin, (uint64_t)u1 * a.cap + c);
-                    acc += x0;
+                    const float x2 = ld(cin, (uint64_t)u2 * a.cap + c);
+                    const float x3 = ld(cin, (uint64_t)u3 * a.cap + c);
+                    acc += x0; // edge order
                     acc += x1;
+                    acc += x2;
+                    acc += x3;
                 }
-                if (k < deg) acc += ld(cin, (uint64_t)ldnt(p + k * 64) * a.cap + c);
+                for (; k < deg; k++) acc += ld(cin, (uint64_t)ldnt(p + k * 64) * a.cap + c);
                 ppr_update(a, row, c, acc, t, res, dang);
             }
         } else {
@@ -305,90 +331,86 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a)
 }
 
 // ---- slot management kernels ----
-__global__ void k_ppr_clear_col(float *X, void *C0, void *C1, uint32_t N, uint32_t cap, uint32_t c,
-                                uint32_t fmt)
+// Column (re)initialisation of the slots admitted in one step: one pass over
+// the rows clears X and both contribution buffers for every listed slot.
+__global__ void k_ppr_clear_cols(float *X, void *C0, void *C1, uint32_t N, uint32_t cap,
+                                 const PprAdmit *adm, uint32_t na, uint32_t fmt)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const uint64_t k = (uint64_t)i * cap + c;
-    X[k] = 0.0f;
-    st(C0, fmt, k, 0.0f);
-    st(C1, fmt, k, 0.0f);
+    for (uint32_t q = 0; q < na; q++) {
+        const uint64_t k = (uint64_t)i * cap + adm[q].slot;
+        X[k] = 0.0f;
+        st(C0, fmt, k, 0.0f);
+        st(C1, fmt, k, 0.0f);
+    }
 }
 
-__global__ void k_ppr_seed_col(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap,
-                               uint32_t c, uint32_t fmt, const uint32_t *sv, const float *sw,
-                               uint32_t n, PprCol *cs, double tol, uint32_t max_iter, float damp)
+// One workgroup per admitted slot: x0 = s on the seed rows, contributions
+// s / outdeg in the buffer the next sweep reads, column state.
+__global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap,
+                                uint32_t fmt, const PprAdmit *adm, const uint32_t *sv,
+                                const float *sw, PprCol *cs, float damp)
 {
-    for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+    const PprAdmit d = adm[blockIdx.x];
+    for (uint32_t q = d.e0 + threadIdx.x; q < d.e1; q += blockDim.x) {
         const uint32_t v = sv[q];
-        const uint64_t k = (uint64_t)v * cap + c;
+        const uint64_t k = (uint64_t)v * cap + d.slot;
         X[k] = sw[q];
         const uint32_t od = outdeg[v];
         st(Cin, fmt, k, od ? sw[q] / (float)od : 0.0f);
     }
     if (threadIdx.x == 0) {
         double d0 = 0.0;
-        for (uint32_t q = 0; q < n; q++)
+        for (uint32_t q = d.e0; q < d.e1; q++)
             if (outdeg[sv[q]] == 0) d0 += (double)sw[q];
         PprCol col;
         col.t = (float)((1.0 - (double)damp) + (double)damp * d0);
         col.done = 0;
         col.iters = 0;
-        col.max_iter = max_iter;
-        col.tol = tol;
+        col.max_iter = d.max_iter;
+        col.tol = d.tol;
         col.residual = INFINITY;
         col.dangling = d0;
         col.stalled = 0;
         col.finish = fmt != AG_PREC_FP32 ? 1u : 0u;
-        cs[c] = col;
+        cs[d.slot] = col;
     }
 }
 
-template <typename T>
-__global__ void k_ppr_move_col(T *X, T *C0, T *C1, uint32_t N, uint32_t cap, uint32_t from,
-                               uint32_t to)
+// Stops columns released while running (a cancelled query).
+__global__ void k_ppr_stop_cols(PprCol *cs, const PprAdmit *adm, uint32_t n)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const uint64_t b = (uint64_t)i * cap;
-    (void)X;
-    C0[b + to] = C0[b + from];
-    C1[b + to] = C1[b + from];
+    const uint32_t q = threadIdx.x;
+    if (q < n) cs[adm[q].slot].done = 1;
 }
 
-__global__ void k_ppr_move_x(float *X, uint32_t N, uint32_t cap, uint32_t from, uint32_t to,
-                             PprCol *cs)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) cs[to] = cs[from];
-    if (i >= N) return;
-    const uint64_t b = (uint64_t)i * cap;
-    X[b + to] = X[b + from];
-}
-
-// Migration of one column into an FP32 batch: iterate copied, both
+// Moves stalled columns into an FP32 batch: iterate copied, both FP32
 // contribution buffers recomputed from it (x / outdeg), state carried over.
-__global__ void k_ppr_migrate_col(const float *Xs, uint32_t scap, uint32_t sc, float *Xd, float *C0,
-                                  float *C1, uint32_t dcap, uint32_t dc, const uint32_t *outdeg,
-                                  uint32_t N, const PprCol *cs_src, PprCol *cs_dst)
+// adm[q].slot = source slot, adm[q].e0 = destination slot.
+__global__ void k_ppr_migrate_cols(const float *Xs, uint32_t scap, float *Xd, float *C0, float *C1,
+                                   uint32_t dcap, const PprAdmit *adm, uint32_t na,
+                                   const uint32_t *outdeg, uint32_t N, const PprCol *cs_src,
+                                   PprCol *cs_dst)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        PprCol col = cs_src[sc];
+    if (i < na) {
+        PprCol col = cs_src[adm[i].slot];
         col.stalled = 0;
         col.done = 0;
         col.finish = 0;
-        cs_dst[dc] = col;
+        cs_dst[adm[i].e0] = col;
     }
     if (i >= N) return;
-    const float x = Xs[(uint64_t)i * scap + sc];
     const uint32_t od = outdeg[i];
-    const float c = od ? x / (float)od : 0.0f;
-    const uint64_t k = (uint64_t)i * dcap + dc;
-    Xd[k] = x;
-    C0[k] = c;
-    C1[k] = c;
+    for (uint32_t q = 0; q < na; q++) {
+        const float x = Xs[(uint64_t)i * scap + adm[q].slot];
+        const float c = od ? x / (float)od : 0.0f;
+        const uint64_t k = (uint64_t)i * dcap + adm[q].e0;
+        Xd[k] = x;
+        C0[k] = c;
+        C1[k] = c;
+    }
 }
 
 __global__ void k_ppr_extract(const float *X, const uint32_t *order, uint32_t N, uint32_t cap,
@@ -430,7 +452,8 @@ void launch_cpw(uint32_t cpw, const PprArgs &a, uint32_t grid, hipStream_t s)
     case 4: launch<4, CT>(a, grid, s); break;
     case 8: launch<8, CT>(a, grid, s); break;
     case 16: launch<16, CT>(a, grid, s); break;
-    default: launch<32, CT>(a, grid, s); break;
+    case 32: launch<32, CT>(a, grid, s); break;
+    default: launch<64, CT>(a, grid, s); break;
     }
 }
 
@@ -444,17 +467,18 @@ uint64_t PprBatch::bytes_for(uint32_t N, uint32_t cap, uint32_t fmt)
 }
 
 PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt, float damp)
-    : ctx_(ctx), g_(g), cap_(cap), cpw_(cpw), fmt_(fmt), damp_(damp)
+    : ctx_(ctx), cap_(cap), cpw_(cpw), fmt_(fmt), damp_(damp)
 {
     if (cap == 0 || cap > kMaxCap || cap % 4) fail(AG_ERR_INVALID, "batch capacity must be 4..64, multiple of 4");
-    if (cpw != 4 && cpw != 8 && cpw != 16 && cpw != 32) fail(AG_ERR_INVALID, "cols_per_wave must be 4, 8, 16 or 32");
+    if (cpw != 4 && cpw != 8 && cpw != 16 && cpw != 32 && cpw != 64)
+        fail(AG_ERR_INVALID, "cols_per_wave must be 4, 8, 16, 32 or 64");
     if (fmt > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", fmt);
     if (!(damp > 0.0f && damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
     p_ = ensure_plan(ctx, g, nullptr);
     N_ = p_->N;
     if (N_ == 0) fail(AG_ERR_INVALID, "graph has no vertices");
     hipStream_t s = ctx->stream;
-    grid_ = std::max<uint32_t>(64, (uint32_t)ctx->num_cus * 2);
+    grid_ = std::max<uint32_t>(64, (uint32_t)ctx->num_cus * 8);
     X_.alloc((size_t)N_ * cap * 4);
     C_[0].alloc((size_t)N_ * cap * cbytes(fmt));
     C_[1].alloc((size_t)N_ * cap * cbytes(fmt));
@@ -467,10 +491,12 @@ PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_
     hub_dang_.alloc((size_t)std::max<uint32_t>(p_->n_hub, 1) * kMaxCap * 8);
     seed_bits_.alloc((size_t)((N_ + 31) / 32) * 4);
     out_.alloc((size_t)N_ * 4);
+    desc_.alloc(sizeof(PprAdmit) * kMaxCap);
     AG_HIP(hipMemsetAsync(bstate_.p, 0, sizeof(PprBatchState), s));
     AG_HIP(hipMemsetAsync(cols_.p, 0, cols_.bytes, s));
     q_.resize(cap);
     tags_.assign(cap, ~0ull);
+    used_.assign(cap, 0);
     upload_seed_table();
 }
 
@@ -484,13 +510,14 @@ PprBatch::~PprBatch()
 uint64_t PprBatch::bytes() const
 {
     return X_.bytes + C_[0].bytes + C_[1].bytes + blk_res_.bytes + blk_dang_.bytes +
-           hub_part_.bytes + hub_res_.bytes + hub_dang_.bytes + seed_bits_.bytes + out_.bytes + seed_v_.bytes +
-           seed_c_.bytes + seed_w_.bytes;
+           hub_part_.bytes + hub_res_.bytes + hub_dang_.bytes + seed_bits_.bytes + out_.bytes +
+           seed_v_.bytes + seed_c_.bytes + seed_w_.bytes + desc_.bytes;
 }
 
 void PprBatch::set_cpw(uint32_t cpw)
 {
-    if (cpw != 4 && cpw != 8 && cpw != 16 && cpw != 32) fail(AG_ERR_INVALID, "bad cols_per_wave");
+    if (cpw != 4 && cpw != 8 && cpw != 16 && cpw != 32 && cpw != 64)
+        fail(AG_ERR_INVALID, "bad cols_per_wave");
     cpw_ = cpw;
     if (gexec_) { (void)hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
@@ -530,12 +557,14 @@ void PprBatch::upload_seed_table()
     hipStream_t s = ctx_->stream;
     std::vector<uint32_t> v, c;
     std::vector<float> w;
-    for (uint32_t k = 0; k < K_; k++)
+    for (uint32_t k = 0; k < cap_; k++) {
+        if (!used_[k]) continue;
         for (size_t q = 0; q < q_[k].labels.size(); q++) {
             v.push_back(q_[k].labels[q]);
             c.push_back(k);
             w.push_back(q_[k].w[q]);
         }
+    }
     const size_t n = v.size();
     const size_t need = std::max<size_t>(n, 1) * 4;
     if (seed_v_.bytes < need) {
@@ -557,34 +586,87 @@ void PprBatch::upload_seed_table()
     if (n)
         hipLaunchKernelGGL(k_set_bits, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                            seed_bits_.as<uint32_t>(), seed_v_.as<uint32_t>(), (uint32_t)n);
-    // K and the seed-entry count live on the device so captured sweeps stay valid.
-    PprBatchState bs{K_, 0u, 0u, (uint32_t)n};
+    // K (slot high-water mark) and the entry count live on the device so that
+    // captured sweeps stay valid.
+    PprBatchState bs{hi_, 0u, 0u, (uint32_t)n};
     AG_HIP(hipMemcpyAsync(bstate_.p, &bs, sizeof(bs), hipMemcpyHostToDevice, s));
     AG_HIP(hipGetLastError());
-    // host vectors above are stack temporaries: wait for the copies
+    // host vectors above are temporaries: wait for the copies
     AG_HIP(hipStreamSynchronize(s));
 }
 
 uint32_t PprBatch::admit(const PprSeedSet &q, uint64_t tag)
 {
-    if (K_ >= cap_) fail(AG_ERR_ADMISSION, "no free column slot");
-    hipStream_t s = ctx_->stream;
-    const uint32_t c = K_;
-    hipLaunchKernelGGL(k_ppr_clear_col, dim3(ceil_div(N_, 256)), dim3(256), 0, s, X_.as<float>(),
-                       C_[0].p, C_[1].p, N_, cap_, c, fmt_);
+    uint32_t c = 0;
+    while (c < cap_ && used_[c]) c++;
+    if (c >= cap_) fail(AG_ERR_ADMISSION, "no free column slot");
+    used_[c] = 1;
+    used_count_++;
+    hi_ = std::max(hi_, c + 1);
     q_[c] = q;
     tags_[c] = tag;
-    K_++;
-    upload_seed_table();
-    // seed entries of slot c sit at the end of the table
-    size_t off = 0;
-    for (uint32_t k = 0; k < c; k++) off += q_[k].labels.size();
-    hipLaunchKernelGGL(k_ppr_seed_col, dim3(1), dim3(256), 0, s, X_.as<float>(), C_[cur_].p,
-                       p_->outdeg.as<uint32_t>(), cap_, c, fmt_, seed_v_.as<uint32_t>() + off,
-                       seed_w_.as<float>() + off, (uint32_t)q.labels.size(), cols_.as<PprCol>(),
-                       q.tol, q.max_iter, damp_);
-    AG_HIP(hipGetLastError());
+    pending_.push_back(c);
+    dirty_ = true;
     return c;
+}
+
+void PprBatch::release(uint32_t slot)
+{
+    if (!used(slot)) fail(AG_ERR_INVALID, "slot %u not in use", slot);
+    used_[slot] = 0;
+    used_count_--;
+    q_[slot] = PprSeedSet();
+    tags_[slot] = ~0ull;
+    pending_.erase(std::remove(pending_.begin(), pending_.end(), slot), pending_.end());
+    stop_.push_back(slot);
+    while (hi_ && !used_[hi_ - 1]) hi_--;
+    dirty_ = true;
+}
+
+// Applies the step's slot changes: stops released columns, rebuilds the seed
+// table, and initialises all admitted columns in one pass over the rows.
+void PprBatch::flush()
+{
+    if (!dirty_) return;
+    hipStream_t s = ctx_->stream;
+    std::vector<PprAdmit> adm;
+    if (!stop_.empty()) {
+        std::sort(stop_.begin(), stop_.end());
+        stop_.erase(std::unique(stop_.begin(), stop_.end()), stop_.end());
+        for (uint32_t c : stop_) adm.push_back({c, 0, 0, 0, 0.0});
+        AG_HIP(hipMemcpyAsync(desc_.p, adm.data(), adm.size() * sizeof(PprAdmit),
+                              hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_ppr_stop_cols, dim3(1), dim3(kMaxCap), 0, s, cols_.as<PprCol>(),
+                           desc_.as<PprAdmit>(), (uint32_t)adm.size());
+        AG_HIP(hipStreamSynchronize(s));
+        stop_.clear();
+    }
+    upload_seed_table();
+    if (!pending_.empty()) {
+        // entry ranges of each admitted slot in the table just uploaded
+        adm.clear();
+        uint32_t off = 0;
+        for (uint32_t k = 0; k < cap_; k++) {
+            if (!used_[k]) continue;
+            const uint32_t n = (uint32_t)q_[k].labels.size();
+            if (std::find(pending_.begin(), pending_.end(), k) != pending_.end())
+                adm.push_back({k, off, off + n, q_[k].max_iter, q_[k].tol});
+            off += n;
+        }
+        AG_HIP(hipMemcpyAsync(desc_.p, adm.data(), adm.size() * sizeof(PprAdmit),
+                              hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_ppr_clear_cols, dim3(ceil_div(N_, 256)), dim3(256), 0, s,
+                           X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, desc_.as<PprAdmit>(),
+                           (uint32_t)adm.size(), fmt_);
+        hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)adm.size()), dim3(256), 0, s,
+                           X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, fmt_,
+                           desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(), seed_w_.as<float>(),
+                           cols_.as<PprCol>(), damp_);
+        AG_HIP(hipGetLastError());
+        AG_HIP(hipStreamSynchronize(s)); // adm is a temporary
+        pending_.clear();
+    }
+    dirty_ = false;
 }
 
 void PprBatch::launch_sweep(bool from0)
@@ -635,7 +717,8 @@ void PprBatch::launch_sweep(bool from0)
 
 void PprBatch::sweep(uint32_t n, bool replay)
 {
-    if (K_ == 0 || n == 0) return;
+    flush();
+    if (used_count_ == 0 || n == 0) return;
     hipStream_t s = ctx_->stream;
     hipEvent_t e0, e1;
     AG_HIP(hipEventCreate(&e0));
@@ -675,39 +758,60 @@ void PprBatch::sweep(uint32_t n, bool replay)
 std::vector<PprBatch::Done> PprBatch::poll()
 {
     std::vector<Done> out;
-    if (K_ == 0) return out;
-    std::vector<PprCol> cs(K_);
-    AG_HIP(hipMemcpyAsync(cs.data(), cols_.p, K_ * sizeof(PprCol), hipMemcpyDeviceToHost,
+    if (hi_ == 0) return out;
+    flush();
+    std::vector<PprCol> cs(hi_);
+    AG_HIP(hipMemcpyAsync(cs.data(), cols_.p, hi_ * sizeof(PprCol), hipMemcpyDeviceToHost,
                           ctx_->stream));
     AG_HIP(hipStreamSynchronize(ctx_->stream));
-    for (uint32_t k = 0; k < K_; k++)
-        if (cs[k].done || cs[k].stalled)
+    for (uint32_t k = 0; k < hi_; k++)
+        if (used_[k] && (cs[k].done || cs[k].stalled))
             out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].stalled != 0});
     return out;
 }
 
-uint32_t PprBatch::migrate(uint32_t slot, PprBatch &dst)
+void PprBatch::migrate(const std::vector<uint32_t> &slots, PprBatch &dst)
 {
-    if (slot >= K_) fail(AG_ERR_INVALID, "slot %u not active", slot);
+    if (slots.empty()) return;
     if (dst.fmt_ != AG_PREC_FP32) fail(AG_ERR_INVALID, "migration target must store FP32");
-    if (dst.K_ >= dst.cap_) fail(AG_ERR_ADMISSION, "no free FP32 slot");
+    if (dst.free_slots() < slots.size()) fail(AG_ERR_ADMISSION, "no free FP32 slots");
+    dst.flush();
     hipStream_t s = ctx_->stream;
-    const uint32_t dc = dst.K_;
-    hipLaunchKernelGGL(k_ppr_migrate_col, dim3(ceil_div(N_, 256)), dim3(256), 0, s,
-                       X_.as<float>(), cap_, slot, dst.X_.as<float>(), dst.C_[0].as<float>(),
-                       dst.C_[1].as<float>(), dst.cap_, dc, p_->outdeg.as<uint32_t>(), N_,
+    std::vector<PprAdmit> mv;
+    for (uint32_t c : slots) {
+        if (!used(c)) fail(AG_ERR_INVALID, "slot %u not in use", c);
+        uint32_t d = 0;
+        while (dst.used_[d]) d++;
+        dst.used_[d] = 1;
+        dst.used_count_++;
+        dst.hi_ = std::max(dst.hi_, d + 1);
+        dst.q_[d] = q_[c];
+        dst.tags_[d] = tags_[c];
+        mv.push_back({c, d, 0, 0, 0.0});
+    }
+    AG_HIP(hipMemcpyAsync(desc_.p, mv.data(), mv.size() * sizeof(PprAdmit), hipMemcpyHostToDevice,
+                          s));
+    hipLaunchKernelGGL(k_ppr_migrate_cols, dim3(ceil_div(std::max<uint32_t>(N_, kMaxCap), 256)),
+                       dim3(256), 0, s, X_.as<float>(), cap_, dst.X_.as<float>(),
+                       dst.C_[0].as<float>(), dst.C_[1].as<float>(), dst.cap_,
+                       desc_.as<PprAdmit>(), (uint32_t)mv.size(), p_->outdeg.as<uint32_t>(), N_,
                        cols_.as<PprCol>(), dst.cols_.as<PprCol>());
     AG_HIP(hipGetLastError());
-    dst.q_[dc] = q_[slot];
-    dst.tags_[dc] = tags_[slot];
-    dst.K_++;
-    dst.upload_seed_table();
-    return dc;
+    AG_HIP(hipStreamSynchronize(s)); // mv is a temporary
+    dst.dirty_ = true;
+    for (uint32_t c : slots) {
+        used_[c] = 0;
+        used_count_--;
+        q_[c] = PprSeedSet();
+        tags_[c] = ~0ull;
+    }
+    while (hi_ && !used_[hi_ - 1]) hi_--;
+    dirty_ = true; // stalled columns are already inactive on the device
 }
 
 void PprBatch::extract(uint32_t slot, float *host_out)
 {
-    if (slot >= K_) fail(AG_ERR_INVALID, "slot %u not active", slot);
+    if (!used(slot)) fail(AG_ERR_INVALID, "slot %u not in use", slot);
     if (!host_out) return;
     hipStream_t s = ctx_->stream;
     hipLaunchKernelGGL(k_ppr_extract, dim3(ceil_div(N_, 256)), dim3(256), 0, s, X_.as<float>(),
@@ -715,35 +819,6 @@ void PprBatch::extract(uint32_t slot, float *host_out)
     AG_HIP(hipGetLastError());
     AG_HIP(hipMemcpyAsync(host_out, out_.p, (size_t)N_ * 4, hipMemcpyDeviceToHost, s));
     AG_HIP(hipStreamSynchronize(s));
-}
-
-uint64_t PprBatch::remove(uint32_t slot)
-{
-    if (slot >= K_) fail(AG_ERR_INVALID, "slot %u not active", slot);
-    hipStream_t s = ctx_->stream;
-    const uint32_t last = K_ - 1;
-    uint64_t moved = ~0ull;
-    if (slot != last) {
-        hipLaunchKernelGGL(k_ppr_move_x, dim3(ceil_div(N_, 256)), dim3(256), 0, s, X_.as<float>(),
-                           N_, cap_, last, slot, cols_.as<PprCol>());
-        if (fmt_ == AG_PREC_FP32)
-            hipLaunchKernelGGL(k_ppr_move_col<uint32_t>, dim3(ceil_div(N_, 256)), dim3(256), 0, s,
-                               (uint32_t *)nullptr, C_[0].as<uint32_t>(), C_[1].as<uint32_t>(), N_,
-                               cap_, last, slot);
-        else
-            hipLaunchKernelGGL(k_ppr_move_col<uint16_t>, dim3(ceil_div(N_, 256)), dim3(256), 0, s,
-                               (uint16_t *)nullptr, C_[0].as<uint16_t>(), C_[1].as<uint16_t>(), N_,
-                               cap_, last, slot);
-        AG_HIP(hipGetLastError());
-        q_[slot] = q_[last];
-        tags_[slot] = tags_[last];
-        moved = tags_[slot];
-    }
-    q_[last] = PprSeedSet();
-    tags_[last] = ~0ull;
-    K_--;
-    upload_seed_table();
-    return moved;
 }
 
 PprPipeline::PprPipeline(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt,
@@ -770,9 +845,9 @@ bool PprPipeline::cancel(uint64_t tag)
 {
     for (PprBatch *b : {first_.get(), fp32_.get()}) {
         if (!b) continue;
-        for (uint32_t k = 0; k < b->active(); k++)
-            if (b->tag(k) == tag) {
-                b->remove(k);
+        for (uint32_t k = 0; k < b->capacity(); k++)
+            if (b->used(k) && b->tag(k) == tag) {
+                b->release(k);
                 return true;
             }
     }
@@ -792,7 +867,7 @@ uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt)
     }
     uint32_t best = 32;
     double best_ms = 1e30;
-    for (uint32_t cpw : {4u, 8u, 16u, 32u}) {
+    for (uint32_t cpw : {4u, 8u, 16u, 32u, 64u}) {
         b.set_cpw(cpw);
         b.sweep(1, false);
         b.sweep(2, false);
@@ -895,8 +970,18 @@ int ag_ppr_single(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t prec
         }
         hipStream_t s = ctx->stream;
         const uint32_t n = (uint32_t)ids.size();
-        DevBuf din(n * 4), lab(n * 4), dw(n * 4), bits((size_t)((N + 31) / 32) * 4);
-        DevBuf out((size_t)N * 4);
+        if (p->ppr_ids.bytes < (size_t)n * 4) {
+            const size_t cap = std::max<size_t>((size_t)n * 4, 4096);
+            p->ppr_ids.alloc(cap);
+            p->ppr_labels.alloc(cap);
+            p->ppr_w.alloc(cap);
+        }
+        if (!p->ppr_out.p) {
+            p->ppr_bits.alloc((size_t)((N + 31) / 32) * 4);
+            p->ppr_out.alloc((size_t)N * 4);
+        }
+        DevBuf &din = p->ppr_ids, &lab = p->ppr_labels, &dw = p->ppr_w, &bits = p->ppr_bits,
+               &out = p->ppr_out;
         AG_HIP(hipMemcpyAsync(din.p, ids.data(), n * 4, hipMemcpyHostToDevice, s));
         AG_HIP(hipMemcpyAsync(dw.p, w.data(), n * 4, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_map_labels, dim3(ceil_div(n, 256)), dim3(256), 0, s,

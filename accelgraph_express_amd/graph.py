@@ -266,21 +266,22 @@ def _query(seeds, tol, max_iter):
 
 
 def ppr_single(graph: Graph, seeds, tol: float = 1e-4, max_iter: int = 1000,
-               precision="fp32"):
-    """One query with the single-vector sweep kernel.  Returns (x, stat)."""
+               precision="fp32", return_x: bool = True):
+    """One query with the single-vector sweep kernel.  Returns (x or None, stat)."""
     q, keep = _query(seeds, tol, max_iter)
-    x = np.zeros(graph.num_vertices, np.float32)
+    x = np.zeros(graph.num_vertices, np.float32) if return_x else None
     st = L.PprStat()
     prec = _PRECS[precision] if isinstance(precision, str) else int(precision)
     L.check(L.load().ag_ppr_single(graph.ctx.h, graph.h, C.byref(q), prec,
-                                   _ptr(x, C.c_float), C.byref(st)))
+                                   _ptr(x, C.c_float) if return_x else None, C.byref(st)))
     del keep
     return x, st
 
 
 def ppr_batch(graph: Graph, queries, precision="bf16", cols_per_wave: int = 0,
-              replay: bool = True, damp: float = 0.85):
-    """queries: list of (seeds, tol, max_iter).  Returns (X [nq, N], stats, elapsed_ms)."""
+              replay: bool = True, damp: float = 0.85, return_x: bool = True):
+    """queries: list of (seeds, tol, max_iter).  Returns (X [nq, N] or None, stats,
+    elapsed_ms); with return_x=False the results stay on the device."""
     lib = L.load()
     nq = len(queries)
     arr = (L.PprQuery * nq)()
@@ -295,11 +296,11 @@ def ppr_batch(graph: Graph, queries, precision="bf16", cols_per_wave: int = 0,
     o.cols_per_wave = cols_per_wave
     o.use_graph_replay = 1 if replay else 0
     o.damp = damp
-    X = np.zeros((nq, graph.num_vertices), np.float32)
+    X = np.zeros((nq, graph.num_vertices), np.float32) if return_x else None
     stats = (L.PprStat * nq)()
     ms = C.c_double()
-    L.check(lib.ag_ppr_batch(graph.ctx.h, graph.h, arr, nq, C.byref(o), _ptr(X, C.c_float),
-                             stats, C.byref(ms)))
+    L.check(lib.ag_ppr_batch(graph.ctx.h, graph.h, arr, nq, C.byref(o),
+                             _ptr(X, C.c_float) if return_x else None, stats, C.byref(ms)))
     return X, list(stats), ms.value
 
 

@@ -13,7 +13,19 @@ layout are built once before timing and reported separately.
                (pageRank.c:1024-1085 + the schedule(dynamic,1024) gather of
                pageRank.c:2890-2907) on the host's cores, same graph.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale 22]
+    ppr      = 64 personalized queries on the same graph (1-16 uniform seeds,
+               stop at L1 residual 1e-4): batched BF16-stored columns with FP32
+               finish vs one query at a time with the FP32 single-vector kernel;
+               queries/s and p50/p99 completion latency.
+
+Other configurations (separate runs, one JSON line each):
+    --workload c3   R-MAT s24, STANDARD PageRank to L1 residual 1e-6 in FP32, and
+                    BF16 / FP16 storage with FP32 finish: time to tolerance
+    --workload c5   serving loop over resident s18/s20/s22 graphs with Poisson
+                    arrivals at swept rates: queries/s, p50/p99 latency
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-ppr]
+                       [--workload c2|c3|c5]
 """
 from __future__ import annotations
 
@@ -44,6 +56,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2022)
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-ppr", action="store_true", help="skip the batched-PPR leg")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -57,6 +71,10 @@ def main() -> int:
 
     import accelgraph_express_amd as ag
 
+    if args.workload == "c3":
+        return workload_c3(args)
+    if args.workload == "c5":
+        return workload_c5(args)
     scale, ef, iters = args.scale, args.edge_factor, args.iterations
     t = time.time()
     src, dst = ag.gen_rmat(args.seed, scale, ef)
@@ -132,6 +150,8 @@ def main() -> int:
         parity = {"max_rel_vs_cpu_fp32": float(rel.max()), "tolerance": 1e-5}
         log(f"CPU {cpu['value']} GTEPS; GPU vs CPU max rel {rel.max():.2e}")
 
+    ppr = None if args.no_ppr else ppr_leg(ag, g, N, np)
+
     line = {
         "metric": METRIC,
         "value": round(gteps, 3),
@@ -156,8 +176,161 @@ def main() -> int:
                      "algorithmic_bytes_per_sweep": bytes_sweep},
         "cpu_baseline": cpu,
         "parity": parity,
+        "ppr": ppr,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def _pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
+    """C4: 64 queries, 1-16 uniform seeds each, per-column stop at L1 residual 1e-4.
+
+    Both paths leave the result vectors in HBM (a 16 MiB rank vector per
+    query over PCIe would time the host copy, not the kernels); a sample of
+    queries is downloaded afterwards for the consistency check."""
+    rng = np.random.RandomState(seed)
+    qs = [(rng.randint(0, N, rng.randint(1, 17)).astype(np.uint32), tol, 1000) for _ in range(nq)]
+    cpw = ag.ppr_autotune(g, "bf16")
+    out = {}
+    for prec in ("bf16", "fp32"):
+        ag.ppr_batch(g, qs[:8], precision=prec, cols_per_wave=cpw, return_x=False)  # warm
+        t0 = time.perf_counter()
+        _, st, _ = ag.ppr_batch(g, qs, precision=prec, cols_per_wave=cpw, return_x=False)
+        out[prec] = (time.perf_counter() - t0, st)
+    ag.ppr_single(g, qs[0][0], tol, precision="fp32", return_x=False)  # warm
+    t0 = time.perf_counter()
+    single = [ag.ppr_single(g, s, t, precision="fp32", return_x=False)[1] for s, t, _ in qs]
+    t_single = time.perf_counter() - t0
+    # consistency: a few columns downloaded from both paths
+    Xb, _, _ = ag.ppr_batch(g, qs[:4], precision="bf16", cols_per_wave=cpw)
+    l1 = max(float(np.abs(Xb[i] - ag.ppr_single(g, qs[i][0], tol, precision="fp32")[0]).sum())
+             for i in range(4))
+    t_bf, st_bf = out["bf16"]
+    t_32, st_32 = out["fp32"]
+    lat = [s.latency_ms for s in st_bf]
+    res = {"workload": f"{nq} PPR queries on the same graph, 1-16 seeds, L1 residual <= {tol}, "
+                       "results left in HBM",
+           "qps_batched_bf16": round(nq / t_bf, 2), "qps_batched_fp32": round(nq / t_32, 2),
+           "qps_single_fp32": round(nq / t_single, 2),
+           "speedup_bf16_vs_single": round(t_single / t_bf, 2),
+           "speedup_fp32_vs_single": round(t_single / t_32, 2),
+           "latency_ms_p50": round(_pct(lat, 50), 3), "latency_ms_p99": round(_pct(lat, 99), 3),
+           "cols_per_wave": cpw,
+           "iterations_mean_bf16": round(float(np.mean([s.iterations for s in st_bf])), 2),
+           "iterations_mean_single": round(float(np.mean([s.iterations for s in single])), 2),
+           "max_l1_batched_bf16_vs_single_fp32": l1}
+    log(f"PPR: batched BF16 {res['qps_batched_bf16']} q/s (p99 {res['latency_ms_p99']} ms), "
+        f"batched FP32 {res['qps_batched_fp32']} q/s, single FP32 {res['qps_single_fp32']} q/s; "
+        f"max L1 batched vs single {l1:.2e}")
+    return res
+
+
+def workload_c3(args):
+    """R-MAT s24, STANDARD PageRank run to L1 residual 1e-6."""
+    import numpy as np
+    import torch
+    import accelgraph_express_amd as ag
+    scale, tol = 24, 1e-6
+    src, dst = ag.gen_rmat(args.seed, scale, 16)
+    N, E = 1 << scale, src.size
+    ctx = ag.Context(0)
+    d_src = torch.from_numpy(src.view(np.int32)).to("cuda")
+    d_dst = torch.from_numpy(dst.view(np.int32)).to("cuda")
+    del src, dst
+    g = ag.Graph.build(ctx, None, None, N, device_ptrs=(d_src.data_ptr(), d_dst.data_ptr()),
+                       num_edges=E)
+    del d_src, d_dst
+    torch.cuda.empty_cache()
+    plan_ms = g.prepare()
+    runs = {}
+    ranks = {}
+    for prec in ("fp32", "bf16", "fp16"):
+        g.pagerank_pull(iterations=400, epsilon=tol, mode="standard", precision=prec,
+                        finish_fp32=prec != "fp32")  # warm
+        t0 = time.perf_counter()
+        st = g.pagerank_pull(iterations=400, epsilon=tol, mode="standard", precision=prec,
+                             finish_fp32=prec != "fp32")
+        secs = time.perf_counter() - t0
+        ranks[prec] = st.pageRanks
+        runs[prec] = {"seconds": round(secs, 4), "sweeps": st.sweeps,
+                      "fp32_switch_sweep": st.fp32_switch_sweep,
+                      "residual": st.error_total, "loop_ms": round(st.loop_ms, 3)}
+        log(f"C3 {prec}: {secs * 1e3:.1f} ms, {st.sweeps} sweeps, residual {st.error_total:.2e}, "
+            f"switch at {st.fp32_switch_sweep}")
+    for prec in ("bf16", "fp16"):
+        runs[prec]["l1_vs_fp32"] = float(np.abs(ranks[prec].astype(np.float64) - ranks["fp32"]).sum())
+    line = {"metric": "time to L1 residual 1e-6, STANDARD PageRank (R-MAT s24)",
+            "value": runs["bf16"]["seconds"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": runs["bf16"]["seconds"] * 1e3, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16-stored/f32-accumulate",
+            "data": "synthetic (seeded R-MAT s24, a=0.57 b=0.19 c=0.19, ids permuted)",
+            "config": {"workload": "rmat_s24_ef16_standard_pagerank_to_1e-6",
+                       "num_vertices": N, "num_edges": E, "gather_layout_ms": round(plan_ms, 1)},
+            "runs": runs}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
+    """Serving: resident R-MAT s18/s20/s22 graphs, Poisson arrivals, BF16 columns."""
+    import numpy as np
+    import accelgraph_express_amd as ag
+    from accelgraph_express_amd import _lib as L
+    ctx = ag.Context(0)
+    graphs = []
+    for sc in (18, 20, 22):
+        src, dst = ag.gen_rmat(args.seed + sc, sc, 16)
+        graphs.append(ag.Graph.build(ctx, src, dst, 1 << sc))
+    srv = ag.Server(ctx, precision="bf16")
+    gids = [srv.add_graph(g) for g in graphs]
+    rng = np.random.RandomState(seed)
+    out = []
+    for rate in rates:
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
+        reqs = []
+        for a in arrivals:
+            k = rng.randint(0, 3)
+            n = 1 << (18 + 2 * k)
+            reqs.append((a, gids[k], rng.randint(0, n, rng.randint(1, 17)).astype(np.uint32),
+                         float(10 ** rng.uniform(-6, -4)), n))
+        pending, lat, i = {}, [], 0
+        t0 = time.perf_counter()
+        while i < len(reqs) or pending:
+            now = time.perf_counter() - t0
+            while i < len(reqs) and reqs[i][0] <= now:
+                _, gid, seeds, tol, n = reqs[i]
+                pending[srv.submit(gid, seeds, tol)] = n
+                i += 1
+            if pending:
+                srv.step()
+                for t in list(pending):
+                    code, x, st = srv.poll(t, pending[t])
+                    if code != L.AG_ERR_PENDING:
+                        lat.append(st.latency_ms)
+                        del pending[t]
+            elif i < len(reqs):
+                time.sleep(max(0.0, min(reqs[i][0] - now, 0.01)))
+        wall = time.perf_counter() - t0
+        r = {"offered_qps": rate, "completed": len(lat), "achieved_qps": round(len(lat) / wall, 2),
+             "latency_ms_p50": round(_pct(lat, 50), 3), "latency_ms_p99": round(_pct(lat, 99), 3)}
+        log(f"C5 rate {rate}: {r}")
+        out.append(r)
+    best = max(out, key=lambda r: r["achieved_qps"])
+    line = {"metric": "PPR serving queries/s + p99 latency (resident R-MAT s18/s20/s22)",
+            "value": best["achieved_qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(out),
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16-stored/f32-accumulate",
+            "data": "synthetic (seeded R-MAT graphs, Poisson arrivals)",
+            "config": {"workload": "serving_rmat_s18_s20_s22_poisson",
+                       "tolerance": "log-uniform [1e-6, 1e-4]", "seeds": "1-16 uniform"},
+            "rates": out}
+    print(json.dumps(line), flush=True)
+    srv.close()
     return 0
 
 

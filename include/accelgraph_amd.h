@@ -204,7 +204,7 @@ typedef struct ag_ppr_stat
 typedef struct ag_ppr_options
 {
     uint32_t precision;       /* AG_PREC_* for stored columns */
-    uint32_t cols_per_wave;   /* 0: autotuned from {4, 8, 16, 32} */
+    uint32_t cols_per_wave;   /* 0: autotuned from {4, 8, 16, 32, 64} */
     float damp;               /* 0.85f */
     uint32_t use_graph_replay;/* 1: capture sweeps in a hipGraph and replay */
 } ag_ppr_options;
@@ -221,7 +221,7 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
 int ag_ppr_single(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t precision, float *x,
                   ag_ppr_stat *stat);
 
-/* Chooses cols_per_wave for g by timing {4, 8, 16, 32}; result cached per graph. */
+/* Chooses cols_per_wave for g by timing {4, 8, 16, 32, 64}; result cached per graph. */
 int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cols_per_wave);
 
 /* ---- serving loop -------------------------------------------------------- */

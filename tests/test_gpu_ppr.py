@@ -40,7 +40,7 @@ def test_single_fp32_vs_fp64(ctx):
         assert abs(x.sum() - 1.0) < 1e-4
 
 
-@pytest.mark.parametrize("cpw", [4, 8, 16, 32])
+@pytest.mark.parametrize("cpw", [4, 8, 16, 32, 64])
 def test_batch_fp32_vs_fp64(ctx, cpw):
     g, csr, n = _graph(ctx)
     qs = _queries(n, 20, 5)
@@ -83,7 +83,7 @@ def test_more_queries_than_slots(ctx):
 
 def test_autotune_picks_a_supported_width(ctx):
     g, csr, n = _graph(ctx)
-    assert ag.ppr_autotune(g, "bf16") in (4, 8, 16, 32)
+    assert ag.ppr_autotune(g, "bf16") in (4, 8, 16, 32, 64)
 
 
 @pytest.mark.parametrize("precision", ["bf16", "fp16"])
