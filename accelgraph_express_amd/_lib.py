@@ -79,7 +79,7 @@ class PprQuery(C.Structure):
 
 class PprStat(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("status", C.c_uint32), ("residual", C.c_double),
-                ("latency_ms", C.c_double)]
+                ("latency_ms", C.c_double), ("fine_sweep", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class PprOptions(C.Structure):

@@ -31,11 +31,17 @@ struct Plan {
     Work work;
     // single-query personalized PageRank scratch (reused across calls)
     DevBuf ppr_bits, ppr_ids, ppr_labels, ppr_w, ppr_out;
+    // batched personalized PageRank: fixed-size edge chunks of the rows
+    // [0, n_nonlow), built on first use
+    DevBuf ppr_ch_row, ppr_ch_first, ppr_multi;
+    uint32_t ppr_nch = 0, ppr_nmulti = 0;
+    bool ppr_ready = false;
     std::mutex mu;
     uint64_t bytes() const
     {
         return order.bytes + rank.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
-               chunk_row.bytes + chunk_first.bytes + hub_partial.bytes;
+               chunk_row.bytes + chunk_first.bytes + hub_partial.bytes + ppr_ch_row.bytes +
+               ppr_ch_first.bytes + ppr_multi.bytes;
     }
 };
 

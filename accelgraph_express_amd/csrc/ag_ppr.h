@@ -14,19 +14,19 @@ namespace ag {
 // Per-column device state.
 struct PprCol {
     float t;           // teleport factor for the next sweep: (1 - d) + d * dangling
-    uint32_t done;     // residual <= tol or iters == max_iter
+    uint32_t done;     // residual <= tol (in FP32) or iters == max_iter
     uint32_t iters;
     uint32_t max_iter;
     double tol;
     double residual;
     double dangling;
-    uint32_t stalled;  // low-precision column frozen for its FP32 finish
-    uint32_t finish;   // 1: freeze at the stall / tolerance instead of finishing
+    uint32_t coarse;   // BF16 storage: 1 while sweeps read the high halves only
+    uint32_t fine_at;  // sweep count when the column switched to full FP32 reads
 };
 
 struct PprBatchState {
-    uint32_t K;      // active (compacted) column slots
-    uint32_t pad0;
+    uint32_t K;      // one past the highest used column slot
+    uint32_t nzrows; // seed rows without in-edges
     uint32_t sweeps;
     uint32_t nseeds; // entries in the seed table
 };
@@ -39,17 +39,23 @@ struct PprSeedSet {
     uint32_t max_iter = 1000;
 };
 
+// Column slots of one graph.  Storage AG_PREC_FP32 keeps FP32 contributions;
+// AG_PREC_BF16 keeps each FP32 contribution split into two 16-bit planes (the
+// high halves, read alone as BF16 while a column is coarse, and the low
+// halves, read as well once the column's residual stalls), so a column
+// finishes in exact FP32 in place.
 class PprBatch {
   public:
-    // capacity: column slots (multiple of 4, <= 64); cpw: columns per lane
-    // group in {4, 8, 16, 32, 64}; fmt: AG_PREC_* storage of contributions.
+    // capacity: column slots (multiple of 8, <= 64); cpw: widest column group
+    // one row access of a wave covers, {4, 8, 16, 32, 64} (BF16: at least 8).
     PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t cpw, uint32_t fmt,
              float damp);
     ~PprBatch();
     PprBatch(const PprBatch &) = delete;
     PprBatch &operator=(const PprBatch &) = delete;
 
-    static uint64_t bytes_for(uint32_t N, uint32_t capacity, uint32_t fmt);
+    // Device bytes of a batch of `capacity` columns on g (builds g's plan).
+    static uint64_t bytes_for(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t fmt);
     uint32_t capacity() const { return cap_; }
     uint32_t active() const { return used_count_; }
     uint32_t free_slots() const { return cap_ - used_count_; }
@@ -59,45 +65,59 @@ class PprBatch {
 
     // Maps vertex ids to labels and merges duplicates (weights 1/n each).
     PprSeedSet make_seeds(const uint32_t *seeds, uint32_t n, double tol, uint32_t max_iter);
-    // Takes a free slot for a query; its columns are initialised on the next
-    // sweep (all admissions of a step in one launch).  Returns the slot.
+    // Takes the lowest free slot for a query; its column is initialised on the
+    // next sweep (all admissions of a step in one launch).  Returns the slot.
     uint32_t admit(const PprSeedSet &q, uint64_t tag);
     // Runs `n` sweeps over all active columns (graph replay when enabled).
     void sweep(uint32_t n, bool replay);
-    // Synchronises and returns the slots whose columns finished or stalled.
     struct Done {
         uint32_t slot;
         uint64_t tag;
         uint32_t iters;
         double residual;
-        bool stalled; // low precision: ready to move to an FP32 batch
+        uint32_t fine_at;
     };
+    // Synchronises and returns the slots whose columns finished.
     std::vector<Done> poll();
-    // Moves stalled columns into `dst` (an FP32 batch) in one launch, iterate
-    // and state included; the source slots are released.
-    void migrate(const std::vector<uint32_t> &slots, PprBatch &dst);
     // Copies a finished column (caller's vertex order) to host memory.
     void extract(uint32_t slot, float *host_out);
     // Frees a slot (a running column is stopped first).
     void release(uint32_t slot);
+    // Moves running columns from the top slots into free lower ones when that
+    // narrows the column groups the sweeps walk (no admissions pending).
+    void compact();
+    bool cancel(uint64_t tag);
     bool used(uint32_t slot) const { return slot < cap_ && used_[slot]; }
     uint64_t tag(uint32_t slot) const { return tags_[slot]; }
     double last_sweep_ms() const { return last_ms_; }
 
+    struct Finished {
+        uint64_t tag;
+        uint32_t iters;
+        double residual;
+        uint32_t fine_at;
+    };
+    // Retires finished columns; `out(tag)` names the host buffer for a
+    // column's result (nullptr: leave it on the device).  With `may_compact`
+    // the remaining columns are packed afterwards.
+    template <typename F> std::vector<Finished> retire(F out, bool may_compact);
+
   private:
     void flush();
     void upload_seed_table();
-    void launch_sweep(bool even);
+    void launch_sweep(bool from0);
+    uint32_t lanes_per_row() const;
     ag_ctx *ctx_;
     Plan *p_;
-    uint32_t N_, cap_, cpw_, fmt_;
+    uint32_t N_, cap_, cpw_ = 0, fmt_;
     uint32_t used_count_ = 0, hi_ = 0; // hi_: one past the highest used slot
     float damp_;
-    int cur_ = 0; // contribution buffer read by the next sweep
-    uint32_t grid_;
+    int cur_ = 0;          // contribution buffer read by the next sweep
+    uint32_t graph_lpr_ = 0; // lanes per row the captured graph was built for
     bool dirty_ = true;
-    DevBuf X_, C_[2], cols_, bstate_, blk_res_, blk_dang_, hub_part_, hub_res_, hub_dang_;
-    DevBuf seed_bits_, seed_v_, seed_c_, seed_w_, out_, desc_;
+    int blocks_[5] = {0, 0, 0, 0, 0}; // sweep workgroups per CU by log2(lanes per row)
+    DevBuf X_, C_[2], cols_, bstate_, blk_res_, blk_dang_, part_;
+    DevBuf seed_bits_, seed_v_, seed_c_, seed_w_, zrows_, out_, desc_;
     std::vector<PprSeedSet> q_;    // per slot
     std::vector<uint64_t> tags_;   // per slot
     std::vector<uint8_t> used_;    // per slot
@@ -108,60 +128,20 @@ class PprBatch {
     double last_ms_ = 0.0;
 };
 
-// A graph's column slots in two stages: columns run in the requested storage
-// precision; with BF16/FP16 storage a column that stalls (or meets its
-// tolerance) is frozen and finished in an FP32 batch.  FP32 runs use one stage.
-class PprPipeline {
-  public:
-    PprPipeline(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt, float damp);
-    static uint64_t bytes_for(uint32_t N, uint32_t cap, uint32_t fmt);
-    uint32_t free_slots() const { return first_->free_slots(); }
-    uint32_t active() const { return first_->active() + (fp32_ ? fp32_->active() : 0u); }
-    PprSeedSet make_seeds(const uint32_t *seeds, uint32_t n, double tol, uint32_t max_iter)
-    {
-        return first_->make_seeds(seeds, n, tol, max_iter);
-    }
-    void admit(const PprSeedSet &q, uint64_t tag) { first_->admit(q, tag); }
-    void sweep(uint32_t n, bool replay);
-    struct Finished {
-        uint64_t tag;
-        uint32_t iters;
-        double residual;
-    };
-    // Migrates stalled columns, retires finished ones; `out(tag)` names the
-    // host buffer for a finished column's result (nullptr: drop it).
-    template <typename F> std::vector<Finished> retire(F out);
-    bool cancel(uint64_t tag);
-
-  private:
-    std::unique_ptr<PprBatch> first_, fp32_;
-};
-
 template <typename F>
-std::vector<PprPipeline::Finished> PprPipeline::retire(F out)
+std::vector<PprBatch::Finished> PprBatch::retire(F out, bool may_compact)
 {
     std::vector<Finished> fin;
-    std::vector<uint32_t> moving;
-    for (auto &d : first_->poll()) {
-        if (d.stalled) {
-            if (fp32_ && moving.size() < fp32_->free_slots()) moving.push_back(d.slot);
-            continue; // without room it stays frozen until a later step
-        }
-        first_->extract(d.slot, out(d.tag));
-        fin.push_back({d.tag, d.iters, d.residual});
-        first_->release(d.slot);
+    for (auto &d : poll()) {
+        extract(d.slot, out(d.tag));
+        fin.push_back({d.tag, d.iters, d.residual, d.fine_at});
+        release(d.slot);
     }
-    if (!moving.empty()) first_->migrate(moving, *fp32_);
-    if (fp32_) {
-        for (auto &d : fp32_->poll()) {
-            fp32_->extract(d.slot, out(d.tag));
-            fin.push_back({d.tag, d.iters, d.residual});
-            fp32_->release(d.slot);
-        }
-    }
+    if (may_compact) compact();
     return fin;
 }
 
 uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt);
+void ensure_ppr_chunks(ag_ctx *ctx, Plan *p);
 
 } // namespace ag

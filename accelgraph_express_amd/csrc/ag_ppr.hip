@@ -8,20 +8,36 @@
 // applied to every column at once so that one random row access per edge
 // serves all columns.
 //
-// Layout: vertex-major [N][capacity] (FP32 iterate; contributions stored in
-// FP32, BF16 or FP16), in the degree-sorted labels of the graph's plan.  A
-// wave is split into 64/CPW lane groups; each group covers CPW consecutive
-// columns.  Hub rows are split in fixed edge chunks (one workgroup each),
-// mid rows take one wave each (groups stride over the edges, shuffle
-// reduction), low rows one group each in the plan's SELL-64 slice order.  A
-// sweep is three launches: the sweep kernel, a hub-row kernel that sums the
-// chunk partials in order, and a one-workgroup column reduction; kernel
-// boundaries order them, with no device-scope fence (an L2 write-back +
-// invalidate on gfx950).  Columns retire individually: a done column is
-// frozen (no writes) and the host compacts slots between sweeps.  Residual and dangling sums are FP64, reduced in a fixed order, so a
-// column's result does not depend on which other columns share the batch.
-#include <hip/hip_fp16.h>
-
+// Layout: vertex-major [N][capacity] in the degree-sorted labels of the
+// graph's plan: the FP32 iterate X and two ping-pong contribution buffers.
+// FP32 storage keeps one FP32 word per column.  BF16 storage keeps the same
+// FP32 word split into its high half (a truncated BF16, read as the midpoint
+// of its truncation interval, so unbiased) and its low half; a row is cut in
+// blocks of up to 32 columns, each block holding its columns' high halves and
+// then their low halves, so both halves of a column share one 128-byte line.
+// A coarse column reads the high halves only (half the gather bytes); once its
+// residual stalls or meets the tolerance it also reads the low halves and
+// continues in exact FP32, in place, in the same batch.
+//
+// A lane holds 16 bytes of one row (4 FP32 or 8 16-bit columns, VW); LPR
+// lanes cover the columns of a column group, so one wave gathers 64 / LPR rows
+// at a time.  LPR follows the highest used slot.
+//   * rows with in-degree above the plan's low_max are cut into chunks of
+//     kChunk edges; a chunk is read by kSlots lane groups (slot s takes edges
+//     s, s + kSlots, ... in order) whose sums combine as (s0 + s1) + (s2 + s3).
+//     A one-chunk row is updated at once; longer rows leave per-chunk partials
+//     that k_ppr_rows sums in the same slot order over chunks;
+//   * low rows (the plan's SELL-64 slices) take one lane group each and sum
+//     their edges in order;
+//   * rows without in-edges change only if they are seeds (zrows list).
+// Every column's summation order is fixed by the graph alone (not by the batch
+// width, the lanes per row or the other columns), so a query's result is the
+// same alone and in any batch.  Residual and dangling sums are FP64.  A sweep
+// is three launches: sweep, multi-chunk rows, one-workgroup column reduction;
+// kernel boundaries order them, with no device-scope fence (an L2 write-back +
+// invalidate on gfx950).  Columns retire individually: a done column is frozen
+// (no writes), its slot is reused by the next admission, and a draining batch
+// packs its columns into the low slots so that the sweeps narrow.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -35,283 +51,607 @@ namespace ag {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kMaxCap = 64;
+constexpr uint32_t kSlots = 4;     // lane groups per chunk (fixed summation order)
+constexpr uint32_t kChunk = 256;   // edges per chunk of a row above low_max
+constexpr int kUnroll = 8;         // gathers in flight per lane
+constexpr uint32_t kMid = 0x80008000u; // low halves of two coarse columns: interval midpoints
+// A coarse column turns fine once its L1 residual is below this, or stops
+// falling within 8x of it: the BF16 rounding noise of a sweep is about
+// d * 2^-10 * ||x||_1 ~ 1e-3, so further coarse sweeps would mostly measure
+// noise.  (Early sweeps may grow the residual while the mass spreads; that is
+// not a stall.)
+#ifndef AG_PPR_COARSE_FLOOR
+#define AG_PPR_COARSE_FLOOR 0.01
+#endif
+constexpr double kCoarseFloor = AG_PPR_COARSE_FLOOR;
 
 struct PprArgs {
     uint32_t N, cap;
-    uint32_t n_hub, n_nonlow, n_nonzero, hub_chunk, nb_hub, nmid_units, nlow_units;
+    uint32_t n_nonlow, n_nonzero;
+    uint32_t nch, nmulti;
     float damp;
-    uint32_t fmt;
     const uint32_t *__restrict__ roff;
     const uint32_t *__restrict__ cols;
     const uint32_t *__restrict__ slice_ptr;
     const uint32_t *__restrict__ outdeg;
-    const uint32_t *__restrict__ chunk_row;
-    const uint32_t *__restrict__ chunk_first;
+    const uint32_t *__restrict__ ch_row;   // [nch] row of each chunk
+    const uint32_t *__restrict__ ch_first; // [n_nonlow + 1] first chunk of each row
+    const uint32_t *__restrict__ multi;    // [nmulti] rows of more than one chunk
     float *X;
     const void *Cin;
     void *Cout;
-    const uint32_t *seed_bits, *seed_v, *seed_c;
+    float *part; // [nch][cap] chunk sums of multi-chunk rows
+    const uint32_t *seed_bits, *seed_v, *seed_c, *zrows;
     const float *seed_w;
     PprCol *cs;
     PprBatchState *bs;
-    double *blk_res, *blk_dang; // [nparts][64]
-    double *hub_res, *hub_dang; // [n_hub][64]
-    float *hub_part;            // [hub chunks][cap]
-    uint32_t nparts;
+    double *blk_res, *blk_dang; // [partial rows][kMaxCap]
+    uint32_t part_base;         // first partial row of this launch
 };
 
-// Per-slot descriptor of one admission (or migration: e0 = destination slot).
+// Per-slot descriptor of one admission (or move: e0 = destination slot).
 struct PprAdmit {
     uint32_t slot, e0, e1, max_iter;
     double tol;
 };
 
-__device__ __forceinline__ float ld(const float *p, uint64_t i) { return p[i]; }
-__device__ __forceinline__ float ld(const bf16_t *p, uint64_t i)
+// BF16 storage: byte offsets, within a row of 4 * cap bytes, of the high and
+// the low half of column c (blocks of up to 32 columns: highs, then lows).
+__device__ __forceinline__ uint32_t split_hi(uint32_t cap, uint32_t c)
 {
-    return __uint_as_float((uint32_t)p[i].x << 16);
+    const uint32_t b = c & ~31u;
+    return b * 4 + (c - b) * 2;
 }
-__device__ __forceinline__ float ld(const __half *p, uint64_t i) { return __half2float(p[i]); }
+__device__ __forceinline__ uint32_t split_lo(uint32_t cap, uint32_t c)
+{
+    const uint32_t b = c & ~31u;
+    return split_hi(cap, c) + min(32u, cap - b) * 2;
+}
 
-__device__ __forceinline__ void st(void *buf, uint32_t fmt, uint64_t i, float c)
+// Scalar store of one contribution (slot management kernels).
+__device__ __forceinline__ void st(void *buf, uint32_t fmt, uint32_t cap, uint32_t row, uint32_t col,
+                                   float c)
 {
     if (fmt == AG_PREC_FP32) {
-        static_cast<float *>(buf)[i] = c;
-    } else if (fmt == AG_PREC_BF16) {
-        const uint32_t b = __float_as_uint(c);
-        static_cast<uint16_t *>(buf)[i] = (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+        static_cast<float *>(buf)[(uint64_t)row * cap + col] = c;
     } else {
-        static_cast<__half *>(buf)[i] = __float2half_rn(c);
+        const uint32_t b = __float_as_uint(c);
+        char *r = static_cast<char *>(buf) + (uint64_t)row * cap * 4;
+        *reinterpret_cast<uint16_t *>(r + split_hi(cap, col)) = (uint16_t)(b >> 16);
+        *reinterpret_cast<uint16_t *>(r + split_lo(cap, col)) = (uint16_t)(b & 0xffffu);
     }
 }
 
 __device__ __forceinline__ uint32_t ldnt(const uint32_t *p) { return __builtin_nontemporal_load(p); }
 
-template <int CPW>
-__device__ __forceinline__ float group_sum(float x)
+// ---- FP32 storage: 4 columns per lane ----
+struct F32Rows {
+    static constexpr int W = 4;
+    static constexpr int U = kUnroll;
+    const float *__restrict__ c;
+    uint32_t cap, c0;
+    __device__ __forceinline__ uint4 load(uint32_t u) const
+    {
+        return *reinterpret_cast<const uint4 *>(c + (uint64_t)u * cap + c0);
+    }
+    static __device__ __forceinline__ void add(float (&a)[4], const uint4 r)
+    {
+        a[0] += __uint_as_float(r.x);
+        a[1] += __uint_as_float(r.y);
+        a[2] += __uint_as_float(r.z);
+        a[3] += __uint_as_float(r.w);
+    }
+};
+
+// ---- BF16 storage: 8 columns per lane, high halves (+ low halves when FINE) ----
+template <bool FINE>
+struct SplitRows {
+    static constexpr int W = 8;
+    static constexpr int U = kUnroll;
+    const char *__restrict__ base;
+    uint32_t stride, hoff, loff; // row bytes; the lane's high / low offsets in a row
+    uint32_t fm[4]; // per word: 0xffff on the halves of fine columns
+    struct Raw {
+        uint4 h, l;
+    };
+    __device__ __forceinline__ Raw load(uint32_t u) const
+    {
+        Raw r;
+        const char *p = base + (uint64_t)u * stride;
+        r.h = *reinterpret_cast<const uint4 *>(p + hoff);
+        // a fine wave reads the low halves on every lane (no divergent loads);
+        // lanes without fine columns mask them to the midpoints in add()
+        r.l = FINE ? *reinterpret_cast<const uint4 *>(p + loff) : make_uint4(kMid, kMid, kMid, kMid);
+        return r;
+    }
+    __device__ __forceinline__ void add2(float &a0, float &a1, uint32_t h, uint32_t l, uint32_t m) const
+    {
+        const uint32_t le = FINE ? ((l & m) | (kMid & ~m)) : kMid;
+        a0 += __uint_as_float((h << 16) | (le & 0xffffu));
+        a1 += __uint_as_float((h & 0xffff0000u) | (le >> 16));
+    }
+    __device__ __forceinline__ void add(float (&a)[8], const Raw &r) const
+    {
+        add2(a[0], a[1], r.h.x, r.l.x, fm[0]);
+        add2(a[2], a[3], r.h.y, r.l.y, fm[1]);
+        add2(a[4], a[5], r.h.z, r.l.z, fm[2]);
+        add2(a[6], a[7], r.h.w, r.l.w, fm[3]);
+    }
+};
+
+template <typename R> struct RawOf { using T = uint4; };
+template <bool F> struct RawOf<SplitRows<F>> { using T = typename SplitRows<F>::Raw; };
+
+template <typename R>
+__device__ __forceinline__ typename RawOf<R>::T zero_raw()
 {
-#pragma unroll
-    for (int off = CPW; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-template <int CPW>
-__device__ __forceinline__ double group_sum(double x)
-{
-#pragma unroll
-    for (int off = CPW; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    if constexpr (R::W == 4) {
+        return make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        typename RawOf<R>::T r;
+        r.h = make_uint4(0u, 0u, 0u, 0u);
+        r.l = make_uint4(0u, 0u, 0u, 0u);
+        return r;
+    }
 }
 
-// Column update of label `row` (only called for active columns).
-__device__ __forceinline__ void ppr_update(const PprArgs &a, uint32_t row, uint32_t c, float acc,
-                                          float t, double &res, double &dang)
+template <typename R>
+__device__ __forceinline__ void add_raw(const R &rows, float (&acc)[R::W], const typename RawOf<R>::T &r)
+{
+    if constexpr (R::W == 4) R::add(acc, r);
+    else rows.add(acc, r);
+}
+
+// acc += contributions of sources p[0], p[stride], ..., p[(n-1)*stride], in
+// that order, for the lane's columns.  Indices of the next group are loaded
+// while the current group's rows are in flight.
+template <typename R>
+__device__ __forceinline__ void gather_seq(const R &rows, const uint32_t *__restrict__ p, uint32_t stride,
+                                           uint32_t n, float (&acc)[R::W])
+{
+    using Raw = typename RawOf<R>::T;
+    uint32_t k = 0;
+    if (n >= (uint32_t)R::U) {
+        uint32_t u[R::U];
+#pragma unroll
+        for (int i = 0; i < R::U; i++) u[i] = ldnt(p + (uint64_t)i * stride);
+        for (; k + R::U <= n; k += R::U) {
+            Raw r[R::U];
+#pragma unroll
+            for (int i = 0; i < R::U; i++) r[i] = rows.load(u[i]);
+            if (k + 2 * R::U <= n) {
+#pragma unroll
+                for (int i = 0; i < R::U; i++) u[i] = ldnt(p + (uint64_t)(k + R::U + i) * stride);
+            }
+#pragma unroll
+            for (int i = 0; i < R::U; i++) add_raw(rows, acc, r[i]);
+        }
+    }
+    if (k < n) {
+        // tail: masked loads; FP32 adds zero, BF16 adds 0x00008000 (4.6e-41)
+        // per missing edge, below every value the sums can hold
+        Raw r[R::U];
+#pragma unroll
+        for (int i = 0; i < R::U; i++) {
+            r[i] = zero_raw<R>();
+            if (k + i < n) r[i] = rows.load(ldnt(p + (uint64_t)(k + i) * stride));
+        }
+#pragma unroll
+        for (int i = 0; i < R::U; i++) {
+            if constexpr (R::W == 4) {
+                add_raw(rows, acc, r[i]);
+            } else {
+                if (k + i < n) add_raw(rows, acc, r[i]);
+            }
+        }
+    }
+}
+
+template <typename CT> struct Store;
+template <> struct Store<float> {
+    static constexpr int W = 4;
+    static __device__ __forceinline__ void put(const PprArgs &a, uint32_t, uint32_t, uint64_t base,
+                                               const float (&c)[4])
+    {
+        *reinterpret_cast<float4 *>(static_cast<float *>(a.Cout) + base) = make_float4(c[0], c[1], c[2], c[3]);
+    }
+};
+template <> struct Store<bf16_t> {
+    static constexpr int W = 8;
+    static __device__ __forceinline__ void put(const PprArgs &a, uint32_t row, uint32_t c0, uint64_t,
+                                               const float (&c)[8])
+    {
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t b0 = __float_as_uint(c[2 * j]), b1 = __float_as_uint(c[2 * j + 1]);
+            h[j] = (b0 >> 16) | (b1 & 0xffff0000u);
+            l[j] = (b0 & 0xffffu) | (b1 << 16);
+        }
+        char *r = static_cast<char *>(a.Cout) + (uint64_t)row * a.cap * 4;
+        *reinterpret_cast<uint4 *>(r + split_hi(a.cap, c0)) = make_uint4(h[0], h[1], h[2], h[3]);
+        *reinterpret_cast<uint4 *>(r + split_lo(a.cap, c0)) = make_uint4(l[0], l[1], l[2], l[3]);
+    }
+};
+
+// Column update of label `row` for the lane's VW columns (amask: active ones).
+// res / dang: the lane's FP64 accumulators in LDS, column i at [i * kBlock].
+template <typename CT>
+__device__ __forceinline__ void update_row(const PprArgs &a, uint32_t row, uint32_t c0, uint32_t amask,
+                                           const float (&acc)[Store<CT>::W], const float (&t)[Store<CT>::W],
+                                           double *res, double *dang)
 {
 #pragma clang fp contract(off)
-    float sv = 0.0f;
+    constexpr int VW = Store<CT>::W;
+    float sv[VW];
+#pragma unroll
+    for (int i = 0; i < VW; i++) sv[i] = 0.0f;
     if ((a.seed_bits[row >> 5] >> (row & 31)) & 1u) {
         const uint32_t ns = a.bs->nseeds;
-        for (uint32_t q = 0; q < ns; q++)
-            if (a.seed_v[q] == row && a.seed_c[q] == c) sv += a.seed_w[q];
+        for (uint32_t q = 0; q < ns; q++) {
+            if (a.seed_v[q] != row) continue;
+            const uint32_t c = a.seed_c[q];
+            const float wq = a.seed_w[q];
+#pragma unroll
+            for (int i = 0; i < VW; i++)
+                if (c == c0 + i) sv[i] += wq;
+        }
     }
-    const uint64_t i = (uint64_t)row * a.cap + c;
-    const float old = a.X[i];
-    const float gathered = a.damp * acc;
-    const float nw = gathered + sv * t;
-    a.X[i] = nw;
-    res += fabs((double)nw - (double)old);
+    const uint64_t base = (uint64_t)row * a.cap + c0;
+    float x[VW];
+#pragma unroll
+    for (int i = 0; i < VW; i += 4) {
+        const float4 v = *reinterpret_cast<const float4 *>(a.X + base + i);
+        x[i] = v.x;
+        x[i + 1] = v.y;
+        x[i + 2] = v.z;
+        x[i + 3] = v.w;
+    }
     const uint32_t od = a.outdeg[row];
-    if (od == 0) dang += (double)nw;
-    st(a.Cout, a.fmt, i, od ? nw / (float)od : 0.0f);
+#pragma unroll
+    for (int i = 0; i < VW; i++) {
+        if (!((amask >> i) & 1u)) continue;
+        const float gathered = a.damp * acc[i];
+        const float nw = gathered + sv[i] * t[i];
+        res[i * kBlock] += fabs((double)nw - (double)x[i]);
+        if (od == 0) dang[i * kBlock] += (double)nw;
+        x[i] = nw;
+    }
+#pragma unroll
+    for (int i = 0; i < VW; i += 4)
+        *reinterpret_cast<float4 *>(a.X + base + i) = make_float4(x[i], x[i + 1], x[i + 2], x[i + 3]);
+    // a row without out-edges is never gathered: its contribution stays 0
+    if (od) {
+        float c[VW];
+#pragma unroll
+        for (int i = 0; i < VW; i++) c[i] = x[i] / (float)od;
+        Store<CT>::put(a, row, c0, base, c);
+    }
 }
 
-template <int CPW, typename CT>
-__global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
+// The lane's columns in a column group: active and fine masks, teleport factors.
+template <int VW>
+__device__ __forceinline__ uint32_t column_state(const PprArgs &a, uint32_t K, uint32_t c0, float (&t)[VW],
+                                                 uint32_t &fmask)
 {
-    constexpr int G = 64 / CPW;
-    __shared__ double s_res[kMaxCap], s_dang[kMaxCap];
-    __shared__ double w_res[4][CPW], w_dang[4][CPW];
-    __shared__ float hub_red[4][CPW];
-    const uint32_t K = a.bs->K;
-    if (K == 0) return;
+    uint32_t amask = 0;
+    fmask = 0;
+#pragma unroll
+    for (int i = 0; i < VW; i++) {
+        t[i] = 0.0f;
+        const uint32_t c = c0 + i;
+        if (c < K) {
+            const PprCol &col = a.cs[c];
+            if (!col.done) {
+                amask |= 1u << i;
+                if (!col.coarse) fmask |= 1u << i;
+                t[i] = col.t;
+            }
+        }
+    }
+    return amask;
+}
+
+// Per-column block sums of the lanes' FP64 partials in a fixed order (lanes of
+// equal column, then waves), into s_res / s_dang at the group's columns.
+template <int LPR, int VW>
+__device__ __forceinline__ void flush_columns(uint32_t cg, const double *lres, const double *ldang,
+                                              double (*w_res)[LPR * VW], double (*w_dang)[LPR * VW],
+                                              double *s_res, double *s_dang)
+{
+    constexpr int CPW = LPR * VW;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int g = lane / CPW, cl = lane % CPW;
-    if (threadIdx.x < kMaxCap) { s_res[threadIdx.x] = 0.0; s_dang[threadIdx.x] = 0.0; }
+    double res[VW], dang[VW];
+#pragma unroll
+    for (int i = 0; i < VW; i++) {
+        res[i] = lres[i * kBlock];
+        dang[i] = ldang[i * kBlock];
+    }
+#pragma unroll
+    for (int i = 0; i < VW; i++) {
+#pragma unroll
+        for (int off = LPR; off < 64; off <<= 1) {
+            res[i] += __shfl_xor(res[i], off, 64);
+            dang[i] += __shfl_xor(dang[i], off, 64);
+        }
+    }
+    if (lane < LPR) {
+#pragma unroll
+        for (int i = 0; i < VW; i++) {
+            w_res[w][lane * VW + i] = res[i];
+            w_dang[w][lane * VW + i] = dang[i];
+        }
+    }
     __syncthreads();
-    const CT *__restrict__ cin = static_cast<const CT *>(a.Cin);
-    const uint32_t ncg = (K + CPW - 1) / CPW;
-    const uint32_t U = a.nb_hub + a.nmid_units + a.nlow_units + 1;
-    const uint32_t total = U * ncg;
-    double res = 0.0, dang = 0.0;
-    int cur = -1;
-
-    auto flush = [&](int cg) {
-        double r = group_sum<CPW>(res), d = group_sum<CPW>(dang);
-        if (g == 0) { w_res[w][cl] = r; w_dang[w][cl] = d; }
-        __syncthreads();
-        if (threadIdx.x < CPW) {
-            const int c = cg * CPW + threadIdx.x;
-            if (c < (int)kMaxCap) {
-                s_res[c] += (w_res[0][threadIdx.x] + w_res[1][threadIdx.x]) +
-                            (w_res[2][threadIdx.x] + w_res[3][threadIdx.x]);
-                s_dang[c] += (w_dang[0][threadIdx.x] + w_dang[1][threadIdx.x]) +
-                             (w_dang[2][threadIdx.x] + w_dang[3][threadIdx.x]);
+    if (threadIdx.x < CPW) {
+        const uint32_t c = cg * CPW + threadIdx.x;
+        if (c < kMaxCap) {
+            double r = 0.0, d = 0.0;
+#pragma unroll
+            for (int k = 0; k < kWaves; k++) {
+                r += w_res[k][threadIdx.x];
+                d += w_dang[k][threadIdx.x];
             }
+            s_res[c] = r;
+            s_dang[c] = d;
         }
-        __syncthreads();
-        res = 0.0;
-        dang = 0.0;
-    };
+    }
+    __syncthreads();
+}
 
-    for (uint32_t vb = blockIdx.x; vb < total; vb += gridDim.x) {
-        const uint32_t u = vb / ncg, cg = vb % ncg;
-        if ((int)cg != cur) {
-            if (cur >= 0) flush(cur);
-            cur = (int)cg;
-        }
-        const uint32_t c = cg * CPW + cl;
-        const bool act = c < K && !a.cs[c].done && !a.cs[c].stalled;
-        const float t = act ? a.cs[c].t : 0.0f;
-        if (u < a.nb_hub) {
-            // hub chunk: the workgroup's 4*G lane groups stride over the chunk
-            const uint32_t row = a.chunk_row[u];
-            const uint32_t k = u - a.chunk_first[row];
-            const uint32_t rb = a.roff[row], re = a.roff[row + 1];
-            const uint32_t b0 = rb + k * a.hub_chunk;
-            const uint32_t b1 = min(re, b0 + a.hub_chunk);
-            float acc = 0.0f;
-            if (act) {
-                float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
-                constexpr uint32_t S = 4 * G; // lane groups of the workgroup
-                uint32_t j = b0 + w * G + g;
-                for (; j + 3 * S < b1; j += 4 * S) {
-                    const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + S);
-                    const uint32_t u2 = ldnt(a.cols + j + 2 * S), u3 = ldnt(a.cols + j + 3 * S);
-                    a0 += ld(cin, (uint64_t)u0 * a.cap + c);
-                    a1 += ld(cin, (uint64_t)u1 * a.cap + c);
-                    a2 += ld(cin, (uint64_t)u2 * a.cap + c);
-                    a3 += ld(cin, (uint64_t)u3 * a.cap + c);
+// Work units of one column group for one lane: gathers through `rows`.
+template <int LPR, typename CT, typename R>
+__device__ __forceinline__ void sweep_units(const PprArgs &a, const R &rows, uint32_t c0, uint32_t amask,
+                                            const float (&t)[R::W], double *res, double *dang,
+                                            uint32_t nz)
+{
+    constexpr int VW = R::W;
+    constexpr int RGL = 64 / LPR;    // low rows per wave
+    constexpr int RG = RGL / kSlots; // chunks per wave
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int sub = lane / LPR;
+    const int slot = sub % kSlots, rg = sub / kSlots;
+    const uint32_t wu_ch = (a.nch + RG - 1) / RG;
+    const uint32_t wu_low = (a.n_nonzero - a.n_nonlow + RGL - 1) / RGL;
+    const uint32_t total = wu_ch + wu_low + (nz + RGL - 1) / RGL;
+    const uint32_t gw = blockIdx.x * kWaves + w, nwav = gridDim.x * kWaves;
+    for (uint32_t wu = gw; wu < total; wu += nwav) {
+        if (wu < wu_ch) {
+            const uint32_t k = wu * RG + rg;
+            if (k < a.nch) {
+                const uint32_t row = a.ch_row[k];
+                const uint32_t f0 = a.ch_first[row];
+                const uint32_t b0 = a.roff[row] + (k - f0) * kChunk;
+                const uint32_t b1 = min(a.roff[row + 1], b0 + kChunk);
+                float acc[VW];
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] = 0.0f;
+                if (amask) {
+                    const uint32_t j = b0 + slot;
+                    const uint32_t n = j < b1 ? (b1 - j + kSlots - 1) / kSlots : 0u;
+                    gather_seq(rows, a.cols + j, kSlots, n, acc);
                 }
-                for (; j < b1; j += S) a0 += ld(cin, (uint64_t)ldnt(a.cols + j) * a.cap + c);
-                acc = (a0 + a1) + (a2 + a3);
-            }
-            acc = group_sum<CPW>(acc);
-            if (g == 0) hub_red[w][cl] = acc;
-            __syncthreads();
-            if (threadIdx.x < CPW && cg * CPW + threadIdx.x < a.cap) {
-                const float tot = (hub_red[0][threadIdx.x] + hub_red[1][threadIdx.x]) +
-                                  (hub_red[2][threadIdx.x] + hub_red[3][threadIdx.x]);
-                a.hub_part[(uint64_t)u * a.cap + cg * CPW + threadIdx.x] = tot;
-            }
-            __syncthreads();
-        } else if (u < a.nb_hub + a.nmid_units) {
-            // mid row: one wave, lane groups stride over the row's edges
-            const uint32_t row = a.n_hub + (u - a.nb_hub) * 4 + w;
-            if (row < a.n_nonlow) {
-                const uint32_t rb = a.roff[row], re = a.roff[row + 1];
-                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-                if (act) {
-                    uint32_t j = rb + g;
-                    for (; j + 3 * G < re; j += 4 * G) {
-                        const uint32_t u0 = ldnt(a.cols + j), u1 = ldnt(a.cols + j + G);
-                        const uint32_t u2 = ldnt(a.cols + j + 2 * G), u3 = ldnt(a.cols + j + 3 * G);
-                        a0 += ld(cin, (uint64_t)u0 * a.cap + c);
-                        a1 += ld(cin, (uint64_t)u1 * a.cap + c);
-                        a2 += ld(cin, (uint64_t)u2 * a.cap + c);
-                        a3 += ld(cin, (uint64_t)u3 * a.cap + c);
+                // (s0 + s1) + (s2 + s3); partner lanes share row and columns
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] += __shfl_xor(acc[i], LPR, 64);
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] += __shfl_xor(acc[i], 2 * LPR, 64);
+                if (slot == 0 && amask) {
+                    if (a.ch_first[row + 1] - f0 == 1) {
+                        update_row<CT>(a, row, c0, amask, acc, t, res, dang);
+                    } else {
+                        float *pp = a.part + (uint64_t)k * a.cap + c0;
+#pragma unroll
+                        for (int i = 0; i < VW; i += 4)
+                            *reinterpret_cast<float4 *>(pp + i) =
+                                make_float4(acc[i], acc[i + 1], acc[i + 2], acc[i + 3]);
                     }
-                    for (; j < re; j += G) a0 += ld(cin, (uint64_t)ldnt(a.cols + j) * a.cap + c);
                 }
-                const float acc = group_sum<CPW>((a0 + a1) + (a2 + a3));
-                if (g == 0 && act) ppr_update(a, row, c, acc, t, res, dang);
             }
-        } else if (u < a.nb_hub + a.nmid_units + a.nlow_units) {
-            // low rows: one lane group per row, edges in the plan's order
-            const uint32_t rl = (u - a.nb_hub - a.nmid_units) * (4 * G) + w * G + g;
+        } else if (wu < wu_ch + wu_low) {
+            const uint32_t rl = (wu - wu_ch) * RGL + sub;
             const uint32_t row = a.n_nonlow + rl;
-            if (row < a.n_nonzero && act) {
+            if (row < a.n_nonzero && amask) {
                 const uint32_t deg = a.roff[row + 1] - a.roff[row];
-                const uint32_t *p = a.cols + a.slice_ptr[rl >> 6] + (rl & 63);
-                float acc = 0.0f;
-                uint32_t k = 0;
-                for (; k + 4 <= deg; k += 4) {
-                    const uint32_t u0 = ldnt(p + k * 64), u1 = ldnt(p + (k + 1) * 64);
-                    const uint32_t u2 = ldnt(p + (k + 2) * 64), u3 = ldnt(p + (k + 3) * 64);
-                    const float x0 = ld(cin, (uint64_t)u0 * a.cap + c);
-                    const float x1 = ld(cin, (uint64_t)u1 * a.cap + c);
-                    const float x2 = ld(cin, (uint64_t)u2 * a.cap + c);
-                    const float x3 = ld(cin, (uint64_t)u3 * a.cap + c);
-                    acc += x0; // edge order
-                    acc += x1;
-                    acc += x2;
-                    acc += x3;
-                }
-                for (; k < deg; k++) acc += ld(cin, (uint64_t)ldnt(p + k * 64) * a.cap + c);
-                ppr_update(a, row, c, acc, t, res, dang);
+                float acc[VW];
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] = 0.0f;
+                gather_seq(rows, a.cols + a.slice_ptr[rl >> 6] + (rl & 63), 64u, deg, acc);
+                update_row<CT>(a, row, c0, amask, acc, t, res, dang);
             }
         } else {
             // seeds without in-edges: x' = s_v * t
-            if (threadIdx.x < CPW && act) {
-                const uint32_t ns = a.bs->nseeds;
-                for (uint32_t q = 0; q < ns; q++)
-                    if (a.seed_c[q] == c && a.seed_v[q] >= a.n_nonzero)
-                        ppr_update(a, a.seed_v[q], c, 0.0f, t, res, dang);
+            const uint32_t zi = (wu - wu_ch - wu_low) * RGL + sub;
+            if (zi < nz && amask) {
+                float acc[VW];
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] = 0.0f;
+                update_row<CT>(a, a.zrows[zi], c0, amask, acc, t, res, dang);
             }
         }
     }
-    if (cur >= 0) flush(cur);
-
-    // per-workgroup column partials; k_ppr_reduce sums them in a fixed order
-    if (threadIdx.x < kMaxCap) {
-        a.blk_res[(uint64_t)blockIdx.x * kMaxCap + threadIdx.x] = s_res[threadIdx.x];
-        a.blk_dang[(uint64_t)blockIdx.x * kMaxCap + threadIdx.x] = s_dang[threadIdx.x];
-    }
 }
 
-// Hub rows, one workgroup per row and one lane per column, after the sweep
-// (the kernel boundary orders the chunk partials): chunk partials summed in
-// chunk order, then the column update.
-__global__ __launch_bounds__(kMaxCap) void k_ppr_hub(PprArgs a)
+template <int LPR, typename CT>
+__global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
 {
+    constexpr int VW = Store<CT>::W;
+    constexpr int CPW = LPR * VW;
+    static_assert(64 / LPR / kSlots >= 1, "a chunk needs kSlots lane groups");
+    __shared__ double w_res[kWaves][CPW], w_dang[kWaves][CPW];
+    __shared__ double s_res[kMaxCap], s_dang[kMaxCap];
+    __shared__ double l_res[VW][kBlock], l_dang[VW][kBlock]; // per-lane column sums
     const uint32_t K = a.bs->K;
     if (K == 0) return;
-    const uint32_t row = blockIdx.x, c = threadIdx.x;
-    double res = 0.0, dang = 0.0;
-    if (c < K && !a.cs[c].done && !a.cs[c].stalled) {
-        const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
-        float sum = 0.0f;
-        for (uint32_t q = f0; q < f1; q++) sum += a.hub_part[(uint64_t)q * a.cap + c];
-        ppr_update(a, row, c, sum, a.cs[c].t, res, dang);
+    const int lane = threadIdx.x & 63;
+    const int l = lane % LPR;
+    if (threadIdx.x < kMaxCap) {
+        s_res[threadIdx.x] = 0.0;
+        s_dang[threadIdx.x] = 0.0;
     }
-    a.hub_res[(uint64_t)row * kMaxCap + c] = res;
-    a.hub_dang[(uint64_t)row * kMaxCap + c] = dang;
+    __syncthreads();
+    const uint32_t ncg = (K + CPW - 1) / CPW;
+    const uint32_t nz = a.bs->nzrows;
+    for (uint32_t cg = 0; cg < ncg; cg++) {
+        const uint32_t c0 = cg * CPW + l * VW;
+        float t[VW];
+        uint32_t fmask;
+        const uint32_t amask = column_state<VW>(a, K, c0, t, fmask);
+        double *res = &l_res[0][threadIdx.x], *dang = &l_dang[0][threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < VW; i++) { res[i * kBlock] = 0.0; dang[i * kBlock] = 0.0; }
+        if constexpr (VW == 4) {
+            F32Rows rows{static_cast<const float *>(a.Cin), a.cap, c0};
+            sweep_units<LPR, CT>(a, rows, c0, amask, t, res, dang, nz);
+        } else {
+            // the low halves are read by waves holding a fine column (wave-uniform path)
+            const bool wave_fine = __builtin_amdgcn_ballot_w64((fmask & amask) != 0) != 0;
+            if (wave_fine) {
+                SplitRows<true> rows;
+                rows.base = static_cast<const char *>(a.Cin);
+                rows.stride = a.cap * 4;
+                rows.hoff = split_hi(a.cap, c0);
+                rows.loff = split_lo(a.cap, c0);
+                const uint32_t fa = fmask & amask;
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    rows.fm[j] = (((fa >> (2 * j)) & 1u) ? 0x0000ffffu : 0u) |
+                                 (((fa >> (2 * j + 1)) & 1u) ? 0xffff0000u : 0u);
+                sweep_units<LPR, CT>(a, rows, c0, amask, t, res, dang, nz);
+            } else {
+                SplitRows<false> rows;
+                rows.base = static_cast<const char *>(a.Cin);
+                rows.stride = a.cap * 4;
+                rows.hoff = split_hi(a.cap, c0);
+                rows.loff = split_lo(a.cap, c0);
+#pragma unroll
+                for (int j = 0; j < 4; j++) rows.fm[j] = 0u;
+                sweep_units<LPR, CT>(a, rows, c0, amask, t, res, dang, nz);
+            }
+        }
+        flush_columns<LPR, VW>(cg, res, dang, w_res, w_dang, s_res, s_dang);
+    }
+    if (threadIdx.x < kMaxCap) {
+        const uint64_t o = (uint64_t)(a.part_base + blockIdx.x) * kMaxCap + threadIdx.x;
+        a.blk_res[o] = s_res[threadIdx.x];
+        a.blk_dang[o] = s_dang[threadIdx.x];
+    }
 }
 
-// One workgroup: per column, residual and dangling sums over the sweep and hub
-// partials in a fixed order, then convergence.
-constexpr int kRedBlock = 1024;
-__global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a)
+// Rows of more than one chunk, after the sweep (the kernel boundary orders the
+// chunk partials): kSlots lane groups per row, slot s sums chunks s, s +
+// kSlots, ... in order, combined as (s0 + s1) + (s2 + s3); then the update.
+template <int LPR, typename CT>
+__global__ __launch_bounds__(kBlock) void k_ppr_rows(PprArgs a)
 {
-    __shared__ double q_res[kRedBlock / kMaxCap][kMaxCap], q_dang[kRedBlock / kMaxCap][kMaxCap];
+    constexpr int VW = Store<CT>::W;
+    constexpr int CPW = LPR * VW;
+    constexpr int RG = 64 / LPR / kSlots;
+    __shared__ double w_res[kWaves][CPW], w_dang[kWaves][CPW];
+    __shared__ double s_res[kMaxCap], s_dang[kMaxCap];
+    __shared__ double l_res[VW][kBlock], l_dang[VW][kBlock]; // per-lane column sums
+    const uint32_t K = a.bs->K;
+    if (K == 0) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int l = lane % LPR, sub = lane / LPR;
+    const int slot = sub % kSlots, rg = sub / kSlots;
+    if (threadIdx.x < kMaxCap) {
+        s_res[threadIdx.x] = 0.0;
+        s_dang[threadIdx.x] = 0.0;
+    }
+    __syncthreads();
+    const uint32_t ncg = (K + CPW - 1) / CPW;
+    const uint32_t total = (a.nmulti + RG - 1) / RG;
+    const uint32_t gw = blockIdx.x * kWaves + w, nwav = gridDim.x * kWaves;
+    for (uint32_t cg = 0; cg < ncg; cg++) {
+        const uint32_t c0 = cg * CPW + l * VW;
+        float t[VW];
+        uint32_t fmask;
+        const uint32_t amask = column_state<VW>(a, K, c0, t, fmask);
+        double *res = &l_res[0][threadIdx.x], *dang = &l_dang[0][threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < VW; i++) { res[i * kBlock] = 0.0; dang[i * kBlock] = 0.0; }
+        for (uint32_t wu = gw; wu < total; wu += nwav) {
+            const uint32_t idx = wu * RG + rg;
+            if (idx < a.nmulti) {
+                const uint32_t row = a.multi[idx];
+                const uint32_t f0 = a.ch_first[row], f1 = a.ch_first[row + 1];
+                float acc[VW];
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] = 0.0f;
+                if (amask) {
+                    uint32_t k = f0 + slot;
+                    for (; k + 3 * kSlots < f1; k += 4 * kSlots) {
+                        float4 v[4][VW / 4];
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+#pragma unroll
+                            for (int i = 0; i < VW / 4; i++)
+                                v[e][i] = *reinterpret_cast<const float4 *>(
+                                    a.part + (uint64_t)(k + e * kSlots) * a.cap + c0 + 4 * i);
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+#pragma unroll
+                            for (int i = 0; i < VW / 4; i++) {
+                                acc[4 * i] += v[e][i].x;
+                                acc[4 * i + 1] += v[e][i].y;
+                                acc[4 * i + 2] += v[e][i].z;
+                                acc[4 * i + 3] += v[e][i].w;
+                            }
+                    }
+                    for (; k < f1; k += kSlots) {
+#pragma unroll
+                        for (int i = 0; i < VW / 4; i++) {
+                            const float4 v = *reinterpret_cast<const float4 *>(
+                                a.part + (uint64_t)k * a.cap + c0 + 4 * i);
+                            acc[4 * i] += v.x;
+                            acc[4 * i + 1] += v.y;
+                            acc[4 * i + 2] += v.z;
+                            acc[4 * i + 3] += v.w;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] += __shfl_xor(acc[i], LPR, 64);
+#pragma unroll
+                for (int i = 0; i < VW; i++) acc[i] += __shfl_xor(acc[i], 2 * LPR, 64);
+                if (slot == 0 && amask) update_row<CT>(a, row, c0, amask, acc, t, res, dang);
+            }
+        }
+        flush_columns<LPR, VW>(cg, res, dang, w_res, w_dang, s_res, s_dang);
+    }
+    if (threadIdx.x < kMaxCap) {
+        const uint64_t o = (uint64_t)(a.part_base + blockIdx.x) * kMaxCap + threadIdx.x;
+        a.blk_res[o] = s_res[threadIdx.x];
+        a.blk_dang[o] = s_dang[threadIdx.x];
+    }
+}
+
+// One workgroup: per column, residual and dangling sums over the partial rows
+// of both kernels in a fixed order, then convergence.  A coarse (BF16) column
+// whose residual stalls or meets its tolerance turns fine: from the next sweep
+// on it reads exact FP32 contributions, and it is done at the tolerance there.
+constexpr int kRedBlock = 1024;
+__global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t nparts)
+{
     constexpr uint32_t P = kRedBlock / kMaxCap;
+    __shared__ double q_res[P][kMaxCap], q_dang[P][kMaxCap];
     const uint32_t K = a.bs->K;
     if (K == 0) return;
     const uint32_t c = threadIdx.x % kMaxCap, part = threadIdx.x / kMaxCap;
     double r = 0.0, d = 0.0;
-    for (uint32_t b = part * a.nparts / P; b < (part + 1) * a.nparts / P; b++) {
+    for (uint32_t b = part * nparts / P; b < (part + 1) * nparts / P; b++) {
         r += a.blk_res[(uint64_t)b * kMaxCap + c];
         d += a.blk_dang[(uint64_t)b * kMaxCap + c];
-    }
-    for (uint32_t h = part * a.n_hub / P; h < (part + 1) * a.n_hub / P; h++) {
-        r += a.hub_res[(uint64_t)h * kMaxCap + c];
-        d += a.hub_dang[(uint64_t)h * kMaxCap + c];
     }
     q_res[part][c] = r;
     q_dang[part][c] = d;
     __syncthreads();
     if (threadIdx.x < K) {
         PprCol &col = a.cs[c];
-        if (!col.done && !col.stalled) {
+        if (!col.done) {
             double R = 0.0, D = 0.0;
             for (uint32_t i = 0; i < P; i++) { R += q_res[i][c]; D += q_dang[i][c]; }
             const double prev = col.residual;
@@ -319,37 +659,38 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a)
             col.residual = R;
             col.dangling = D;
             col.t = (float)((1.0 - (double)a.damp) + (double)a.damp * D);
-            if (col.iters >= col.max_iter)
+            if (col.iters >= col.max_iter) {
                 col.done = 1;
-            else if (col.finish && (R <= col.tol || (col.iters >= 3 && R > 0.97 * prev)))
-                col.stalled = 1; // low-precision floor reached: finish in FP32
-            else if (R <= col.tol)
+            } else if (col.coarse && (R <= col.tol || R <= kCoarseFloor ||
+                                       (R > 0.97 * prev && R <= 8.0 * kCoarseFloor))) {
+                col.coarse = 0; // storage floor reached: read full FP32 from now on
+                col.fine_at = col.iters;
+            } else if (R <= col.tol) {
                 col.done = 1;
+            }
         }
     }
     if (threadIdx.x == 0) a.bs->sweeps += 1;
 }
 
 // ---- slot management kernels ----
-// Column (re)initialisation of the slots admitted in one step: one pass over
-// the rows clears X and both contribution buffers for every listed slot.
+// Column (re)initialisation of the slots admitted in one step: X and both
+// contribution buffers cleared, one thread per (row, admitted slot).
 __global__ void k_ppr_clear_cols(float *X, void *C0, void *C1, uint32_t N, uint32_t cap,
                                  const PprAdmit *adm, uint32_t na, uint32_t fmt)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    for (uint32_t q = 0; q < na; q++) {
-        const uint64_t k = (uint64_t)i * cap + adm[q].slot;
-        X[k] = 0.0f;
-        st(C0, fmt, k, 0.0f);
-        st(C1, fmt, k, 0.0f);
-    }
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)N * na) return;
+    const uint32_t row = (uint32_t)(i / na), q = (uint32_t)(i % na);
+    const uint64_t k = (uint64_t)row * cap + adm[q].slot;
+    X[k] = 0.0f;
+    st(C0, fmt, cap, row, adm[q].slot, 0.0f);
+    st(C1, fmt, cap, row, adm[q].slot, 0.0f);
 }
 
 // One workgroup per admitted slot: x0 = s on the seed rows, contributions
 // s / outdeg in the buffer the next sweep reads, column state.
-__global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap,
-                                uint32_t fmt, const PprAdmit *adm, const uint32_t *sv,
+__global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap, uint32_t fmt, const PprAdmit *adm, const uint32_t *sv,
                                 const float *sw, PprCol *cs, float damp)
 {
     const PprAdmit d = adm[blockIdx.x];
@@ -358,7 +699,7 @@ __global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uin
         const uint64_t k = (uint64_t)v * cap + d.slot;
         X[k] = sw[q];
         const uint32_t od = outdeg[v];
-        st(Cin, fmt, k, od ? sw[q] / (float)od : 0.0f);
+        st(Cin, fmt, cap, v, d.slot, od ? sw[q] / (float)od : 0.0f);
     }
     if (threadIdx.x == 0) {
         double d0 = 0.0;
@@ -372,8 +713,8 @@ __global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uin
         col.tol = d.tol;
         col.residual = INFINITY;
         col.dangling = d0;
-        col.stalled = 0;
-        col.finish = fmt != AG_PREC_FP32 ? 1u : 0u;
+        col.coarse = fmt != AG_PREC_FP32 ? 1u : 0u;
+        col.fine_at = 0;
         cs[d.slot] = col;
     }
 }
@@ -385,32 +726,38 @@ __global__ void k_ppr_stop_cols(PprCol *cs, const PprAdmit *adm, uint32_t n)
     if (q < n) cs[adm[q].slot].done = 1;
 }
 
-// Moves stalled columns into an FP32 batch: iterate copied, both FP32
-// contribution buffers recomputed from it (x / outdeg), state carried over.
-// adm[q].slot = source slot, adm[q].e0 = destination slot.
-__global__ void k_ppr_migrate_cols(const float *Xs, uint32_t scap, float *Xd, float *C0, float *C1,
-                                   uint32_t dcap, const PprAdmit *adm, uint32_t na,
-                                   const uint32_t *outdeg, uint32_t N, const PprCol *cs_src,
-                                   PprCol *cs_dst)
+// Packs running columns into lower slots of the same batch: iterate and both
+// contribution buffers (both halves) copied, so rows a sweep never rewrites
+// (no in-edges, not a seed) keep the moved column's values, not those of the
+// slot's previous column.  adm[q].slot = source, adm[q].e0 = destination; one
+// thread per (row, move).
+__global__ void k_ppr_move_cols(float *X, void *C0, void *C1, uint32_t cap, uint32_t fmt,
+                                const PprAdmit *adm, uint32_t na, uint32_t N)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < na) {
-        PprCol col = cs_src[adm[i].slot];
-        col.stalled = 0;
-        col.done = 0;
-        col.finish = 0;
-        cs_dst[adm[i].e0] = col;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)N * na) return;
+    const uint32_t row = (uint32_t)(i / na), q = (uint32_t)(i % na);
+    const uint64_t s = (uint64_t)row * cap + adm[q].slot, d = (uint64_t)row * cap + adm[q].e0;
+    X[d] = X[s];
+    if (fmt == AG_PREC_FP32) {
+        float *c0 = static_cast<float *>(C0), *c1 = static_cast<float *>(C1);
+        c0[d] = c0[s];
+        c1[d] = c1[s];
+    } else {
+        const uint32_t hs = split_hi(cap, adm[q].slot), ls = split_lo(cap, adm[q].slot);
+        const uint32_t hd = split_hi(cap, adm[q].e0), ld = split_lo(cap, adm[q].e0);
+        for (void *buf : {C0, C1}) {
+            char *r = static_cast<char *>(buf) + (uint64_t)row * cap * 4;
+            *reinterpret_cast<uint16_t *>(r + hd) = *reinterpret_cast<const uint16_t *>(r + hs);
+            *reinterpret_cast<uint16_t *>(r + ld) = *reinterpret_cast<const uint16_t *>(r + ls);
+        }
     }
-    if (i >= N) return;
-    const uint32_t od = outdeg[i];
-    for (uint32_t q = 0; q < na; q++) {
-        const float x = Xs[(uint64_t)i * scap + adm[q].slot];
-        const float c = od ? x / (float)od : 0.0f;
-        const uint64_t k = (uint64_t)i * dcap + adm[q].e0;
-        Xd[k] = x;
-        C0[k] = c;
-        C1[k] = c;
-    }
+}
+
+__global__ void k_ppr_move_state(PprCol *cs, const PprAdmit *adm, uint32_t na)
+{
+    const uint32_t q = threadIdx.x;
+    if (q < na) cs[adm[q].e0] = cs[adm[q].slot];
 }
 
 __global__ void k_ppr_extract(const float *X, const uint32_t *order, uint32_t N, uint32_t cap,
@@ -439,56 +786,137 @@ __global__ void k_map_labels(const uint32_t *rank, const uint32_t *ids, uint32_t
     if (q < n) out[q] = rank[ids[q]];
 }
 
-template <int CPW, typename CT>
-void launch(const PprArgs &a, uint32_t grid, hipStream_t s)
-{
-    hipLaunchKernelGGL((k_ppr_sweep<CPW, CT>), dim3(grid), dim3(kBlock), 0, s, a);
-}
+uint32_t vec_width(uint32_t fmt) { return fmt == AG_PREC_FP32 ? 4u : 8u; }
 
-template <typename CT>
-void launch_cpw(uint32_t cpw, const PprArgs &a, uint32_t grid, hipStream_t s)
+// Launch of the sweep pair for one (lanes per row, storage type).
+template <int LPR, typename CT>
+void launch_lpr(const PprArgs &a, uint32_t g1, uint32_t g2, hipStream_t s)
 {
-    switch (cpw) {
-    case 4: launch<4, CT>(a, grid, s); break;
-    case 8: launch<8, CT>(a, grid, s); break;
-    case 16: launch<16, CT>(a, grid, s); break;
-    case 32: launch<32, CT>(a, grid, s); break;
-    default: launch<64, CT>(a, grid, s); break;
+    hipLaunchKernelGGL((k_ppr_sweep<LPR, CT>), dim3(g1), dim3(kBlock), 0, s, a);
+    if (a.nmulti) {
+        PprArgs b = a;
+        b.part_base = g1;
+        hipLaunchKernelGGL((k_ppr_rows<LPR, CT>), dim3(g2), dim3(kBlock), 0, s, b);
     }
 }
 
-uint32_t cbytes(uint32_t fmt) { return fmt == AG_PREC_FP32 ? 4u : 2u; }
+template <typename CT>
+void launch_ct(uint32_t lpr, const PprArgs &a, uint32_t g1, uint32_t g2, hipStream_t s)
+{
+    switch (lpr) {
+    case 1: launch_lpr<1, CT>(a, g1, g2, s); break;
+    case 2: launch_lpr<2, CT>(a, g1, g2, s); break;
+    case 4: launch_lpr<4, CT>(a, g1, g2, s); break;
+    case 8: launch_lpr<8, CT>(a, g1, g2, s); break;
+    default:
+        if constexpr (sizeof(CT) == 4) launch_lpr<16, CT>(a, g1, g2, s);
+        else fail(AG_ERR_INVALID, "BF16 columns: at most 64 columns per wave");
+        break;
+    }
+}
+
+template <int LPR, typename CT>
+int sweep_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ppr_sweep<LPR, CT>, kBlock, 0) != hipSuccess ||
+        nb <= 0)
+        nb = 2;
+    return std::min(nb, 16);
+}
+
+template <typename CT>
+int blocks_per_cu(uint32_t lpr)
+{
+    switch (lpr) {
+    case 1: return sweep_blocks_per_cu<1, CT>();
+    case 2: return sweep_blocks_per_cu<2, CT>();
+    case 4: return sweep_blocks_per_cu<4, CT>();
+    case 8: return sweep_blocks_per_cu<8, CT>();
+    default:
+        if constexpr (sizeof(CT) == 4) return sweep_blocks_per_cu<16, CT>();
+        return 2;
+    }
+}
+
+uint32_t log2u(uint32_t v)
+{
+    uint32_t r = 0;
+    while ((1u << (r + 1)) <= v) r++;
+    return r;
+}
 
 } // namespace
 
-uint64_t PprBatch::bytes_for(uint32_t N, uint32_t cap, uint32_t fmt)
+// Chunk table of the rows [0, n_nonlow) (in-degree above the plan's low_max):
+// kChunk edges per chunk, and the list of rows of more than one chunk.
+void ensure_ppr_chunks(ag_ctx *ctx, Plan *p)
 {
-    return (uint64_t)N * cap * (4ull + 2ull * cbytes(fmt)) + (uint64_t)N * 4 + (1u << 20);
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->ppr_ready) return;
+    hipStream_t s = ctx->stream;
+    const uint32_t nr = p->n_nonlow;
+    std::vector<uint32_t> roff(nr + 1, 0u);
+    AG_HIP(hipMemcpyAsync(roff.data(), p->roff.p, (size_t)(nr + 1) * 4, hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> first(nr + 1), rows, multi;
+    uint32_t nch = 0;
+    for (uint32_t r = 0; r < nr; r++) {
+        first[r] = nch;
+        const uint32_t deg = roff[r + 1] - roff[r];
+        const uint32_t c = std::max<uint32_t>(1, ceil_div(deg, kChunk));
+        for (uint32_t k = 0; k < c; k++) rows.push_back(r);
+        if (c > 1) multi.push_back(r);
+        nch += c;
+    }
+    first[nr] = nch;
+    p->ppr_ch_first.alloc((size_t)(nr + 1) * 4);
+    p->ppr_ch_row.alloc((size_t)std::max<uint32_t>(nch, 1) * 4);
+    p->ppr_multi.alloc((size_t)std::max<size_t>(multi.size(), 1) * 4);
+    AG_HIP(hipMemcpyAsync(p->ppr_ch_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
+    if (nch) AG_HIP(hipMemcpyAsync(p->ppr_ch_row.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, s));
+    if (!multi.empty())
+        AG_HIP(hipMemcpyAsync(p->ppr_multi.p, multi.data(), multi.size() * 4, hipMemcpyHostToDevice, s));
+    AG_HIP(hipStreamSynchronize(s));
+    p->ppr_nch = nch;
+    p->ppr_nmulti = (uint32_t)multi.size();
+    p->ppr_ready = true;
+}
+
+uint64_t PprBatch::bytes_for(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t fmt)
+{
+    if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
+        fail(AG_ERR_INVALID, "batched columns store FP32 or BF16 (precision %u)", fmt);
+    Plan *p = ensure_plan(ctx, g, nullptr);
+    ensure_ppr_chunks(ctx, p);
+    const uint64_t N = p->N;
+    // X + two contribution buffers of 4 bytes per column (FP32, or split BF16)
+    return N * cap * 12ull + N / 8 + (uint64_t)p->ppr_nch * cap * 4 + (1u << 20);
 }
 
 PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt, float damp)
-    : ctx_(ctx), cap_(cap), cpw_(cpw), fmt_(fmt), damp_(damp)
+    : ctx_(ctx), cap_(cap), fmt_(fmt), damp_(damp)
 {
-    if (cap == 0 || cap > kMaxCap || cap % 4) fail(AG_ERR_INVALID, "batch capacity must be 4..64, multiple of 4");
-    if (cpw != 4 && cpw != 8 && cpw != 16 && cpw != 32 && cpw != 64)
-        fail(AG_ERR_INVALID, "cols_per_wave must be 4, 8, 16, 32 or 64");
-    if (fmt > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", fmt);
+    if (cap == 0 || cap > kMaxCap || cap % 8) fail(AG_ERR_INVALID, "batch capacity must be 8..64, multiple of 8");
+    if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
+        fail(AG_ERR_INVALID, "batched columns store FP32 or BF16 (precision %u); FP16 storage is "
+                             "available to the single-query path", fmt);
     if (!(damp > 0.0f && damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
     p_ = ensure_plan(ctx, g, nullptr);
+    ensure_ppr_chunks(ctx, p_);
     N_ = p_->N;
     if (N_ == 0) fail(AG_ERR_INVALID, "graph has no vertices");
+    set_cpw(cpw);
     hipStream_t s = ctx->stream;
-    grid_ = std::max<uint32_t>(64, (uint32_t)ctx->num_cus * 8);
     X_.alloc((size_t)N_ * cap * 4);
-    C_[0].alloc((size_t)N_ * cap * cbytes(fmt));
-    C_[1].alloc((size_t)N_ * cap * cbytes(fmt));
+    C_[0].alloc((size_t)N_ * cap * 4);
+    C_[1].alloc((size_t)N_ * cap * 4);
     cols_.alloc(sizeof(PprCol) * kMaxCap);
     bstate_.alloc(sizeof(PprBatchState));
-    blk_res_.alloc((size_t)grid_ * kMaxCap * 8);
-    blk_dang_.alloc((size_t)grid_ * kMaxCap * 8);
-    hub_part_.alloc((size_t)std::max<uint32_t>(p_->hub_chunks, 1) * cap * 4);
-    hub_res_.alloc((size_t)std::max<uint32_t>(p_->n_hub, 1) * kMaxCap * 8);
-    hub_dang_.alloc((size_t)std::max<uint32_t>(p_->n_hub, 1) * kMaxCap * 8);
+    const size_t nparts = (size_t)ctx->num_cus * 16 * 2;
+    blk_res_.alloc(nparts * kMaxCap * 8);
+    blk_dang_.alloc(nparts * kMaxCap * 8);
+    part_.alloc((size_t)std::max<uint32_t>(p_->ppr_nch, 1) * cap * 4);
     seed_bits_.alloc((size_t)((N_ + 31) / 32) * 4);
     out_.alloc((size_t)N_ * 4);
     desc_.alloc(sizeof(PprAdmit) * kMaxCap);
@@ -509,18 +937,31 @@ PprBatch::~PprBatch()
 
 uint64_t PprBatch::bytes() const
 {
-    return X_.bytes + C_[0].bytes + C_[1].bytes + blk_res_.bytes + blk_dang_.bytes +
-           hub_part_.bytes + hub_res_.bytes + hub_dang_.bytes + seed_bits_.bytes + out_.bytes +
-           seed_v_.bytes + seed_c_.bytes + seed_w_.bytes + desc_.bytes;
+    return X_.bytes + C_[0].bytes + C_[1].bytes + blk_res_.bytes + blk_dang_.bytes + part_.bytes +
+           seed_bits_.bytes + out_.bytes + seed_v_.bytes + seed_c_.bytes + seed_w_.bytes +
+           zrows_.bytes + desc_.bytes;
 }
 
+// Widest column group: VW columns (16 bytes of one row) per lane times a
+// power of two of lanes, at most 64 columns.
 void PprBatch::set_cpw(uint32_t cpw)
 {
     if (cpw != 4 && cpw != 8 && cpw != 16 && cpw != 32 && cpw != 64)
-        fail(AG_ERR_INVALID, "bad cols_per_wave");
-    cpw_ = cpw;
+        fail(AG_ERR_INVALID, "cols_per_wave must be 4, 8, 16, 32 or 64");
+    cpw_ = std::max(cpw, vec_width(fmt_));
+    graph_lpr_ = 0; // recapture
     if (gexec_) { (void)hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+}
+
+// Lanes per row for the current slots: the column group covers the used
+// slots, up to cols_per_wave.
+uint32_t PprBatch::lanes_per_row() const
+{
+    const uint32_t vw = vec_width(fmt_);
+    uint32_t cols = vw;
+    while (cols < hi_ && cols < cpw_) cols <<= 1;
+    return std::min(cols, cpw_) / vw;
 }
 
 PprSeedSet PprBatch::make_seeds(const uint32_t *seeds, uint32_t n, double tol, uint32_t max_iter)
@@ -555,7 +996,7 @@ PprSeedSet PprBatch::make_seeds(const uint32_t *seeds, uint32_t n, double tol, u
 void PprBatch::upload_seed_table()
 {
     hipStream_t s = ctx_->stream;
-    std::vector<uint32_t> v, c;
+    std::vector<uint32_t> v, c, z;
     std::vector<float> w;
     for (uint32_t k = 0; k < cap_; k++) {
         if (!used_[k]) continue;
@@ -563,8 +1004,12 @@ void PprBatch::upload_seed_table()
             v.push_back(q_[k].labels[q]);
             c.push_back(k);
             w.push_back(q_[k].w[q]);
+            if (q_[k].labels[q] >= p_->n_nonzero) z.push_back(q_[k].labels[q]);
         }
     }
+    // seed rows without in-edges, each once (one lane group updates a row)
+    std::sort(z.begin(), z.end());
+    z.erase(std::unique(z.begin(), z.end()), z.end());
     const size_t n = v.size();
     const size_t need = std::max<size_t>(n, 1) * 4;
     if (seed_v_.bytes < need) {
@@ -572,6 +1017,8 @@ void PprBatch::upload_seed_table()
         seed_v_.alloc(nb);
         seed_c_.alloc(nb);
         seed_w_.alloc(nb);
+        zrows_.alloc(nb);
+        graph_lpr_ = 0; // the captured sweeps hold the old pointers
         if (gexec_) { (void)hipGraphExecDestroy(gexec_); gexec_ = nullptr; }
         if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
     }
@@ -580,15 +1027,16 @@ void PprBatch::upload_seed_table()
         AG_HIP(hipMemcpyAsync(seed_c_.p, c.data(), n * 4, hipMemcpyHostToDevice, s));
         AG_HIP(hipMemcpyAsync(seed_w_.p, w.data(), n * 4, hipMemcpyHostToDevice, s));
     }
+    if (!z.empty()) AG_HIP(hipMemcpyAsync(zrows_.p, z.data(), z.size() * 4, hipMemcpyHostToDevice, s));
     const uint64_t words = (N_ + 31) / 32;
     hipLaunchKernelGGL(k_fill_u32, dim3(ceil_div(words, 256)), dim3(256), 0, s,
                        seed_bits_.as<uint32_t>(), words, 0u);
     if (n)
         hipLaunchKernelGGL(k_set_bits, dim3(ceil_div(n, 256)), dim3(256), 0, s,
                            seed_bits_.as<uint32_t>(), seed_v_.as<uint32_t>(), (uint32_t)n);
-    // K (slot high-water mark) and the entry count live on the device so that
+    // K (slot high-water mark) and the table sizes live on the device so that
     // captured sweeps stay valid.
-    PprBatchState bs{hi_, 0u, 0u, (uint32_t)n};
+    PprBatchState bs{hi_, (uint32_t)z.size(), 0u, (uint32_t)n};
     AG_HIP(hipMemcpyAsync(bstate_.p, &bs, sizeof(bs), hipMemcpyHostToDevice, s));
     AG_HIP(hipGetLastError());
     // host vectors above are temporaries: wait for the copies
@@ -619,6 +1067,58 @@ void PprBatch::release(uint32_t slot)
     tags_[slot] = ~0ull;
     pending_.erase(std::remove(pending_.begin(), pending_.end(), slot), pending_.end());
     stop_.push_back(slot);
+    while (hi_ && !used_[hi_ - 1]) hi_--;
+    dirty_ = true;
+}
+
+bool PprBatch::cancel(uint64_t tag)
+{
+    for (uint32_t k = 0; k < cap_; k++)
+        if (used_[k] && tags_[k] == tag) {
+            release(k);
+            return true;
+        }
+    return false;
+}
+
+void PprBatch::compact()
+{
+    if (!pending_.empty() || used_count_ == 0 || used_count_ == hi_) return;
+    const uint32_t vw = vec_width(fmt_);
+    auto width = [&](uint32_t h) {
+        uint32_t c = vw;
+        while (c < h) c <<= 1;
+        return c;
+    };
+    // only when the packed slots need narrower column groups
+    if (width(used_count_) >= width(hi_)) return;
+    std::vector<uint32_t> src, dst;
+    for (uint32_t k = hi_; k-- > used_count_;)
+        if (used_[k]) src.push_back(k);
+    for (uint32_t k = 0; k < used_count_ && dst.size() < src.size(); k++)
+        if (!used_[k]) dst.push_back(k);
+    std::vector<PprAdmit> mv;
+    for (size_t i = 0; i < src.size(); i++) mv.push_back({src[i], dst[i], 0, 0, 0.0});
+    hipStream_t s = ctx_->stream;
+    AG_HIP(hipMemcpyAsync(desc_.p, mv.data(), mv.size() * sizeof(PprAdmit), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_ppr_move_cols, dim3(ceil_div((uint64_t)N_ * mv.size(), 256)), dim3(256), 0, s,
+                       X_.as<float>(), C_[0].p, C_[1].p, cap_, fmt_, desc_.as<PprAdmit>(),
+                       (uint32_t)mv.size(), N_);
+    hipLaunchKernelGGL(k_ppr_move_state, dim3(1), dim3(kMaxCap), 0, s, cols_.as<PprCol>(),
+                       desc_.as<PprAdmit>(), (uint32_t)mv.size());
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipStreamSynchronize(s)); // mv is a temporary
+    for (size_t i = 0; i < src.size(); i++) {
+        const uint32_t a = src[i], b = dst[i];
+        used_[b] = 1;
+        q_[b] = q_[a];
+        tags_[b] = tags_[a];
+        used_[a] = 0;
+        q_[a] = PprSeedSet();
+        tags_[a] = ~0ull;
+        // b now holds a running column: it must not be stopped by an earlier release
+        stop_.erase(std::remove(stop_.begin(), stop_.end(), b), stop_.end());
+    }
     while (hi_ && !used_[hi_ - 1]) hi_--;
     dirty_ = true;
 }
@@ -655,8 +1155,8 @@ void PprBatch::flush()
         }
         AG_HIP(hipMemcpyAsync(desc_.p, adm.data(), adm.size() * sizeof(PprAdmit),
                               hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_ppr_clear_cols, dim3(ceil_div(N_, 256)), dim3(256), 0, s,
-                           X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, desc_.as<PprAdmit>(),
+        hipLaunchKernelGGL(k_ppr_clear_cols, dim3(ceil_div((uint64_t)N_ * adm.size(), 256)), dim3(256),
+                           0, s, X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, desc_.as<PprAdmit>(),
                            (uint32_t)adm.size(), fmt_);
         hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)adm.size()), dim3(256), 0, s,
                            X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, fmt_,
@@ -673,45 +1173,45 @@ void PprBatch::launch_sweep(bool from0)
 {
     PprArgs a;
     std::memset(&a, 0, sizeof(a));
-    const uint32_t G = 64 / cpw_;
     a.N = N_;
     a.cap = cap_;
-    a.n_hub = p_->n_hub;
     a.n_nonlow = p_->n_nonlow;
     a.n_nonzero = p_->n_nonzero;
-    a.hub_chunk = p_->hub_chunk;
-    a.nb_hub = p_->hub_chunks;
-    a.nmid_units = ceil_div(p_->n_nonlow - p_->n_hub, 4);
-    a.nlow_units = ceil_div(p_->n_nonzero - p_->n_nonlow, 4 * G);
+    a.nch = p_->ppr_nch;
+    a.nmulti = p_->ppr_nmulti;
     a.damp = damp_;
-    a.fmt = fmt_;
     a.roff = p_->roff.as<uint32_t>();
     a.cols = p_->cols.as<uint32_t>();
     a.slice_ptr = p_->slice_ptr.as<uint32_t>();
     a.outdeg = p_->outdeg.as<uint32_t>();
-    a.chunk_row = p_->chunk_row.as<uint32_t>();
-    a.chunk_first = p_->chunk_first.as<uint32_t>();
+    a.ch_row = p_->ppr_ch_row.as<uint32_t>();
+    a.ch_first = p_->ppr_ch_first.as<uint32_t>();
+    a.multi = p_->ppr_multi.as<uint32_t>();
     a.X = X_.as<float>();
     a.Cin = C_[from0 ? 0 : 1].p;
     a.Cout = C_[from0 ? 1 : 0].p;
+    a.part = part_.as<float>();
     a.seed_bits = seed_bits_.as<uint32_t>();
     a.seed_v = seed_v_.as<uint32_t>();
     a.seed_c = seed_c_.as<uint32_t>();
     a.seed_w = seed_w_.as<float>();
+    a.zrows = zrows_.as<uint32_t>();
     a.cs = cols_.as<PprCol>();
     a.bs = bstate_.as<PprBatchState>();
     a.blk_res = blk_res_.as<double>();
     a.blk_dang = blk_dang_.as<double>();
-    a.hub_part = hub_part_.as<float>();
-    a.hub_res = hub_res_.as<double>();
-    a.hub_dang = hub_dang_.as<double>();
-    a.nparts = grid_;
+    a.part_base = 0;
+    const uint32_t lpr = lanes_per_row();
+    int &nb = blocks_[log2u(lpr)];
+    if (!nb) nb = fmt_ == AG_PREC_FP32 ? blocks_per_cu<float>(lpr) : blocks_per_cu<bf16_t>(lpr);
+    const uint32_t g1 = (uint32_t)ctx_->num_cus * (uint32_t)nb;
+    const uint32_t rows_per_block = (64 / lpr / kSlots) * kWaves;
+    const uint32_t g2 = std::max<uint32_t>(1, std::min<uint32_t>(g1, ceil_div(p_->ppr_nmulti, rows_per_block)));
     hipStream_t s = ctx_->stream;
-    if (fmt_ == AG_PREC_FP32) launch_cpw<float>(cpw_, a, grid_, s);
-    else if (fmt_ == AG_PREC_BF16) launch_cpw<bf16_t>(cpw_, a, grid_, s);
-    else launch_cpw<__half>(cpw_, a, grid_, s);
-    if (p_->n_hub) hipLaunchKernelGGL(k_ppr_hub, dim3(p_->n_hub), dim3(kMaxCap), 0, s, a);
-    hipLaunchKernelGGL(k_ppr_reduce, dim3(1), dim3(kRedBlock), 0, s, a);
+    if (fmt_ == AG_PREC_FP32) launch_ct<float>(lpr, a, g1, g2, s);
+    else launch_ct<bf16_t>(lpr, a, g1, g2, s);
+    const uint32_t nparts = g1 + (a.nmulti ? g2 : 0u);
+    hipLaunchKernelGGL(k_ppr_reduce, dim3(1), dim3(kRedBlock), 0, s, a, nparts);
     AG_HIP(hipGetLastError());
 }
 
@@ -727,14 +1227,22 @@ void PprBatch::sweep(uint32_t n, bool replay)
     uint32_t done = 0;
     if (replay) {
         // A captured pair of sweeps (C0 -> C1 -> C0), replayed while the
-        // current buffer is C0.
+        // current buffer is C0; recaptured when the lanes per row change.
         if (cur_ == 1) { launch_sweep(false); cur_ = 0; done++; }
+        const uint32_t lpr = lanes_per_row();
+        if (gexec_ && graph_lpr_ != lpr) {
+            (void)hipGraphExecDestroy(gexec_);
+            (void)hipGraphDestroy(graph_);
+            gexec_ = nullptr;
+            graph_ = nullptr;
+        }
         if (n - done >= 2 && !gexec_) {
             AG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
             launch_sweep(true);
             launch_sweep(false);
             AG_HIP(hipStreamEndCapture(s, &graph_));
             AG_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+            graph_lpr_ = lpr;
         }
         while (n - done >= 2) {
             AG_HIP(hipGraphLaunch(gexec_, s));
@@ -765,48 +1273,8 @@ std::vector<PprBatch::Done> PprBatch::poll()
                           ctx_->stream));
     AG_HIP(hipStreamSynchronize(ctx_->stream));
     for (uint32_t k = 0; k < hi_; k++)
-        if (used_[k] && (cs[k].done || cs[k].stalled))
-            out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].stalled != 0});
+        if (used_[k] && cs[k].done) out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].fine_at});
     return out;
-}
-
-void PprBatch::migrate(const std::vector<uint32_t> &slots, PprBatch &dst)
-{
-    if (slots.empty()) return;
-    if (dst.fmt_ != AG_PREC_FP32) fail(AG_ERR_INVALID, "migration target must store FP32");
-    if (dst.free_slots() < slots.size()) fail(AG_ERR_ADMISSION, "no free FP32 slots");
-    dst.flush();
-    hipStream_t s = ctx_->stream;
-    std::vector<PprAdmit> mv;
-    for (uint32_t c : slots) {
-        if (!used(c)) fail(AG_ERR_INVALID, "slot %u not in use", c);
-        uint32_t d = 0;
-        while (dst.used_[d]) d++;
-        dst.used_[d] = 1;
-        dst.used_count_++;
-        dst.hi_ = std::max(dst.hi_, d + 1);
-        dst.q_[d] = q_[c];
-        dst.tags_[d] = tags_[c];
-        mv.push_back({c, d, 0, 0, 0.0});
-    }
-    AG_HIP(hipMemcpyAsync(desc_.p, mv.data(), mv.size() * sizeof(PprAdmit), hipMemcpyHostToDevice,
-                          s));
-    hipLaunchKernelGGL(k_ppr_migrate_cols, dim3(ceil_div(std::max<uint32_t>(N_, kMaxCap), 256)),
-                       dim3(256), 0, s, X_.as<float>(), cap_, dst.X_.as<float>(),
-                       dst.C_[0].as<float>(), dst.C_[1].as<float>(), dst.cap_,
-                       desc_.as<PprAdmit>(), (uint32_t)mv.size(), p_->outdeg.as<uint32_t>(), N_,
-                       cols_.as<PprCol>(), dst.cols_.as<PprCol>());
-    AG_HIP(hipGetLastError());
-    AG_HIP(hipStreamSynchronize(s)); // mv is a temporary
-    dst.dirty_ = true;
-    for (uint32_t c : slots) {
-        used_[c] = 0;
-        used_count_--;
-        q_[c] = PprSeedSet();
-        tags_[c] = ~0ull;
-    }
-    while (hi_ && !used_[hi_ - 1]) hi_--;
-    dirty_ = true; // stalled columns are already inactive on the device
 }
 
 void PprBatch::extract(uint32_t slot, float *host_out)
@@ -821,53 +1289,22 @@ void PprBatch::extract(uint32_t slot, float *host_out)
     AG_HIP(hipStreamSynchronize(s));
 }
 
-PprPipeline::PprPipeline(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt,
-                         float damp)
-{
-    first_.reset(new PprBatch(ctx, g, cap, cpw, fmt, damp));
-    if (fmt != AG_PREC_FP32) fp32_.reset(new PprBatch(ctx, g, cap, cpw, AG_PREC_FP32, damp));
-}
-
-uint64_t PprPipeline::bytes_for(uint32_t N, uint32_t cap, uint32_t fmt)
-{
-    uint64_t b = PprBatch::bytes_for(N, cap, fmt);
-    if (fmt != AG_PREC_FP32) b += PprBatch::bytes_for(N, cap, AG_PREC_FP32);
-    return b;
-}
-
-void PprPipeline::sweep(uint32_t n, bool replay)
-{
-    if (first_->active()) first_->sweep(n, replay);
-    if (fp32_ && fp32_->active()) fp32_->sweep(n, replay);
-}
-
-bool PprPipeline::cancel(uint64_t tag)
-{
-    for (PprBatch *b : {first_.get(), fp32_.get()}) {
-        if (!b) continue;
-        for (uint32_t k = 0; k < b->capacity(); k++)
-            if (b->used(k) && b->tag(k) == tag) {
-                b->release(k);
-                return true;
-            }
-    }
-    return false;
-}
-
 uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt)
 {
-    if (fmt > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", fmt);
+    if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
+        fail(AG_ERR_INVALID, "batched columns store FP32 or BF16 (precision %u)", fmt);
     if (g->ppr_cpw[fmt]) return g->ppr_cpw[fmt];
     Plan *p = ensure_plan(ctx, g, nullptr);
     const uint32_t N = p->N;
-    PprBatch b(ctx, g, kMaxCap, 32, fmt, 0.85f);
+    PprBatch b(ctx, g, kMaxCap, 64, fmt, 0.85f);
     for (uint32_t q = 0; q < kMaxCap; q++) {
         const uint32_t seed = (uint32_t)(((uint64_t)q * 2654435761u) % N);
         b.admit(b.make_seeds(&seed, 1, 0.0, 1u << 30), q);
     }
-    uint32_t best = 32;
+    uint32_t best = 64;
     double best_ms = 1e30;
     for (uint32_t cpw : {4u, 8u, 16u, 32u, 64u}) {
+        if (cpw < vec_width(fmt)) continue;
         b.set_cpw(cpw);
         b.sweep(1, false);
         b.sweep(2, false);
@@ -915,30 +1352,32 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
         if (opt) o = *opt; else ag_ppr_options_default(&o);
         if (nq == 0) return;
         const uint32_t cpw = o.cols_per_wave ? o.cols_per_wave : ppr_autotune(ctx, g, o.precision);
-        const uint32_t cap = std::min<uint32_t>(kMaxCap, ((nq + 3) / 4) * 4);
-        PprPipeline pl(ctx, g, cap, cpw, o.precision, o.damp);
+        const uint32_t cap = std::min<uint32_t>(kMaxCap, ((nq + 7) / 8) * 8);
+        PprBatch b(ctx, g, cap, cpw, o.precision, o.damp);
         const uint32_t N = g->N;
         std::vector<PprSeedSet> sets(nq);
         for (uint32_t i = 0; i < nq; i++)
-            sets[i] = pl.make_seeds(q[i].seeds, q[i].nseeds, q[i].tol, q[i].max_iter);
+            sets[i] = b.make_seeds(q[i].seeds, q[i].nseeds, q[i].tol, q[i].max_iter);
         AG_HIP(hipStreamSynchronize(ctx->stream));
         const auto t0 = std::chrono::steady_clock::now();
         uint32_t next = 0, finished = 0;
-        while (next < nq && pl.free_slots()) { pl.admit(sets[next], next); next++; }
+        while (next < nq && b.free_slots()) { b.admit(sets[next], next); next++; }
         while (finished < nq) {
-            pl.sweep(4, o.use_graph_replay != 0);
-            auto fin = pl.retire([&](uint64_t tag) { return x ? x + (size_t)tag * N : nullptr; });
+            b.sweep(4, o.use_graph_replay != 0);
+            auto fin = b.retire([&](uint64_t tag) { return x ? x + (size_t)tag * N : nullptr; },
+                                next >= nq);
             for (auto &f : fin) {
                 if (stats) {
                     stats[f.tag].iterations = f.iters;
                     stats[f.tag].residual = f.residual;
+                    stats[f.tag].fine_sweep = f.fine_at;
                     stats[f.tag].status = AG_OK;
                     stats[f.tag].latency_ms = std::chrono::duration<double, std::milli>(
                         std::chrono::steady_clock::now() - t0).count();
                 }
                 finished++;
             }
-            while (next < nq && pl.free_slots()) { pl.admit(sets[next], next); next++; }
+            while (next < nq && b.free_slots()) { b.admit(sets[next], next); next++; }
         }
         AG_HIP(hipStreamSynchronize(ctx->stream));
         if (elapsed_ms)
@@ -1013,6 +1452,7 @@ int ag_ppr_single(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t prec
             stat->iterations = r.iterations;
             stat->status = AG_OK;
             stat->residual = r.error_total;
+            stat->fine_sweep = r.fp32_switch_sweep;
             stat->latency_ms = std::chrono::duration<double, std::milli>(
                 std::chrono::steady_clock::now() - t0).count();
         }

@@ -37,7 +37,7 @@ struct Query {
 
 struct GraphEntry {
     ag_graph *g = nullptr;
-    std::unique_ptr<ag::PprPipeline> batch;
+    std::unique_ptr<ag::PprBatch> batch;
     uint64_t reserved = 0;
     uint32_t cpw = 0;
 };
@@ -60,24 +60,24 @@ using namespace ag;
 
 namespace {
 
-// Creates the graph's batch with the largest capacity (multiple of 4, at most
-// max_columns) whose column state fits the remaining budget.  False: no room.
+// Creates the graph's batch with the largest capacity (multiple of 8, at most
+// max_columns rounded up to 8) whose column state fits the remaining budget.
+// False: no room.
 bool ensure_batch(ag_server *s, GraphEntry &ge)
 {
     if (ge.batch) return true;
-    const uint32_t N = ge.g->N;
-    uint32_t cap = std::min<uint32_t>(64, std::max<uint32_t>(4, s->cfg.max_columns / 4 * 4));
-    while (cap >= 4 && s->reserved + PprPipeline::bytes_for(N, cap, s->cfg.precision) >
+    uint32_t cap = std::min<uint32_t>(64, std::max<uint32_t>(8, (s->cfg.max_columns + 7) / 8 * 8));
+    while (cap >= 8 && s->reserved + PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision) >
                            s->cfg.hbm_budget_bytes)
-        cap -= 4;
-    if (cap < 4) return false;
+        cap -= 8;
+    if (cap < 8) return false;
     s->alloc_attempts++;
     if (s->cfg.inject_fail_alloc && s->alloc_attempts == s->cfg.inject_fail_alloc)
         return false; // injected allocation failure: the query stays queued
     if (!ge.cpw) ge.cpw = s->cfg.cols_per_wave ? s->cfg.cols_per_wave
                                                : ppr_autotune(s->ctx, ge.g, s->cfg.precision);
-    ge.batch.reset(new PprPipeline(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp));
-    ge.reserved = PprPipeline::bytes_for(N, cap, s->cfg.precision);
+    ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp));
+    ge.reserved = PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision);
     s->reserved += ge.reserved;
     return true;
 }
@@ -118,25 +118,6 @@ void admit(ag_server *s)
     }
 }
 
-void retire(ag_server *s, uint32_t gi)
-{
-    GraphEntry &ge = s->graphs[gi];
-    if (!ge.batch) return;
-    auto fin = ge.batch->retire([&](uint64_t tag) -> float * {
-        auto qi = s->queries.find(tag);
-        if (qi == s->queries.end() || qi->second.state != QState::Running) return nullptr;
-        qi->second.result.resize(ge.g->N);
-        return qi->second.result.data();
-    });
-    for (auto &f : fin) {
-        auto qi = s->queries.find(f.tag);
-        if (qi == s->queries.end() || qi->second.state != QState::Running) continue;
-        qi->second.stat.iterations = f.iters;
-        qi->second.stat.residual = f.residual;
-        finish(qi->second, QState::Done, AG_OK);
-    }
-}
-
 bool graph_has_queued(ag_server *s, uint32_t gi)
 {
     for (uint64_t t : s->queue) {
@@ -146,6 +127,29 @@ bool graph_has_queued(ag_server *s, uint32_t gi)
     }
     return false;
 }
+
+void retire(ag_server *s, uint32_t gi)
+{
+    GraphEntry &ge = s->graphs[gi];
+    if (!ge.batch) return;
+    auto fin = ge.batch->retire(
+        [&](uint64_t tag) -> float * {
+            auto qi = s->queries.find(tag);
+            if (qi == s->queries.end() || qi->second.state != QState::Running) return nullptr;
+            qi->second.result.resize(ge.g->N);
+            return qi->second.result.data();
+        },
+        !graph_has_queued(s, gi));
+    for (auto &f : fin) {
+        auto qi = s->queries.find(f.tag);
+        if (qi == s->queries.end() || qi->second.state != QState::Running) continue;
+        qi->second.stat.iterations = f.iters;
+        qi->second.stat.residual = f.residual;
+        qi->second.stat.fine_sweep = f.fine_at;
+        finish(qi->second, QState::Done, AG_OK);
+    }
+}
+
 
 } // namespace
 
@@ -172,7 +176,8 @@ int ag_serve_create(ag_ctx *ctx, const ag_serve_config *cfg, ag_server **out)
         if (!s->cfg.hbm_budget_bytes) s->cfg.hbm_budget_bytes = 8ull << 30;
         if (!s->cfg.max_columns) s->cfg.max_columns = 64;
         if (s->cfg.damp == 0.0f) s->cfg.damp = 0.85f;
-        if (s->cfg.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision");
+        if (s->cfg.precision != AG_PREC_FP32 && s->cfg.precision != AG_PREC_BF16)
+            fail(AG_ERR_INVALID, "served columns store FP32 or BF16 (precision %u)", s->cfg.precision);
         *out = s.release();
     });
 }

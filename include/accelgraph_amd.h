@@ -199,19 +199,27 @@ typedef struct ag_ppr_stat
     uint32_t status;       /* AG_OK or AG_ERR_* */
     double residual;
     double latency_ms;     /* serving: submit -> done */
+    uint32_t fine_sweep;   /* 16-bit storage: sweeps done before the column read full
+                              FP32 contributions (0: FP32 throughout) */
+    uint32_t reserved;
 } ag_ppr_stat;
 
 typedef struct ag_ppr_options
 {
-    uint32_t precision;       /* AG_PREC_* for stored columns */
+    uint32_t precision;       /* AG_PREC_FP32, or AG_PREC_BF16: each FP32 contribution
+                                 split into a high (BF16) and a low 16-bit plane; a
+                                 column reads the high plane alone until its residual
+                                 stalls, then both (exact FP32) until it converges */
     uint32_t cols_per_wave;   /* 0: autotuned from {4, 8, 16, 32, 64} */
     float damp;               /* 0.85f */
     uint32_t use_graph_replay;/* 1: capture sweeps in a hipGraph and replay */
 } ag_ppr_options;
 void ag_ppr_options_default(ag_ppr_options *opt);
 
-/* Runs nq queries as multi-right-hand-side columns with per-column retirement.
- * x (host, nq * N floats, row q = query q) may be NULL; stats (nq) may be NULL. */
+/* Runs nq queries as multi-right-hand-side columns with per-column retirement
+ * (at most 64 columns at a time; later queries take the slots that free up).
+ * x (host, nq * N floats, row q = query q) may be NULL; stats (nq) may be NULL.
+ * AG_PREC_FP16 is refused here (AG_ERR_INVALID); ag_ppr_single accepts it. */
 int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
                  const ag_ppr_options *opt, float *x, ag_ppr_stat *stats, double *elapsed_ms);
 
@@ -221,7 +229,8 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
 int ag_ppr_single(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t precision, float *x,
                   ag_ppr_stat *stat);
 
-/* Chooses cols_per_wave for g by timing {4, 8, 16, 32, 64}; result cached per graph. */
+/* Chooses cols_per_wave for g by timing {4, 8, 16, 32, 64} (BF16: from 8) on a
+ * full 64-column batch; result cached per graph and precision. */
 int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cols_per_wave);
 
 /* ---- serving loop -------------------------------------------------------- */
@@ -231,7 +240,7 @@ typedef struct ag_serve_config
 {
     uint64_t hbm_budget_bytes; /* admission limit for column state (0: 8 GiB) */
     uint32_t max_columns;      /* per-graph column slots in flight (0: 64) */
-    uint32_t precision;        /* AG_PREC_* */
+    uint32_t precision;        /* AG_PREC_FP32 or AG_PREC_BF16 (as ag_ppr_options) */
     uint32_t cols_per_wave;    /* 0: autotune */
     uint32_t inject_fail_alloc;/* test hook: refuse the n-th admission (0: off) */
     uint32_t inject_fail_query;/* test hook: fail ticket n (0: off) */
