@@ -35,6 +35,7 @@ AG_PREC_FP32 = 0
 AG_PREC_BF16 = 1
 AG_PREC_FP16 = 2
 AG_PR_FLAG_FINISH_FP32 = 0x1
+AG_PR_FLAG_TIME_SWEEPS = 0x2
 
 u32p = C.POINTER(C.c_uint32)
 f32p = C.POINTER(C.c_float)
@@ -58,7 +59,8 @@ class PrResult(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("sweeps", C.c_uint32),
                 ("active_vertices", C.c_uint32), ("fp32_switch_sweep", C.c_uint32),
                 ("error_total", C.c_double), ("pr_sum", C.c_double), ("loop_ms", C.c_double),
-                ("total_ms", C.c_double), ("plan_ms", C.c_double)]
+                ("total_ms", C.c_double), ("plan_ms", C.c_double),
+                ("sweep_kernel_ms", C.c_double)]
 
 
 class PlanInfo(C.Structure):

@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "ag_plan.h"
 
@@ -47,6 +48,14 @@ constexpr int kZeroRowsPerUnit = kSweepBlock * 4;
 constexpr uint32_t kLdsBytes = 160 * 1024;   // LDS per CU on gfx950
 constexpr uint32_t kStaticLds = 2048;        // static shared arrays of the sweep
 constexpr uint32_t kFmtF32 = 0, kFmtBF16 = 1; // 2 = FP16
+#ifndef AG_PR_DEPTH
+#define AG_PR_DEPTH 8
+#endif
+#ifndef AG_PR_SELL
+#define AG_PR_SELL 8
+#endif
+constexpr int kDepth = AG_PR_DEPTH; // loads in flight per lane, CSR rows (power of two)
+constexpr int kSell = AG_PR_SELL;   // loads in flight per lane, SELL rows
 
 } // namespace
 
@@ -214,7 +223,7 @@ __device__ __forceinline__ float gather(const CT *__restrict__ g, uint32_t u, ui
 }
 
 // Rows [.., row_end) with G lanes per row (G = 16, 32 or 64): lanes of a
-// group stride over the row's edges, four loads in flight each, then a
+// group stride over the row's edges, kDepth loads in flight each, then a
 // shuffle reduction inside the group and one lane updates the vertex.
 template <int G, typename CT>
 __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uint32_t row_end,
@@ -224,29 +233,34 @@ __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uin
 {
     const int gl = lane & (G - 1);
     const bool valid = row < row_end;
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    float sp[kDepth];
+#pragma unroll
+    for (int t = 0; t < kDepth; t++) sp[t] = 0.f;
     if (valid) {
         const uint32_t rb = a.roff[row], re = a.roff[row + 1];
         // indices of the next batch are in flight while the current batch gathers
         uint32_t j = rb + gl;
-        uint32_t c0 = j < re ? ldnt(a.cols + j) : 0u;
-        uint32_t c1 = j + G < re ? ldnt(a.cols + j + G) : 0u;
-        uint32_t c2 = j + 2 * G < re ? ldnt(a.cols + j + 2 * G) : 0u;
-        uint32_t c3 = j + 3 * G < re ? ldnt(a.cols + j + 3 * G) : 0u;
-        for (; j < re; j += 4 * G) {
-            const uint32_t jn = j + 4 * G;
-            const uint32_t n0 = jn < re ? ldnt(a.cols + jn) : 0u;
-            const uint32_t n1 = jn + G < re ? ldnt(a.cols + jn + G) : 0u;
-            const uint32_t n2 = jn + 2 * G < re ? ldnt(a.cols + jn + 2 * G) : 0u;
-            const uint32_t n3 = jn + 3 * G < re ? ldnt(a.cols + jn + 3 * G) : 0u;
-            s0 += gather(cin, c0, hot);
-            if (j + G < re) s1 += gather(cin, c1, hot);
-            if (j + 2 * G < re) s2 += gather(cin, c2, hot);
-            if (j + 3 * G < re) s3 += gather(cin, c3, hot);
-            c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        uint32_t c[kDepth];
+#pragma unroll
+        for (int t = 0; t < kDepth; t++) c[t] = j + t * G < re ? ldnt(a.cols + j + t * G) : 0u;
+        for (; j < re; j += kDepth * G) {
+            const uint32_t jn = j + kDepth * G;
+            uint32_t n[kDepth];
+#pragma unroll
+            for (int t = 0; t < kDepth; t++) n[t] = jn + t * G < re ? ldnt(a.cols + jn + t * G) : 0u;
+#pragma unroll
+            for (int t = 0; t < kDepth; t++)
+                if (j + t * G < re) sp[t] += gather(cin, c[t], hot);
+#pragma unroll
+            for (int t = 0; t < kDepth; t++) c[t] = n[t];
         }
     }
-    float s = (s0 + s1) + (s2 + s3);
+    // pairwise over the partial sums: ((s0 + s1) + (s2 + s3)) + ...
+#pragma unroll
+    for (int w = 1; w < kDepth; w <<= 1)
+#pragma unroll
+        for (int t = 0; t + w < kDepth; t += 2 * w) sp[t] += sp[t + w];
+    float s = sp[0];
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if (valid && gl == 0) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang);
@@ -272,25 +286,29 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                 const uint32_t k = c - a.chunk_first[row];
                 const uint32_t b0 = a.roff[row] + k * a.hub_chunk;
                 const uint32_t b1 = min(a.roff[row + 1], b0 + a.hub_chunk);
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+                float sp[kDepth];
+#pragma unroll
+                for (int t = 0; t < kDepth; t++) sp[t] = 0.f;
                 uint32_t j = b0 + lane;
-                uint32_t c0 = j < b1 ? ldnt(a.cols + j) : 0u;
-                uint32_t c1 = j + 64 < b1 ? ldnt(a.cols + j + 64) : 0u;
-                uint32_t c2 = j + 128 < b1 ? ldnt(a.cols + j + 128) : 0u;
-                uint32_t c3 = j + 192 < b1 ? ldnt(a.cols + j + 192) : 0u;
-                for (; j < b1; j += 256) {
-                    const uint32_t jn = j + 256;
-                    const uint32_t n0 = jn < b1 ? ldnt(a.cols + jn) : 0u;
-                    const uint32_t n1 = jn + 64 < b1 ? ldnt(a.cols + jn + 64) : 0u;
-                    const uint32_t n2 = jn + 128 < b1 ? ldnt(a.cols + jn + 128) : 0u;
-                    const uint32_t n3 = jn + 192 < b1 ? ldnt(a.cols + jn + 192) : 0u;
-                    s0 += gather(cin, c0, hot);
-                    if (j + 64 < b1) s1 += gather(cin, c1, hot);
-                    if (j + 128 < b1) s2 += gather(cin, c2, hot);
-                    if (j + 192 < b1) s3 += gather(cin, c3, hot);
-                    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+                uint32_t ci[kDepth];
+#pragma unroll
+                for (int t = 0; t < kDepth; t++) ci[t] = j + t * 64 < b1 ? ldnt(a.cols + j + t * 64) : 0u;
+                for (; j < b1; j += kDepth * 64) {
+                    const uint32_t jn = j + kDepth * 64;
+                    uint32_t n[kDepth];
+#pragma unroll
+                    for (int t = 0; t < kDepth; t++) n[t] = jn + t * 64 < b1 ? ldnt(a.cols + jn + t * 64) : 0u;
+#pragma unroll
+                    for (int t = 0; t < kDepth; t++)
+                        if (j + t * 64 < b1) sp[t] += gather(cin, ci[t], hot);
+#pragma unroll
+                    for (int t = 0; t < kDepth; t++) ci[t] = n[t];
                 }
-                const float s = wave_sum((s0 + s1) + (s2 + s3));
+#pragma unroll
+                for (int w2 = 1; w2 < kDepth; w2 <<= 1)
+#pragma unroll
+                    for (int t = 0; t + w2 < kDepth; t += 2 * w2) sp[t] += sp[t + w2];
+                const float s = wave_sum(sp[0]);
                 // chunk partial; k_pr_finalize sums a row's chunks in chunk order
                 if (lane == 0) a.hub_partial[c] = s;
             }
@@ -314,24 +332,24 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                 const uint32_t width = __shfl(deg, 0, 64); // rows sorted: lane 0 is the widest
                 const uint32_t *p = a.cols + a.slice_ptr[sl] + lane;
                 float s = 0.0f;
-                // batch of 8 indices in flight while the previous batch gathers;
-                // each lane adds in the reference's edge order
-                uint32_t u[8];
+                // batch of kSell indices in flight while the previous batch
+                // gathers; each lane adds in the reference's edge order
+                uint32_t u[kSell];
 #pragma unroll
-                for (int t = 0; t < 8; t++) u[t] = (uint32_t)t < deg ? ldnt(p + t * 64) : 0u;
-                for (uint32_t k = 0; k < width; k += 8) {
-                    uint32_t nx[8];
+                for (int t = 0; t < kSell; t++) u[t] = (uint32_t)t < deg ? ldnt(p + t * 64) : 0u;
+                for (uint32_t k = 0; k < width; k += kSell) {
+                    uint32_t nx[kSell];
 #pragma unroll
-                    for (int t = 0; t < 8; t++)
-                        nx[t] = k + 8 + t < deg ? ldnt(p + (k + 8 + t) * 64) : 0u;
-                    float x[8];
+                    for (int t = 0; t < kSell; t++)
+                        nx[t] = k + kSell + t < deg ? ldnt(p + (k + kSell + t) * 64) : 0u;
+                    float x[kSell];
 #pragma unroll
-                    for (int t = 0; t < 8; t++) x[t] = k + t < deg ? gather(cin, u[t], hot) : 0.f;
+                    for (int t = 0; t < kSell; t++) x[t] = k + t < deg ? gather(cin, u[t], hot) : 0.f;
 #pragma unroll
-                    for (int t = 0; t < 8; t++)
+                    for (int t = 0; t < kSell; t++)
                         if (k + t < deg) s += x[t];
 #pragma unroll
-                    for (int t = 0; t < 8; t++) u[t] = nx[t];
+                    for (int t = 0; t < kSell; t++) u[t] = nx[t];
                 }
                 if (valid) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang);
             }
@@ -908,11 +926,17 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     AG_HIP(hipGetLastError());
     AG_HIP(hipEventRecord(ev[1], s));
     const uint32_t grid = p->work.grid;
+    // optional per-sweep events around the sweep kernels (finalize excluded)
+    const bool timed = (o.flags & AG_PR_FLAG_TIME_SWEEPS) != 0;
+    std::vector<hipEvent_t> sev(timed ? 2 * (size_t)max_iter : 0);
+    for (auto &e : sev) AG_HIP(hipEventCreate(&e));
+    struct SevGuard { std::vector<hipEvent_t> &v; ~SevGuard() { for (auto e : v) (void)hipEventDestroy(e); } } sg{sev};
     const bool can_stop = o.epsilon > 0.0;
     const uint32_t group = can_stop ? 16u : max_iter;
     for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
         const uint32_t it1 = std::min(max_iter, it0 + group);
         for (uint32_t it = it0; it < it1; it++) {
+            if (timed) AG_HIP(hipEventRecord(sev[2 * it], s));
             if (o.precision == AG_PREC_BF16)
                 hipLaunchKernelGGL(k_pr_sweep<bf16_t>, dim3(grid), dim3(kSweepBlock),
                                    p->hot_bytes, s, a, it);
@@ -922,6 +946,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
             if (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32))
                 hipLaunchKernelGGL(k_pr_sweep<float>, dim3(grid), dim3(kSweepBlock),
                                    p->hot_bytes, s, a, it);
+            if (timed) AG_HIP(hipEventRecord(sev[2 * it + 1], s));
             hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
         }
         AG_HIP(hipGetLastError());
@@ -957,6 +982,11 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     r.loop_ms = ms;
     AG_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
     r.total_ms = ms;
+    r.sweep_kernel_ms = 0.0;
+    for (uint32_t it = 0; timed && it < st.sweeps && it < max_iter; it++) {
+        AG_HIP(hipEventElapsedTime(&ms, sev[2 * it], sev[2 * it + 1]));
+        r.sweep_kernel_ms += ms;
+    }
     r.iterations = st.iterations;
     r.sweeps = st.sweeps;
     r.active_vertices = st.last_active;

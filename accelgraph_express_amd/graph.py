@@ -212,14 +212,18 @@ class Graph:
             active_per_iter=act[:r.sweeps].copy(), error_per_iter=err[:r.sweeps].copy())
 
     def pagerank_pull_device(self, d_ranks_ptr: int, iterations: int = 20, epsilon: float = 0.0,
-                             mode="reference", precision="fp32", finish_fp32=False) -> L.PrResult:
+                             mode="reference", precision="fp32", finish_fp32=False,
+                             time_sweeps: bool = False) -> L.PrResult:
+        """Ranks to a device buffer of N floats; time_sweeps fills
+        sweep_kernel_ms (HIP events around each sweep kernel)."""
         lib = L.load()
         o = L.PrOptions()
         lib.ag_pr_options_default(C.byref(o))
         o.iterations, o.epsilon = iterations, epsilon
         o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
         o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
-        o.flags = L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0
+        o.flags = (L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0) | \
+                  (L.AG_PR_FLAG_TIME_SWEEPS if time_sweeps else 0)
         r = L.PrResult()
         L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))

@@ -120,6 +120,7 @@ int ag_vertex_map_host(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uin
 #define AG_PREC_FP16 2u    /* stored FP16 (RNE), FP32 accumulate */
 
 #define AG_PR_FLAG_FINISH_FP32 0x1u /* low precision: switch to FP32 storage when the residual stalls */
+#define AG_PR_FLAG_TIME_SWEEPS 0x2u /* time the sweep kernels alone (HIP events around each) */
 
 typedef struct ag_pr_options
 {
@@ -146,6 +147,8 @@ typedef struct ag_pr_result
     double loop_ms;           /* device time of the sweep loop (HIP events) */
     double total_ms;          /* device time init + loop + normalise */
     double plan_ms;           /* gather-layout build time (0 if cached) */
+    double sweep_kernel_ms;   /* AG_PR_FLAG_TIME_SWEEPS: device time of the sweep kernels
+                                 alone, summed over sweeps (finalize launches excluded) */
 } ag_pr_result;
 
 /* Runs PageRank on a resident graph.  ranks (host, N floats) may be NULL;
