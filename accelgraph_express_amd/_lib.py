@@ -147,6 +147,8 @@ _SIGS = {
     "ag_serve_cancel": (C.c_int, [vp, C.c_uint64]),
     "ag_serve_step": (C.c_int, [vp, u32p]),
     "ag_serve_poll": (C.c_int, [vp, C.c_uint64, f32p, C.c_uint32, C.POINTER(PprStat)]),
+    "ag_serve_peek": (C.c_int, [vp, C.c_uint64, C.POINTER(f32p), u32p, C.POINTER(PprStat)]),
+    "ag_serve_release": (C.c_int, [vp, C.c_uint64]),
     "ag_serve_reserved_bytes": (C.c_uint64, [vp]),
     "ag_gen_uniform": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p]),
     "ag_gen_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_double,

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <mutex>
+#include <vector>
 
 #include "ag_internal.h"
 
@@ -36,6 +37,7 @@ struct Plan {
     DevBuf ppr_ch_row, ppr_ch_first, ppr_multi;
     uint32_t ppr_nch = 0, ppr_nmulti = 0;
     bool ppr_ready = false;
+    std::vector<uint32_t> rank_host; // host copy of rank (seed ids -> labels), on first use
     std::mutex mu;
     uint64_t bytes() const
     {

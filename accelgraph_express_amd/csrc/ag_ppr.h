@@ -69,7 +69,8 @@ class PprBatch {
     // next sweep (all admissions of a step in one launch).  Returns the slot.
     uint32_t admit(const PprSeedSet &q, uint64_t tag);
     // Runs `n` sweeps over all active columns (graph replay when enabled).
-    void sweep(uint32_t n, bool replay);
+    // With `timed` it waits for them and records the mean sweep time.
+    void sweep(uint32_t n, bool replay, bool timed = true);
     struct Done {
         uint32_t slot;
         uint64_t tag;
@@ -81,6 +82,10 @@ class PprBatch {
     std::vector<Done> poll();
     // Copies a finished column (caller's vertex order) to host memory.
     void extract(uint32_t slot, float *host_out);
+    // Writes a finished column in vertex order to `dst`, memory the device can
+    // write (device memory or mapped pinned host memory); asynchronous, in
+    // stream order before any later reuse of the slot.
+    void extract_to(uint32_t slot, float *dst);
     // Frees a slot (a running column is stopped first).
     void release(uint32_t slot);
     // Moves running columns from the top slots into free lower ones when that
@@ -97,10 +102,12 @@ class PprBatch {
         double residual;
         uint32_t fine_at;
     };
-    // Retires finished columns; `out(tag)` names the host buffer for a
-    // column's result (nullptr: leave it on the device).  With `may_compact`
-    // the remaining columns are packed afterwards.
-    template <typename F> std::vector<Finished> retire(F out, bool may_compact);
+    // Retires finished columns; `out(tag)` names the buffer for a column's
+    // result (nullptr: none): host memory filled synchronously, or with
+    // `device_out` device-writable memory filled asynchronously.  With
+    // `may_compact` the remaining columns are packed afterwards.
+    template <typename F>
+    std::vector<Finished> retire(F out, bool may_compact, bool device_out = false);
 
   private:
     void flush();
@@ -129,11 +136,16 @@ class PprBatch {
 };
 
 template <typename F>
-std::vector<PprBatch::Finished> PprBatch::retire(F out, bool may_compact)
+std::vector<PprBatch::Finished> PprBatch::retire(F out, bool may_compact, bool device_out)
 {
     std::vector<Finished> fin;
     for (auto &d : poll()) {
-        extract(d.slot, out(d.tag));
+        float *dst = out(d.tag);
+        if (device_out) {
+            if (dst) extract_to(d.slot, dst);
+        } else {
+            extract(d.slot, dst);
+        }
         fin.push_back({d.tag, d.iters, d.residual, d.fine_at});
         release(d.slot);
     }
@@ -143,5 +155,7 @@ std::vector<PprBatch::Finished> PprBatch::retire(F out, bool may_compact)
 
 uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt);
 void ensure_ppr_chunks(ag_ctx *ctx, Plan *p);
+// Host copy of the plan's rank array (vertex id -> label), made on first use.
+const std::vector<uint32_t> &rank_host(ag_ctx *ctx, Plan *p);
 
 } // namespace ag

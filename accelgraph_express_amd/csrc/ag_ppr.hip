@@ -768,6 +768,15 @@ __global__ void k_ppr_extract(const float *X, const uint32_t *order, uint32_t N,
     out[order[i]] = X[(uint64_t)i * cap + c];
 }
 
+// Vertex-order writer (coalesced writes, suited to host memory over the bus).
+__global__ void k_ppr_extract_vo(const float *X, const uint32_t *rank, uint32_t N, uint32_t cap,
+                                 uint32_t c, float *out)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    out[v] = X[(uint64_t)rank[v] * cap + c];
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t v)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -883,6 +892,18 @@ void ensure_ppr_chunks(ag_ctx *ctx, Plan *p)
     p->ppr_ready = true;
 }
 
+const std::vector<uint32_t> &rank_host(ag_ctx *ctx, Plan *p)
+{
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->rank_host.size() != p->N) {
+        std::vector<uint32_t> r(p->N);
+        AG_HIP(hipMemcpyAsync(r.data(), p->rank.p, (size_t)p->N * 4, hipMemcpyDeviceToHost, ctx->stream));
+        AG_HIP(hipStreamSynchronize(ctx->stream));
+        p->rank_host.swap(r);
+    }
+    return p->rank_host;
+}
+
 uint64_t PprBatch::bytes_for(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t fmt)
 {
     if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
@@ -972,22 +993,13 @@ PprSeedSet PprBatch::make_seeds(const uint32_t *seeds, uint32_t n, double tol, u
         if (seeds[i] >= N_) fail(AG_ERR_INVALID, "seed vertex %u >= num_vertices %u", seeds[i], N_);
         cnt[seeds[i]]++;
     }
-    std::vector<uint32_t> ids;
     PprSeedSet q;
+    // vertex ids -> labels through a host copy of the plan's rank array
+    const std::vector<uint32_t> &rank = rank_host(ctx_, p_);
     for (auto &kv : cnt) {
-        ids.push_back(kv.first);
+        q.labels.push_back(rank[kv.first]);
         q.w.push_back((float)((double)kv.second / (double)n));
     }
-    // vertex ids -> labels (device map through the plan's rank array)
-    DevBuf din(ids.size() * 4), dout(ids.size() * 4);
-    q.labels.resize(ids.size());
-    AG_HIP(hipMemcpyAsync(din.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, ctx_->stream));
-    hipLaunchKernelGGL(k_map_labels, dim3(ceil_div(ids.size(), 256)), dim3(256), 0, ctx_->stream,
-                       p_->rank.as<uint32_t>(), din.as<uint32_t>(), (uint32_t)ids.size(),
-                       dout.as<uint32_t>());
-    AG_HIP(hipMemcpyAsync(q.labels.data(), dout.p, ids.size() * 4, hipMemcpyDeviceToHost,
-                          ctx_->stream));
-    AG_HIP(hipStreamSynchronize(ctx_->stream));
     q.tol = tol;
     q.max_iter = max_iter ? max_iter : 1000;
     return q;
@@ -1215,15 +1227,17 @@ void PprBatch::launch_sweep(bool from0)
     AG_HIP(hipGetLastError());
 }
 
-void PprBatch::sweep(uint32_t n, bool replay)
+void PprBatch::sweep(uint32_t n, bool replay, bool timed)
 {
     flush();
     if (used_count_ == 0 || n == 0) return;
     hipStream_t s = ctx_->stream;
-    hipEvent_t e0, e1;
-    AG_HIP(hipEventCreate(&e0));
-    AG_HIP(hipEventCreate(&e1));
-    AG_HIP(hipEventRecord(e0, s));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (timed) {
+        AG_HIP(hipEventCreate(&e0));
+        AG_HIP(hipEventCreate(&e1));
+        AG_HIP(hipEventRecord(e0, s));
+    }
     uint32_t done = 0;
     if (replay) {
         // A captured pair of sweeps (C0 -> C1 -> C0), replayed while the
@@ -1254,6 +1268,7 @@ void PprBatch::sweep(uint32_t n, bool replay)
         cur_ ^= 1;
         done++;
     }
+    if (!timed) return;
     AG_HIP(hipEventRecord(e1, s));
     AG_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1275,6 +1290,15 @@ std::vector<PprBatch::Done> PprBatch::poll()
     for (uint32_t k = 0; k < hi_; k++)
         if (used_[k] && cs[k].done) out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].fine_at});
     return out;
+}
+
+void PprBatch::extract_to(uint32_t slot, float *dst)
+{
+    if (!used(slot)) fail(AG_ERR_INVALID, "slot %u not in use", slot);
+    if (!dst) return;
+    hipLaunchKernelGGL(k_ppr_extract_vo, dim3(ceil_div(N_, 256)), dim3(256), 0, ctx_->stream,
+                       X_.as<float>(), p_->rank.as<uint32_t>(), N_, cap_, slot, dst);
+    AG_HIP(hipGetLastError());
 }
 
 void PprBatch::extract(uint32_t slot, float *host_out)
@@ -1363,7 +1387,7 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
         uint32_t next = 0, finished = 0;
         while (next < nq && b.free_slots()) { b.admit(sets[next], next); next++; }
         while (finished < nq) {
-            b.sweep(4, o.use_graph_replay != 0);
+            b.sweep(4, o.use_graph_replay != 0, false);
             auto fin = b.retire([&](uint64_t tag) { return x ? x + (size_t)tag * N : nullptr; },
                                 next >= nq);
             for (auto &f : fin) {

@@ -2,10 +2,11 @@
 //
 // Continuous batching: every graph with work owns one PprBatch (column slots
 // in HBM).  ag_serve_step admits queued queries into free slots (FIFO, by HBM
-// budget: a batch's column state is reserved when the batch is created and
-// released when it drains), runs sweeps for every graph with active columns,
-// and retires converged columns immediately, so queries that arrive while
-// others are running join the next sweep.  Host code only; the sweeps are the
+// budget: a batch's column state is reserved when the batch is created; a
+// drained batch stays resident until another graph needs its memory), runs
+// sweeps for every graph with active columns, and retires converged columns
+// immediately, so queries that arrive while others are running join the next
+// sweep.  Host code only; the sweeps are the
 // kernels of ag_ppr.hip.  The reference has no serving path.
 #include <algorithm>
 #include <chrono>
@@ -22,6 +23,15 @@ using Clock = std::chrono::steady_clock;
 
 enum class QState { Queued, Running, Done, Failed, Cancelled };
 
+// A finished query's result: pinned, device-mapped host memory that the GPU
+// fills in vertex order; `ready` is recorded after the writing kernel.
+struct ResultBuf {
+    float *host = nullptr;
+    float *dev = nullptr;
+    uint32_t n = 0;
+    hipEvent_t ready = nullptr;
+};
+
 struct Query {
     uint64_t ticket = 0;
     uint32_t graph = 0;
@@ -30,7 +40,7 @@ struct Query {
     uint32_t max_iter = 1000;
     QState state = QState::Queued;
     int code = AG_OK;
-    std::vector<float> result;
+    ResultBuf *res = nullptr;
     ag_ppr_stat stat{};
     Clock::time_point t_submit;
 };
@@ -53,23 +63,53 @@ struct ag_server {
     uint64_t next_ticket = 1;
     uint64_t reserved = 0;
     uint32_t alloc_attempts = 0;
+    // result buffers: all of them, and the free ones by length
+    std::vector<std::unique_ptr<ResultBuf>> bufs;
+    std::unordered_map<uint32_t, std::vector<ResultBuf *>> free_bufs;
     std::mutex mu;
+    ~ag_server()
+    {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        for (auto &b : bufs) {
+            if (b->ready) (void)hipEventDestroy(b->ready);
+            if (b->host) (void)hipHostFree(b->host);
+        }
+    }
 };
 
 using namespace ag;
 
 namespace {
 
+void release_batch(ag_server *s, GraphEntry &ge)
+{
+    ge.batch.reset();
+    s->reserved -= ge.reserved;
+    ge.reserved = 0;
+}
+
 // Creates the graph's batch with the largest capacity (multiple of 8, at most
-// max_columns rounded up to 8) whose column state fits the remaining budget.
-// False: no room.
+// max_columns rounded up to 8) whose column state fits the budget.  Drained
+// batches of other graphs stay resident until their memory is needed: they
+// are evicted (largest first) before the capacity is reduced.  False: no room.
 bool ensure_batch(ag_server *s, GraphEntry &ge)
 {
     if (ge.batch) return true;
-    uint32_t cap = std::min<uint32_t>(64, std::max<uint32_t>(8, (s->cfg.max_columns + 7) / 8 * 8));
-    while (cap >= 8 && s->reserved + PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision) >
-                           s->cfg.hbm_budget_bytes)
-        cap -= 8;
+    const uint32_t want = std::min<uint32_t>(64, std::max<uint32_t>(8, (s->cfg.max_columns + 7) / 8 * 8));
+    auto fits = [&](uint32_t cap) {
+        return s->reserved + PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision) <=
+               s->cfg.hbm_budget_bytes;
+    };
+    while (!fits(want)) {
+        GraphEntry *idle = nullptr;
+        for (auto &o : s->graphs)
+            if (&o != &ge && o.batch && o.batch->active() == 0 && (!idle || o.reserved > idle->reserved))
+                idle = &o;
+        if (!idle) break;
+        release_batch(s, *idle);
+    }
+    uint32_t cap = want;
+    while (cap >= 8 && !fits(cap)) cap -= 8;
     if (cap < 8) return false;
     s->alloc_attempts++;
     if (s->cfg.inject_fail_alloc && s->alloc_attempts == s->cfg.inject_fail_alloc)
@@ -82,11 +122,27 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
     return true;
 }
 
-void release_batch(ag_server *s, GraphEntry &ge)
+ResultBuf *take_buf(ag_server *s, uint32_t n)
 {
-    ge.batch.reset();
-    s->reserved -= ge.reserved;
-    ge.reserved = 0;
+    auto &fl = s->free_bufs[n];
+    if (!fl.empty()) {
+        ResultBuf *b = fl.back();
+        fl.pop_back();
+        return b;
+    }
+    std::unique_ptr<ResultBuf> b(new ResultBuf());
+    b->n = n;
+    if (hipHostMalloc((void **)&b->host, (size_t)n * 4, hipHostMallocMapped) != hipSuccess)
+        fail(AG_ERR_OOM, "pinned result buffer of %u floats", n);
+    AG_HIP(hipHostGetDevicePointer((void **)&b->dev, b->host, 0));
+    AG_HIP(hipEventCreateWithFlags(&b->ready, hipEventDisableTiming));
+    s->bufs.push_back(std::move(b));
+    return s->bufs.back().get();
+}
+
+void give_buf(ag_server *s, ResultBuf *b)
+{
+    if (b) s->free_bufs[b->n].push_back(b);
 }
 
 void finish(Query &q, QState st, int code)
@@ -132,17 +188,20 @@ void retire(ag_server *s, uint32_t gi)
 {
     GraphEntry &ge = s->graphs[gi];
     if (!ge.batch) return;
+    // results are written by the GPU straight into pinned host buffers, in
+    // stream order, without waiting; poll / peek wait on the buffer's event
     auto fin = ge.batch->retire(
         [&](uint64_t tag) -> float * {
             auto qi = s->queries.find(tag);
             if (qi == s->queries.end() || qi->second.state != QState::Running) return nullptr;
-            qi->second.result.resize(ge.g->N);
-            return qi->second.result.data();
+            qi->second.res = take_buf(s, ge.g->N);
+            return qi->second.res->dev;
         },
-        !graph_has_queued(s, gi));
+        !graph_has_queued(s, gi), true);
     for (auto &f : fin) {
         auto qi = s->queries.find(f.tag);
         if (qi == s->queries.end() || qi->second.state != QState::Running) continue;
+        if (qi->second.res) AG_HIP(hipEventRecord(qi->second.res->ready, s->ctx->stream));
         qi->second.stat.iterations = f.iters;
         qi->second.stat.residual = f.residual;
         qi->second.stat.fine_sweep = f.fine_at;
@@ -250,14 +309,10 @@ int ag_serve_step(ag_server *s, uint32_t *active)
         admit(s);
         for (uint32_t gi = 0; gi < s->graphs.size(); gi++) {
             GraphEntry &ge = s->graphs[gi];
-            if (ge.batch && ge.batch->active()) ge.batch->sweep(2, true);
+            // sweeps of all graphs are queued back to back; retire() waits
+            if (ge.batch && ge.batch->active()) ge.batch->sweep(2, true, false);
         }
-        for (uint32_t gi = 0; gi < s->graphs.size(); gi++) {
-            GraphEntry &ge = s->graphs[gi];
-            retire(s, gi);
-            if (ge.batch && ge.batch->active() == 0 && !graph_has_queued(s, gi))
-                release_batch(s, ge);
-        }
+        for (uint32_t gi = 0; gi < s->graphs.size(); gi++) retire(s, gi);
         admit(s);
         if (active) {
             uint32_t n = 0;
@@ -282,14 +337,57 @@ int ag_serve_poll(ag_server *s, uint64_t ticket, float *x, uint32_t n, ag_ppr_st
             return;
         }
         if (st) *st = q.stat;
-        if (q.state == QState::Done && x) {
-            if (n < q.result.size()) fail(AG_ERR_INVALID, "output holds %u < %zu values", n, q.result.size());
-            std::memcpy(x, q.result.data(), q.result.size() * 4);
+        if (q.state == QState::Done && x && q.res) {
+            if (n < q.res->n) fail(AG_ERR_INVALID, "output holds %u < %u values", n, q.res->n);
+            AG_HIP(hipEventSynchronize(q.res->ready));
+            std::memcpy(x, q.res->host, (size_t)q.res->n * 4);
         }
         code = q.code;
+        give_buf(s, q.res);
         s->queries.erase(it);
     });
     return rc != AG_OK ? rc : code;
+}
+
+int ag_serve_peek(ag_server *s, uint64_t ticket, const float **x, uint32_t *n, ag_ppr_stat *st)
+{
+    if (!s) return AG_ERR_INVALID;
+    int code = AG_OK;
+    int rc = guard([&] {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->queries.find(ticket);
+        if (it == s->queries.end()) fail(AG_ERR_NOT_FOUND, "unknown ticket");
+        Query &q = it->second;
+        if (q.state == QState::Queued || q.state == QState::Running) {
+            code = AG_ERR_PENDING;
+            return;
+        }
+        if (st) *st = q.stat;
+        if (x) *x = nullptr;
+        if (n) *n = 0;
+        if (q.state == QState::Done && q.res) {
+            AG_HIP(hipEventSynchronize(q.res->ready));
+            if (x) *x = q.res->host;
+            if (n) *n = q.res->n;
+        }
+        code = q.code;
+    });
+    return rc != AG_OK ? rc : code;
+}
+
+int ag_serve_release(ag_server *s, uint64_t ticket)
+{
+    return guard([&] {
+        if (!s) fail(AG_ERR_INVALID, "null server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->queries.find(ticket);
+        if (it == s->queries.end()) fail(AG_ERR_NOT_FOUND, "unknown ticket");
+        Query &q = it->second;
+        if (q.state == QState::Queued || q.state == QState::Running)
+            fail(AG_ERR_PENDING, "ticket %llu has not finished", (unsigned long long)ticket);
+        give_buf(s, q.res);
+        s->queries.erase(it);
+    });
 }
 
 uint64_t ag_serve_reserved_bytes(const ag_server *s) { return s ? s->reserved : 0; }

@@ -360,11 +360,25 @@ class Server:
         return n.value
 
     def poll(self, ticket: int, n: int):
-        """Returns (code, x or None, stat).  code AG_ERR_PENDING while running."""
-        x = np.zeros(n, np.float32)
+        """Returns (code, x or None, stat).  code AG_ERR_PENDING while running
+        (x untouched: np.empty maps no pages until the result is copied in)."""
+        x = np.empty(n, np.float32)
         st = L.PprStat()
         code = L.load().ag_serve_poll(self.h, ticket, _ptr(x, C.c_float), n, C.byref(st))
         return code, (x if code == L.AG_OK else None), st
+
+    def peek(self, ticket: int):
+        """Zero-copy result: (code, x, stat) where x is a numpy view of the
+        pinned host buffer the GPU wrote (valid until release(ticket))."""
+        p = L.f32p()
+        n = C.c_uint32()
+        st = L.PprStat()
+        code = L.load().ag_serve_peek(self.h, ticket, C.byref(p), C.byref(n), C.byref(st))
+        x = np.ctypeslib.as_array(p, shape=(n.value,)) if code == L.AG_OK and p else None
+        return code, x, st
+
+    def release(self, ticket: int) -> None:
+        L.check(L.load().ag_serve_release(self.h, ticket))
 
     @property
     def reserved_bytes(self) -> int:

@@ -304,7 +304,11 @@ def workload_c3(args):
 
 
 def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
-    """Serving: resident R-MAT s18/s20/s22 graphs, Poisson arrivals, BF16 columns."""
+    """Serving: resident R-MAT s18/s20/s22 graphs, Poisson arrivals, BF16 columns.
+
+    A query is complete when its result vector (N floats, vertex order) sits in
+    host memory: the GPU writes it into a pinned buffer that the client reads
+    in place (ag_serve_peek) and then releases."""
     import numpy as np
     import accelgraph_express_amd as ag
     from accelgraph_express_amd import _lib as L
@@ -336,9 +340,11 @@ def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
             if pending:
                 srv.step()
                 for t in list(pending):
-                    code, x, st = srv.poll(t, pending[t])
+                    # zero-copy: the GPU wrote the result into pinned host memory
+                    code, x, st = srv.peek(t)
                     if code != L.AG_ERR_PENDING:
                         lat.append(st.latency_ms)
+                        srv.release(t)
                         del pending[t]
             elif i < len(reqs):
                 time.sleep(max(0.0, min(reqs[i][0] - now, 0.01)))

@@ -267,6 +267,13 @@ int ag_serve_step(ag_server *s, uint32_t *active);
 /* Result of a finished query (x: N floats, may be NULL).  AG_ERR_PENDING if
  * not finished.  The result is released after a successful poll. */
 int ag_serve_poll(ag_server *s, uint64_t ticket, float *x, uint32_t n, ag_ppr_stat *st);
+/* Zero-copy view of a finished query's result: *x points at *n floats (vertex
+ * order) in pinned host memory written by the GPU, valid until
+ * ag_serve_release(ticket).  AG_ERR_PENDING while the query runs; the query's
+ * status code once it has ended (then *x is NULL unless it succeeded). */
+int ag_serve_peek(ag_server *s, uint64_t ticket, const float **x, uint32_t *n, ag_ppr_stat *st);
+/* Frees a finished query and its result buffer (after ag_serve_peek). */
+int ag_serve_release(ag_server *s, uint64_t ticket);
 /* Bytes of HBM currently reserved for column state. */
 uint64_t ag_serve_reserved_bytes(const ag_server *s);
 

@@ -106,7 +106,19 @@ def test_hbm_budget_admission(ctx):
     assert 0 < srv.reserved_bytes <= budget
     res = _drain(srv, list(tickets), {t: n for t in tickets})
     assert all(c == L.AG_OK for c, _, _ in res.values())
-    assert srv.reserved_bytes == 0  # batch released once drained
+    held = srv.reserved_bytes
+    assert 0 < held <= budget  # the drained batch stays resident for the next query
+    # a query on another graph of the same size evicts the idle batch instead
+    # of waiting forever (the budget holds one batch)
+    s2, d2 = ag.gen_rmat(7, 12, 16)
+    g2 = ag.Graph.build(ctx, s2, d2, n)
+    gid2 = srv.add_graph(g2)
+    t2 = srv.submit(gid2, [1, 2, 3], 1e-5)
+    res2 = _drain(srv, [t2], {t2: n})
+    assert res2[t2][0] == L.AG_OK
+    assert 0 < srv.reserved_bytes <= budget
+    ref, _, _ = oracle.ppr_f64(oracle.build_csr(s2, d2, n), np.array([1, 2, 3], np.uint32), 1e-5)
+    assert np.abs(res2[t2][1] - ref).sum() <= 2e-5
     # a graph whose column state never fits stays queued, and nothing breaks
     tiny = ag.Server(ctx, hbm_budget_bytes=1024, precision="fp32", cols_per_wave=8)
     g2 = tiny.add_graph(graphs[0][0])
@@ -150,4 +162,32 @@ def test_bad_requests(ctx):
     with pytest.raises(ag.AGError):
         srv.submit(gid + 7, [1], 1e-4)
     assert srv.poll(12345, 10)[0] == L.AG_ERR_NOT_FOUND
+    srv.close()
+
+
+def test_peek_is_zero_copy_and_matches_poll(ctx):
+    """peek returns a view of the pinned buffer the GPU wrote (same values as a
+    poll copy of the same query), valid until release; release frees it."""
+    graphs = _setup(ctx)
+    g, csr, n = graphs[0]
+    srv = ag.Server(ctx, precision="fp32", cols_per_wave=8)
+    gid = srv.add_graph(g)
+    seeds = np.array([4, 40, 400], np.uint32)
+    ta, tb = srv.submit(gid, seeds, 1e-6), srv.submit(gid, seeds, 1e-6)
+    assert srv.peek(ta)[0] == L.AG_ERR_PENDING
+    for _ in range(5000):
+        srv.step()
+        if srv.peek(ta)[0] != L.AG_ERR_PENDING and srv.peek(tb)[0] != L.AG_ERR_PENDING:
+            break
+    code, xa, sa = srv.peek(ta)
+    assert code == L.AG_OK and xa.shape == (n,)
+    code_b, xb, sb = srv.poll(tb, n)
+    assert code_b == L.AG_OK
+    np.testing.assert_array_equal(xa, xb)
+    ref, _, _ = oracle.ppr_f64(csr, seeds, 1e-6)
+    assert np.abs(xa - ref).sum() <= 2e-5
+    srv.release(ta)
+    with pytest.raises(ag.AGError):
+        srv.release(ta)
+    assert srv.peek(ta)[0] == L.AG_ERR_NOT_FOUND
     srv.close()
