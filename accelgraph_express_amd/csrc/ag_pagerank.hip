@@ -567,36 +567,47 @@ __global__ void k_degree_keys(const uint32_t *__restrict__ indeg, uint32_t N, ui
     ids[v] = v;
 }
 
-__global__ void k_relabel_vertex(const uint32_t *__restrict__ order,
+// Relabels vertex order[i] as i and counts the bins.  Grid-strided; each
+// wave counts with ballots, a workgroup sums its waves in LDS and adds its six
+// counts with one atomic each (per-wave atomics on six shared counters
+// serialise at the L2).
+__global__ __launch_bounds__(256) void k_relabel_vertex(const uint32_t *__restrict__ order,
                                  const uint32_t *__restrict__ skeys, uint32_t maxdeg,
                                  const uint32_t *__restrict__ fwd_outdeg, uint32_t N,
                                  uint32_t *__restrict__ rank, uint32_t *__restrict__ rdeg,
                                  uint32_t *__restrict__ outdeg_r, uint32_t low_max,
                                  uint32_t hub_min, uint32_t *__restrict__ counts)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t d = 0, od = 1;
-    if (i < N) {
-        const uint32_t v = order[i];
-        rank[v] = i;
-        d = maxdeg - skeys[i];
-        rdeg[i] = d;
-        od = fwd_outdeg[v];
-        outdeg_r[i] = od;
+    __shared__ uint32_t s_cnt[4][6];
+    uint32_t c[6] = {0, 0, 0, 0, 0, 0};
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < N; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t d = 0, od = 1;
+        if (i < N) {
+            const uint32_t v = order[i];
+            rank[v] = i;
+            d = maxdeg - skeys[i];
+            rdeg[i] = d;
+            od = fwd_outdeg[v];
+            outdeg_r[i] = od;
+        }
+        c[0] += (uint32_t)__popcll(__ballot(i < N && d >= hub_min));
+        c[1] += (uint32_t)__popcll(__ballot(i < N && d > low_max));
+        c[2] += (uint32_t)__popcll(__ballot(i < N && d > 0));
+        c[3] += (uint32_t)__popcll(__ballot(i < N && od == 0));
+        c[4] += (uint32_t)__popcll(__ballot(i < N && d >= kWaveRowMin));
+        c[5] += (uint32_t)__popcll(__ballot(i < N && d >= kHalfRowMin));
     }
-    const uint64_t hub = __ballot(i < N && d >= hub_min);
-    const uint64_t nonlow = __ballot(i < N && d > low_max);
-    const uint64_t nonzero = __ballot(i < N && d > 0);
-    const uint64_t dangl = __ballot(i < N && od == 0);
-    const uint64_t wide = __ballot(i < N && d >= kWaveRowMin);
-    const uint64_t half = __ballot(i < N && d >= kHalfRowMin);
-    if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&counts[4], (uint32_t)__popcll(wide));
-        atomicAdd(&counts[5], (uint32_t)__popcll(half));
-        atomicAdd(&counts[0], (uint32_t)__popcll(hub));
-        atomicAdd(&counts[1], (uint32_t)__popcll(nonlow));
-        atomicAdd(&counts[2], (uint32_t)__popcll(nonzero));
-        atomicAdd(&counts[3], (uint32_t)__popcll(dangl));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) s_cnt[w][k] = c[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const uint32_t t = s_cnt[0][threadIdx.x] + s_cnt[1][threadIdx.x] + s_cnt[2][threadIdx.x] +
+                           s_cnt[3][threadIdx.x];
+        // counts[]: 0 hub, 1 nonlow, 2 nonzero, 3 dangling, 4 wide, 5 half
+        if (t) atomicAdd(&counts[threadIdx.x], t);
     }
 }
 
@@ -696,7 +707,8 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     radix_sort_pairs(ctx, keys.as<uint32_t>(), ids.as<uint32_t>(), skeys.as<uint32_t>(),
                      p->order.as<uint32_t>(), N, kbits);
     AG_HIP(hipMemsetAsync(counts.p, 0, 32, s));
-    hipLaunchKernelGGL(k_relabel_vertex, dim3(nb), dim3(256), 0, s, p->order.as<uint32_t>(),
+    hipLaunchKernelGGL(k_relabel_vertex, dim3(std::min<uint32_t>(nb, (uint32_t)ctx->num_cus * 8)),
+                       dim3(256), 0, s, p->order.as<uint32_t>(),
                        skeys.as<uint32_t>(), maxdeg, g->fwd_outdeg.as<uint32_t>(), N,
                        p->rank.as<uint32_t>(), rdeg.as<uint32_t>(), p->outdeg.as<uint32_t>(),
                        tn.low_max, tn.hub_min, counts.as<uint32_t>());

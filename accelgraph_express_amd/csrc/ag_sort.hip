@@ -178,22 +178,26 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_apply(uint32_t *__restric
 // the lanes holding equal digits are found with `nbits` ballots; the rank of a
 // lane is the wave's running count for its digit plus the number of equal-digit
 // lanes below it.  The lowest lane of each equal-digit group then advances the
-// running count.  After all steps the per-wave counts are turned into a
-// cross-wave exclusive prefix, which keeps the order wave 0 < 1 < 2 < 3, i.e.
-// the input order.
+// running count.  The tile is then reordered by digit in LDS (wave 0 < 1 < 2 <
+// 3 inside a digit, i.e. input order) and written out run by run, so that keys
+// of one digit leave as consecutive words instead of one scattered word each.
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
     uint32_t nbits, const uint32_t *__restrict__ hist)
 {
-    __shared__ uint32_t wcnt[kWaves][kRadix];
+    __shared__ uint32_t wcnt[kWaves][kRadix]; // per-wave digit counts, then in-tile positions
+    __shared__ uint32_t gbase[kRadix];        // global position of the tile's first key of digit d
+    __shared__ uint32_t wsum[kWaves];
+    __shared__ uint32_t sk[kTile], sv[kTile];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t mask = (1u << nbits) - 1u;
 #pragma unroll
     for (int i = 0; i < kWaves; i++) wcnt[i][threadIdx.x] = 0;
     __syncthreads();
 
-    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)w * (kTile / kWaves);
+    const uint64_t tile0 = (uint64_t)blockIdx.x * kTile;
+    const uint64_t base = tile0 + (uint64_t)w * (kTile / kWaves);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t k[kItems], v[kItems], r[kItems];
 #pragma unroll
@@ -221,13 +225,24 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
         r[i] = rank;
     }
     __syncthreads();
+    // digit d (one thread each): tile count, then the exclusive scan over
+    // digits gives the digit's first in-tile position
+    const uint32_t d0 = threadIdx.x;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) tot += wcnt[i][d0];
+    const uint32_t incl = wave_incl_scan(tot, lane);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t loc = incl - tot;
+    for (int i = 0; i < w; i++) loc += wsum[i];
+    gbase[d0] = hist[(uint64_t)blockIdx.x * kRadix + d0] - loc;
     {
-        const uint32_t d = threadIdx.x;
-        uint32_t run = hist[(uint64_t)blockIdx.x * kRadix + d];
+        uint32_t run = loc;
 #pragma unroll
         for (int i = 0; i < kWaves; i++) {
-            uint32_t c = wcnt[i][d];
-            wcnt[i][d] = run;
+            const uint32_t c = wcnt[i][d0];
+            wcnt[i][d0] = run;
             run += c;
         }
     }
@@ -236,10 +251,22 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     for (int i = 0; i < kItems; i++) {
         const uint64_t idx = base + (uint64_t)i * 64 + lane;
         if (idx < n) {
-            const uint32_t d = (k[i] >> shift) & mask;
-            const uint32_t pos = wcnt[w][d] + r[i];
-            kout[pos] = k[i];
-            vout[pos] = v[i];
+            const uint32_t lp = wcnt[w][(k[i] >> shift) & mask] + r[i];
+            sk[lp] = k[i];
+            sv[lp] = v[i];
+        }
+    }
+    __syncthreads();
+    // in-tile position j of digit d goes to gbase[d] + j (consecutive per run)
+    const uint32_t cnt = (uint32_t)min<uint64_t>(kTile, n - tile0);
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+        const uint32_t j = (uint32_t)i * kBlock + threadIdx.x;
+        if (j < cnt) {
+            const uint32_t key = sk[j];
+            const uint32_t pos = gbase[(key >> shift) & mask] + j;
+            kout[pos] = key;
+            vout[pos] = sv[j];
         }
     }
 }
