@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -97,9 +98,33 @@ struct ag_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 0;
     std::string arch;
+    // Scratch arena of the builders (radix passes, scans, uploads): buffers
+    // grow on demand and are kept until the context is destroyed, so a build
+    // does not pay hipMalloc / hipFree (which waits for the device) per step.
+    // The mutex serialises the host threads that use it.
+    static constexpr int kScratchSlots = 16;
+    std::recursive_mutex scratch_mu;
+    ag::DevBuf scratch[kScratchSlots];
 };
 
 namespace ag {
+
+// Scratch slots (see ag_ctx::scratch).
+enum ScratchSlot : int {
+    kScrHist = 0, kScrCsum, kScrSortK, kScrSortV, kScrScan0, // kScrScan0 .. +3: scan levels
+    kScrMax = 8, kScrPairA, kScrPairB, kScrInSrc, kScrInDst,
+};
+// Device buffer of at least `bytes` in `slot`; growing waits for the stream
+// first, since the old buffer may still be read by queued work.
+inline void *scratch(ag_ctx *ctx, int slot, size_t bytes)
+{
+    DevBuf &b = ctx->scratch[slot];
+    if (b.bytes < bytes) {
+        AG_HIP(hipStreamSynchronize(ctx->stream));
+        b.alloc(bytes);
+    }
+    return b.p;
+}
 
 // Gather layout of a graph for the pull kernels (built lazily, cached).
 struct Plan;

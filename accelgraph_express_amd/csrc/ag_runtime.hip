@@ -172,16 +172,16 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
         hipStream_t s = ctx->stream;
         std::unique_ptr<ag_graph> g(new ag_graph());
         g->E = E;
+        std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
 
-        DevBuf in_src, in_dst;
         const uint32_t *ds = src, *dd = dst;
         if (!(flags & AG_BUILD_DEVICE_INPUT) && E) {
-            in_src.alloc((size_t)E * 4);
-            in_dst.alloc((size_t)E * 4);
-            AG_HIP(hipMemcpyAsync(in_src.p, src, (size_t)E * 4, hipMemcpyHostToDevice, s));
-            AG_HIP(hipMemcpyAsync(in_dst.p, dst, (size_t)E * 4, hipMemcpyHostToDevice, s));
-            ds = in_src.as<uint32_t>();
-            dd = in_dst.as<uint32_t>();
+            void *in_src = scratch(ctx, kScrInSrc, (size_t)E * 4);
+            void *in_dst = scratch(ctx, kScrInDst, (size_t)E * 4);
+            AG_HIP(hipMemcpyAsync(in_src, src, (size_t)E * 4, hipMemcpyHostToDevice, s));
+            AG_HIP(hipMemcpyAsync(in_dst, dst, (size_t)E * 4, hipMemcpyHostToDevice, s));
+            ds = static_cast<const uint32_t *>(in_src);
+            dd = static_cast<const uint32_t *>(in_dst);
         }
         const uint32_t maxid = device_max2(ctx, ds, dd, E);
         if (N == 0) {
@@ -198,10 +198,10 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
             ko.alloc((size_t)E * 4);
             vo.alloc((size_t)E * 4);
             if (by_src_dst) {
-                DevBuf ta((size_t)E * 4), tb((size_t)E * 4);
-                radix_sort_pairs(ctx, v, k, tb.as<uint32_t>(), ta.as<uint32_t>(), E, bits);
-                radix_sort_pairs(ctx, ta.as<uint32_t>(), tb.as<uint32_t>(), ko.as<uint32_t>(),
-                                 vo.as<uint32_t>(), E, bits);
+                uint32_t *ta = static_cast<uint32_t *>(scratch(ctx, kScrPairA, (size_t)E * 4));
+                uint32_t *tb = static_cast<uint32_t *>(scratch(ctx, kScrPairB, (size_t)E * 4));
+                radix_sort_pairs(ctx, v, k, tb, ta, E, bits);
+                radix_sort_pairs(ctx, ta, tb, ko.as<uint32_t>(), vo.as<uint32_t>(), E, bits);
             } else {
                 radix_sort_pairs(ctx, k, v, ko.as<uint32_t>(), vo.as<uint32_t>(), E, bits);
             }

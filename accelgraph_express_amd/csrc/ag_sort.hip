@@ -139,7 +139,19 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_scan(uint32_t *__restrict
     __shared__ uint32_t ws[kRadix / 64];
     const uint32_t d = threadIdx.x;
     uint32_t run = 0;
-    for (uint32_t c = 0; c < nchunks; c++) {
+    // eight independent loads in flight per step (the loop is latency-bound)
+    uint32_t c = 0;
+    for (; c + 8 <= nchunks; c += 8) {
+        uint32_t x[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = csum[(uint64_t)(c + i) * kRadix + d];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            csum[(uint64_t)(c + i) * kRadix + d] = run;
+            run += x[i];
+        }
+    }
+    for (; c < nchunks; c++) {
         uint32_t x = csum[(uint64_t)c * kRadix + d];
         csum[(uint64_t)c * kRadix + d] = run;
         run += x;
@@ -153,7 +165,8 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_scan(uint32_t *__restrict
     uint32_t wpref = 0;
     for (int i = 0; i < w; i++) wpref += ws[i];
     const uint32_t digit_base = wpref + incl - run;
-    for (uint32_t c = 0; c < nchunks; c++) csum[(uint64_t)c * kRadix + d] += digit_base;
+#pragma unroll 8
+    for (uint32_t k = 0; k < nchunks; k++) csum[(uint64_t)k * kRadix + d] += digit_base;
 }
 
 // Phase 3: rewrite each tile histogram entry as its global exclusive offset.
@@ -304,31 +317,44 @@ __global__ __launch_bounds__(kBlock) void k_max2(const uint32_t *__restrict__ a,
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+    __shared__ uint32_t wm[kWaves];
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kWaves; i++) m = max(m, wm[i]);
+        atomicMax(out, m); // one atomic per workgroup
+    }
 }
 
 } // namespace
 
-void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n)
+// One level of the exclusive scan; block sums of level l live in scratch slot
+// kScrScan0 + l (four levels cover 2^48 elements).
+static void scan_level(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n, int level)
 {
     hipStream_t s = ctx->stream;
     if (n == 0) {
         AG_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), s));
         return;
     }
+    if (level > 3) fail(AG_ERR_INVALID, "scan of %u elements is too deep", n);
     const uint32_t nb = ceil_div(n, kTile);
-    DevBuf sums((size_t)(nb + 1) * sizeof(uint32_t));
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, sums.as<uint32_t>());
+    uint32_t *sums = static_cast<uint32_t *>(scratch(ctx, kScrScan0 + level, (size_t)(nb + 1) * 4));
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, sums);
     AG_HIP(hipGetLastError());
     if (nb == 1) {
-        AG_HIP(hipMemsetAsync(sums.p, 0, sizeof(uint32_t), s));
+        AG_HIP(hipMemsetAsync(sums, 0, sizeof(uint32_t), s));
     } else {
-        scan_exclusive_u32(ctx, sums.as<uint32_t>(), sums.as<uint32_t>(), nb);
+        scan_level(ctx, sums, sums, nb, level + 1);
     }
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, out, n,
-                       sums.as<uint32_t>(), nb);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, out, n, sums, nb);
     AG_HIP(hipGetLastError());
-    // `sums` is freed at scope exit: hipFree synchronises with pending work.
+}
+
+void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n)
+{
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    scan_level(ctx, in, out, n, 0);
 }
 
 void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
@@ -336,6 +362,7 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
 {
     hipStream_t s = ctx->stream;
     if (n == 0) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
     const uint32_t passes = (key_bits + 7) / 8;
     if (passes == 0) {
         AG_HIP(hipMemcpyAsync(kout, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
@@ -345,12 +372,12 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
     const uint32_t ntiles = ceil_div(n, kTile);
     const uint32_t tpc = ceil_div(ntiles, 256);
     const uint32_t nchunks = ceil_div(ntiles, tpc);
-    DevBuf hist((size_t)ntiles * kRadix * 4);
-    DevBuf csum((size_t)nchunks * kRadix * 4);
-    DevBuf tk, tv;
+    uint32_t *hist = static_cast<uint32_t *>(scratch(ctx, kScrHist, (size_t)ntiles * kRadix * 4));
+    uint32_t *csum = static_cast<uint32_t *>(scratch(ctx, kScrCsum, (size_t)nchunks * kRadix * 4));
+    uint32_t *tk = nullptr, *tv = nullptr;
     if (passes > 1) {
-        tk.alloc((size_t)n * 4);
-        tv.alloc((size_t)n * 4);
+        tk = static_cast<uint32_t *>(scratch(ctx, kScrSortK, (size_t)n * 4));
+        tv = static_cast<uint32_t *>(scratch(ctx, kScrSortV, (size_t)n * 4));
     }
     // Ping-pong so that the last pass lands in (kout, vout).
     const uint32_t *ck = kin, *cv = vin;
@@ -358,18 +385,17 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
         const uint32_t shift = 8 * p;
         const uint32_t nbits = std::min<uint32_t>(8, key_bits - shift);
         const bool to_out = ((passes - 1 - p) % 2) == 0;
-        uint32_t *ok = to_out ? kout : tk.as<uint32_t>();
-        uint32_t *ov = to_out ? vout : tv.as<uint32_t>();
+        uint32_t *ok = to_out ? kout : tk;
+        uint32_t *ov = to_out ? vout : tv;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ck, n, shift,
-                           (1u << nbits) - 1u, hist.as<uint32_t>());
-        hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nchunks), dim3(kRadix), 0, s,
-                           hist.as<uint32_t>(), ntiles, tpc, csum.as<uint32_t>());
-        hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum.as<uint32_t>(),
-                           nchunks);
-        hipLaunchKernelGGL(k_hist_chunk_apply, dim3(nchunks), dim3(kRadix), 0, s,
-                           hist.as<uint32_t>(), ntiles, tpc, csum.as<uint32_t>());
+                           (1u << nbits) - 1u, hist);
+        hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
+                           csum);
+        hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum, nchunks);
+        hipLaunchKernelGGL(k_hist_chunk_apply, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
+                           csum);
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, ck, cv, ok, ov, n,
-                           shift, nbits, hist.as<uint32_t>());
+                           shift, nbits, hist);
         AG_HIP(hipGetLastError());
         ck = ok;
         cv = ov;
@@ -399,14 +425,14 @@ void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N
 static uint32_t device_max_impl(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n)
 {
     if (n == 0) return 0;
-    DevBuf out(4);
-    AG_HIP(hipMemsetAsync(out.p, 0, 4, ctx->stream));
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    uint32_t *out = static_cast<uint32_t *>(scratch(ctx, kScrMax, 4));
+    AG_HIP(hipMemsetAsync(out, 0, 4, ctx->stream));
     const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock * 8), 4096);
-    hipLaunchKernelGGL(k_max2, dim3(blocks), dim3(kBlock), 0, ctx->stream, a, b, n,
-                       out.as<uint32_t>());
+    hipLaunchKernelGGL(k_max2, dim3(blocks), dim3(kBlock), 0, ctx->stream, a, b, n, out);
     AG_HIP(hipGetLastError());
     uint32_t m = 0;
-    AG_HIP(hipMemcpyAsync(&m, out.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+    AG_HIP(hipMemcpyAsync(&m, out, 4, hipMemcpyDeviceToHost, ctx->stream));
     AG_HIP(hipStreamSynchronize(ctx->stream));
     return m;
 }
