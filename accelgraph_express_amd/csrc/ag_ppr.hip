@@ -629,31 +629,39 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows(PprArgs a)
     }
 }
 
-// One workgroup: per column, residual and dangling sums over the partial rows
-// of both kernels in a fixed order, then convergence.  A coarse (BF16) column
-// whose residual stalls or meets its tolerance turns fine: from the next sweep
-// on it reads exact FP32 contributions, and it is done at the tolerance there.
-constexpr int kRedBlock = 1024;
+// One workgroup per column slot: residual and dangling sums over the partial
+// rows of both kernels in a fixed order (thread t takes rows t, t + 256, ...,
+// then a fixed tree), then convergence.  A coarse (BF16) column whose residual
+// stalls or meets its tolerance turns fine: from the next sweep on it reads
+// exact FP32 contributions, and it is done at the tolerance there.
+constexpr int kRedBlock = 256;
 __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t nparts)
 {
-    constexpr uint32_t P = kRedBlock / kMaxCap;
-    __shared__ double q_res[P][kMaxCap], q_dang[P][kMaxCap];
+    __shared__ double w_r[kRedBlock / 64], w_d[kRedBlock / 64];
     const uint32_t K = a.bs->K;
-    if (K == 0) return;
-    const uint32_t c = threadIdx.x % kMaxCap, part = threadIdx.x / kMaxCap;
+    const uint32_t c = blockIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && K) a.bs->sweeps += 1;
+    if (c >= K) return;
     double r = 0.0, d = 0.0;
-    for (uint32_t b = part * nparts / P; b < (part + 1) * nparts / P; b++) {
+    for (uint32_t b = threadIdx.x; b < nparts; b += kRedBlock) {
         r += a.blk_res[(uint64_t)b * kMaxCap + c];
         d += a.blk_dang[(uint64_t)b * kMaxCap + c];
     }
-    q_res[part][c] = r;
-    q_dang[part][c] = d;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        r += __shfl_xor(r, off, 64);
+        d += __shfl_xor(d, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        w_r[threadIdx.x >> 6] = r;
+        w_d[threadIdx.x >> 6] = d;
+    }
     __syncthreads();
-    if (threadIdx.x < K) {
+    if (threadIdx.x == 0) {
         PprCol &col = a.cs[c];
         if (!col.done) {
-            double R = 0.0, D = 0.0;
-            for (uint32_t i = 0; i < P; i++) { R += q_res[i][c]; D += q_dang[i][c]; }
+            const double R = (w_r[0] + w_r[1]) + (w_r[2] + w_r[3]);
+            const double D = (w_d[0] + w_d[1]) + (w_d[2] + w_d[3]);
             const double prev = col.residual;
             col.iters += 1;
             col.residual = R;
@@ -670,7 +678,6 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t np
             }
         }
     }
-    if (threadIdx.x == 0) a.bs->sweeps += 1;
 }
 
 // ---- slot management kernels ----
@@ -1223,7 +1230,7 @@ void PprBatch::launch_sweep(bool from0)
     if (fmt_ == AG_PREC_FP32) launch_ct<float>(lpr, a, g1, g2, s);
     else launch_ct<bf16_t>(lpr, a, g1, g2, s);
     const uint32_t nparts = g1 + (a.nmulti ? g2 : 0u);
-    hipLaunchKernelGGL(k_ppr_reduce, dim3(1), dim3(kRedBlock), 0, s, a, nparts);
+    hipLaunchKernelGGL(k_ppr_reduce, dim3(kMaxCap), dim3(kRedBlock), 0, s, a, nparts);
     AG_HIP(hipGetLastError());
 }
 
