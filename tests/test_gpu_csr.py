@@ -67,6 +67,25 @@ def test_edge_cases(ctx):
              rng.randint(0, 70000, m).astype(np.uint32), 70000)
 
 
+def test_random_small_graphs_bit_exact(ctx):
+    """300 random edge lists on at most 6 vertices with at most 10 edges, every
+    sort mode, against the oracle (which the CPU tests pin to the closed forms)."""
+    rng = np.random.RandomState(91)
+    print("case: 300 small random graphs", flush=True)
+    for i in range(300):
+        n = rng.randint(1, 7)
+        m = rng.randint(0, 11)
+        src = rng.randint(0, n, m).astype(np.uint32)
+        dst = rng.randint(0, n, m).astype(np.uint32)
+        sort = i % 4
+        g = ag.Graph.build(ctx, src, dst, num_vertices=n, sort=sort)
+        got = g.download()
+        want = oracle.build_csr(src, dst, n, sort, False)
+        for k in oracle.CSR_KEYS:
+            np.testing.assert_array_equal(got[k], want[k], err_msg=f"{k} graph {i}")
+        g.free()
+
+
 def test_vertex_out_of_range_rejected(ctx):
     with pytest.raises(ag.AGError) as e:
         ag.Graph.build(ctx, np.array([1, 9], np.uint32), np.array([0, 0], np.uint32), 5)

@@ -87,6 +87,21 @@ def test_brute_force_tiny_graphs():
     assert count == 1 + 9 + 81 + 729 + 6561
 
 
+def test_random_small_graphs_all_modes():
+    """4,000 random edge lists on at most 6 vertices with at most 10 edges
+    (duplicates and self-loops included), every sort mode and both inverse
+    conventions, against the closed forms."""
+    rng = np.random.RandomState(77)
+    for _ in range(500):
+        n = rng.randint(1, 7)
+        m = rng.randint(0, 11)
+        src = rng.randint(0, n, m)
+        dst = rng.randint(0, n, m)
+        for mode in range(4):
+            for from_input in (False, True):
+                _check(src, dst, n, mode, from_input)
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("from_input", [False, True])
 def test_random_graphs_all_modes(mode, from_input):
