@@ -6,8 +6,9 @@ Python host-side mirror used by tests and ``bench.py``.
 """
 from . import _lib
 from ._lib import AGError, load
-from .graph import (Context, Graph, PageRankStats, Server, gen_rmat, gen_uniform, get_tuning,
-                    ppr_autotune, ppr_batch, ppr_single, set_tuning)
+from .graph import (Context, Graph, PageRankStats, Server, csr_file_info, gen_rmat, gen_uniform,
+                    get_tuning, ppr_autotune, ppr_batch, ppr_single, set_tuning)
 
-__all__ = ["AGError", "Context", "Graph", "PageRankStats", "Server", "gen_rmat", "gen_uniform",
+__all__ = ["AGError", "Context", "Graph", "PageRankStats", "Server", "csr_file_info", "gen_rmat",
+           "gen_uniform",
            "get_tuning", "ppr_autotune", "ppr_batch", "ppr_single", "set_tuning", "load", "_lib"]

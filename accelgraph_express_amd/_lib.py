@@ -121,6 +121,9 @@ _SIGS = {
     "ag_graph_device_bytes": (C.c_uint64, [vp]),
     "ag_graph_view": (C.c_int, [vp, C.POINTER(CsrView)]),
     "ag_graph_download": (C.c_int, [vp, vp] + [u32p] * 10),
+    "ag_graph_save": (C.c_int, [vp, vp, C.c_char_p]),
+    "ag_graph_load": (C.c_int, [vp, C.c_char_p, C.POINTER(vp)]),
+    "ag_csr_file_info": (C.c_int, [C.c_char_p, u32p, u32p, u32p]),
     "ag_sort_pairs_host": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
     "ag_vertex_map_host": (C.c_int, [vp, u32p, C.c_uint32, C.c_uint32, u32p, u32p]),
     "ag_pr_options_default": (None, [C.POINTER(PrOptions)]),

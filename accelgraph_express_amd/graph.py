@@ -14,6 +14,7 @@ names follow the reference's C API where one exists:
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -132,6 +133,17 @@ class Graph:
                                       _ptr(d), C.byref(h)))
         return cls(ctx, h)
 
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "Graph":
+        """Loads an AGCSRv1 file (see ag_graph_save) onto the device."""
+        h = C.c_void_p()
+        L.check(L.load().ag_graph_load(ctx.h, os.fsencode(path), C.byref(h)))
+        return cls(ctx, h)
+
+    def save(self, path: str) -> None:
+        """Writes the graph's CSR arrays to an AGCSRv1 file."""
+        L.check(L.load().ag_graph_save(self.ctx.h, self.h, os.fsencode(path)))
+
     def free(self) -> None:
         if getattr(self, "h", None):
             L.load().ag_graph_free(self.h)
@@ -228,6 +240,14 @@ class Graph:
         L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))
         return r
+
+
+def csr_file_info(path: str) -> dict:
+    """Header of an AGCSRv1 file after its size and checksum are verified
+    (host only)."""
+    n, e, m = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    L.check(L.load().ag_csr_file_info(os.fsencode(path), C.byref(n), C.byref(e), C.byref(m)))
+    return {"num_vertices": n.value, "num_edges": e.value, "sections": m.value}
 
 
 def set_tuning(low_max: int = 0, hub_min: int = 0, hub_chunk: int = 0,

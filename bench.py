@@ -63,6 +63,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ppr", action="store_true", help="skip the batched-PPR leg")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--serialize", default=None, metavar="DIR",
+                    help="also time saving / loading the graph as an AGCSRv1 file in DIR")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -110,6 +112,23 @@ def main() -> int:
     info = g.plan_info()
     del d_src, d_dst
     log(f"CSR build {csr_ms:.1f} ms, gather layout {plan_ms:.1f} ms, plan {info}")
+    ser = None
+    if args.serialize:
+        path = os.path.join(args.serialize, f"rmat_s{scale}.agcsr")
+        t = time.perf_counter()
+        g.save(path)
+        save_s = time.perf_counter() - t
+        size = os.path.getsize(path)
+        t = time.perf_counter()
+        h = ag.Graph.load(ctx, path)
+        load_s = time.perf_counter() - t
+        same = all(np.array_equal(x, y) for x, y in zip(g.download().values(), h.download().values()))
+        h.free()
+        os.remove(path)
+        ser = {"file_bytes": size, "save_s": round(save_s, 3), "load_s": round(load_s, 3),
+               "load_GBps": round(size / load_s / 1e9, 2), "round_trip_bit_exact": same}
+        log(f"AGCSRv1 file {size / 1e9:.2f} GB: save {save_s:.2f} s, load {load_s:.2f} s "
+            f"(incl. checksum and upload), round trip bit-exact {same}")
 
     out = torch.empty(N, dtype=torch.float32, device="cuda")
     for _ in range(args.warmup):
@@ -173,7 +192,7 @@ def main() -> int:
         "config": {"workload": f"rmat_s{scale}_ef{ef}_pull_pagerank_fp32_{iters}sweeps",
                    "num_vertices": N, "num_edges": E, "iterations": iters, "epsilon": 0.0,
                    "csr_build_ms": round(csr_ms, 2), "gather_layout_ms": round(plan_ms, 2),
-                   "plan": info},
+                   "plan": info, **({"serialization": ser} if ser else {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_pr_sweep",

@@ -101,6 +101,22 @@ int ag_graph_download(ag_ctx *ctx, const ag_graph *g,
                       uint32_t *inv_edges_idx, uint32_t *inv_sorted_src,
                       uint32_t *inv_sorted_dest);
 
+/* CSR binary serialization (no counterpart file format in the reference: its
+ * graphCSR-binary layout, README.md:295-306, lives in the absent OpenGraph
+ * module).  Writes the graph's CSR arrays to `path` in the AGCSRv1 layout:
+ * a 32-byte header (magic "AGCSRv1\0", N, E, section mask, reserved,
+ * FNV-1a-64 of the payload) followed by the present sections as
+ * little-endian uint32 arrays (bit 0..9: sorted src, sorted dest,
+ * out_degree, edges_idx, offsets, then the same five for the inverse). */
+int ag_graph_save(ag_ctx *ctx, const ag_graph *g, const char *path);
+/* Loads an AGCSRv1 file onto the device: checks the size against the header,
+ * the checksum, offsets[N] == E and vertex ids < N (AG_ERR_INVALID on any
+ * mismatch, AG_ERR_IO on file errors). */
+int ag_graph_load(ag_ctx *ctx, const char *path, ag_graph **out);
+/* Host-only check of an AGCSRv1 file (header, size, checksum): no device. */
+int ag_csr_file_info(const char *path, uint32_t *num_vertices, uint32_t *num_edges,
+                     uint32_t *sections);
+
 /* Stable sort of (key, val) pairs by key (keys < key_limit) on the device:
  * the radix passes used by the builder, exposed for sortRunAlgorithms.  All
  * four pointers are host memory; in-place is allowed. */
