@@ -31,6 +31,7 @@ AG_BUILD_DEVICE_INPUT = 0x400
 
 AG_PR_REFERENCE = 0
 AG_PR_STANDARD = 1
+AG_PR_DATA_DRIVEN = 2
 AG_PREC_FP32 = 0
 AG_PREC_BF16 = 1
 AG_PREC_FP16 = 2

@@ -290,19 +290,26 @@ void freePageRankStats(struct PageRankStats *st)
 
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
 {
-    // pageRank.c:874-938: case 0 and `default` are the pull kernel; the push,
-    // fixed-point, quantised and data-driven variants are outside this
-    // library and fall through to pull like the reference's default branch.
+    // pageRank.c:874-938: case 6 is the data-driven pull; case 0 and
+    // `default` are the pull kernel.  The push, fixed-point and quantised
+    // variants are outside this library and fall through to pull like the
+    // reference's default branch.
+    if (arguments && arguments->pushpull == 6) return pageRankDataDrivenPullGraphCSR(arguments, graph);
     return pageRankPullGraphCSR(arguments, graph);
 }
 
-// pageRankPullGraphCSR (pageRank.c:942-1108) on the GPU.  time_total spans the
-// device copy of the CSR (when not resident yet), the gather-layout build and
-// the sweep loop, as the reference's spans WED mapping, AFU setup and loop.
-struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+} // extern "C"
+
+namespace {
+
+// Shared body of the pull entry points: time_total spans the device copy of
+// the CSR (when not resident yet), the gather-layout build and the sweep loop,
+// as the reference's spans WED mapping, AFU setup and loop.
+struct PageRankStats *run_pull(struct Arguments *arguments, struct GraphCSR *graph,
+                               uint32_t mode, const char *fn, const char *title)
 {
     if (!arguments || !graph) return nullptr;
-    ag_ctx *c = ctx_or_null("pageRankPullGraphCSR");
+    ag_ctx *c = ctx_or_null(fn);
     if (!c) return nullptr;
     struct PageRankStats *st = newPageRankStatsGraphCSR(graph);
     if (!st) return nullptr;
@@ -311,7 +318,7 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
         std::printf(" -----------------------------------------------------\n");
         std::printf("| %-51s | \n", "          ---->>> MI355X (gfx950) <<<----");
         std::printf(" -----------------------------------------------------\n");
-        std::printf("| %-51s | \n", "Starting Page Rank Pull (tolerance/epsilon)");
+        std::printf("| %-51s | \n", title);
         std::printf(" -----------------------------------------------------\n");
         std::printf("| %-51.13lf | \n", arguments->epsilon);
         std::printf(" -----------------------------------------------------\n");
@@ -329,13 +336,13 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
         struct EdgeList *ie = graph->sorted_edges_array;
 #endif
         if (!iv || !ie || !graph->vertices) {
-            std::fprintf(stderr, "[accelgraph-amd] pageRankPullGraphCSR: graph has no inverse CSR\n");
+            std::fprintf(stderr, "[accelgraph-amd] %s: graph has no inverse CSR\n", fn);
             freePageRankStats(st);
             return nullptr;
         }
         if (ag_graph_from_csr(c, graph->num_vertices, graph->num_edges, graph->vertices->out_degree,
                               iv->out_degree, iv->edges_idx, ie->edges_array_dest, &dg) != AG_OK) {
-            report("pageRankPullGraphCSR");
+            report(fn);
             freePageRankStats(st);
             return nullptr;
         }
@@ -345,13 +352,13 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
     ag_pr_options_default(&o);
     o.iterations = arguments->iterations;
     o.epsilon = arguments->epsilon;
-    o.mode = AG_PR_REFERENCE;
+    o.mode = mode;
     o.precision = AG_PREC_FP32;
     std::vector<uint32_t> act(std::max<uint32_t>(o.iterations, 1));
     std::vector<double> err(std::max<uint32_t>(o.iterations, 1));
     ag_pr_result r;
     if (ag_pagerank_pull(c, dg, &o, st->pageRanks, act.data(), err.data(), &r) != AG_OK) {
-        report("pageRankPullGraphCSR");
+        report(fn);
         freePageRankStats(st);
         return nullptr;
     }
@@ -372,6 +379,27 @@ struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct G
         std::printf(" -----------------------------------------------------\n");
     }
     return st;
+}
+
+} // namespace
+
+extern "C" {
+
+// pageRankPullGraphCSR (pageRank.c:942-1108) on the GPU.
+struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    return run_pull(arguments, graph, AG_PR_REFERENCE, "pageRankPullGraphCSR",
+                    "Starting Page Rank Pull (tolerance/epsilon)");
+}
+
+// pageRankDataDrivenPullGraphCSR (pageRank.c:2810-2968) on the GPU: the
+// worklist pull.  Same table and PageRankStats; activeVertices is the number
+// of vertices that took a new rank in the last sweep.
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments,
+                                                     struct GraphCSR *graph)
+{
+    return run_pull(arguments, graph, AG_PR_DATA_DRIVEN, "pageRankDataDrivenPullGraphCSR",
+                    "Starting Page Rank Pull DD (tolerance/epsilon)");
 }
 
 void pageRankPrint(float *pr, uint32_t n)

@@ -54,6 +54,7 @@ constexpr uint32_t kFmtF32 = 0, kFmtBF16 = 1; // 2 = FP16
 #ifndef AG_PR_SELL
 #define AG_PR_SELL 8
 #endif
+constexpr uint32_t kDdFlag = 0x80000000u; // DATA_DRIVEN: "updated" mark on a contribution
 constexpr int kDepth = AG_PR_DEPTH; // loads in flight per lane, CSR rows (power of two)
 constexpr int kSell = AG_PR_SELL;   // loads in flight per lane, SELL rows
 
@@ -82,9 +83,10 @@ struct SweepArgs {
     uint32_t n_hub, n_w, n_h, n_nonlow, n_nonzero; // bin boundaries (labels)
     uint32_t nb_hub, nb_w, nb_h, nb_q, nb_low, nb_zero, nvblocks; // work units per bin
     uint32_t hub_chunk, hub_chunks;
-    // FP32 REFERENCE mode: zero in-degree rows keep pr = base and contribution
-    // base/outdeg in both buffers (written at init); their |delta| is 0, so
-    // they count as active exactly when epsilon <= 0.
+    // FP32 REFERENCE mode (and DATA_DRIVEN with epsilon > 0): zero in-degree
+    // rows keep pr = base and contribution base/outdeg in both buffers
+    // (written at init); their |delta| is 0, so they count as active exactly
+    // when epsilon <= 0.
     uint32_t skip_zero;
     float f16_scale, f16_inv; // FP16 storage scale (power of two) and its inverse
     uint32_t mode;
@@ -172,12 +174,40 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 // REFERENCE mode; x' = tele + d*s in STANDARD mode), then the contribution the
 // next sweep reads (pageRank.c:1027-1032).  Written without contraction so
 // that base + damp*next rounds twice, as the reference's C does.
+//
+// DATA_DRIVEN (pageRank.c:2886-2926): `flag` has bit 31 set when the row is
+// on this sweep's worklist, i.e. some in-neighbour took a new rank in the
+// previous sweep (or this is the first sweep).  Such a row adds |delta|/N to
+// the error; it takes the new rank only when |delta| >= epsilon, and then its
+// contribution carries bit 31 (the sign bit; contributions are >= 0) so that
+// the next sweep's gathers find its out-neighbours.  That is the reference's
+// workListNext marking over forward edges, read from the pull side.
 __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, float s,
                                               float tele, void *cout, uint32_t wfmt,
-                                              double &err, uint32_t &act, double &dang)
+                                              double &err, uint32_t &act, double &dang,
+                                              uint32_t flag = 0u)
 {
 #pragma clang fp contract(off)
     const float old = a.pr[i];
+    if (a.mode == AG_PR_DATA_DRIVEN) {
+        float keep = old;
+        uint32_t mark = 0u;
+        if (flag & kDdFlag) {
+            const float nw = a.base + a.damp * s;
+            const double e = fabs((double)(nw - old));
+            err += e / (double)a.N;
+            if (e >= a.epsilon) {
+                a.pr[i] = nw;
+                keep = nw;
+                act += 1u;
+                mark = kDdFlag;
+            }
+        }
+        const uint32_t od = a.outdeg[i];
+        const float c = od ? keep / (float)od : 0.0f;
+        static_cast<uint32_t *>(cout)[i] = __float_as_uint(c) | mark;
+        return;
+    }
     float nw;
     if (a.mode == AG_PR_REFERENCE) {
         nw = a.base + a.damp * s;
@@ -222,17 +252,34 @@ __device__ __forceinline__ float gather(const CT *__restrict__ g, uint32_t u, ui
     return u < hot ? load_c(lt, u) : load_c(g, u);
 }
 
+// Adds one gathered contribution.  DATA_DRIVEN contributions carry the
+// "updated" mark in the sign bit: the sum takes |x| (a free source modifier)
+// and the marks are ORed.
+template <bool DD>
+__device__ __forceinline__ void acc(float &s, uint32_t &fl, float x)
+{
+    if (DD) {
+        fl |= __float_as_uint(x);
+        s += __builtin_fabsf(x);
+    } else {
+        s += x;
+    }
+}
+
 // Rows [.., row_end) with G lanes per row (G = 16, 32 or 64): lanes of a
 // group stride over the row's edges, kDepth loads in flight each, then a
 // shuffle reduction inside the group and one lane updates the vertex.
-template <int G, typename CT>
+// `first` is kDdFlag on a DATA_DRIVEN run's first sweep (every row is on the
+// worklist), else 0.
+template <int G, typename CT, bool DD>
 __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uint32_t row_end,
                                            const CT *__restrict__ cin, void *cout, uint32_t wfmt,
                                            float tele, uint32_t hot, float rscale, double &err,
-                                           uint32_t &act, double &dang, int lane)
+                                           uint32_t &act, double &dang, int lane, uint32_t first)
 {
     const int gl = lane & (G - 1);
     const bool valid = row < row_end;
+    uint32_t fl = first;
     float sp[kDepth];
 #pragma unroll
     for (int t = 0; t < kDepth; t++) sp[t] = 0.f;
@@ -250,7 +297,7 @@ __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uin
             for (int t = 0; t < kDepth; t++) n[t] = jn + t * G < re ? ldnt(a.cols + jn + t * G) : 0u;
 #pragma unroll
             for (int t = 0; t < kDepth; t++)
-                if (j + t * G < re) sp[t] += gather(cin, c[t], hot);
+                if (j + t * G < re) acc<DD>(sp[t], fl, gather(cin, c[t], hot));
 #pragma unroll
             for (int t = 0; t < kDepth; t++) c[t] = n[t];
         }
@@ -263,13 +310,18 @@ __device__ __forceinline__ void group_rows(const SweepArgs &a, uint32_t row, uin
     float s = sp[0];
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (valid && gl == 0) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang);
+    if (DD) {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
+    }
+    if (valid && gl == 0)
+        update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang, fl);
 }
 
-template <typename CT>
+template <typename CT, bool DD>
 __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restrict__ cin,
                            void *cout, float tele, uint32_t hot, double &err, uint32_t &act,
-                           double &dang)
+                           double &dang, uint32_t first)
 {
     // FP16 words hold contributions times f16_scale: gathered sums scale back
     const float rscale = FmtOf<CT>::v == 2u ? a.f16_inv : 1.0f;
@@ -289,6 +341,7 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                 float sp[kDepth];
 #pragma unroll
                 for (int t = 0; t < kDepth; t++) sp[t] = 0.f;
+                uint32_t fl = 0u;
                 uint32_t j = b0 + lane;
                 uint32_t ci[kDepth];
 #pragma unroll
@@ -300,7 +353,7 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                     for (int t = 0; t < kDepth; t++) n[t] = jn + t * 64 < b1 ? ldnt(a.cols + jn + t * 64) : 0u;
 #pragma unroll
                     for (int t = 0; t < kDepth; t++)
-                        if (j + t * 64 < b1) sp[t] += gather(cin, ci[t], hot);
+                        if (j + t * 64 < b1) acc<DD>(sp[t], fl, gather(cin, ci[t], hot));
 #pragma unroll
                     for (int t = 0; t < kDepth; t++) ci[t] = n[t];
                 }
@@ -310,17 +363,28 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                     for (int t = 0; t + w2 < kDepth; t += 2 * w2) sp[t] += sp[t + w2];
                 const float s = wave_sum(sp[0]);
                 // chunk partial; k_pr_finalize sums a row's chunks in chunk order
-                if (lane == 0) a.hub_partial[c] = s;
+                // (DATA_DRIVEN: the chunk's OR of marks rides in the sign bit)
+                if (DD) {
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
+                    if (lane == 0)
+                        a.hub_partial[c] = __uint_as_float(__float_as_uint(s) | (fl & kDdFlag));
+                } else if (lane == 0) {
+                    a.hub_partial[c] = s;
+                }
             }
         } else if (vb < u_h) {
             const uint32_t row = a.n_hub + (vb - u_w) * kSweepWaves + w;
-            group_rows<64>(a, row, a.n_w, cin, cout, wfmt, tele, hot, rscale, err, act, dang, lane);
+            group_rows<64, CT, DD>(a, row, a.n_w, cin, cout, wfmt, tele, hot, rscale, err, act, dang,
+                                    lane, first);
         } else if (vb < u_q) {
             const uint32_t row = a.n_w + (vb - u_h) * (2 * kSweepWaves) + w * 2 + (lane >> 5);
-            group_rows<32>(a, row, a.n_h, cin, cout, wfmt, tele, hot, rscale, err, act, dang, lane);
+            group_rows<32, CT, DD>(a, row, a.n_h, cin, cout, wfmt, tele, hot, rscale, err, act, dang,
+                                    lane, first);
         } else if (vb < u_l) {
             const uint32_t row = a.n_h + (vb - u_q) * (4 * kSweepWaves) + w * 4 + (lane >> 4);
-            group_rows<16>(a, row, a.n_nonlow, cin, cout, wfmt, tele, hot, rscale, err, act, dang, lane);
+            group_rows<16, CT, DD>(a, row, a.n_nonlow, cin, cout, wfmt, tele, hot, rscale, err, act, dang,
+                                    lane, first);
         } else if (vb < u_z) {
             // ---- low rows: SELL-64 slice per wave, lane per row, edges in order ----
             const uint32_t sl = (vb - u_l) * kSweepWaves + w;
@@ -332,6 +396,7 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                 const uint32_t width = __shfl(deg, 0, 64); // rows sorted: lane 0 is the widest
                 const uint32_t *p = a.cols + a.slice_ptr[sl] + lane;
                 float s = 0.0f;
+                uint32_t fl = first;
                 // batch of kSell indices in flight while the previous batch
                 // gathers; each lane adds in the reference's edge order
                 uint32_t u[kSell];
@@ -347,11 +412,11 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
                     for (int t = 0; t < kSell; t++) x[t] = k + t < deg ? gather(cin, u[t], hot) : 0.f;
 #pragma unroll
                     for (int t = 0; t < kSell; t++)
-                        if (k + t < deg) s += x[t];
+                        if (k + t < deg) acc<DD>(s, fl, x[t]);
 #pragma unroll
                     for (int t = 0; t < kSell; t++) u[t] = nx[t];
                 }
-                if (valid) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang);
+                if (valid) update_vertex(a, row, s * rscale, tele, cout, wfmt, err, act, dang, fl);
             }
         } else if (!a.skip_zero) {
             // ---- zero in-degree rows ----
@@ -359,7 +424,7 @@ __device__ void sweep_body(const SweepArgs &a, uint32_t wfmt, const CT *__restri
 #pragma unroll
             for (int q = 0; q < kZeroRowsPerUnit / kSweepBlock; q++) {
                 const uint32_t row = r0 + q * kSweepBlock + threadIdx.x;
-                if (row < a.N) update_vertex(a, row, 0.0f, tele, cout, wfmt, err, act, dang);
+                if (row < a.N) update_vertex(a, row, 0.0f, tele, cout, wfmt, err, act, dang, first);
             }
         }
     }
@@ -386,7 +451,7 @@ __device__ __forceinline__ void load_hot(const CT *__restrict__ cin, uint32_t ho
 // kernel's register footprint to one instantiation).  A mixed-precision run
 // launches the low-precision and the FP32 kernel every sweep; the one whose
 // format does not match the sweep's exits at once.
-template <typename CT>
+template <typename CT, bool DD>
 __global__ __launch_bounds__(kSweepBlock) void k_pr_sweep(SweepArgs a, uint32_t it)
 {
     __shared__ double s_err[kSweepWaves], s_dang[kSweepWaves];
@@ -402,7 +467,7 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_sweep(SweepArgs a, uint32_t 
     uint32_t act = 0;
     const uint32_t hot = min(a.N, a.hot_bytes / (uint32_t)sizeof(CT));
     load_hot(cin, hot);
-    sweep_body<CT>(a, wfmt, cin, cout, tele, hot, err, act, dang);
+    sweep_body<CT, DD>(a, wfmt, cin, cout, tele, hot, err, act, dang, (DD && it == 0) ? kDdFlag : 0u);
 
     // ---- block partials; k_pr_finalize reduces them in a fixed order ----
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -439,11 +504,16 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     const float tele = st->tele;
     double err = 0.0, dang = 0.0;
     uint32_t act = 0;
+    const bool dd = a.mode == AG_PR_DATA_DRIVEN;
     for (uint32_t row = threadIdx.x; row < a.n_hub; row += kFinBlock) {
         const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
         float sum = 0.0f;
-        for (uint32_t q = f0; q < f1; q++) sum += a.hub_partial[q];
-        update_vertex(a, row, sum * rscale, tele, cout, wfmt, err, act, dang);
+        uint32_t fl = (dd && it == 0) ? kDdFlag : 0u;
+        if (dd)
+            for (uint32_t q = f0; q < f1; q++) acc<true>(sum, fl, a.hub_partial[q]);
+        else
+            for (uint32_t q = f0; q < f1; q++) sum += a.hub_partial[q];
+        update_vertex(a, row, sum * rscale, tele, cout, wfmt, err, act, dang, fl);
     }
     for (uint32_t b = threadIdx.x; b < a.nparts; b += kFinBlock) {
         err += a.blk_err[b];
@@ -474,7 +544,7 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
         st->fp32_switch = it + 2;
     }
     if (it + 2 < a.max_iter + 2) a.fmt[it + 2] = nf;
-    if (a.mode == AG_PR_REFERENCE) {
+    if (a.mode == AG_PR_REFERENCE || dd) {
         if (A_ == 0) { st->done = 1; st->iterations = it; }
         else st->iterations = it + 1;
     } else if (a.mode == kModePPR) {
@@ -675,9 +745,10 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     if (p->hot_bytes) {
         static bool attr_set = false;
         if (!attr_set) {
-            const void *ks[3] = {(const void *)k_pr_sweep<float>,
-                                 (const void *)k_pr_sweep<bf16_t>,
-                                 (const void *)k_pr_sweep<__half>};
+            const void *ks[4] = {(const void *)k_pr_sweep<float, false>,
+                                 (const void *)k_pr_sweep<bf16_t, false>,
+                                 (const void *)k_pr_sweep<__half, false>,
+                                 (const void *)k_pr_sweep<float, true>};
             for (const void *k : ks)
                 AG_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(kLdsBytes - kStaticLds)));
@@ -862,7 +933,12 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.mode = o.mode;
     a.max_iter = max_iter;
     a.finish_fp32 = (o.flags & AG_PR_FLAG_FINISH_FP32) ? 1u : 0u;
-    a.skip_zero = (o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ? 1u : 0u;
+    // DATA_DRIVEN zero rows never change rank; with epsilon > 0 they never take
+    // the "updated" mark either, so both buffers can hold them from init.
+    a.skip_zero = ((o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ||
+                   (o.mode == AG_PR_DATA_DRIVEN && o.epsilon > 0.0))
+                      ? 1u
+                      : 0u;
     // STANDARD ranks are ~1/N: scale FP16 words by 2^ceil(log2 N)
     a.f16_scale = o.mode == AG_PR_STANDARD ? std::ldexp(1.0f, (int)bits_for(p->N)) : 1.0f;
     a.f16_inv = 1.0f / a.f16_scale;
@@ -907,8 +983,12 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     std::lock_guard<std::mutex> lk(p->mu);
     const uint32_t N = p->N;
     const uint32_t max_iter = o.iterations;
-    if (o.mode != AG_PR_REFERENCE && o.mode != AG_PR_STANDARD && o.mode != kModePPR)
+    if (o.mode != AG_PR_REFERENCE && o.mode != AG_PR_STANDARD && o.mode != AG_PR_DATA_DRIVEN &&
+        o.mode != kModePPR)
         fail(AG_ERR_INVALID, "unknown PageRank mode %u", o.mode);
+    const bool dd = o.mode == AG_PR_DATA_DRIVEN;
+    if (dd && (o.precision != AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32)))
+        fail(AG_ERR_INVALID, "data-driven pull stores FP32 contributions only");
     if ((o.mode == kModePPR) != (seeds != nullptr))
         fail(AG_ERR_INVALID, "personalized mode needs a seed table");
     if (o.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", o.precision);
@@ -925,7 +1005,8 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         a.seed_w = seeds->w;
         a.nseeds = seeds->n;
     }
-    const float x0 = (o.mode == AG_PR_REFERENCE) ? a.base : (float)(1.0 / (double)N);
+    const bool ref_scale = o.mode == AG_PR_REFERENCE || dd; // pr0 = base, ranks pr/N
+    const float x0 = ref_scale ? a.base : (float)(1.0 / (double)N);
     const double dang0 = (double)p->n_dangling * (1.0 / (double)N);
     const float tele0 = (float)((1.0 - (double)o.damp) / N + (double)o.damp * dang0 / N);
     hipEvent_t ev[4];
@@ -949,14 +1030,17 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         const uint32_t it1 = std::min(max_iter, it0 + group);
         for (uint32_t it = it0; it < it1; it++) {
             if (timed) AG_HIP(hipEventRecord(sev[2 * it], s));
-            if (o.precision == AG_PREC_BF16)
-                hipLaunchKernelGGL(k_pr_sweep<bf16_t>, dim3(grid), dim3(kSweepBlock),
+            if (dd)
+                hipLaunchKernelGGL((k_pr_sweep<float, true>), dim3(grid), dim3(kSweepBlock),
+                                   p->hot_bytes, s, a, it);
+            else if (o.precision == AG_PREC_BF16)
+                hipLaunchKernelGGL((k_pr_sweep<bf16_t, false>), dim3(grid), dim3(kSweepBlock),
                                    p->hot_bytes, s, a, it);
             else if (o.precision == AG_PREC_FP16)
-                hipLaunchKernelGGL(k_pr_sweep<__half>, dim3(grid), dim3(kSweepBlock),
+                hipLaunchKernelGGL((k_pr_sweep<__half, false>), dim3(grid), dim3(kSweepBlock),
                                    p->hot_bytes, s, a, it);
-            if (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32))
-                hipLaunchKernelGGL(k_pr_sweep<float>, dim3(grid), dim3(kSweepBlock),
+            if (!dd && (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32)))
+                hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(grid), dim3(kSweepBlock),
                                    p->hot_bytes, s, a, it);
             if (timed) AG_HIP(hipEventRecord(sev[2 * it + 1], s));
             hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
@@ -973,7 +1057,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     const uint32_t nbf = ceil_div(N, kBlock);
     double *parts = p->work.sum_part.as<double>();
     hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, p->work.pr.as<float>(),
-                       p->order.as<uint32_t>(), N, o.mode == AG_PR_REFERENCE ? 1u : 0u, d_out,
+                       p->order.as<uint32_t>(), N, ref_scale ? 1u : 0u, d_out,
                        parts);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
     AG_HIP(hipGetLastError());

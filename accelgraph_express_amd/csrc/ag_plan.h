@@ -49,8 +49,9 @@ struct Plan {
 
 
 // Internal PageRank mode: single-query personalized PageRank (teleport and
-// dangling mass to the seed set), used by ag_ppr_single.
-constexpr uint32_t kModePPR = 2;
+// dangling mass to the seed set), used by ag_ppr_single.  Outside the public
+// AG_PR_* range.
+constexpr uint32_t kModePPR = 0x100;
 
 // Seed table of one personalized query, in label space (device arrays).
 struct SeedTable {

@@ -82,7 +82,8 @@ class PageRankStats:
         return np.arange(self.num_vertices, dtype=np.uint32)
 
 
-_MODES = {"reference": L.AG_PR_REFERENCE, "standard": L.AG_PR_STANDARD}
+_MODES = {"reference": L.AG_PR_REFERENCE, "standard": L.AG_PR_STANDARD,
+          "data_driven": L.AG_PR_DATA_DRIVEN}
 _PRECS = {"fp32": L.AG_PREC_FP32, "bf16": L.AG_PREC_BF16, "fp16": L.AG_PREC_FP16}
 _SORTS = {"radix_src": 0, "radix_src_dst": 1, "count_src": 2, "count_src_dst": 3}
 

@@ -130,6 +130,9 @@ int ag_vertex_map_host(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uin
 /* ---- pull PageRank ------------------------------------------------------ */
 #define AG_PR_REFERENCE 0u /* pageRank.c:942-1108 formulation */
 #define AG_PR_STANDARD  1u /* x0 = 1/N, dangling mass spread uniformly, FP64 dangling sum */
+#define AG_PR_DATA_DRIVEN 2u /* pageRank.c:2810-2968 worklist pull: only rows with an updated
+                                in-neighbour are recomputed, a row takes its new rank only when
+                                the change is >= epsilon (FP32 storage only) */
 
 #define AG_PREC_FP32 0u    /* contributions stored FP32 */
 #define AG_PREC_BF16 1u    /* stored BF16 (RNE), FP32 accumulate */
@@ -144,7 +147,7 @@ typedef struct ag_pr_options
     uint32_t mode;       /* AG_PR_* */
     uint32_t precision;  /* AG_PREC_* */
     uint32_t flags;      /* AG_PR_FLAG_* */
-    double epsilon;      /* REFERENCE: per-vertex active threshold (0: fixed sweeps);
+    double epsilon;      /* REFERENCE / DATA_DRIVEN: per-vertex active threshold (0: fixed sweeps);
                             STANDARD: stop when the L1 residual <= epsilon (0: fixed) */
     float damp;          /* 0.85f */
     float reserved;
@@ -158,7 +161,8 @@ typedef struct ag_pr_result
     uint32_t sweeps;          /* sweeps executed on the device */
     uint32_t active_vertices; /* active count of the last sweep */
     uint32_t fp32_switch_sweep; /* first FP32-stored sweep (precision modes), else 0 */
-    double error_total;       /* REFERENCE: mean |delta|; STANDARD: L1 residual */
+    double error_total;       /* REFERENCE: mean |delta|; DATA_DRIVEN: |delta|/N summed over
+                                 the last worklist; STANDARD: L1 residual */
     double pr_sum;            /* sum of returned ranks (FP64) */
     double loop_ms;           /* device time of the sweep loop (HIP events) */
     double total_ms;          /* device time init + loop + normalise */

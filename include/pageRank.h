@@ -40,11 +40,17 @@ extern "C" {
 struct PageRankStats *newPageRankStatsGraphCSR(struct GraphCSR *graph);
 void freePageRankStats(struct PageRankStats *stats);
 
-/* Dispatch on arguments->pushpull (pageRank.c:874-938): 0 and every variant
- * outside this library's scope run the pull kernel, as the reference's
- * `default:` branch does. */
+/* Dispatch on arguments->pushpull (pageRank.c:874-938): 6 runs the
+ * data-driven pull; 0 and every variant outside this library's scope run the
+ * pull kernel, as the reference's `default:` branch does. */
 struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
 struct PageRankStats *pageRankPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph);
+/* pageRank.c:2810-2968 (declared at pageRank.h:84): worklist pull.  Only
+ * vertices with an in-neighbour updated in the previous sweep are recomputed;
+ * a vertex takes its new rank only when |new - old| >= epsilon.  The first
+ * sweep recomputes every vertex; the loop stops when no vertex is updated. */
+struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments,
+                                                     struct GraphCSR *graph);
 
 void pageRankPrint(float *pageRankArray, uint32_t num_vertices);
 

@@ -273,6 +273,71 @@ uint32_t oracle_ppr_f64(uint32_t N, const uint32_t *fwd_outdeg, const uint32_t *
     return it;
 }
 
+/* Data-driven pull, after pageRank.c:2810-2968 (DIRECTED: gather over the
+ * inverse CSR, marks over the forward CSR).  The contribution vector is taken
+ * from the ranks as they stand at the start of the sweep (:2876-2883), so the
+ * in-place rank writes of one sweep are seen only by the next (Jacobi). */
+uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
+                                     const uint32_t *fwd_idx, const uint32_t *fwd_dest,
+                                     const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                     const uint32_t *inv_dest, uint32_t iterations,
+                                     double epsilon, float *pr, double *error_total,
+                                     double *pr_sum, uint32_t *active_iter, double *err_iter,
+                                     uint32_t *work_iter)
+{
+    const float damp = 0.85f;
+    const float base = 1.0f - damp;
+    float *contrib = (float *)xcalloc(N ? N : 1, sizeof(float));
+    uint8_t *cur = (uint8_t *)xcalloc(N ? N : 1, 1);
+    uint8_t *nxt = (uint8_t *)xcalloc(N ? N : 1, 1);
+    double err_tot = 0.0;
+    uint32_t it;
+
+    for (uint32_t v = 0; v < N; v++) { pr[v] = base; cur[v] = 1; } /* :2862-2870 */
+
+    for (it = 0; it < iterations; it++) {
+        uint32_t active = 0, work = 0;
+        err_tot = 0.0;
+        for (uint32_t v = 0; v < N; v++)
+            contrib[v] = fwd_outdeg[v] ? pr[v] / (float)fwd_outdeg[v] : 0.0f;
+        for (uint32_t v = 0; v < N; v++) {
+            if (!cur[v]) continue;
+            work++;
+            float s = 0.0f;
+            uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+            for (uint32_t j = j0; j < j1; j++) s += contrib[inv_dest[j]];
+            float old = pr[v];
+            float nw = base + damp * s;
+            double e = fabs(nw - old);
+            err_tot += e / N;
+            if (e >= epsilon) {                                     /* :2910-2926 */
+                pr[v] = nw;
+                uint32_t k0 = fwd_idx[v], k1 = fwd_idx[v] + fwd_outdeg[v];
+                for (uint32_t k = k0; k < k1; k++) nxt[fwd_dest[k]] = 1;
+                active++;
+            }
+        }
+        uint8_t *t = cur; cur = nxt; nxt = t;                        /* :2930-2931 */
+        memset(nxt, 0, N);
+        if (active_iter) active_iter[it] = active;
+        if (err_iter) err_iter[it] = err_tot;
+        if (work_iter) work_iter[it] = work;
+        if (active == 0) break;
+    }
+
+    double sum = 0.0;
+    for (uint32_t v = 0; v < N; v++) {
+        pr[v] = pr[v] / N;
+        sum += pr[v];
+    }
+    if (error_total) *error_total = err_tot;
+    if (pr_sum) *pr_sum = sum;
+    free(contrib);
+    free(cur);
+    free(nxt);
+    return it;
+}
+
 /* ------------------------------------------------------------------------- */
 /* OpenMP baselines                                                          */
 /* ------------------------------------------------------------------------- */

@@ -80,6 +80,22 @@ uint32_t oracle_ppr_f64(uint32_t N, const uint32_t *fwd_outdeg, const uint32_t *
                         const uint32_t *seeds, uint32_t nseeds, double tol,
                         uint32_t max_iter, double *x, double *residual);
 
+/* Reference data-driven pull PageRank (pageRank.c:2810-2968), FP32, serial.
+ * Sweep t gathers only the rows on the worklist; a row whose change is
+ * >= epsilon takes its new rank and puts its forward out-neighbours on the
+ * next worklist, a row below epsilon keeps its old rank.  The first worklist
+ * holds every vertex.  Returns the reported iteration count; ranks[] receives
+ * pr/N.  active_iter/err_iter/work_iter (may be NULL, length `iterations`)
+ * receive per sweep the updated count, the error total (summed over the
+ * worklist) and the worklist size. */
+uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
+                                     const uint32_t *fwd_idx, const uint32_t *fwd_dest,
+                                     const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                     const uint32_t *inv_dest, uint32_t iterations,
+                                     double epsilon, float *ranks, double *error_total,
+                                     double *pr_sum, uint32_t *active_iter, double *err_iter,
+                                     uint32_t *work_iter);
+
 /* OpenMP CPU baselines (reported beside the GPU, never the product path). */
 double oracle_csr_build_omp(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
                             uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,

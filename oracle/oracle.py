@@ -40,6 +40,8 @@ def load() -> C.CDLL:
         "oracle_pagerank_pull_f32": (U, [U, u32p, u32p, u32p, u32p, U, D, f32p, f64p, f64p,
                                          u32p, f64p]),
         "oracle_pagerank_pull_f64": (U, [U, u32p, u32p, u32p, u32p, U, D, f64p, f64p]),
+        "oracle_pagerank_dd_pull_f32": (U, [U] + [u32p] * 6 + [U, D, f32p, f64p, f64p, u32p,
+                                                              f64p, u32p]),
         "oracle_pagerank_standard_f64": (U, [U, u32p, u32p, u32p, u32p, D, U, D, f64p, f64p]),
         "oracle_ppr_f64": (U, [U, u32p, u32p, u32p, u32p, D, u32p, U, D, U, f64p, f64p]),
         "oracle_csr_build_omp": (D, [U, U, u32p, u32p] + [u32p] * 8 + [I]),
@@ -122,6 +124,27 @@ def pagerank_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4):
                                          _p(act), _p(err, C.c_double))
     sweeps = min(iterations, it + 1) if iterations else 0
     return pr, int(it), et.value, ps.value, act[:sweeps].copy(), err[:sweeps].copy()
+
+
+def pagerank_dd_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4):
+    """Data-driven pull.  Returns (ranks f32, iterations, error_total, pr_sum,
+    active[], err[], work[]) with per-sweep updated counts, error totals and
+    worklist sizes."""
+    n = csr["num_vertices"]
+    od, idg, iidx, idst = _inv(csr)
+    fidx = _u(csr["vertices.edges_idx"])
+    fdst = _u(csr["sorted_edges_array.edges_array_dest"])
+    pr = np.zeros(n, np.float32)
+    et, ps = C.c_double(), C.c_double()
+    k = max(iterations, 1)
+    act, err, work = np.zeros(k, np.uint32), np.zeros(k, np.float64), np.zeros(k, np.uint32)
+    it = load().oracle_pagerank_dd_pull_f32(n, _p(od), _p(fidx), _p(fdst), _p(idg), _p(iidx),
+                                            _p(idst), iterations, epsilon, _p(pr, C.c_float),
+                                            C.byref(et), C.byref(ps), _p(act),
+                                            _p(err, C.c_double), _p(work))
+    sweeps = min(iterations, it + 1) if iterations else 0
+    return (pr, int(it), et.value, ps.value, act[:sweeps].copy(), err[:sweeps].copy(),
+            work[:sweeps].copy())
 
 
 def pagerank_pull_f64(csr: dict, iterations: int = 20, epsilon: float = 1e-4):

@@ -28,7 +28,8 @@ static int same(const uint32_t *a, const uint32_t *b, uint32_t n)
 int main(int argc, char **argv)
 {
     if (argc < 6) {
-        fprintf(stderr, "usage: %s <edge list> <format> <iterations> <epsilon> <out>\n", argv[0]);
+        fprintf(stderr, "usage: %s <edge list> <format> <iterations> <epsilon> <out> [pushpull]\n",
+                argv[0]);
         return 2;
     }
     struct Arguments args;
@@ -37,7 +38,7 @@ int main(int argc, char **argv)
     args.fnameb_format = (uint32_t)atoi(argv[2]);
     args.iterations = (uint32_t)atoi(argv[3]);
     args.epsilon = atof(argv[4]);
-    args.pushpull = 0;
+    args.pushpull = argc > 6 ? (uint32_t)atoi(argv[6]) : 0; /* 6: data-driven pull */
     args.sort = 0;
 
     struct GraphCSR *a = graphCSRPreProcessingStep(&args);

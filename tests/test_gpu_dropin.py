@@ -18,12 +18,12 @@ BIN = os.path.join(ROOT, "tests", "c", "_bin", "dropin_pagerank")
 REL_TOL = 1e-5
 
 
-def _run(path, fmt, iterations, epsilon, tmp_path):
+def _run(path, fmt, iterations, epsilon, tmp_path, pushpull=0):
     out = str(tmp_path / "out.bin")
     print(f"case: dropin {os.path.basename(os.path.dirname(path))}/{os.path.basename(path)} "
-          f"fmt={fmt}", flush=True)
+          f"fmt={fmt} pushpull={pushpull}", flush=True)
     env = dict(os.environ, AG_QUIET="1")
-    r = subprocess.run([BIN, path, str(fmt), str(iterations), str(epsilon), out],
+    r = subprocess.run([BIN, path, str(fmt), str(iterations), str(epsilon), out, str(pushpull)],
                        capture_output=True, text=True, env=env, timeout=120)
     assert r.returncode == 0, r.stderr
     raw = open(out, "rb").read()
@@ -77,3 +77,22 @@ def test_c1_size_binary_file(tmp_path):
     ref, it, err, *_ = oracle.pagerank_pull_f32(csr, 100, 1e-4)
     assert r["it"] == it
     assert np.abs(r["ra"].astype(np.float64) - ref).max() / ref.min() <= REL_TOL * 10
+
+
+@pytest.mark.parametrize("rel,fmt", [("v51_e1021/graph", 0), ("v300_e2730/graph.wbin", 2),
+                                     ("graphbrew/graph.bin", 1)])
+def test_data_driven_pull_dispatch(rel, fmt, tmp_path):
+    """pushpull 6 routes pageRankGraphCSR to pageRankDataDrivenPullGraphCSR
+    (pageRank.c:900-902): iteration count and ranks follow the data-driven
+    oracle, which differs from plain pull once epsilon stops sub-epsilon
+    updates."""
+    path = os.path.join(GOLDEN, "reference_graphs", rel)
+    src, dst = _read_list(path, fmt)
+    csr = oracle.build_csr(src, dst)
+    for eps in (0.0, 1e-4):
+        r = _run(path, fmt, 50, eps, tmp_path, pushpull=6)
+        ref, it, *_ = oracle.pagerank_dd_pull_f32(csr, 50, eps)
+        assert r["it"] == it == r["itb"]
+        for ranks in (r["ra"], r["rb"]):
+            rel_err = np.abs(ranks.astype(np.float64) - ref) / ref
+            assert rel_err.max() <= REL_TOL
