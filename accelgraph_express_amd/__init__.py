@@ -7,8 +7,9 @@ Python host-side mirror used by tests and ``bench.py``.
 from . import _lib
 from ._lib import AGError, load
 from .graph import (Context, Graph, PageRankStats, Server, csr_file_info, gen_rmat, gen_uniform,
-                    get_tuning, ppr_autotune, ppr_batch, ppr_single, set_tuning)
+                    get_tuning, ppr_autotune, ppr_batch, ppr_single, set_tuning, spmv_input)
 
 __all__ = ["AGError", "Context", "Graph", "PageRankStats", "Server", "csr_file_info", "gen_rmat",
            "gen_uniform",
-           "get_tuning", "ppr_autotune", "ppr_batch", "ppr_single", "set_tuning", "load", "_lib"]
+           "get_tuning", "ppr_autotune", "ppr_batch", "ppr_single", "set_tuning", "spmv_input",
+           "load", "_lib"]

@@ -56,6 +56,11 @@ class PrOptions(C.Structure):
                 ("reserved", C.c_float)]
 
 
+class SpmvResult(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("reserved", C.c_uint32), ("y_sum", C.c_double),
+                ("loop_ms", C.c_double), ("total_ms", C.c_double), ("plan_ms", C.c_double)]
+
+
 class PrResult(C.Structure):
     _fields_ = [("iterations", C.c_uint32), ("sweeps", C.c_uint32),
                 ("active_vertices", C.c_uint32), ("fp32_switch_sweep", C.c_uint32),
@@ -133,6 +138,8 @@ _SIGS = {
     "ag_pagerank_pull_device": (C.c_int, [vp, vp, C.POINTER(PrOptions), vp,
                                           C.POINTER(PrResult)]),
     "ag_pagerank_prepare": (C.c_int, [vp, vp, f64p]),
+    "ag_spmv_pull": (C.c_int, [vp, vp, f32p, f32p, C.c_uint32, C.POINTER(SpmvResult)]),
+    "ag_spmv_pull_device": (C.c_int, [vp, vp, vp, vp, C.c_uint32, C.POINTER(SpmvResult)]),
     "ag_pagerank_plan_info": (C.c_int, [vp, C.POINTER(PlanInfo)]),
     "ag_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "ag_get_tuning": (C.c_int, [C.POINTER(Tuning)]),

@@ -10,6 +10,7 @@
 
 #include "accelgraph_amd.h"
 #include "graphCSR.h"
+#include "SPMV.h"
 #include "pageRank.h"
 #include "sortRun.h"
 #include "vertex.h"
@@ -302,6 +303,34 @@ struct PageRankStats *pageRankGraphCSR(struct Arguments *arguments, struct Graph
 
 namespace {
 
+// The device copy of a host GraphCSR, uploaded on first use and kept in
+// graph->ag_device (released by graphCSRFree).  The gathers need the forward
+// out-degree and the inverse (DIRECTED, graphConfig.h) degree / index / dest
+// arrays, as the reference's WED hands the compute units (capienv.c:396-438).
+ag_graph *resident(ag_ctx *c, struct GraphCSR *graph, const char *fn)
+{
+    ag_graph *dg = (ag_graph *)graph->ag_device;
+    if (dg) return dg;
+#if DIRECTED
+    struct Vertex *iv = graph->inverse_vertices;
+    struct EdgeList *ie = graph->inverse_sorted_edges_array;
+#else
+    struct Vertex *iv = graph->vertices;
+    struct EdgeList *ie = graph->sorted_edges_array;
+#endif
+    if (!iv || !ie || !graph->vertices) {
+        std::fprintf(stderr, "[accelgraph-amd] %s: graph has no inverse CSR\n", fn);
+        return nullptr;
+    }
+    if (ag_graph_from_csr(c, graph->num_vertices, graph->num_edges, graph->vertices->out_degree,
+                          iv->out_degree, iv->edges_idx, ie->edges_array_dest, &dg) != AG_OK) {
+        report(fn);
+        return nullptr;
+    }
+    graph->ag_device = dg;
+    return dg;
+}
+
 // Shared body of the pull entry points: time_total spans the device copy of
 // the CSR (when not resident yet), the gather-layout build and the sweep loop,
 // as the reference's spans WED mapping, AFU setup and loop.
@@ -326,27 +355,10 @@ struct PageRankStats *run_pull(struct Arguments *arguments, struct GraphCSR *gra
         std::printf(" -----------------------------------------------------\n");
     }
     const double t0 = now_s();
-    ag_graph *dg = (ag_graph *)graph->ag_device;
+    ag_graph *dg = resident(c, graph, fn);
     if (!dg) {
-#if DIRECTED
-        struct Vertex *iv = graph->inverse_vertices;
-        struct EdgeList *ie = graph->inverse_sorted_edges_array;
-#else
-        struct Vertex *iv = graph->vertices;
-        struct EdgeList *ie = graph->sorted_edges_array;
-#endif
-        if (!iv || !ie || !graph->vertices) {
-            std::fprintf(stderr, "[accelgraph-amd] %s: graph has no inverse CSR\n", fn);
-            freePageRankStats(st);
-            return nullptr;
-        }
-        if (ag_graph_from_csr(c, graph->num_vertices, graph->num_edges, graph->vertices->out_degree,
-                              iv->out_degree, iv->edges_idx, ie->edges_array_dest, &dg) != AG_OK) {
-            report(fn);
-            freePageRankStats(st);
-            return nullptr;
-        }
-        graph->ag_device = dg;
+        freePageRankStats(st);
+        return nullptr;
     }
     ag_pr_options o;
     ag_pr_options_default(&o);
@@ -400,6 +412,96 @@ struct PageRankStats *pageRankDataDrivenPullGraphCSR(struct Arguments *arguments
 {
     return run_pull(arguments, graph, AG_PR_DATA_DRIVEN, "pageRankDataDrivenPullGraphCSR",
                     "Starting Page Rank Pull DD (tolerance/epsilon)");
+}
+
+// newSPMVStatsGraphCSR (SPMV.c:51-73): both vectors zeroed.
+struct SPMVStats *newSPMVStatsGraphCSR(struct GraphCSR *graph)
+{
+    if (!graph) return nullptr;
+    struct SPMVStats *st = (struct SPMVStats *)std::calloc(1, sizeof(struct SPMVStats));
+    if (!st) return nullptr;
+    st->num_vertices = graph->num_vertices;
+    const size_t n = (size_t)std::max<uint32_t>(graph->num_vertices, 1);
+    st->vector_output = (float *)std::calloc(n, 4);
+    st->vector_input = (float *)std::calloc(n, 4);
+    if (!st->vector_output || !st->vector_input) {
+        freeSPMVStats(st);
+        return nullptr;
+    }
+    return st;
+}
+
+void freeSPMVStats(struct SPMVStats *st)
+{
+    if (!st) return;
+    std::free(st->vector_output);
+    std::free(st->vector_input);
+    std::free(st);
+}
+
+struct SPMVStats *SPMVGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    // SPMV.c:596-620: case 0 and `default` are the pull kernel; push and
+    // fixed point are outside this library and fall through to pull.
+    return SPMVPullGraphCSR(arguments, graph);
+}
+
+// SPMVPullGraphCSR (SPMV.c:622-750) on the GPU.  time_total spans the device
+// copy of the CSR (when not resident yet), the gather-layout build and the
+// `iterations` products, as the reference's spans AFU setup and its loop.
+struct SPMVStats *SPMVPullGraphCSR(struct Arguments *arguments, struct GraphCSR *graph)
+{
+    if (!arguments || !graph) return nullptr;
+    ag_ctx *c = ctx_or_null("SPMVPullGraphCSR");
+    if (!c) return nullptr;
+    struct SPMVStats *st = newSPMVStatsGraphCSR(graph);
+    if (!st) return nullptr;
+    const bool q = quiet();
+    if (!q) {
+        std::printf(" -----------------------------------------------------\n");
+        std::printf("| %-51s | \n", "          ---->>> MI355X (gfx950) <<<----");
+        std::printf(" -----------------------------------------------------\n");
+        std::printf("| %-51s | \n", "Starting SPMV-PULL");
+        std::printf(" -----------------------------------------------------\n");
+        std::printf("| %-21s | %-27s | \n", "Iteration", "Time (S)");
+        std::printf(" -----------------------------------------------------\n");
+    }
+    // benchmark input vector (SPMV.c:699-706)
+    for (uint32_t v = 0; v < graph->num_vertices; v++) {
+        if (graph->vertices->out_degree[v])
+            st->vector_input[v] = (1.0f / graph->vertices->out_degree[v]);
+        else
+            st->vector_input[v] = 0.001f;
+    }
+    const double t0 = now_s();
+    ag_graph *dg = resident(c, graph, "SPMVPullGraphCSR");
+    if (!dg) {
+        freeSPMVStats(st);
+        return nullptr;
+    }
+    ag_spmv_result r;
+    if (ag_spmv_pull(c, dg, st->vector_input, st->vector_output, arguments->iterations, &r) !=
+        AG_OK) {
+        report("SPMVPullGraphCSR");
+        freeSPMVStats(st);
+        return nullptr;
+    }
+    st->iterations = arguments->iterations;
+    st->time_total = now_s() - t0;
+    if (!q) {
+        const double per = r.iterations ? r.loop_ms / 1e3 / r.iterations : 0.0;
+        for (uint32_t i = 0; i < st->iterations; i++)
+            std::printf("| %-21u | %-27f | \n", i, per);
+        double sum = 0.0; // SPMV.c:735-739
+        for (uint32_t v = 0; v < graph->num_vertices; v++)
+            sum += ((int)(st->vector_output[v] * 100 + .5) / 100.0);
+        std::printf(" -----------------------------------------------------\n");
+        std::printf("| %-15s | %-15s | %-15s | \n", "Iterations", "Sum", "Time (S)");
+        std::printf(" -----------------------------------------------------\n");
+        std::printf("| %-15u | %-15lf | %-15f | \n", st->iterations, sum, st->time_total);
+        std::printf(" -----------------------------------------------------\n");
+    }
+    return st;
 }
 
 void pageRankPrint(float *pr, uint32_t n)

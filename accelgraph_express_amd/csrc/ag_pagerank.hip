@@ -188,6 +188,10 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
                                               uint32_t flag = 0u)
 {
 #pragma clang fp contract(off)
+    if (a.mode == kModeSpmv) { // SPMV pull: y[i] = gathered sum
+        a.pr[i] = s;
+        return;
+    }
     const float old = a.pr[i];
     if (a.mode == AG_PR_DATA_DRIVEN) {
         float keep = old;
@@ -591,6 +595,22 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
     const float c = od ? x / (float)od : 0.0f;
     store_c(a.c0, fmt0, i, c, a.f16_scale);
     if (a.skip_zero && i >= a.n_nonzero) store_c(a.c1, fmt0, i, c, a.f16_scale);
+}
+
+// SPMV pull set-up: resets the sweep state, marks every sweep FP32, and loads
+// the caller's input vector into label space (c0 = x[order[i]]).
+__global__ void k_spmv_init(SweepArgs a, const float *__restrict__ x,
+                            const uint32_t *__restrict__ order)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        SweepState *st = a.st;
+        st->done = 0; st->ticket = 0; st->iterations = 0; st->sweeps = 0;
+        st->last_active = 0; st->fp32_switch = 0;
+        st->last_err = 0.0; st->prev_err = 0.0; st->dangling = 0.0; st->tele = 0.0f;
+    }
+    if (i <= a.max_iter + 1) a.fmt[i] = kFmtF32;
+    if (i < a.N) static_cast<float *>(a.c0)[i] = x[order[i]];
 }
 
 // Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
@@ -1103,6 +1123,69 @@ void run_pagerank_seeded(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o, const
     run_pagerank_impl(ctx, g, o, &st, d_out, nullptr, nullptr, res);
 }
 
+// SPMV pull (SPMV.c:622-750): `iterations` sweeps of y = A^T x over the
+// gather layout, as the reference restarts its compute unit once per
+// iteration on the same input.  d_x / d_y are device buffers in caller order.
+static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
+                     uint32_t iterations, ag_spmv_result *res)
+{
+    hipStream_t s = ctx->stream;
+    ag_spmv_result r;
+    std::memset(&r, 0, sizeof(r));
+    Plan *p = ensure_plan(ctx, g, &r.plan_ms);
+    std::lock_guard<std::mutex> lk(p->mu);
+    const uint32_t N = p->N;
+    if (N == 0) {
+        r.iterations = iterations;
+        if (res) *res = r;
+        return;
+    }
+    if (iterations == 0) { // the compute unit never runs: the output stays zero
+        AG_HIP(hipMemsetAsync(d_y, 0, (size_t)N * 4, s));
+        AG_HIP(hipStreamSynchronize(s));
+        if (res) *res = r;
+        return;
+    }
+    ensure_work(ctx, p, iterations);
+    ag_pr_options o;
+    ag_pr_options_default(&o);
+    o.mode = kModeSpmv;
+    o.epsilon = 0.0;
+    SweepArgs a = make_args(p, o, iterations);
+    a.c1 = a.c0; // every sweep reads the same input
+    a.skip_zero = 0;
+    hipEvent_t ev[4];
+    for (auto &e : ev) AG_HIP(hipEventCreate(&e));
+    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } eg{ev};
+    AG_HIP(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_spmv_init, dim3(ceil_div((uint64_t)std::max(N, iterations + 2), 256)),
+                       dim3(256), 0, s, a, d_x, p->order.as<uint32_t>());
+    AG_HIP(hipEventRecord(ev[1], s));
+    for (uint32_t it = 0; it < iterations; it++) {
+        hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(p->work.grid), dim3(kSweepBlock),
+                           p->hot_bytes, s, a, it);
+        hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
+    }
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipEventRecord(ev[2], s));
+    const uint32_t nbf = ceil_div(N, kBlock);
+    double *parts = p->work.sum_part.as<double>();
+    hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, p->work.pr.as<float>(),
+                       p->order.as<uint32_t>(), N, 0u, d_y, parts);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipEventRecord(ev[3], s));
+    AG_HIP(hipMemcpyAsync(&r.y_sum, parts + nbf, 8, hipMemcpyDeviceToHost, s));
+    AG_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    AG_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    r.loop_ms = ms;
+    AG_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
+    r.total_ms = ms;
+    r.iterations = iterations;
+    if (res) *res = r;
+}
+
 } // namespace ag
 
 using namespace ag;
@@ -1166,6 +1249,37 @@ int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *
                                   ctx->stream));
             AG_HIP(hipStreamSynchronize(ctx->stream));
         }
+    });
+}
+
+int ag_spmv_pull(ag_ctx *ctx, ag_graph *g, const float *x, float *y, uint32_t iterations,
+                 ag_spmv_result *res)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g) fail(AG_ERR_INVALID, "null graph");
+        if (g->N && (!x || !y)) fail(AG_ERR_INVALID, "null vector");
+        if (g->N == 0) {
+            run_spmv(ctx, g, nullptr, nullptr, iterations, res);
+            return;
+        }
+        DevBuf dx((size_t)g->N * 4), dy((size_t)g->N * 4);
+        AG_HIP(hipMemcpyAsync(dx.p, x, (size_t)g->N * 4, hipMemcpyHostToDevice, ctx->stream));
+        run_spmv(ctx, g, dx.as<float>(), dy.as<float>(), iterations, res);
+        AG_HIP(hipMemcpyAsync(y, dy.p, (size_t)g->N * 4, hipMemcpyDeviceToHost, ctx->stream));
+        AG_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int ag_spmv_pull_device(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
+                        uint32_t iterations, ag_spmv_result *res)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g) fail(AG_ERR_INVALID, "null graph");
+        if (g->N && (!d_x || !d_y)) fail(AG_ERR_INVALID, "null vector");
+        if (d_x == d_y && g->N) fail(AG_ERR_INVALID, "input and output must not alias");
+        run_spmv(ctx, g, d_x, d_y, iterations, res);
     });
 }
 

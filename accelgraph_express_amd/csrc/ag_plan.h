@@ -52,6 +52,9 @@ struct Plan {
 // dangling mass to the seed set), used by ag_ppr_single.  Outside the public
 // AG_PR_* range.
 constexpr uint32_t kModePPR = 0x100;
+// Internal sweep mode for SPMV pull: the row's gathered sum is the output
+// (no damping, no contribution written); the input sits in label space in c0.
+constexpr uint32_t kModeSpmv = 0x101;
 
 // Seed table of one personalized query, in label space (device arrays).
 struct SeedTable {

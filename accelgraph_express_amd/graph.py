@@ -9,7 +9,10 @@ names follow the reference's C API where one exists:
 * ``Graph.download``       the GraphCSR arrays (``vertices->out_degree`` ...,
                            ``sorted_edges_array->edges_array_src`` ...)
 * ``Graph.pagerank_pull``  pageRankPullGraphCSR (pageRank.c:942-1108) and the
-                           PageRankStats fields it fills (pageRank.h:19-30)
+                           PageRankStats fields it fills (pageRank.h:19-30);
+                           mode="data_driven" is pageRankDataDrivenPullGraphCSR
+                           (pageRank.c:2810-2968)
+* ``Graph.spmv_pull``      SPMVPullGraphCSR (SPMV.c:622-750), unweighted
 """
 from __future__ import annotations
 
@@ -224,6 +227,26 @@ class Graph:
             total_ms=r.total_ms, plan_ms=r.plan_ms, fp32_switch_sweep=r.fp32_switch_sweep,
             active_per_iter=act[:r.sweeps].copy(), error_per_iter=err[:r.sweeps].copy())
 
+    def spmv_pull(self, x, iterations: int = 1):
+        """y[v] = sum of x[u] over the in-edges u -> v (FP32), run `iterations`
+        times on the same input.  Returns (y, SpmvResult)."""
+        n = self.num_vertices
+        xs = np.ascontiguousarray(x, np.float32)
+        if xs.size != n:
+            raise ValueError(f"x has {xs.size} entries, graph has {n} vertices")
+        y = np.zeros(n, np.float32)
+        r = L.SpmvResult()
+        L.check(L.load().ag_spmv_pull(self.ctx.h, self.h, _ptr(xs, C.c_float), _ptr(y, C.c_float),
+                                      iterations, C.byref(r)))
+        return y, r
+
+    def spmv_pull_device(self, d_x_ptr: int, d_y_ptr: int, iterations: int = 1) -> L.SpmvResult:
+        """Same on device buffers of N floats (not aliased)."""
+        r = L.SpmvResult()
+        L.check(L.load().ag_spmv_pull_device(self.ctx.h, self.h, C.c_void_p(d_x_ptr),
+                                             C.c_void_p(d_y_ptr), iterations, C.byref(r)))
+        return r
+
     def pagerank_pull_device(self, d_ranks_ptr: int, iterations: int = 20, epsilon: float = 0.0,
                              mode="reference", precision="fp32", finish_fp32=False,
                              time_sweeps: bool = False) -> L.PrResult:
@@ -241,6 +264,16 @@ class Graph:
         L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))
         return r
+
+
+def spmv_input(out_degree) -> np.ndarray:
+    """The reference's SPMV benchmark input (SPMV.c:699-706): 1/out_degree in
+    FP32, 0.001 for a vertex without out-edges."""
+    od = np.asarray(out_degree, np.uint32)
+    x = np.full(od.size, np.float32(0.001), np.float32)
+    nz = od > 0
+    x[nz] = np.float32(1.0) / od[nz].astype(np.float32)
+    return x
 
 
 def csr_file_info(path: str) -> dict:

@@ -62,6 +62,8 @@ def main() -> int:
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ppr", action="store_true", help="skip the batched-PPR leg")
+    ap.add_argument("--no-dd", action="store_true",
+                    help="skip the data-driven pull and SPMV lines")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
     ap.add_argument("--serialize", default=None, metavar="DIR",
                     help="also time saving / loading the graph as an AGCSRv1 file in DIR")
@@ -151,6 +153,42 @@ def main() -> int:
     log(f"step {ms_step:.3f} ms, sweep kernel {sweep_ms * 1e3:.1f} us, {gteps:.1f} GTEPS, "
         f"{achieved:.0f} GB/s algorithmic")
 
+    # data-driven pull (pushpull 6) on the same graph at the reference CLI's
+    # epsilon: the worklist test rides on the gather, so a sweep costs a pull sweep
+    dd = None
+    if not args.no_dd:
+        g.pagerank_pull_device(out.data_ptr(), iterations=iters, epsilon=1e-4,
+                               mode="data_driven")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = g.pagerank_pull_device(out.data_ptr(), iterations=iters, epsilon=1e-4,
+                                   mode="data_driven", time_sweeps=True)
+        torch.cuda.synchronize()
+        dd_ms = (time.perf_counter() - t) * 1e3
+        dd = {"epsilon": 1e-4, "iterations": r.iterations, "sweeps": r.sweeps,
+              "ms": round(dd_ms, 3), "sweep_us": round(r.sweep_kernel_ms / max(r.sweeps, 1) * 1e3, 2),
+              "updated_last_sweep": r.active_vertices}
+        log(f"data-driven pull eps 1e-4: {r.sweeps} sweeps in {dd_ms:.2f} ms "
+            f"({dd['sweep_us']} us per sweep kernel), {r.active_vertices} updated in the last")
+
+    # SPMV pull (SPMVPullGraphCSR) on the same graph and the reference's
+    # benchmark input, `iters` products
+    spmv = None
+    if not args.no_dd:
+        fwd_od = np.bincount(src, minlength=N).astype(np.uint32)
+        dx = torch.from_numpy(ag.spmv_input(fwd_od)).to("cuda")
+        dy = torch.empty(N, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        g.spmv_pull_device(dx.data_ptr(), dy.data_ptr(), iters)
+        r = g.spmv_pull_device(dx.data_ptr(), dy.data_ptr(), iters)
+        per = r.loop_ms / iters
+        spmv = {"iterations": iters, "loop_ms": round(r.loop_ms, 3),
+                "ms_per_product": round(per, 4),
+                "GTEPS": round(E / (per / 1e3) / 1e9, 2), "y_sum": r.y_sum}
+        log(f"SPMV pull: {iters} products in {r.loop_ms:.2f} ms ({per * 1e3:.1f} us each, "
+            f"{spmv['GTEPS']} GTEPS)")
+        del dx, dy
+
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"pr_sweep_rmat_s{scale}_traffic.json")
     if os.path.exists(tf):
@@ -192,7 +230,9 @@ def main() -> int:
         "config": {"workload": f"rmat_s{scale}_ef{ef}_pull_pagerank_fp32_{iters}sweeps",
                    "num_vertices": N, "num_edges": E, "iterations": iters, "epsilon": 0.0,
                    "csr_build_ms": round(csr_ms, 2), "gather_layout_ms": round(plan_ms, 2),
-                   "plan": info, **({"serialization": ser} if ser else {})},
+                   "plan": info, **({"serialization": ser} if ser else {}),
+                   **({"data_driven_pull": dd} if dd else {}),
+                   **({"spmv_pull": spmv} if spmv else {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_pr_sweep",

@@ -180,6 +180,28 @@ int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *
 int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
                             float *d_ranks, ag_pr_result *res);
 
+/* ---- SPMV pull (SPMV.c:622-750, SPMVPullGraphCSR) ---------------------- */
+typedef struct ag_spmv_result
+{
+    uint32_t iterations; /* sweeps run (the reference restarts its compute unit per iteration) */
+    uint32_t reserved;
+    double y_sum;        /* FP64 sum of y */
+    double loop_ms;      /* device time of the sweeps (HIP events) */
+    double total_ms;     /* device time including input relabel and output un-relabel */
+    double plan_ms;      /* gather-layout build time (0 if cached) */
+} ag_spmv_result;
+
+/* y[v] = sum of x[u] over v's in-edges u -> v (unweighted, the reference's
+ * WEIGHTED 0 build), FP32 accumulation, repeated `iterations` times on the
+ * same input (0: y is all zero, as the reference's output buffer starts).
+ * Host buffers of N floats. */
+int ag_spmv_pull(ag_ctx *ctx, ag_graph *g, const float *x, float *y, uint32_t iterations,
+                 ag_spmv_result *res);
+
+/* Same on device buffers (N floats each, not aliased). */
+int ag_spmv_pull_device(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
+                        uint32_t iterations, ag_spmv_result *res);
+
 /* Builds (or rebuilds) the degree-binned gather layout of g now, so that a
  * timed call does not include it.  Returns its build time in *ms. */
 int ag_pagerank_prepare(ag_ctx *ctx, ag_graph *g, double *ms);

@@ -338,6 +338,30 @@ uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
     return it;
 }
 
+void oracle_spmv_input(uint32_t N, const uint32_t *fwd_outdeg, float *x)
+{
+    for (uint32_t v = 0; v < N; v++)
+        x[v] = fwd_outdeg[v] ? (1.0f / fwd_outdeg[v]) : 0.001f;
+}
+
+void oracle_spmv_pull_f32(uint32_t N, const uint32_t *inv_deg, const uint32_t *inv_idx,
+                          const uint32_t *inv_dest, const float *x, float *y)
+{
+    for (uint32_t v = 0; v < N; v++) {
+        float s = 0.0f;
+        uint32_t j0 = inv_idx[v], j1 = inv_idx[v] + inv_deg[v];
+        for (uint32_t j = j0; j < j1; j++) s += x[inv_dest[j]];
+        y[v] = s;
+    }
+}
+
+double oracle_spmv_sum(uint32_t N, const float *y)
+{
+    double sum = 0.0;
+    for (uint32_t v = 0; v < N; v++) sum += ((int)(y[v] * 100 + .5) / 100.0);
+    return sum;
+}
+
 /* ------------------------------------------------------------------------- */
 /* OpenMP baselines                                                          */
 /* ------------------------------------------------------------------------- */

@@ -96,6 +96,17 @@ uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
                                      double *pr_sum, uint32_t *active_iter, double *err_iter,
                                      uint32_t *work_iter);
 
+/* SPMVPullGraphCSR (SPMV.c:622-750), unweighted (graphConfig.h:7
+ * WEIGHTED 0): the input vector of :699-706, x[v] = 1/out_degree(v) (FP32
+ * divide) or 0.001f without out-edges ... */
+void oracle_spmv_input(uint32_t N, const uint32_t *fwd_outdeg, float *x);
+/* ... y[v] = sum of x over v's inverse edge range, FP32, in edge order (the
+ * gather the SPMV compute unit runs with auxiliary1 = x, auxiliary2 = y). */
+void oracle_spmv_pull_f32(uint32_t N, const uint32_t *inv_deg, const uint32_t *inv_idx,
+                          const uint32_t *inv_dest, const float *x, float *y);
+/* The printed checksum (:735-739): sum of (int)(y*100 + .5) / 100.0. */
+double oracle_spmv_sum(uint32_t N, const float *y);
+
 /* OpenMP CPU baselines (reported beside the GPU, never the product path). */
 double oracle_csr_build_omp(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
                             uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,

@@ -40,6 +40,9 @@ def load() -> C.CDLL:
         "oracle_pagerank_pull_f32": (U, [U, u32p, u32p, u32p, u32p, U, D, f32p, f64p, f64p,
                                          u32p, f64p]),
         "oracle_pagerank_pull_f64": (U, [U, u32p, u32p, u32p, u32p, U, D, f64p, f64p]),
+        "oracle_spmv_input": (None, [U, u32p, f32p]),
+        "oracle_spmv_pull_f32": (None, [U, u32p, u32p, u32p, f32p, f32p]),
+        "oracle_spmv_sum": (D, [U, f32p]),
         "oracle_pagerank_dd_pull_f32": (U, [U] + [u32p] * 6 + [U, D, f32p, f64p, f64p, u32p,
                                                               f64p, u32p]),
         "oracle_pagerank_standard_f64": (U, [U, u32p, u32p, u32p, u32p, D, U, D, f64p, f64p]),
@@ -145,6 +148,31 @@ def pagerank_dd_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4)
     sweeps = min(iterations, it + 1) if iterations else 0
     return (pr, int(it), et.value, ps.value, act[:sweeps].copy(), err[:sweeps].copy(),
             work[:sweeps].copy())
+
+
+def spmv_input(csr: dict) -> np.ndarray:
+    """The reference's benchmark input vector: 1/out_degree, 0.001 without out-edges."""
+    n = csr["num_vertices"]
+    x = np.zeros(n, np.float32)
+    load().oracle_spmv_input(n, _p(_u(csr["vertices.out_degree"])), _p(x, C.c_float))
+    return x
+
+
+def spmv_pull_f32(csr: dict, x) -> np.ndarray:
+    """y[v] = sum of x over v's in-edges (inverse CSR order), FP32."""
+    n = csr["num_vertices"]
+    _, idg, iidx, idst = _inv(csr)
+    xs = np.ascontiguousarray(x, np.float32)
+    assert xs.size == n
+    y = np.zeros(n, np.float32)
+    load().oracle_spmv_pull_f32(n, _p(idg), _p(iidx), _p(idst), _p(xs, C.c_float),
+                                _p(y, C.c_float))
+    return y
+
+
+def spmv_sum(y) -> float:
+    ys = np.ascontiguousarray(y, np.float32)
+    return float(load().oracle_spmv_sum(ys.size, _p(ys, C.c_float)))
 
 
 def pagerank_pull_f64(csr: dict, iterations: int = 20, epsilon: float = 1e-4):

@@ -1,0 +1,148 @@
+"""GPU SPMV pull (SPMVPullGraphCSR, SPMV.c:622-750, unweighted) against the
+oracle.
+
+Bar: rows of in-degree <= low_max sum sequentially in the reference's edge
+order, so on graphs with only such rows the output equals the FP32 oracle bit
+for bit.  On skewed graphs hub and mid rows sum in another order: every output
+is within REL_TOL = 1e-5 relative of the FP64 product (positive inputs), and
+within REL_TOL of the FP32 oracle except where that oracle's own sequential
+sum is further from FP64.  Signed inputs: within k * 2^-22 * sum|terms| of
+FP64 (k = in-degree).  Repeating the product (`iterations`) leaves the output
+unchanged; iterations 0 leaves it zero, as the reference's output starts."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+import accelgraph_express_amd as ag
+import oracle
+
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+REL_TOL = 1e-5
+BIN = os.path.join(ROOT, "tests", "c", "_bin", "dropin_spmv")
+
+
+def _y64(src, dst, x, n):
+    return np.bincount(dst, weights=x[src].astype(np.float64), minlength=n)
+
+
+def _case(ctx, src, dst, n, x=None, iterations=1):
+    print(f"case: spmv N={n} E={len(src)} iterations={iterations}", flush=True)
+    g = ag.Graph.build(ctx, src, dst, n)
+    csr = oracle.build_csr(src, dst, n)
+    if x is None:
+        x = oracle.spmv_input(csr)
+    y, r = g.spmv_pull(x, iterations)
+    return g, csr, x, y, r
+
+
+def test_low_rows_exact(ctx):
+    src, dst = ag.gen_uniform(99, 4096, 20000)
+    g, csr, x, y, r = _case(ctx, src, dst, 4096)
+    g.prepare()
+    info = g.plan_info()
+    assert info["n_mid"] == 0 and info["n_hub"] == 0
+    np.testing.assert_array_equal(y, oracle.spmv_pull_f32(csr, x))
+    np.testing.assert_array_equal(x, ag.spmv_input(csr["vertices.out_degree"]))
+    assert r.iterations == 1
+    assert abs(r.y_sum - y.astype(np.float64).sum()) <= 1e-9 * r.y_sum
+
+
+def test_c1_uniform_exact(ctx):
+    src, dst = ag.gen_uniform(27491095, 65536, 1048576)
+    g, csr, x, y, r = _case(ctx, src, dst, 65536, iterations=20)
+    ref = oracle.spmv_pull_f32(csr, x)
+    np.testing.assert_array_equal(y, ref)
+    assert oracle.spmv_sum(y) == oracle.spmv_sum(ref)
+    assert r.iterations == 20 and r.loop_ms > 0
+
+
+def test_skewed_bins(ctx):
+    ag.set_tuning(low_max=24, hub_min=512, hub_chunk=256)
+    try:
+        src, dst = ag.gen_rmat(77, 16, 16)
+        n = 1 << 16
+        g, csr, x, y, r = _case(ctx, src, dst, n, iterations=3)
+        g.prepare()
+        info = g.plan_info()
+        assert min(info["n_hub"], info["n_mid"], info["n_low"], info["n_zero"]) > 0
+    finally:
+        ag.set_tuning(low_max=64, hub_min=4096, hub_chunk=4096)
+    y64 = _y64(src, dst, x, n)
+    ref = oracle.spmv_pull_f32(csr, x)
+    nz = y64 > 0
+    rel64 = np.abs(y[nz] - y64[nz]) / y64[nz]
+    assert rel64.max() <= REL_TOL
+    assert np.all(y[~nz] == 0)
+    rel32 = np.abs(y[nz].astype(np.float64) - ref[nz]) / y64[nz]
+    off = rel32 > REL_TOL
+    oracle_err = np.abs(ref[nz].astype(np.float64) - y64[nz]) / y64[nz]
+    assert np.all(rel64[off] <= oracle_err[off])
+
+
+def test_signed_input_and_repeats(ctx):
+    src, dst = ag.gen_rmat(31, 15, 16)
+    n = 1 << 15
+    x = np.random.RandomState(2).randn(n).astype(np.float32)
+    g, csr, _, y1, _ = _case(ctx, src, dst, n, x=x, iterations=1)
+    y3, _ = g.spmv_pull(x, 3)
+    np.testing.assert_array_equal(y1, y3)
+    y64 = _y64(src, dst, x, n)
+    mag = np.bincount(dst, weights=np.abs(x[src]).astype(np.float64), minlength=n)
+    k = np.bincount(dst, minlength=n)
+    assert np.all(np.abs(y1 - y64) <= k * 2.0 ** -22 * mag + 1e-30)
+    y0, r0 = g.spmv_pull(x, 0)
+    assert not y0.any() and r0.iterations == 0
+
+
+def test_device_buffers_and_refusals(ctx):
+    src, dst = ag.gen_uniform(7, 10000, 80000)
+    g = ag.Graph.build(ctx, src, dst, 10000)
+    x = torch.rand(10000, device="cuda")
+    y = torch.empty(10000, device="cuda")
+    torch.cuda.synchronize()  # torch's stream is not the library's
+    r = g.spmv_pull_device(x.data_ptr(), y.data_ptr(), 2)
+    torch.cuda.synchronize()
+    want, _ = g.spmv_pull(x.cpu().numpy(), 1)
+    np.testing.assert_array_equal(y.cpu().numpy(), want)
+    assert abs(r.y_sum - want.astype(np.float64).sum()) <= 1e-9 * r.y_sum
+    with pytest.raises(ag.AGError):
+        g.spmv_pull_device(x.data_ptr(), x.data_ptr(), 1)
+    with pytest.raises(ValueError):
+        g.spmv_pull(np.zeros(5, np.float32))
+    e = ag.Graph.build(ctx, np.zeros(0, np.uint32), np.zeros(0, np.uint32), 10)
+    ye, _ = e.spmv_pull(np.ones(10, np.float32))
+    assert not ye.any()
+
+
+@pytest.mark.parametrize("rel,fmt", [("v51_e1021/graph", 0), ("v300_e2730/graph.wbin", 2),
+                                     ("graphbrew/graph.bin", 1)])
+def test_dropin_spmv(rel, fmt, tmp_path):
+    """A C program against the reference-named SPMV.h: SPMVGraphCSR builds the
+    benchmark input, and the output follows the oracle on the reference's own
+    fixture graphs, in both calls (the second on the resident graph)."""
+    path = os.path.join(GOLDEN, "reference_graphs", rel)
+    out = str(tmp_path / "spmv.bin")
+    print(f"case: dropin_spmv {rel}", flush=True)
+    r = subprocess.run([BIN, path, str(fmt), "5", out], capture_output=True, text=True,
+                       env=dict(os.environ, AG_QUIET="1"), timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = open(out, "rb").read()
+    V, it = np.frombuffer(raw[:8], np.uint32)
+    x = np.frombuffer(raw[8:8 + 4 * V], np.float32)
+    ya = np.frombuffer(raw[8 + 4 * V:8 + 8 * V], np.float32)
+    yb = np.frombuffer(raw[8 + 8 * V:8 + 12 * V], np.float32)
+    if fmt == 0:
+        a = np.loadtxt(path, dtype=np.int64).reshape(-1, 2)
+    else:
+        a = np.fromfile(path, dtype="<i4").reshape(-1, 3 if fmt == 2 else 2)[:, :2]
+    csr = oracle.build_csr(a[:, 0].astype(np.uint32), a[:, 1].astype(np.uint32))
+    assert it == 5 and V == csr["num_vertices"]
+    np.testing.assert_array_equal(x, oracle.spmv_input(csr))
+    ref = oracle.spmv_pull_f32(csr, x)
+    np.testing.assert_allclose(ya, ref, rtol=REL_TOL)
+    np.testing.assert_array_equal(ya, yb)
