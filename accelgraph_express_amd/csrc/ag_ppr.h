@@ -111,6 +111,7 @@ class PprBatch {
 
   private:
     void flush();
+    void choose_layout();
     void upload_seed_table();
     void launch_sweep(bool from0);
     uint32_t lanes_per_row() const;
@@ -121,6 +122,8 @@ class PprBatch {
     float damp_;
     int cur_ = 0;          // contribution buffer read by the next sweep
     uint32_t graph_lpr_ = 0; // lanes per row the captured graph was built for
+    uint32_t blk_ = 64;      // BF16 split block (columns per highs/lows block)
+    uint32_t graph_blk_ = 0; // split block the captured graph was built for
     bool dirty_ = true;
     int blocks_[5] = {0, 0, 0, 0, 0}; // sweep workgroups per CU by log2(lanes per row)
     DevBuf X_, C_[2], cols_, bstate_, blk_res_, blk_dang_, part_;

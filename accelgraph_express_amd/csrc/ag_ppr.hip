@@ -69,6 +69,7 @@ constexpr double kCoarseFloor = AG_PPR_COARSE_FLOOR;
 
 struct PprArgs {
     uint32_t N, cap;
+    uint32_t blk; // BF16 split block (32 or 64 columns)
     uint32_t n_nonlow, n_nonzero;
     uint32_t nch, nmulti;
     float damp;
@@ -98,29 +99,35 @@ struct PprAdmit {
 };
 
 // BF16 storage: byte offsets, within a row of 4 * cap bytes, of the high and
-// the low half of column c (blocks of up to 32 columns: highs, then lows).
-__device__ __forceinline__ uint32_t split_hi(uint32_t cap, uint32_t c)
+// the low half of column c.  Columns form blocks of up to `blk` (32 or 64):
+// the block's high halves, then its low halves.
+//   * blk 64: a full row (cap 64) is one 128-byte line of highs followed by
+//     one of lows, so a coarse sweep reads one line per gathered row;
+//   * blk 32: columns 0-31 keep both halves in one line, so a narrow fine
+//     sweep (a drained batch packed into the low slots) reads one line.
+// The batch switches between the two as its used slots widen and narrow.
+__device__ __forceinline__ uint32_t split_hi(uint32_t cap, uint32_t blk, uint32_t c)
 {
-    const uint32_t b = c & ~31u;
+    const uint32_t b = c & ~(blk - 1);
     return b * 4 + (c - b) * 2;
 }
-__device__ __forceinline__ uint32_t split_lo(uint32_t cap, uint32_t c)
+__device__ __forceinline__ uint32_t split_lo(uint32_t cap, uint32_t blk, uint32_t c)
 {
-    const uint32_t b = c & ~31u;
-    return split_hi(cap, c) + min(32u, cap - b) * 2;
+    const uint32_t b = c & ~(blk - 1);
+    return split_hi(cap, blk, c) + min(blk, cap - b) * 2;
 }
 
 // Scalar store of one contribution (slot management kernels).
-__device__ __forceinline__ void st(void *buf, uint32_t fmt, uint32_t cap, uint32_t row, uint32_t col,
-                                   float c)
+__device__ __forceinline__ void st(void *buf, uint32_t fmt, uint32_t cap, uint32_t blk, uint32_t row,
+                                   uint32_t col, float c)
 {
     if (fmt == AG_PREC_FP32) {
         static_cast<float *>(buf)[(uint64_t)row * cap + col] = c;
     } else {
         const uint32_t b = __float_as_uint(c);
         char *r = static_cast<char *>(buf) + (uint64_t)row * cap * 4;
-        *reinterpret_cast<uint16_t *>(r + split_hi(cap, col)) = (uint16_t)(b >> 16);
-        *reinterpret_cast<uint16_t *>(r + split_lo(cap, col)) = (uint16_t)(b & 0xffffu);
+        *reinterpret_cast<uint16_t *>(r + split_hi(cap, blk, col)) = (uint16_t)(b >> 16);
+        *reinterpret_cast<uint16_t *>(r + split_lo(cap, blk, col)) = (uint16_t)(b & 0xffffu);
     }
 }
 
@@ -271,8 +278,8 @@ template <> struct Store<bf16_t> {
             l[j] = (b0 & 0xffffu) | (b1 << 16);
         }
         char *r = static_cast<char *>(a.Cout) + (uint64_t)row * a.cap * 4;
-        *reinterpret_cast<uint4 *>(r + split_hi(a.cap, c0)) = make_uint4(h[0], h[1], h[2], h[3]);
-        *reinterpret_cast<uint4 *>(r + split_lo(a.cap, c0)) = make_uint4(l[0], l[1], l[2], l[3]);
+        *reinterpret_cast<uint4 *>(r + split_hi(a.cap, a.blk, c0)) = make_uint4(h[0], h[1], h[2], h[3]);
+        *reinterpret_cast<uint4 *>(r + split_lo(a.cap, a.blk, c0)) = make_uint4(l[0], l[1], l[2], l[3]);
     }
 };
 
@@ -512,8 +519,8 @@ __global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
                 SplitRows<true> rows;
                 rows.base = static_cast<const char *>(a.Cin);
                 rows.stride = a.cap * 4;
-                rows.hoff = split_hi(a.cap, c0);
-                rows.loff = split_lo(a.cap, c0);
+                rows.hoff = split_hi(a.cap, a.blk, c0);
+                rows.loff = split_lo(a.cap, a.blk, c0);
                 const uint32_t fa = fmask & amask;
 #pragma unroll
                 for (int j = 0; j < 4; j++)
@@ -524,8 +531,8 @@ __global__ __launch_bounds__(kBlock) void k_ppr_sweep(PprArgs a)
                 SplitRows<false> rows;
                 rows.base = static_cast<const char *>(a.Cin);
                 rows.stride = a.cap * 4;
-                rows.hoff = split_hi(a.cap, c0);
-                rows.loff = split_lo(a.cap, c0);
+                rows.hoff = split_hi(a.cap, a.blk, c0);
+                rows.loff = split_lo(a.cap, a.blk, c0);
 #pragma unroll
                 for (int j = 0; j < 4; j++) rows.fm[j] = 0u;
                 sweep_units<LPR, CT>(a, rows, c0, amask, t, res, dang, nz);
@@ -684,20 +691,21 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t np
 // Column (re)initialisation of the slots admitted in one step: X and both
 // contribution buffers cleared, one thread per (row, admitted slot).
 __global__ void k_ppr_clear_cols(float *X, void *C0, void *C1, uint32_t N, uint32_t cap,
-                                 const PprAdmit *adm, uint32_t na, uint32_t fmt)
+                                 uint32_t blk, const PprAdmit *adm, uint32_t na, uint32_t fmt)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)N * na) return;
     const uint32_t row = (uint32_t)(i / na), q = (uint32_t)(i % na);
     const uint64_t k = (uint64_t)row * cap + adm[q].slot;
     X[k] = 0.0f;
-    st(C0, fmt, cap, row, adm[q].slot, 0.0f);
-    st(C1, fmt, cap, row, adm[q].slot, 0.0f);
+    st(C0, fmt, cap, blk, row, adm[q].slot, 0.0f);
+    st(C1, fmt, cap, blk, row, adm[q].slot, 0.0f);
 }
 
 // One workgroup per admitted slot: x0 = s on the seed rows, contributions
 // s / outdeg in the buffer the next sweep reads, column state.
-__global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap, uint32_t fmt, const PprAdmit *adm, const uint32_t *sv,
+__global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap,
+                                uint32_t blk, uint32_t fmt, const PprAdmit *adm, const uint32_t *sv,
                                 const float *sw, PprCol *cs, float damp)
 {
     const PprAdmit d = adm[blockIdx.x];
@@ -706,7 +714,7 @@ __global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uin
         const uint64_t k = (uint64_t)v * cap + d.slot;
         X[k] = sw[q];
         const uint32_t od = outdeg[v];
-        st(Cin, fmt, cap, v, d.slot, od ? sw[q] / (float)od : 0.0f);
+        st(Cin, fmt, cap, blk, v, d.slot, od ? sw[q] / (float)od : 0.0f);
     }
     if (threadIdx.x == 0) {
         double d0 = 0.0;
@@ -738,8 +746,8 @@ __global__ void k_ppr_stop_cols(PprCol *cs, const PprAdmit *adm, uint32_t n)
 // (no in-edges, not a seed) keep the moved column's values, not those of the
 // slot's previous column.  adm[q].slot = source, adm[q].e0 = destination; one
 // thread per (row, move).
-__global__ void k_ppr_move_cols(float *X, void *C0, void *C1, uint32_t cap, uint32_t fmt,
-                                const PprAdmit *adm, uint32_t na, uint32_t N)
+__global__ void k_ppr_move_cols(float *X, void *C0, void *C1, uint32_t cap, uint32_t blk,
+                                uint32_t fmt, const PprAdmit *adm, uint32_t na, uint32_t N)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)N * na) return;
@@ -751,13 +759,48 @@ __global__ void k_ppr_move_cols(float *X, void *C0, void *C1, uint32_t cap, uint
         c0[d] = c0[s];
         c1[d] = c1[s];
     } else {
-        const uint32_t hs = split_hi(cap, adm[q].slot), ls = split_lo(cap, adm[q].slot);
-        const uint32_t hd = split_hi(cap, adm[q].e0), ld = split_lo(cap, adm[q].e0);
+        const uint32_t hs = split_hi(cap, blk, adm[q].slot), ls = split_lo(cap, blk, adm[q].slot);
+        const uint32_t hd = split_hi(cap, blk, adm[q].e0), ld = split_lo(cap, blk, adm[q].e0);
         for (void *buf : {C0, C1}) {
             char *r = static_cast<char *>(buf) + (uint64_t)row * cap * 4;
             *reinterpret_cast<uint16_t *>(r + hd) = *reinterpret_cast<const uint16_t *>(r + hs);
             *reinterpret_cast<uint16_t *>(r + ld) = *reinterpret_cast<const uint16_t *>(r + ls);
         }
+    }
+}
+
+// Re-lays every row of a split-BF16 buffer from `from`-column blocks to
+// `to`-column blocks, in place: a workgroup stages kRelayRows rows in LDS,
+// then writes each 32-bit word of the new layout from its two halves.
+constexpr uint32_t kRelayBlock = 256, kRelayBytes = 4096;
+__global__ __launch_bounds__(kRelayBlock) void k_ppr_relayout(void *C, uint32_t N, uint32_t cap,
+                                                             uint32_t from, uint32_t to)
+{
+    __shared__ uint16_t h[kRelayBytes / 2];
+    const uint32_t rb = cap * 4;           // row bytes
+    const uint32_t rows = kRelayBytes / rb; // rows per workgroup (cap <= 64: >= 16)
+    const uint64_t r0 = (uint64_t)blockIdx.x * rows;
+    const uint32_t nr = (uint32_t)min<uint64_t>(rows, N - r0);
+    uint32_t *g = reinterpret_cast<uint32_t *>(static_cast<char *>(C) + r0 * rb);
+    uint32_t *lw = reinterpret_cast<uint32_t *>(h);
+    const uint32_t words = nr * cap; // 32-bit words staged
+    for (uint32_t i = threadIdx.x; i < words; i += kRelayBlock) lw[i] = g[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < words; i += kRelayBlock) {
+        const uint32_t row = i / cap, w = i - row * cap;
+        uint32_t out = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 2; k++) {
+            // 16-bit index j of the new layout -> column and half
+            const uint32_t j = 2 * w + k;
+            const uint32_t b = (j / (2 * to)) * to; // first column of j's block
+            const uint32_t wb = min(to, cap - b), r = j - 2 * b;
+            const bool hi = r < wb;
+            const uint32_t col = hi ? b + r : b + r - wb;
+            const uint32_t src = (hi ? split_hi(cap, from, col) : split_lo(cap, from, col)) / 2;
+            out |= (uint32_t)h[row * (rb / 2) + src] << (16 * k);
+        }
+        g[i] = out;
     }
 }
 
@@ -1121,7 +1164,7 @@ void PprBatch::compact()
     hipStream_t s = ctx_->stream;
     AG_HIP(hipMemcpyAsync(desc_.p, mv.data(), mv.size() * sizeof(PprAdmit), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_ppr_move_cols, dim3(ceil_div((uint64_t)N_ * mv.size(), 256)), dim3(256), 0, s,
-                       X_.as<float>(), C_[0].p, C_[1].p, cap_, fmt_, desc_.as<PprAdmit>(),
+                       X_.as<float>(), C_[0].p, C_[1].p, cap_, blk_, fmt_, desc_.as<PprAdmit>(),
                        (uint32_t)mv.size(), N_);
     hipLaunchKernelGGL(k_ppr_move_state, dim3(1), dim3(kMaxCap), 0, s, cols_.as<PprCol>(),
                        desc_.as<PprAdmit>(), (uint32_t)mv.size());
@@ -1142,11 +1185,35 @@ void PprBatch::compact()
     dirty_ = true;
 }
 
+// Split-BF16 block for the used slots: 64-column blocks while more than 48
+// slots are in use (coarse sweeps read one line per row), 32-column blocks
+// once at most 32 are (fine sweeps of the packed columns read one line); in
+// between the current layout stays.  Running columns are re-laid in place in
+// both contribution buffers (rows no sweep rewrites keep their values).
+void PprBatch::choose_layout()
+{
+    if (fmt_ != AG_PREC_BF16 || cap_ <= 32) return; // one block either way
+    uint32_t want = blk_;
+    if (blk_ == 64 && hi_ <= 32) want = 32;
+    if (blk_ == 32 && hi_ > 48) want = 64;
+    if (want == blk_) return;
+    if (used_count_ > pending_.size()) { // running columns hold data
+        hipStream_t s = ctx_->stream;
+        const uint32_t rows = kRelayBytes / (cap_ * 4);
+        for (int b = 0; b < 2; b++)
+            hipLaunchKernelGGL(k_ppr_relayout, dim3(ceil_div(N_, rows)), dim3(kRelayBlock), 0, s,
+                               C_[b].p, N_, cap_, blk_, want);
+        AG_HIP(hipGetLastError());
+    }
+    blk_ = want;
+}
+
 // Applies the step's slot changes: stops released columns, rebuilds the seed
 // table, and initialises all admitted columns in one pass over the rows.
 void PprBatch::flush()
 {
     if (!dirty_) return;
+    choose_layout();
     hipStream_t s = ctx_->stream;
     std::vector<PprAdmit> adm;
     if (!stop_.empty()) {
@@ -1175,10 +1242,10 @@ void PprBatch::flush()
         AG_HIP(hipMemcpyAsync(desc_.p, adm.data(), adm.size() * sizeof(PprAdmit),
                               hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_ppr_clear_cols, dim3(ceil_div((uint64_t)N_ * adm.size(), 256)), dim3(256),
-                           0, s, X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, desc_.as<PprAdmit>(),
-                           (uint32_t)adm.size(), fmt_);
+                           0, s, X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, blk_,
+                           desc_.as<PprAdmit>(), (uint32_t)adm.size(), fmt_);
         hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)adm.size()), dim3(256), 0, s,
-                           X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, fmt_,
+                           X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, blk_, fmt_,
                            desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(), seed_w_.as<float>(),
                            cols_.as<PprCol>(), damp_);
         AG_HIP(hipGetLastError());
@@ -1194,6 +1261,7 @@ void PprBatch::launch_sweep(bool from0)
     std::memset(&a, 0, sizeof(a));
     a.N = N_;
     a.cap = cap_;
+    a.blk = blk_;
     a.n_nonlow = p_->n_nonlow;
     a.n_nonzero = p_->n_nonzero;
     a.nch = p_->ppr_nch;
@@ -1248,10 +1316,11 @@ void PprBatch::sweep(uint32_t n, bool replay, bool timed)
     uint32_t done = 0;
     if (replay) {
         // A captured pair of sweeps (C0 -> C1 -> C0), replayed while the
-        // current buffer is C0; recaptured when the lanes per row change.
+        // current buffer is C0; recaptured when the lanes per row or the
+        // split layout change.
         if (cur_ == 1) { launch_sweep(false); cur_ = 0; done++; }
         const uint32_t lpr = lanes_per_row();
-        if (gexec_ && graph_lpr_ != lpr) {
+        if (gexec_ && (graph_lpr_ != lpr || graph_blk_ != blk_)) {
             (void)hipGraphExecDestroy(gexec_);
             (void)hipGraphDestroy(graph_);
             gexec_ = nullptr;
@@ -1264,6 +1333,7 @@ void PprBatch::sweep(uint32_t n, bool replay, bool timed)
             AG_HIP(hipStreamEndCapture(s, &graph_));
             AG_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
             graph_lpr_ = lpr;
+            graph_blk_ = blk_;
         }
         while (n - done >= 2) {
             AG_HIP(hipGraphLaunch(gexec_, s));

@@ -138,6 +138,25 @@ def test_bf16_batch_alone_equals_mixed(ctx):
         np.testing.assert_array_equal(Xa[0], X[i])
 
 
+def test_bf16_wide_batch_relayout_alone_equals_mixed(ctx):
+    """A 64-column BF16 batch stores 64-column split blocks (one line of high
+    halves per row); once it drains into at most 32 slots the rows are re-laid
+    into 32-column blocks in place.  Tolerances spread over two decades so the
+    columns retire unevenly and the survivors are packed and re-laid while
+    running: every checked result equals the query run alone, bit for bit."""
+    g, csr, n = _graph(ctx)
+    rng = np.random.RandomState(41)
+    qs = [(rng.randint(0, n, rng.randint(1, 17)).astype(np.uint32),
+           float(10 ** rng.uniform(-6, -4)), 1000) for _ in range(64)]
+    X, st, _ = ag.ppr_batch(g, qs, precision="bf16", cols_per_wave=64)
+    its = np.array([s.iterations for s in st])
+    assert its.max() > its.min() + 3  # uneven: the batch drains through compaction
+    for i in list(np.argsort(its)[-12:]) + [0, 31, 63]:
+        Xa, sa, _ = ag.ppr_batch(g, [qs[i]], precision="bf16", cols_per_wave=64)
+        assert sa[0].iterations == st[i].iterations, i
+        np.testing.assert_array_equal(Xa[0], X[i])
+
+
 def test_single_bf16_finishes_in_fp32(ctx):
     g, csr, n = _graph(ctx)
     seeds = np.array([5, 77, 4000], np.uint32)

@@ -191,3 +191,28 @@ def test_peek_is_zero_copy_and_matches_poll(ctx):
         srv.release(ta)
     assert srv.peek(ta)[0] == L.AG_ERR_NOT_FOUND
     srv.close()
+
+
+def test_bf16_layout_switches_while_running(ctx):
+    """A 64-slot BF16 batch that starts narrow (32-column split blocks), widens
+    past 48 running columns (re-laid to 64-column blocks while running) and
+    drains again (re-laid back): every result equals the query alone."""
+    graphs = _setup(ctx)
+    g, csr, n = graphs[1]
+    rng = np.random.RandomState(12)
+    qs = [(rng.randint(0, n, rng.randint(1, 17)).astype(np.uint32),
+           float(10 ** rng.uniform(-6, -5))) for _ in range(64)]
+    srv = ag.Server(ctx, precision="bf16", cols_per_wave=64, max_columns=64)
+    gid = srv.add_graph(g)
+    tickets = [srv.submit(gid, s, tol) for s, tol in qs[:10]]
+    srv.step()
+    srv.step()
+    tickets += [srv.submit(gid, s, tol) for s, tol in qs[10:]]
+    res = _drain(srv, list(tickets), {t: n for t in tickets})
+    for i in (0, 5, 9, 10, 40, 63):
+        code, x, st = res[tickets[i]]
+        assert code == L.AG_OK
+        Xa, sa, _ = ag.ppr_batch(g, [(qs[i][0], qs[i][1], 1000)], precision="bf16",
+                                 cols_per_wave=64)
+        np.testing.assert_array_equal(x, Xa[0])
+    srv.close()
