@@ -139,6 +139,12 @@ _SIGS = {
                                           C.POINTER(PrResult)]),
     "ag_pagerank_prepare": (C.c_int, [vp, vp, f64p]),
     "ag_spmv_pull": (C.c_int, [vp, vp, f32p, f32p, C.c_uint32, C.POINTER(SpmvResult)]),
+    "ag_reorder_labels": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_uint32, u32p]),
+    "ag_reorder_labels_device": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32,
+                                           C.c_uint32, vp]),
+    "ag_relabel_edges": (C.c_int, [vp, u32p, C.c_uint32, u32p, u32p, C.c_uint32]),
+    "ag_relabel_edges_device": (C.c_int, [vp, vp, C.c_uint32, vp, vp, C.c_uint32]),
     "ag_spmv_pull_device": (C.c_int, [vp, vp, vp, vp, C.c_uint32, C.POINTER(SpmvResult)]),
     "ag_pagerank_plan_info": (C.c_int, [vp, C.POINTER(PlanInfo)]),
     "ag_set_tuning": (C.c_int, [C.POINTER(Tuning)]),

@@ -2,6 +2,7 @@
 // implemented over the accelgraph_amd.h C ABI.  Every compute step runs on the
 // GPU; when no device is visible the calls print the reason and return NULL
 // (there is no CPU fallback).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +17,10 @@
 #include "vertex.h"
 
 namespace {
+
+// Seed of the random relabelling (mode 10): the reference CLI's MT19937
+// seed (accel-graph.c, argumentsSetDefaults).
+constexpr uint32_t kRelabelSeed = 27491095u;
 
 bool quiet()
 {
@@ -218,17 +223,108 @@ struct GraphCSR *graphCSRFromEdgeList(const struct EdgeList *el, uint32_t sort)
     return g;
 }
 
+} // extern "C"
+
+namespace {
+
+// Mode 11 of the relabelling options: labels from a text file, one unsigned
+// integer per vertex, vertex v's new label on line v (the format of the
+// reference's 04_test_graphs/TEST/graphbrew/graph_Rabbit.org.labels).
+bool read_label_file(const char *path, uint32_t N, std::vector<uint32_t> &lab)
+{
+    FILE *f = path ? std::fopen(path, "r") : nullptr;
+    if (!f) {
+        std::fprintf(stderr, "[accelgraph-amd] relabelling: cannot open label file %s\n",
+                     path ? path : "(null: arguments->fnamel)");
+        return false;
+    }
+    lab.assign(N, 0);
+    uint32_t n = 0;
+    unsigned long x;
+    while (n < N && std::fscanf(f, "%lu", &x) == 1) lab[n++] = (uint32_t)x;
+    std::fclose(f);
+    if (n != N) {
+        std::fprintf(stderr, "[accelgraph-amd] relabelling: %s holds %u labels, the graph %u vertices\n",
+                     path, n, N);
+        return false;
+    }
+    return true;
+}
+
+// The -l / -L / -O layers (Arguments.lmode, lmode_l2, lmode_l3): each layer
+// relabels the edge list left by the previous one; `total` composes them
+// (total[v] = label of original vertex v after every layer).
+bool apply_relabel_layers(ag_ctx *c, const struct Arguments *a, struct EdgeList *el,
+                          std::vector<uint32_t> &total)
+{
+    const uint32_t N = el->num_vertices, E = el->num_edges;
+    total.resize(N);
+    for (uint32_t v = 0; v < N; v++) total[v] = v;
+    const uint32_t modes[3] = {a->lmode, a->lmode_l2, a->lmode_l3};
+    for (uint32_t m : modes) {
+        if (m == 0) continue;
+        std::vector<uint32_t> lab(N);
+        if (m == 11) {
+            if (!read_label_file(a->fnamel, N, lab)) return false;
+        } else if (ag_reorder_labels(c, el->edges_array_src, el->edges_array_dest, E, N, m,
+                                     kRelabelSeed, lab.data()) != AG_OK) {
+            report("graphCSRPreProcessingStep (relabel)");
+            return false;
+        }
+        if (ag_relabel_edges(c, lab.data(), N, el->edges_array_src, el->edges_array_dest, E) !=
+            AG_OK) {
+            report("graphCSRPreProcessingStep (relabel)");
+            return false;
+        }
+        for (uint32_t v = 0; v < N; v++) total[v] = lab[total[v]];
+    }
+    return true;
+}
+
+bool set_labels(struct EdgeList *el, const std::vector<uint32_t> &total)
+{
+    const size_t N = total.size();
+    el->label_array = (uint32_t *)std::malloc(std::max<size_t>(N, 1) * 4);
+    el->inverse_label_array = (uint32_t *)std::malloc(std::max<size_t>(N, 1) * 4);
+    if (!el->label_array || !el->inverse_label_array) return false;
+    for (size_t v = 0; v < N; v++) {
+        el->label_array[v] = total[v];
+        el->inverse_label_array[total[v]] = (uint32_t)v;
+    }
+    return true;
+}
+
+} // namespace
+
+extern "C" {
+
+// Reads the edge list, applies the relabelling layers, builds both CSR
+// directions on the GPU, and fills label_array (original id -> label) and
+// inverse_label_array on both sorted edge lists (identity without layers).
 struct GraphCSR *graphCSRPreProcessingStep(struct Arguments *arguments)
 {
     if (!arguments || !arguments->fnameb) return nullptr;
+    ag_ctx *c = ctx_or_null("graphCSRPreProcessingStep");
+    if (!c) return nullptr;
     struct EdgeList *el = readEdgeListsFile(arguments->fnameb, arguments->fnameb_format, 0,
                                             arguments->symmetric);
     if (!el) {
         report("graphCSRPreProcessingStep");
         return nullptr;
     }
+    std::vector<uint32_t> total;
+    if (!apply_relabel_layers(c, arguments, el, total)) {
+        freeEdgeList(el);
+        return nullptr;
+    }
     struct GraphCSR *g = graphCSRFromEdgeList(el, arguments->sort);
     freeEdgeList(el);
+    if (g && total.size() == g->num_vertices &&
+        !(set_labels(g->sorted_edges_array, total) &&
+          set_labels(g->inverse_sorted_edges_array, total))) {
+        graphCSRFree(g);
+        return nullptr;
+    }
     return g;
 }
 

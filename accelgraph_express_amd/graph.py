@@ -13,6 +13,8 @@ names follow the reference's C API where one exists:
                            mode="data_driven" is pageRankDataDrivenPullGraphCSR
                            (pageRank.c:2810-2968)
 * ``Graph.spmv_pull``      SPMVPullGraphCSR (SPMV.c:622-750), unweighted
+* ``reorder_labels`` / ``relabel_edges``  the -l/-L/-O locality relabelling
+                           (README.md:362-411)
 """
 from __future__ import annotations
 
@@ -264,6 +266,30 @@ class Graph:
         L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))
         return r
+
+
+REORDER_MODES = {"none": 0, "out_degree": 1, "in_degree": 2, "inout_degree": 3, "dbg_out": 4,
+                 "dbg_in": 5, "hubsort_out": 6, "hubsort_in": 7, "hubcluster_out": 8,
+                 "hubcluster_in": 9, "random": 10}
+
+
+def reorder_labels(ctx: Context, src, dst, num_vertices: int, mode, seed: int = 0) -> np.ndarray:
+    """labels[v] = new label of vertex v under a locality relabelling mode
+    (the reference's -l/-L/-O options 0-10), computed on the GPU."""
+    s, d = _u32(src), _u32(dst)
+    m = REORDER_MODES[mode] if isinstance(mode, str) else int(mode)
+    lab = np.zeros(max(num_vertices, 1), np.uint32)
+    L.check(L.load().ag_reorder_labels(ctx.h, _ptr(s), _ptr(d), s.size, num_vertices, m, seed,
+                                       _ptr(lab)))
+    return lab[:num_vertices]
+
+
+def relabel_edges(ctx: Context, labels, src, dst):
+    """(labels[src], labels[dst]) on the GPU; labels must be a permutation."""
+    lab = _u32(labels)
+    s, d = _u32(src).copy(), _u32(dst).copy()
+    L.check(L.load().ag_relabel_edges(ctx.h, _ptr(lab), lab.size, _ptr(s), _ptr(d), s.size))
+    return s, d
 
 
 def spmv_input(out_degree) -> np.ndarray:

@@ -180,6 +180,40 @@ int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *
 int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
                             float *d_ranks, ag_pr_result *res);
 
+/* ---- locality relabelling (the reference's -l/-L/-O options, README.md:362-411) ----
+ * labels[v] = new label of vertex v: the EdgeList label_array the reference's
+ * algorithms read (BFS.c:301, SSSP.c:583).  The reference's implementation is
+ * in the unavailable OpenGraph reorder module; these modes restate the
+ * published algorithms.  Degrees count the edge list (out: as src, in: as
+ * dst); avg = E / N (uint32); a hub has degree > avg. */
+#define AG_REORDER_NONE            0u  /* identity */
+#define AG_REORDER_OUT_DEGREE      1u  /* descending degree, ties by id */
+#define AG_REORDER_IN_DEGREE       2u
+#define AG_REORDER_INOUT_DEGREE    3u
+#define AG_REORDER_DBG_OUT         4u  /* 8 degree groups at avg * 2^k, hottest first, id order inside */
+#define AG_REORDER_DBG_IN          5u
+#define AG_REORDER_HUBSORT_OUT     6u  /* hubs by descending degree, then the rest in id order */
+#define AG_REORDER_HUBSORT_IN      7u
+#define AG_REORDER_HUBCLUSTER_OUT  8u  /* hubs in id order, then the rest in id order */
+#define AG_REORDER_HUBCLUSTER_IN   9u
+#define AG_REORDER_RANDOM         10u  /* seeded random permutation */
+
+/* Labels for `mode` from a host edge list (ids < num_vertices). */
+int ag_reorder_labels(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t num_edges,
+                      uint32_t num_vertices, uint32_t mode, uint32_t seed, uint32_t *labels);
+/* Same on device arrays. */
+int ag_reorder_labels_device(ag_ctx *ctx, const uint32_t *d_src, const uint32_t *d_dst,
+                             uint32_t num_edges, uint32_t num_vertices, uint32_t mode,
+                             uint32_t seed, uint32_t *d_labels);
+/* src[e] = labels[src[e]], dst[e] = labels[dst[e]] in place (host arrays).
+ * Refuses labels that are not a permutation of [0, num_vertices) and ids out
+ * of range (AG_ERR_INVALID), leaving the arrays unchanged. */
+int ag_relabel_edges(ag_ctx *ctx, const uint32_t *labels, uint32_t num_vertices, uint32_t *src,
+                     uint32_t *dst, uint32_t num_edges);
+/* Same on device arrays. */
+int ag_relabel_edges_device(ag_ctx *ctx, const uint32_t *d_labels, uint32_t num_vertices,
+                            uint32_t *d_src, uint32_t *d_dst, uint32_t num_edges);
+
 /* ---- SPMV pull (SPMV.c:622-750, SPMVPullGraphCSR) ---------------------- */
 typedef struct ag_spmv_result
 {

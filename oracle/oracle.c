@@ -362,6 +362,64 @@ double oracle_spmv_sum(uint32_t N, const float *y)
     return sum;
 }
 
+/* Relabelling: a key per vertex, then ids ordered by (key, id).  The keys
+ * are encoded so that ascending order is the mode's order. */
+typedef struct { uint64_t key; uint32_t id; } reorder_item;
+
+static int reorder_cmp(const void *a, const void *b)
+{
+    const reorder_item *x = (const reorder_item *)a, *y = (const reorder_item *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->id < y->id ? -1 : (x->id > y->id ? 1 : 0);
+}
+
+static uint32_t lowbias32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+int oracle_reorder_labels(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
+                          uint32_t mode, uint32_t seed, uint32_t *labels)
+{
+    if (mode > 10) return -1;
+    uint32_t *deg = (uint32_t *)xcalloc(N ? N : 1, sizeof(uint32_t));
+    const int use_out = mode == 1 || mode == 3 || mode == 4 || mode == 6 || mode == 8;
+    const int use_in = mode == 2 || mode == 3 || mode == 5 || mode == 7 || mode == 9;
+    for (uint32_t e = 0; e < E; e++) {
+        if (use_out) deg[src[e]]++;
+        if (use_in) deg[dst[e]]++;
+    }
+    const uint64_t avg = N ? E / N : 0;
+    reorder_item *it = (reorder_item *)xcalloc(N ? N : 1, sizeof(reorder_item));
+    for (uint32_t v = 0; v < N; v++) {
+        const uint64_t d = deg[v];
+        uint64_t k = 0;
+        switch (mode) {
+        case 0: k = 0; break;
+        case 1: case 2: case 3: k = UINT64_MAX - d; break;
+        case 4: case 5:
+            k = d >= 32 * avg ? 0 : d >= 16 * avg ? 1 : d >= 8 * avg ? 2 : d >= 4 * avg ? 3
+              : d >= 2 * avg ? 4 : d >= avg ? 5 : 2 * d >= avg ? 6 : 7;
+            break;
+        case 6: case 7: k = d > avg ? UINT64_MAX - 1 - d : UINT64_MAX; break;
+        case 8: case 9: k = d > avg ? 0 : 1; break;
+        case 10: k = lowbias32(v * 0x9E3779B9u ^ seed); break;
+        }
+        it[v].key = k;
+        it[v].id = v;
+    }
+    qsort(it, N, sizeof(reorder_item), reorder_cmp);
+    for (uint32_t i = 0; i < N; i++) labels[it[i].id] = i;
+    free(it);
+    free(deg);
+    return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* OpenMP baselines                                                          */
 /* ------------------------------------------------------------------------- */

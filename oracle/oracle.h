@@ -107,6 +107,26 @@ void oracle_spmv_pull_f32(uint32_t N, const uint32_t *inv_deg, const uint32_t *i
 /* The printed checksum (:735-739): sum of (int)(y*100 + .5) / 100.0. */
 double oracle_spmv_sum(uint32_t N, const float *y);
 
+/* Locality relabelling, the reference's -l/-L/-O modes 0-10 (README.md:362-411;
+ * the implementation is in the absent OpenGraph reorder module, so this
+ * restates the published algorithms with the conventions below).  labels[v]
+ * = new label of vertex v (the EdgeList label_array of BFS.c:301).  Degrees
+ * count the edge list (out: src, in: dst, in+out: both); avg = E / N in
+ * uint32 (the structs' avg_degree); a hub has degree > avg.
+ *   1-3   degree sort: descending degree, ties by ascending id
+ *   4-5   DBG (Faldu et al., IISWC 2019): 8 groups, d >= 32avg, >= 16avg,
+ *         >= 8avg, >= 4avg, >= 2avg, >= avg, 2d >= avg, rest; groups in that
+ *         order, ascending id inside a group
+ *   6-7   HUBSort (Zhang et al., 2017): hubs by descending degree (ties by
+ *         id), then the other vertices in id order
+ *   8-9   HUBCluster (Balaji and Lucia, IISWC 2018): hubs in id order, then
+ *         the other vertices in id order
+ *   10    random: ascending 32-bit key lowbias32(v * 0x9E3779B9 ^ seed),
+ *         ties by id
+ * Returns 0, or -1 for an unknown mode. */
+int oracle_reorder_labels(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
+                          uint32_t mode, uint32_t seed, uint32_t *labels);
+
 /* OpenMP CPU baselines (reported beside the GPU, never the product path). */
 double oracle_csr_build_omp(uint32_t N, uint32_t E, const uint32_t *src, const uint32_t *dst,
                             uint32_t *fwd_src, uint32_t *fwd_dst, uint32_t *fwd_outdeg,

@@ -41,6 +41,7 @@ def load() -> C.CDLL:
                                          u32p, f64p]),
         "oracle_pagerank_pull_f64": (U, [U, u32p, u32p, u32p, u32p, U, D, f64p, f64p]),
         "oracle_spmv_input": (None, [U, u32p, f32p]),
+        "oracle_reorder_labels": (I, [U, U, u32p, u32p, U, U, u32p]),
         "oracle_spmv_pull_f32": (None, [U, u32p, u32p, u32p, f32p, f32p]),
         "oracle_spmv_sum": (D, [U, f32p]),
         "oracle_pagerank_dd_pull_f32": (U, [U] + [u32p] * 6 + [U, D, f32p, f64p, f64p, u32p,
@@ -148,6 +149,16 @@ def pagerank_dd_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4)
     sweeps = min(iterations, it + 1) if iterations else 0
     return (pr, int(it), et.value, ps.value, act[:sweeps].copy(), err[:sweeps].copy(),
             work[:sweeps].copy())
+
+
+def reorder_labels(src, dst, n: int, mode: int, seed: int = 0) -> np.ndarray:
+    """labels[v] = new label of v under relabelling `mode` (0-10)."""
+    s, d = _u(src), _u(dst)
+    lab = np.zeros(max(n, 1), np.uint32)
+    rc = load().oracle_reorder_labels(n, s.size, _p(s), _p(d), mode, seed, _p(lab))
+    if rc != 0:
+        raise ValueError(f"unknown relabelling mode {mode}")
+    return lab[:n]
 
 
 def spmv_input(csr: dict) -> np.ndarray:
