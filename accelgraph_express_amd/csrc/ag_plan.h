@@ -38,12 +38,15 @@ struct Plan {
     uint32_t ppr_nch = 0, ppr_nmulti = 0;
     bool ppr_ready = false;
     std::vector<uint32_t> rank_host; // host copy of rank (seed ids -> labels), on first use
+    // column buffers (X, C0, C1) of the last finished batch, lent to the next
+    DevBuf ppr_keep[3];
     std::mutex mu;
     uint64_t bytes() const
     {
         return order.bytes + rank.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
                chunk_row.bytes + chunk_first.bytes + hub_partial.bytes + ppr_ch_row.bytes +
-               ppr_ch_first.bytes + ppr_multi.bytes;
+               ppr_ch_first.bytes + ppr_multi.bytes + ppr_keep[0].bytes + ppr_keep[1].bytes +
+               ppr_keep[2].bytes;
     }
 };
 

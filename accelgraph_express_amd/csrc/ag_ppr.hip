@@ -979,9 +979,18 @@ PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_
     if (N_ == 0) fail(AG_ERR_INVALID, "graph has no vertices");
     set_cpw(cpw);
     hipStream_t s = ctx->stream;
-    X_.alloc((size_t)N_ * cap * 4);
-    C_[0].alloc((size_t)N_ * cap * 4);
-    C_[1].alloc((size_t)N_ * cap * 4);
+    {
+        // the column buffers of the last finished batch on this graph are lent
+        // to this one (no hipMalloc / hipFree of ~12 bytes per vertex and
+        // column per batch); every admitted column is cleared before use
+        const size_t need = (size_t)N_ * cap * 4;
+        DevBuf *mine[3] = {&X_, &C_[0], &C_[1]};
+        std::lock_guard<std::mutex> lk(p_->mu);
+        for (int i = 0; i < 3; i++)
+            if (p_->ppr_keep[i].bytes >= need) *mine[i] = std::move(p_->ppr_keep[i]);
+    }
+    for (DevBuf *b : {&X_, &C_[0], &C_[1]})
+        if (!b->p) b->alloc((size_t)N_ * cap * 4);
     cols_.alloc(sizeof(PprCol) * kMaxCap);
     bstate_.alloc(sizeof(PprBatchState));
     const size_t nparts = (size_t)ctx->num_cus * 16 * 2;
@@ -1004,6 +1013,11 @@ PprBatch::~PprBatch()
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     (void)hipStreamSynchronize(ctx_->stream);
+    // keep the larger column buffers on the graph for the next batch
+    DevBuf *mine[3] = {&X_, &C_[0], &C_[1]};
+    std::lock_guard<std::mutex> lk(p_->mu);
+    for (int i = 0; i < 3; i++)
+        if (mine[i]->bytes > p_->ppr_keep[i].bytes) p_->ppr_keep[i] = std::move(*mine[i]);
 }
 
 uint64_t PprBatch::bytes() const
