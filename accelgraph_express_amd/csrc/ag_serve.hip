@@ -15,6 +15,7 @@
 #include <mutex>
 #include <unordered_map>
 
+#include "ag_admit.h"
 #include "ag_ppr.h"
 
 namespace {
@@ -96,21 +97,19 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
 {
     if (ge.batch) return true;
     const uint32_t want = std::min<uint32_t>(64, std::max<uint32_t>(8, (s->cfg.max_columns + 7) / 8 * 8));
-    auto fits = [&](uint32_t cap) {
-        return s->reserved + PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision) <=
-               s->cfg.hbm_budget_bytes;
-    };
-    while (!fits(want)) {
-        GraphEntry *idle = nullptr;
-        for (auto &o : s->graphs)
-            if (&o != &ge && o.batch && o.batch->active() == 0 && (!idle || o.reserved > idle->reserved))
-                idle = &o;
-        if (!idle) break;
-        release_batch(s, *idle);
-    }
-    uint32_t cap = want;
-    while (cap >= 8 && !fits(cap)) cap -= 8;
-    if (cap < 8) return false;
+    std::vector<AdmitHolder> holders;
+    std::vector<GraphEntry *> who;
+    for (auto &o : s->graphs)
+        if (&o != &ge && o.batch) {
+            holders.push_back({o.reserved, o.batch->active() == 0});
+            who.push_back(&o);
+        }
+    const AdmitPlan plan = plan_admission(
+        s->cfg.hbm_budget_bytes, s->reserved, want, holders,
+        [&](uint32_t cap) { return PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision); });
+    if (plan.cap == 0) return false;
+    for (size_t i : plan.evict) release_batch(s, *who[i]);
+    const uint32_t cap = plan.cap;
     s->alloc_attempts++;
     if (s->cfg.inject_fail_alloc && s->alloc_attempts == s->cfg.inject_fail_alloc)
         return false; // injected allocation failure: the query stays queued
