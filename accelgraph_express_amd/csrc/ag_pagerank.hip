@@ -541,9 +541,18 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     st->last_err = E_;
     st->last_active = A_;
     st->sweeps = it + 1;
-    // Storage format of the contributions written by the next sweep.
+    // Storage format of the contributions written by the next sweep.  A
+    // low-precision run turns FP32 once its residual reaches the storage
+    // format's rounding noise, d * 2^-(m+1) with m explicit mantissa bits
+    // (BF16 7, FP16 10): round-to-nearest storage is a deterministic map, so
+    // the rounded iteration converges to a fixed point of its own and its
+    // residual keeps falling below that noise while the iterate stays up to
+    // ~1e-4 from the true fixed point.  A stall (residual not falling) also
+    // switches.
     uint32_t nf = a.fmt[it + 1];
-    if (nf != kFmtF32 && a.finish_fp32 && it >= 2 && E_ > 0.97 * st->prev_err) {
+    const double noise = (double)a.damp * (nf == kFmtBF16 ? 0x1p-8 : 0x1p-11);
+    if (nf != kFmtF32 && a.finish_fp32 && it >= 2 &&
+        (E_ <= noise || E_ > 0.97 * st->prev_err)) {
         nf = kFmtF32;
         st->fp32_switch = it + 2;
     }

@@ -187,3 +187,28 @@ def test_graph_from_host_csr(ctx):
         ag.Graph.from_csr(ctx, 1 << 14, csr["vertices.out_degree"],
                           csr["inverse_vertices.out_degree"], bad,
                           csr["inverse_sorted_edges_array.edges_array_dest"])
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16"])
+def test_low_precision_finish_reaches_fp32_accuracy(ctx, prec):
+    """Round-to-nearest storage is a deterministic map, so a low-precision
+    iteration can converge to a fixed point of its own whose residual falls
+    below any tolerance while the iterate stays ~1e-4 from the true fixed
+    point.  With the FP32 finish the run must switch (at the storage format's
+    rounding noise at the latest) and end within the FP32 bound: the FP64
+    contraction bound ||x - T(x)||_1 / (1 - d) <= 2e-5 at residual 1e-6."""
+    src, dst = ag.gen_rmat(2024, 18, 16)
+    n = 1 << 18
+    g = ag.Graph.build(ctx, src, dst, n)
+    st = g.pagerank_pull(iterations=300, epsilon=1e-6, mode="standard", precision=prec,
+                         finish_fp32=True)
+    assert 0 < st.fp32_switch_sweep <= st.sweeps
+    x = st.pageRanks.astype(np.float64)
+    od = np.bincount(src, minlength=n).astype(np.float64)
+    contrib = np.divide(x, od, out=np.zeros(n), where=od > 0)
+    tx = 0.15 / n + 0.85 * (np.bincount(dst, weights=contrib[src], minlength=n)
+                            + x[od == 0].sum() / n)
+    bound = np.abs(x - tx).sum() / 0.15
+    print(f"{prec}: switch at {st.fp32_switch_sweep} of {st.sweeps}, bound {bound:.2e}")
+    assert bound <= 2e-5
+    assert abs(x.sum() - 1.0) <= 1e-5
