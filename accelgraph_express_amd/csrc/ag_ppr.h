@@ -80,6 +80,12 @@ class PprBatch {
     };
     // Synchronises and returns the slots whose columns finished.
     std::vector<Done> poll();
+    // Sweeps until the first running column can meet its tolerance, from the
+    // residuals seen by the last poll (each sweep assumed to cut the residual
+    // by at least 0.6), clamped to [1, 8]; 4 before any residual is known.  A
+    // finished column is frozen on the device, so sweeping past it does not
+    // change its result; polling less often saves the host round trips.
+    uint32_t sweeps_hint() const { return hint_; }
     // Copies a finished column (caller's vertex order) to host memory.
     void extract(uint32_t slot, float *host_out);
     // Writes a finished column in vertex order to `dst`, memory the device can
@@ -136,6 +142,7 @@ class PprBatch {
     hipGraphExec_t gexec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     double last_ms_ = 0.0;
+    uint32_t hint_ = 4;
 };
 
 template <typename F>

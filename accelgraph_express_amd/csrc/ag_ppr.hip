@@ -1378,8 +1378,21 @@ std::vector<PprBatch::Done> PprBatch::poll()
     AG_HIP(hipMemcpyAsync(cs.data(), cols_.p, hi_ * sizeof(PprCol), hipMemcpyDeviceToHost,
                           ctx_->stream));
     AG_HIP(hipStreamSynchronize(ctx_->stream));
-    for (uint32_t k = 0; k < hi_; k++)
-        if (used_[k] && cs[k].done) out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].fine_at});
+    uint32_t hint = 8;
+    bool known = false;
+    for (uint32_t k = 0; k < hi_; k++) {
+        if (!used_[k]) continue;
+        if (cs[k].done) {
+            out.push_back({k, tags_[k], cs[k].iters, cs[k].residual, cs[k].fine_at});
+        } else if (std::isfinite(cs[k].residual) && cs[k].residual > 0.0 && cs[k].tol > 0.0) {
+            const double need = std::ceil(std::log(cs[k].tol / cs[k].residual) / std::log(0.6));
+            hint = std::min<uint32_t>(hint, (uint32_t)std::max(1.0, std::min(8.0, need)));
+            known = true;
+        } else {
+            hint = std::min<uint32_t>(hint, 4);
+        }
+    }
+    hint_ = known || hint < 8 ? hint : 4;
     return out;
 }
 
@@ -1478,7 +1491,7 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
         uint32_t next = 0, finished = 0;
         while (next < nq && b.free_slots()) { b.admit(sets[next], next); next++; }
         while (finished < nq) {
-            b.sweep(4, o.use_graph_replay != 0, false);
+            b.sweep(b.sweeps_hint(), o.use_graph_replay != 0, false);
             auto fin = b.retire([&](uint64_t tag) { return x ? x + (size_t)tag * N : nullptr; },
                                 next >= nq);
             for (auto &f : fin) {
