@@ -20,6 +20,10 @@ struct Work {
     DevBuf pr, c0, c1, blk_err, blk_act, blk_dang, state, err_iter, act_iter, fmt, sum_part, out;
 };
 
+// Buffers of a personalized-PageRank batch that a finished batch leaves to
+// the next one on the same graph (see PprBatch).
+constexpr int kKeepBufs = 11;
+
 struct Plan {
     uint32_t N = 0, E = 0;
     uint32_t n_hub = 0, n_w = 0, n_h = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
@@ -38,15 +42,21 @@ struct Plan {
     uint32_t ppr_nch = 0, ppr_nmulti = 0;
     bool ppr_ready = false;
     std::vector<uint32_t> rank_host; // host copy of rank (seed ids -> labels), on first use
-    // column buffers (X, C0, C1) of the last finished batch, lent to the next
-    DevBuf ppr_keep[3];
+    // buffers of the last finished PprBatch (columns, state, partials,
+    // staging), lent to the next batch on this graph
+    DevBuf ppr_keep[kKeepBufs];
     std::mutex mu;
     uint64_t bytes() const
     {
         return order.bytes + rank.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
                chunk_row.bytes + chunk_first.bytes + hub_partial.bytes + ppr_ch_row.bytes +
-               ppr_ch_first.bytes + ppr_multi.bytes + ppr_keep[0].bytes + ppr_keep[1].bytes +
-               ppr_keep[2].bytes;
+               ppr_ch_first.bytes + ppr_multi.bytes + keep_bytes();
+    }
+    uint64_t keep_bytes() const
+    {
+        uint64_t b = 0;
+        for (const DevBuf &k : ppr_keep) b += k.bytes;
+        return b;
     }
 };
 

@@ -980,26 +980,27 @@ PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_
     set_cpw(cpw);
     hipStream_t s = ctx->stream;
     {
-        // the column buffers of the last finished batch on this graph are lent
-        // to this one (no hipMalloc / hipFree of ~12 bytes per vertex and
-        // column per batch); every admitted column is cleared before use
-        const size_t need = (size_t)N_ * cap * 4;
-        DevBuf *mine[3] = {&X_, &C_[0], &C_[1]};
-        std::lock_guard<std::mutex> lk(p_->mu);
-        for (int i = 0; i < 3; i++)
-            if (p_->ppr_keep[i].bytes >= need) *mine[i] = std::move(p_->ppr_keep[i]);
+        // The buffers of the last finished batch on this graph are lent to
+        // this one, so a repeated batch call makes no hipMalloc / hipFree (the
+        // column buffers alone are ~12 bytes per vertex and column).  Every
+        // admitted column is cleared before use, and every other buffer is
+        // (re)written before it is read.
+        const size_t nparts = (size_t)ctx->num_cus * 16 * 2;
+        const size_t need[kKeepBufs] = {
+            (size_t)N_ * cap * 4, (size_t)N_ * cap * 4, (size_t)N_ * cap * 4,
+            sizeof(PprCol) * kMaxCap, sizeof(PprBatchState), nparts * kMaxCap * 8,
+            nparts * kMaxCap * 8, (size_t)std::max<uint32_t>(p_->ppr_nch, 1) * cap * 4,
+            (size_t)((N_ + 31) / 32) * 4, (size_t)N_ * 4, sizeof(PprAdmit) * kMaxCap};
+        DevBuf *mine[kKeepBufs] = {&X_,        &C_[0], &C_[1], &cols_,      &bstate_, &blk_res_,
+                                   &blk_dang_, &part_, &seed_bits_, &out_, &desc_};
+        {
+            std::lock_guard<std::mutex> lk(p_->mu);
+            for (int i = 0; i < kKeepBufs; i++)
+                if (p_->ppr_keep[i].bytes >= need[i]) *mine[i] = std::move(p_->ppr_keep[i]);
+        }
+        for (int i = 0; i < kKeepBufs; i++)
+            if (!mine[i]->p) mine[i]->alloc(need[i]);
     }
-    for (DevBuf *b : {&X_, &C_[0], &C_[1]})
-        if (!b->p) b->alloc((size_t)N_ * cap * 4);
-    cols_.alloc(sizeof(PprCol) * kMaxCap);
-    bstate_.alloc(sizeof(PprBatchState));
-    const size_t nparts = (size_t)ctx->num_cus * 16 * 2;
-    blk_res_.alloc(nparts * kMaxCap * 8);
-    blk_dang_.alloc(nparts * kMaxCap * 8);
-    part_.alloc((size_t)std::max<uint32_t>(p_->ppr_nch, 1) * cap * 4);
-    seed_bits_.alloc((size_t)((N_ + 31) / 32) * 4);
-    out_.alloc((size_t)N_ * 4);
-    desc_.alloc(sizeof(PprAdmit) * kMaxCap);
     AG_HIP(hipMemsetAsync(bstate_.p, 0, sizeof(PprBatchState), s));
     AG_HIP(hipMemsetAsync(cols_.p, 0, cols_.bytes, s));
     q_.resize(cap);
@@ -1013,10 +1014,11 @@ PprBatch::~PprBatch()
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     (void)hipStreamSynchronize(ctx_->stream);
-    // keep the larger column buffers on the graph for the next batch
-    DevBuf *mine[3] = {&X_, &C_[0], &C_[1]};
+    // keep the larger buffers on the graph for the next batch
+    DevBuf *mine[kKeepBufs] = {&X_,        &C_[0], &C_[1], &cols_,      &bstate_, &blk_res_,
+                               &blk_dang_, &part_, &seed_bits_, &out_, &desc_};
     std::lock_guard<std::mutex> lk(p_->mu);
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < kKeepBufs; i++)
         if (mine[i]->bytes > p_->ppr_keep[i].bytes) p_->ppr_keep[i] = std::move(*mine[i]);
 }
 
