@@ -6,8 +6,8 @@
    N=65,536, E=1,048,576, seed 27491095) and sampled FP32 ranks after 20
    sweeps, as produced by the oracle, for regression of the GPU path.
 
-Run in the build container (it reads /root/reference); the GPU box only reads
-the committed output.
+Run where the reference tree is mounted (it reads /root/reference); the GPU
+tests only read the committed output.
 """
 import hashlib
 import json

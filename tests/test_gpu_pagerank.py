@@ -1,6 +1,6 @@
 """GPU pull PageRank parity against the oracle.
 
-Bar (north star): after a fixed sweep count the FP32 ranks are within 1e-5
+Bar: after a fixed sweep count the FP32 ranks are within 1e-5
 relative per vertex of the oracle's FP32 restatement of pageRankPullGraphCSR
 (pageRank.c:942-1108); iteration counts match; per-sweep active counts match
 exactly at epsilon 0 and within a handful of threshold-straddling vertices
