@@ -308,8 +308,15 @@ int ag_serve_step(ag_server *s, uint32_t *active)
         admit(s);
         for (uint32_t gi = 0; gi < s->graphs.size(); gi++) {
             GraphEntry &ge = s->graphs[gi];
-            // sweeps of all graphs are queued back to back; retire() waits
-            if (ge.batch && ge.batch->active()) ge.batch->sweep(2, true, false);
+            // sweeps of all graphs are queued back to back; retire() waits.
+            // Two sweeps per step keep admissions prompt; a full batch with
+            // queries waiting admits nothing before its first column
+            // finishes, so it sweeps until then.
+            if (!ge.batch || !ge.batch->active()) continue;
+            uint32_t n = 2;
+            if (ge.batch->free_slots() == 0 && graph_has_queued(s, gi))
+                n = std::max(2u, ge.batch->sweeps_hint());
+            ge.batch->sweep(n, true, false);
         }
         for (uint32_t gi = 0; gi < s->graphs.size(); gi++) retire(s, gi);
         admit(s);

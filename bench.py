@@ -437,6 +437,19 @@ def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
     srv = ag.Server(ctx, precision="bf16")
     gids = [srv.add_graph(g) for g in graphs]
     rng = np.random.RandomState(seed)
+    # warm-up, untimed: each graph's gather layout, columns-per-wave autotune,
+    # resident batch and pinned result buffers are made by its first queries
+    warm = {}
+    for k, gid in enumerate(gids):
+        n = 1 << (18 + 2 * k)
+        for _ in range(4):
+            warm[srv.submit(gid, rng.randint(0, n, 4).astype(np.uint32), 1e-4)] = n
+    while warm:
+        srv.step()
+        for t in list(warm):
+            if srv.peek(t)[0] != L.AG_ERR_PENDING:
+                srv.release(t)
+                del warm[t]
     out = []
     for rate in rates:
         arrivals = np.cumsum(rng.exponential(1.0 / rate, per_rate))
@@ -473,7 +486,7 @@ def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
     best = max(out, key=lambda r: r["achieved_qps"])
     line = {"metric": "PPR serving queries/s + p99 latency (resident R-MAT s18/s20/s22)",
             "value": best["achieved_qps"], "unit": "queries/s", "n_gpus": 1, "steps": len(out),
-            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+            "warmup": 12, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "bf16-stored/f32-accumulate",
             "data": "synthetic (seeded R-MAT graphs, Poisson arrivals)",
             "config": {"workload": "serving_rmat_s18_s20_s22_poisson",
