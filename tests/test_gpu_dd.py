@@ -180,3 +180,20 @@ def test_refusals(ctx):
         g.pagerank_pull(20, 1e-4, mode="data_driven", precision="bf16")
     with pytest.raises(ag.AGError):
         g.pagerank_pull(20, 1e-4, mode=7)
+
+
+def test_edge_cases(ctx):
+    """Empty graph, a single self-loop, and epsilon < 0 (every recomputed vertex
+    counts as updated), each against the oracle."""
+    cases = [(np.zeros(0, np.uint32), np.zeros(0, np.uint32), 5, 1e-4),
+             (np.array([0], np.uint32), np.array([0], np.uint32), 1, 1e-4),
+             (np.array([0, 1, 2, 2], np.uint32), np.array([1, 2, 0, 2], np.uint32), 3, -1.0)]
+    for src, dst, n, eps in cases:
+        print(f"case: data-driven edge N={n} E={src.size} eps={eps}", flush=True)
+        g = ag.Graph.build(ctx, src, dst, n)
+        csr = oracle.build_csr(src, dst, n)
+        st = g.pagerank_pull(30, eps, mode="data_driven")
+        pr, it, err, psum, act, errs, work = oracle.pagerank_dd_pull_f32(csr, 30, eps)
+        assert st.iterations == it
+        np.testing.assert_array_equal(st.active_per_iter, act)
+        np.testing.assert_array_equal(st.pageRanks, pr)

@@ -127,3 +127,13 @@ def test_dropin_layers(rel, fmt, layers, tmp_path):
     ref, *_ = oracle.pagerank_pull_f32(oracle.build_csr(src, dst, n), 20, 0.0)
     rel_err = np.abs(p["ranks"][total].astype(np.float64) - ref) / ref
     assert rel_err.max() <= 1e-5
+
+
+def test_degenerate_graphs(ctx):
+    # one vertex, self-loops only, and a vertex id range with isolated vertices
+    for src, dst, n in ((np.array([0], np.uint32), np.array([0], np.uint32), 1),
+                        (np.array([3, 3, 3], np.uint32), np.array([3, 3, 3], np.uint32), 5),
+                        (np.array([0, 9], np.uint32), np.array([9, 0], np.uint32), 10)):
+        for mode in range(11):
+            got = ag.reorder_labels(ctx, src, dst, n, mode, seed=5)
+            np.testing.assert_array_equal(got, oracle.reorder_labels(src, dst, n, mode, seed=5))

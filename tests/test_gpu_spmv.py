@@ -146,3 +146,12 @@ def test_dropin_spmv(rel, fmt, tmp_path):
     ref = oracle.spmv_pull_f32(csr, x)
     np.testing.assert_allclose(ya, ref, rtol=REL_TOL)
     np.testing.assert_array_equal(ya, yb)
+
+
+def test_self_loops_and_duplicates(ctx):
+    src = np.array([0, 0, 0, 1, 2, 2], np.uint32)
+    dst = np.array([0, 0, 1, 1, 2, 0], np.uint32)
+    g, csr, x, y, r = _case(ctx, src, dst, 3)
+    np.testing.assert_array_equal(y, oracle.spmv_pull_f32(csr, x))
+    # y0 = 2 x0 + x2, y1 = x0 + x1, y2 = x2
+    np.testing.assert_allclose(y, [2 * x[0] + x[2], x[0] + x[1], x[2]], rtol=1e-7)
