@@ -589,17 +589,21 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows(PprArgs a)
 #pragma unroll
                 for (int i = 0; i < VW; i++) acc[i] = 0.0f;
                 if (amask) {
+                    // RU chunk partials per step (16 float4 loads in flight per
+                    // lane): the largest hubs' chains of ~150 partials per slot
+                    // set this kernel's time.  Sums stay in chunk order.
+                    constexpr int RU = 64 / VW;
                     uint32_t k = f0 + slot;
-                    for (; k + 3 * kSlots < f1; k += 4 * kSlots) {
-                        float4 v[4][VW / 4];
+                    for (; k + (RU - 1) * kSlots < f1; k += RU * kSlots) {
+                        float4 v[RU][VW / 4];
 #pragma unroll
-                        for (int e = 0; e < 4; e++)
+                        for (int e = 0; e < RU; e++)
 #pragma unroll
                             for (int i = 0; i < VW / 4; i++)
                                 v[e][i] = *reinterpret_cast<const float4 *>(
                                     a.part + (uint64_t)(k + e * kSlots) * a.cap + c0 + 4 * i);
 #pragma unroll
-                        for (int e = 0; e < 4; e++)
+                        for (int e = 0; e < RU; e++)
 #pragma unroll
                             for (int i = 0; i < VW / 4; i++) {
                                 acc[4 * i] += v[e][i].x;
