@@ -92,14 +92,14 @@ class PprStat(C.Structure):
 
 class PprOptions(C.Structure):
     _fields_ = [("precision", C.c_uint32), ("cols_per_wave", C.c_uint32), ("damp", C.c_float),
-                ("use_graph_replay", C.c_uint32)]
+                ("use_graph_replay", C.c_uint32), ("extrapolate", C.c_uint32)]
 
 
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_columns", C.c_uint32),
                 ("precision", C.c_uint32), ("cols_per_wave", C.c_uint32),
                 ("inject_fail_alloc", C.c_uint32), ("inject_fail_query", C.c_uint32),
-                ("damp", C.c_float), ("reserved", C.c_uint32)]
+                ("damp", C.c_float), ("extrapolate", C.c_uint32)]
 
 
 class CsrView(C.Structure):
@@ -155,6 +155,8 @@ _SIGS = {
     "ag_ppr_autotune": (C.c_int, [vp, vp, C.c_uint32, u32p]),
     "ag_ppr_single": (C.c_int, [vp, vp, C.POINTER(PprQuery), C.c_uint32, f32p,
                                 C.POINTER(PprStat)]),
+    "ag_ppr_single_opts": (C.c_int, [vp, vp, C.POINTER(PprQuery), C.POINTER(PprOptions), f32p,
+                                     C.POINTER(PprStat)]),
     "ag_serve_config_default": (None, [C.POINTER(ServeConfig)]),
     "ag_serve_create": (C.c_int, [vp, C.POINTER(ServeConfig), C.POINTER(vp)]),
     "ag_serve_destroy": (C.c_int, [vp]),

@@ -70,12 +70,14 @@ struct SweepState {
     uint32_t sweeps;
     uint32_t last_active;
     uint32_t fp32_switch;
-    uint32_t pad0, pad1;
+    uint32_t since; // personalized, extrapolating: plain sweeps since the last extrapolated step
+    uint32_t pad1;
     double last_err;
     double prev_err;
     double dangling;
     float tele;
-    float pad2;
+    float w;        // > 0: the next sweep stores x + w (T(x) - x) instead of T(x)
+    double rho;     // last residual ratio
 };
 
 struct SweepArgs {
@@ -92,6 +94,7 @@ struct SweepArgs {
     uint32_t mode;
     uint32_t max_iter;
     uint32_t finish_fp32;
+    uint32_t extrap; // personalized mode: Aitken-type extrapolated steps (st->w)
     float damp, base;
     double epsilon;
     double inv_n;
@@ -227,7 +230,13 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
     } else {
         nw = tele + a.damp * s;
     }
-    a.pr[i] = nw;
+    // stored iterate: T(x), or on an extrapolated sweep x + w (T(x) - x)
+    float xo = nw;
+    if (a.extrap) {
+        const float w = a.st->w;
+        if (w != 0.0f) xo = nw + w * (nw - old);
+    }
+    a.pr[i] = xo;
     const uint32_t od = a.outdeg[i];
     if (a.mode == AG_PR_REFERENCE) {
         const double e = fabs((double)(nw - old));
@@ -235,9 +244,9 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
         act += (e >= a.epsilon) ? 1u : 0u;
     } else {
         err += fabs((double)nw - (double)old);
-        if (od == 0) dang += (double)nw;
+        if (od == 0) dang += (double)xo;
     }
-    const float c = od ? nw / (float)od : 0.0f;
+    const float c = od ? xo / (float)od : 0.0f;
     store_c(cout, wfmt, i, c, a.f16_scale);
 }
 
@@ -564,7 +573,27 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
         st->dangling = D_;
         st->tele = (float)((1.0 - (double)a.damp) + (double)a.damp * D_);
         st->iterations = it + 1;
-        if (E_ <= a.epsilon) st->done = 1;
+        // after an extrapolated step the stored iterate is not T(x): its
+        // residual E_ certifies nothing, so no stop on it
+        const bool stepped = st->w != 0.0f;
+        st->w = 0.0f;
+        if (E_ <= a.epsilon && !stepped) {
+            st->done = 1;
+        } else if (a.extrap) {
+            // the batch's rule: a steady residual ratio rho on two consecutive
+            // plain sweeps (within 2%), at least three plain sweeps apart
+            if (stepped) {
+                st->since = 0;
+                st->rho = 0.0;
+            } else {
+                const double prev = st->prev_err;
+                const double rho = it > 0 && prev > 0.0 ? E_ / prev : 0.0;
+                st->since += 1;
+                if (st->since >= 3 && rho >= 0.2 && rho <= 0.95 && fabs(rho - st->rho) <= 0.02 * rho)
+                    st->w = (float)(rho / (1.0 - rho));
+                st->rho = rho;
+            }
+        }
     } else {
         st->dangling = D_;
         st->tele = (float)((1.0 - (double)a.damp) * a.inv_n + (double)a.damp * D_ * a.inv_n);
@@ -581,6 +610,7 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
         st->done = 0; st->ticket = 0; st->iterations = 0; st->sweeps = 0;
         st->last_active = 0; st->fp32_switch = 0;
         st->last_err = 0.0; st->prev_err = 0.0; st->dangling = 0.0; st->tele = tele0;
+        st->since = 0; st->w = 0.0f; st->rho = 0.0;
         if (a.mode == kModePPR) {
             // dangling mass of x0 = s: seed weight on vertices without out-edges
             double d0 = 0.0;
@@ -617,6 +647,7 @@ __global__ void k_spmv_init(SweepArgs a, const float *__restrict__ x,
         st->done = 0; st->ticket = 0; st->iterations = 0; st->sweeps = 0;
         st->last_active = 0; st->fp32_switch = 0;
         st->last_err = 0.0; st->prev_err = 0.0; st->dangling = 0.0; st->tele = 0.0f;
+        st->since = 0; st->w = 0.0f; st->rho = 0.0;
     }
     if (i <= a.max_iter + 1) a.fmt[i] = kFmtF32;
     if (i < a.N) static_cast<float *>(a.c0)[i] = x[order[i]];
@@ -962,6 +993,7 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.mode = o.mode;
     a.max_iter = max_iter;
     a.finish_fp32 = (o.flags & AG_PR_FLAG_FINISH_FP32) ? 1u : 0u;
+    a.extrap = (o.flags & AG_PR_FLAG_EXTRAPOLATE) ? 1u : 0u;
     // DATA_DRIVEN zero rows never change rank; with epsilon > 0 they never take
     // the "updated" mark either, so both buffers can hold them from init.
     a.skip_zero = ((o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ||
@@ -1020,6 +1052,8 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         fail(AG_ERR_INVALID, "data-driven pull stores FP32 contributions only");
     if ((o.mode == kModePPR) != (seeds != nullptr))
         fail(AG_ERR_INVALID, "personalized mode needs a seed table");
+    if ((o.flags & AG_PR_FLAG_EXTRAPOLATE) && o.mode != kModePPR)
+        fail(AG_ERR_INVALID, "extrapolated steps are offered for personalized PageRank only");
     if (o.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", o.precision);
     if (!(o.damp > 0.0f && o.damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
     if (N == 0) {

@@ -22,6 +22,9 @@ struct PprCol {
     double dangling;
     uint32_t coarse;   // BF16 storage: 1 while sweeps read the high halves only
     uint32_t fine_at;  // sweep count when the column switched to full FP32 reads
+    float w;           // > 0: the next sweep takes an extrapolated step of weight w
+    uint32_t since;    // plain sweeps since the start or the last extrapolated step
+    double rho;        // last residual ratio R_k / R_(k-1)
 };
 
 struct PprBatchState {
@@ -48,8 +51,10 @@ class PprBatch {
   public:
     // capacity: column slots (multiple of 8, <= 64); cpw: widest column group
     // one row access of a wave covers, {4, 8, 16, 32, 64} (BF16: at least 8).
+    // extrapolate: columns take Aitken-type extrapolated steps once their
+    // residual falls geometrically (fewer sweeps; same stop rule and bound).
     PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t cpw, uint32_t fmt,
-             float damp);
+             float damp, bool extrapolate = false);
     ~PprBatch();
     PprBatch(const PprBatch &) = delete;
     PprBatch &operator=(const PprBatch &) = delete;
@@ -126,6 +131,7 @@ class PprBatch {
     uint32_t N_, cap_, cpw_ = 0, fmt_;
     uint32_t used_count_ = 0, hi_ = 0; // hi_: one past the highest used slot
     float damp_;
+    bool extrap_;
     int cur_ = 0;          // contribution buffer read by the next sweep
     uint32_t graph_lpr_ = 0; // lanes per row the captured graph was built for
     uint32_t blk_ = 64;      // BF16 split block (columns per highs/lows block)

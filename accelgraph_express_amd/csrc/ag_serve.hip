@@ -115,7 +115,8 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
         return false; // injected allocation failure: the query stays queued
     if (!ge.cpw) ge.cpw = s->cfg.cols_per_wave ? s->cfg.cols_per_wave
                                                : ppr_autotune(s->ctx, ge.g, s->cfg.precision);
-    ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp));
+    ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp,
+                                s->cfg.extrapolate != 0));
     ge.reserved = PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision);
     s->reserved += ge.reserved;
     return true;

@@ -350,22 +350,32 @@ def _query(seeds, tol, max_iter):
 
 
 def ppr_single(graph: Graph, seeds, tol: float = 1e-4, max_iter: int = 1000,
-               precision="fp32", return_x: bool = True):
-    """One query with the single-vector sweep kernel.  Returns (x or None, stat)."""
+               precision="fp32", return_x: bool = True, extrapolate: bool = False,
+               damp: float = 0.85):
+    """One query with the single-vector sweep kernel.  Returns (x or None, stat).
+    extrapolate: Aitken-type extrapolated steps once the residual falls by a
+    steady ratio (ag_ppr_options.extrapolate)."""
+    lib = L.load()
     q, keep = _query(seeds, tol, max_iter)
     x = np.zeros(graph.num_vertices, np.float32) if return_x else None
     st = L.PprStat()
-    prec = _PRECS[precision] if isinstance(precision, str) else int(precision)
-    L.check(L.load().ag_ppr_single(graph.ctx.h, graph.h, C.byref(q), prec,
+    o = L.PprOptions()
+    lib.ag_ppr_options_default(C.byref(o))
+    o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
+    o.extrapolate = 1 if extrapolate else 0
+    o.damp = damp
+    L.check(lib.ag_ppr_single_opts(graph.ctx.h, graph.h, C.byref(q), C.byref(o),
                                    _ptr(x, C.c_float) if return_x else None, C.byref(st)))
     del keep
     return x, st
 
 
 def ppr_batch(graph: Graph, queries, precision="bf16", cols_per_wave: int = 0,
-              replay: bool = True, damp: float = 0.85, return_x: bool = True):
+              replay: bool = True, damp: float = 0.85, return_x: bool = True,
+              extrapolate: bool = False):
     """queries: list of (seeds, tol, max_iter).  Returns (X [nq, N] or None, stats,
-    elapsed_ms); with return_x=False the results stay on the device."""
+    elapsed_ms); with return_x=False the results stay on the device.
+    extrapolate: per-column Aitken-type extrapolated steps."""
     lib = L.load()
     nq = len(queries)
     arr = (L.PprQuery * nq)()
@@ -379,6 +389,7 @@ def ppr_batch(graph: Graph, queries, precision="bf16", cols_per_wave: int = 0,
     o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
     o.cols_per_wave = cols_per_wave
     o.use_graph_replay = 1 if replay else 0
+    o.extrapolate = 1 if extrapolate else 0
     o.damp = damp
     X = np.zeros((nq, graph.num_vertices), np.float32) if return_x else None
     stats = (L.PprStat * nq)()
@@ -400,7 +411,7 @@ class Server:
 
     def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, max_columns: int = 0,
                  precision="bf16", cols_per_wave: int = 0, inject_fail_alloc: int = 0,
-                 inject_fail_query: int = 0):
+                 inject_fail_query: int = 0, extrapolate: bool = False):
         lib = L.load()
         cfg = L.ServeConfig()
         lib.ag_serve_config_default(C.byref(cfg))
@@ -412,6 +423,7 @@ class Server:
         cfg.cols_per_wave = cols_per_wave
         cfg.inject_fail_alloc = inject_fail_alloc
         cfg.inject_fail_query = inject_fail_query
+        cfg.extrapolate = 1 if extrapolate else 0
         h = C.c_void_p()
         L.check(lib.ag_serve_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

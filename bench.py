@@ -65,6 +65,8 @@ def main() -> int:
     ap.add_argument("--no-dd", action="store_true",
                     help="skip the data-driven pull and SPMV lines")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--extrapolate", action="store_true",
+                    help="c5: columns take Aitken-type extrapolated steps")
     ap.add_argument("--serialize", default=None, metavar="DIR",
                     help="also time saving / loading the graph as an AGCSRv1 file in DIR")
     ap.add_argument("--copy-probe", action="store_true",
@@ -302,26 +304,28 @@ def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
     # leg) would otherwise decide a 50 ms measurement.
     reps = 3
 
-    def batched(prec, t):
+    def batched(prec, t, ex=False):
         qs = [(s, t, 1000) for s in seeds]
         # warm run of the same shape (kernels loaded, graph-held buffers sized)
-        ag.ppr_batch(g, qs, precision=prec, cols_per_wave=cpw[prec], return_x=False)
+        ag.ppr_batch(g, qs, precision=prec, cols_per_wave=cpw[prec], return_x=False,
+                     extrapolate=ex)
         runs = []
         for _ in range(reps):
             t0 = time.perf_counter()
             _, st, ms = ag.ppr_batch(g, qs, precision=prec, cols_per_wave=cpw[prec],
-                                     return_x=False)
+                                     return_x=False, extrapolate=ex)
             runs.append((time.perf_counter() - t0, ms, st))
         runs.sort(key=lambda r: r[0])
         wall, ms, st = runs[reps // 2]
         return wall, st, ms
 
-    def single(t):
-        ag.ppr_single(g, seeds[0], t, precision="fp32", return_x=False)  # warm
+    def single(t, ex=False):
+        ag.ppr_single(g, seeds[0], t, precision="fp32", return_x=False, extrapolate=ex)  # warm
         runs = []
         for _ in range(reps):
             t0 = time.perf_counter()
-            st = [ag.ppr_single(g, s, t, precision="fp32", return_x=False)[1] for s in seeds]
+            st = [ag.ppr_single(g, s, t, precision="fp32", return_x=False, extrapolate=ex)[1]
+                  for s in seeds]
             runs.append((time.perf_counter() - t0, st))
         runs.sort(key=lambda r: r[0])
         return runs[reps // 2]
@@ -329,12 +333,13 @@ def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
     sample = range(4)
     ref = [ag.ppr_single(g, seeds[i], 1e-9, precision="fp32")[0].astype(np.float64) for i in sample]
 
-    def l1_err(prec, t):
+    def l1_err(prec, t, ex=False):
         if prec == "single":
-            xs = [ag.ppr_single(g, seeds[i], t, precision="fp32")[0] for i in sample]
+            xs = [ag.ppr_single(g, seeds[i], t, precision="fp32", extrapolate=ex)[0]
+                  for i in sample]
         else:
             X, _, _ = ag.ppr_batch(g, [(seeds[i], t, 1000) for i in sample], precision=prec,
-                                   cols_per_wave=cpw[prec])
+                                   cols_per_wave=cpw[prec], extrapolate=ex)
             xs = list(X)
         return max(float(np.abs(x - r).sum()) for x, r in zip(xs, ref))
 
@@ -344,7 +349,11 @@ def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
         t_bf, st_bf, ms_bf = batched("bf16", t)
         t_32, st_32, ms_32 = batched("fp32", t)
         t_1, st_1 = single(t)
+        # the same with Aitken-type extrapolated steps, batched and single
+        t_bx, st_bx, ms_bx = batched("bf16", t, ex=True)
+        t_1x, st_1x = single(t, ex=True)
         lat = [s.latency_ms for s in st_bf]
+        lat_x = [s.latency_ms for s in st_bx]
         r = {"residual_tol": t,
              "qps_batched_bf16": round(nq / t_bf, 2), "qps_batched_fp32": round(nq / t_32, 2),
              "qps_single_fp32": round(nq / t_1, 2),
@@ -359,11 +368,31 @@ def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
                                  "single": round(float(np.mean([s.iterations for s in st_1])), 2)},
              "bf16_fine_sweep_mean": round(float(np.mean([s.fine_sweep for s in st_bf])), 2),
              "max_l1_vs_converged": {"bf16": l1_err("bf16", t), "fp32": l1_err("fp32", t),
-                                     "single": l1_err("single", t)}}
+                                     "single": l1_err("single", t)},
+             "extrapolated": {
+                 "qps_batched_bf16": round(nq / t_bx, 2), "qps_single_fp32": round(nq / t_1x, 2),
+                 "speedup_vs_single_extrapolated": round(t_1x / t_bx, 2),
+                 "speedup_vs_single": round(t_1 / t_bx, 2),
+                 "batch_ms": {"bf16_wall": round(t_bx * 1e3, 2),
+                              "bf16_in_library": round(ms_bx, 2)},
+                 "latency_ms_p50_bf16": round(_pct(lat_x, 50), 3),
+                 "latency_ms_p99_bf16": round(_pct(lat_x, 99), 3),
+                 "iterations_mean": {
+                     "bf16": round(float(np.mean([s.iterations for s in st_bx])), 2),
+                     "single": round(float(np.mean([s.iterations for s in st_1x])), 2)},
+                 "iterations_max": {"bf16": int(max(s.iterations for s in st_bx)),
+                                    "single": int(max(s.iterations for s in st_1x))},
+                 "max_l1_vs_converged": {"bf16": l1_err("bf16", t, True),
+                                         "single": l1_err("single", t, True)}}}
         res[key] = r
+        x = r["extrapolated"]
         log(f"PPR tol {t:g}: batched BF16 {r['qps_batched_bf16']} q/s "
             f"(p99 {r['latency_ms_p99_bf16']} ms), batched FP32 {r['qps_batched_fp32']} q/s, "
             f"single FP32 {r['qps_single_fp32']} q/s; max L1 vs converged {r['max_l1_vs_converged']}")
+        log(f"  extrapolated: batched BF16 {x['qps_batched_bf16']} q/s (p99 "
+            f"{x['latency_ms_p99_bf16']} ms), single FP32 {x['qps_single_fp32']} q/s, "
+            f"sweeps mean {x['iterations_mean']} max {x['iterations_max']}; "
+            f"max L1 vs converged {x['max_l1_vs_converged']}")
     return res
 
 
@@ -442,7 +471,7 @@ def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
     for sc in (18, 20, 22):
         src, dst = ag.gen_rmat(args.seed + sc, sc, 16)
         graphs.append(ag.Graph.build(ctx, src, dst, 1 << sc))
-    srv = ag.Server(ctx, precision="bf16")
+    srv = ag.Server(ctx, precision="bf16", extrapolate=args.extrapolate)
     gids = [srv.add_graph(g) for g in graphs]
     rng = np.random.RandomState(seed)
     # warm-up, untimed: each graph's gather layout, columns-per-wave autotune,
@@ -498,7 +527,8 @@ def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
             "vs_baseline": None, "dtype": "bf16-stored/f32-accumulate",
             "data": "synthetic (seeded R-MAT graphs, Poisson arrivals)",
             "config": {"workload": "serving_rmat_s18_s20_s22_poisson",
-                       "tolerance": "log-uniform [1e-6, 1e-4]", "seeds": "1-16 uniform"},
+                       "tolerance": "log-uniform [1e-6, 1e-4]", "seeds": "1-16 uniform",
+                       "extrapolated_steps": bool(args.extrapolate)},
             "rates": out}
     print(json.dumps(line), flush=True)
     srv.close()

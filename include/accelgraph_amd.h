@@ -140,6 +140,7 @@ int ag_vertex_map_host(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uin
 
 #define AG_PR_FLAG_FINISH_FP32 0x1u /* low precision: switch to FP32 storage when the residual stalls */
 #define AG_PR_FLAG_TIME_SWEEPS 0x2u /* time the sweep kernels alone (HIP events around each) */
+#define AG_PR_FLAG_EXTRAPOLATE 0x4u /* personalized PageRank only (ag_ppr_single_opts) */
 
 typedef struct ag_pr_options
 {
@@ -292,6 +293,10 @@ typedef struct ag_ppr_options
     uint32_t cols_per_wave;   /* 0: autotuned from {4, 8, 16, 32, 64} */
     float damp;               /* 0.85f */
     uint32_t use_graph_replay;/* 1: capture sweeps in a hipGraph and replay */
+    uint32_t extrapolate;     /* 1: a column whose residual falls by a steady ratio rho
+                                 takes the step x + rho / (1 - rho) (T(x) - x)
+                                 (Aitken-type); the stop rule still reads a plain
+                                 sweep's residual.  0 (default): power iteration */
 } ag_ppr_options;
 void ag_ppr_options_default(ag_ppr_options *opt);
 
@@ -307,6 +312,10 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
  * contributions in 16 bits and finishes in FP32 once the residual stalls. */
 int ag_ppr_single(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t precision, float *x,
                   ag_ppr_stat *stat);
+/* ag_ppr_single with options: precision, damp and extrapolate are honoured
+ * (cols_per_wave and use_graph_replay do not apply to one vector). */
+int ag_ppr_single_opts(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, const ag_ppr_options *opt,
+                       float *x, ag_ppr_stat *stat);
 
 /* Chooses cols_per_wave for g by timing {4, 8, 16, 32, 64} (BF16: from 8) on a
  * full 64-column batch; result cached per graph and precision. */
@@ -324,7 +333,7 @@ typedef struct ag_serve_config
     uint32_t inject_fail_alloc;/* test hook: refuse the n-th admission (0: off) */
     uint32_t inject_fail_query;/* test hook: fail ticket n (0: off) */
     float damp;
-    uint32_t reserved;
+    uint32_t extrapolate;      /* as ag_ppr_options.extrapolate (0: off) */
 } ag_serve_config;
 void ag_serve_config_default(ag_serve_config *cfg);
 
