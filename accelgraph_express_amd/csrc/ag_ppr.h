@@ -91,6 +91,9 @@ class PprBatch {
     // finished column is frozen on the device, so sweeping past it does not
     // change its result; polling less often saves the host round trips.
     uint32_t sweeps_hint() const { return hint_; }
+    // Sweeps until the slowest running column can meet its tolerance, from the
+    // same estimate (64 when unknown); compact() skips short tails.
+    uint32_t tail_sweeps() const { return tail_; }
     // Copies a finished column (caller's vertex order) to host memory.
     void extract(uint32_t slot, float *host_out);
     // Writes a finished column in vertex order to `dst`, memory the device can
@@ -149,6 +152,7 @@ class PprBatch {
     hipGraph_t graph_ = nullptr;
     double last_ms_ = 0.0;
     uint32_t hint_ = 4;
+    uint32_t tail_ = 64;
 };
 
 template <typename F>

@@ -66,6 +66,9 @@ constexpr uint32_t kMid = 0x80008000u; // low halves of two coarse columns: inte
 #define AG_PPR_COARSE_FLOOR 0.01
 #endif
 constexpr double kCoarseFloor = AG_PPR_COARSE_FLOOR;
+// Fewest remaining sweeps (estimated at 0.6x residual per sweep for the
+// slowest running column) for which a draining batch packs its columns.
+constexpr uint32_t kCompactMinTail = 3;
 
 struct PprArgs {
     uint32_t N, cap;
@@ -1205,6 +1208,10 @@ bool PprBatch::cancel(uint64_t tag)
 void PprBatch::compact()
 {
     if (!pending_.empty() || used_count_ == 0 || used_count_ == hi_) return;
+    // Moving columns costs about one wide sweep, plus one per buffer when the
+    // split layout switches with it (s22: 1.0 ms + 2 x 0.9 ms against 2.4 ms
+    // fine sweeps that narrow to 0.9 ms): not worth it for the last two sweeps.
+    if (tail_ < kCompactMinTail) return;
     const uint32_t vw = vec_width(fmt_);
     auto width = [&](uint32_t h) {
         uint32_t c = vw;
@@ -1424,7 +1431,7 @@ std::vector<PprBatch::Done> PprBatch::poll()
     AG_HIP(hipMemcpyAsync(cs.data(), cols_.p, hi_ * sizeof(PprCol), hipMemcpyDeviceToHost,
                           ctx_->stream));
     AG_HIP(hipStreamSynchronize(ctx_->stream));
-    uint32_t hint = 8;
+    uint32_t hint = 8, tail = 0;
     bool known = false;
     for (uint32_t k = 0; k < hi_; k++) {
         if (!used_[k]) continue;
@@ -1433,12 +1440,15 @@ std::vector<PprBatch::Done> PprBatch::poll()
         } else if (std::isfinite(cs[k].residual) && cs[k].residual > 0.0 && cs[k].tol > 0.0) {
             const double need = std::ceil(std::log(cs[k].tol / cs[k].residual) / std::log(0.6));
             hint = std::min<uint32_t>(hint, (uint32_t)std::max(1.0, std::min(8.0, need)));
+            tail = std::max<uint32_t>(tail, (uint32_t)std::max(1.0, std::min(64.0, need)));
             known = true;
         } else {
             hint = std::min<uint32_t>(hint, 4);
+            tail = 64;
         }
     }
     hint_ = known || hint < 8 ? hint : 4;
+    tail_ = tail;
     return out;
 }
 
