@@ -64,7 +64,7 @@ def main() -> int:
     ap.add_argument("--no-ppr", action="store_true", help="skip the batched-PPR leg")
     ap.add_argument("--no-dd", action="store_true",
                     help="skip the data-driven pull and SPMV lines")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
     ap.add_argument("--extrapolate", action="store_true",
                     help="c5: columns take Aitken-type extrapolated steps")
     ap.add_argument("--serialize", default=None, metavar="DIR",
@@ -84,6 +84,8 @@ def main() -> int:
 
     if args.workload == "c3":
         return workload_c3(args)
+    if args.workload == "c4":
+        return workload_c4(args)
     if args.workload == "c5":
         return workload_c5(args)
     scale, ef, iters = args.scale, args.edge_factor, args.iterations
@@ -407,6 +409,31 @@ def _standard_error_bound(src, dst, n, x, damp=0.85):
     tx = (1 - damp) / n + damp * (np.bincount(dst, weights=contrib[src], minlength=n)
                                   + x[od == 0].sum() / n)
     return float(np.abs(x - tx).sum() / (1 - damp))
+
+
+def workload_c4(args):
+    """The PPR leg alone: 64 queries on the R-MAT s22 graph (plain and
+    extrapolated, batched and single), as in the default run."""
+    import numpy as np
+    import accelgraph_express_amd as ag
+    scale = args.scale
+    src, dst = ag.gen_rmat(args.seed, scale, args.edge_factor)
+    N = 1 << scale
+    ctx = ag.Context(0)
+    g = ag.Graph.build(ctx, src, dst, N)
+    g.prepare()
+    res = ppr_leg(ag, g, N, np)
+    r = res["tol_1e-4"]
+    line = {"metric": "batched PPR queries/s (64 queries, L1 residual 1e-4, R-MAT s%d)" % scale,
+            "value": r["extrapolated"]["qps_batched_bf16"], "unit": "queries/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "ms_per_step": r["extrapolated"]["batch_ms"]["bf16_wall"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16-stored/f32-accumulate",
+            "data": "synthetic (seeded R-MAT, 1-16 uniform seeds per query)",
+            "config": {"workload": f"ppr64_rmat_s{scale}", "extrapolated_steps": True},
+            "ppr": res}
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def workload_c3(args):
