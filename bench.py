@@ -289,7 +289,7 @@ def _pct(v, q):
 
 
 def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
-    """C4: 64 queries, 1-16 uniform seeds each, per-column stop at L1 residual
+    """64 PPR queries, 1-16 uniform seeds each, per-column stop at L1 residual
     `tol`, all 64 as the columns of one batch vs one query at a time.
 
     Both paths leave the result vectors in HBM (a 16 MiB rank vector per query
@@ -465,13 +465,13 @@ def workload_c3(args):
         runs[prec] = {"seconds": round(secs, 4), "sweeps": st.sweeps,
                       "fp32_switch_sweep": st.fp32_switch_sweep,
                       "residual": st.error_total, "loop_ms": round(st.loop_ms, 3)}
-        log(f"C3 {prec}: {secs * 1e3:.1f} ms, {st.sweeps} sweeps, residual {st.error_total:.2e}, "
+        log(f"s24 STANDARD {prec}: {secs * 1e3:.1f} ms, {st.sweeps} sweeps, residual {st.error_total:.2e}, "
             f"switch at {st.fp32_switch_sweep}")
     for prec in ("bf16", "fp16"):
         runs[prec]["l1_vs_fp32"] = float(np.abs(ranks[prec].astype(np.float64) - ranks["fp32"]).sum())
     for prec in ("fp32", "bf16", "fp16"):
         runs[prec]["fp64_error_bound"] = _standard_error_bound(src, dst, N, ranks[prec])
-        log(f"C3 {prec}: L1 error vs the FP64 fixed point <= {runs[prec]['fp64_error_bound']:.2e}")
+        log(f"s24 STANDARD {prec}: L1 error vs the FP64 fixed point <= {runs[prec]['fp64_error_bound']:.2e}")
     line = {"metric": "time to L1 residual 1e-6, STANDARD PageRank (R-MAT s24)",
             "value": runs["bf16"]["seconds"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
             "ms_per_step": runs["bf16"]["seconds"] * 1e3, "higher_is_better": False,
@@ -545,7 +545,7 @@ def workload_c5(args, rates=(200.0, 800.0, 3200.0), per_rate=300, seed=9):
         wall = time.perf_counter() - t0
         r = {"offered_qps": rate, "completed": len(lat), "achieved_qps": round(len(lat) / wall, 2),
              "latency_ms_p50": round(_pct(lat, 50), 3), "latency_ms_p99": round(_pct(lat, 99), 3)}
-        log(f"C5 rate {rate}: {r}")
+        log(f"serving, offered {rate} q/s: {r}")
         out.append(r)
     best = max(out, key=lambda r: r["achieved_qps"])
     line = {"metric": "PPR serving queries/s + p99 latency (resident R-MAT s18/s20/s22)",
