@@ -296,6 +296,35 @@ __global__ void k_segment_bounds(const uint32_t *__restrict__ keys, uint32_t n,
     if (i == (uint64_t)n - 1 || keys[i + 1] != v) end[v] = (uint32_t)(i + 1);
 }
 
+// The same, four keys per thread from one 16-byte load (keys 16-byte aligned);
+// the neighbours across the group come from the adjacent words.
+__global__ void k_segment_bounds4(const uint32_t *__restrict__ keys, uint32_t n,
+                                  uint32_t *__restrict__ idx, uint32_t *__restrict__ end)
+{
+    const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (i0 >= n) return;
+    const uint32_t m = (uint32_t)min<uint64_t>(4, n - i0);
+    uint32_t k[4];
+    if (m == 4) {
+        const uint4 q = *reinterpret_cast<const uint4 *>(keys + i0);
+        k[0] = q.x; k[1] = q.y; k[2] = q.z; k[3] = q.w;
+    } else {
+        for (uint32_t j = 0; j < 4; j++) k[j] = j < m ? keys[i0 + j] : 0u;
+    }
+    const bool has_prev = i0 > 0, has_next = i0 + m < n;
+    const uint32_t prev = has_prev ? keys[i0 - 1] : 0u;
+    const uint32_t next = has_next ? keys[i0 + m] : 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4; j++) {
+        if (j >= m) break;
+        const uint32_t v = k[j];
+        const bool first = j == 0 ? (!has_prev || prev != v) : k[j - 1] != v;
+        const bool last = j + 1 == m ? (!has_next || next != v) : k[j + 1] != v;
+        if (first) idx[v] = (uint32_t)(i0 + j);
+        if (last) end[v] = (uint32_t)(i0 + j + 1);
+    }
+}
+
 __global__ void k_segment_degree(const uint32_t *__restrict__ idx, uint32_t *__restrict__ deg,
                                  uint32_t N)
 {
@@ -305,13 +334,26 @@ __global__ void k_segment_degree(const uint32_t *__restrict__ idx, uint32_t *__r
     deg[v] = e ? e - idx[v] : 0u;
 }
 
+__device__ __forceinline__ uint32_t max4(uint4 q)
+{
+    return max(max(q.x, q.y), max(q.z, q.w));
+}
+
+// vec: a and b (when given) are 16-byte aligned; the first n / 4 * 4 words
+// are read four at a time, the rest one at a time.
 __global__ __launch_bounds__(kBlock) void k_max2(const uint32_t *__restrict__ a,
                                                  const uint32_t *__restrict__ b, uint32_t n,
-                                                 uint32_t *__restrict__ out)
+                                                 uint32_t vec, uint32_t *__restrict__ out)
 {
     uint32_t m = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t n4 = vec ? n / 4 : 0;
+    for (uint64_t i = t; i < n4; i += stride) {
+        m = max(m, max4(reinterpret_cast<const uint4 *>(a)[i]));
+        if (b) m = max(m, max4(reinterpret_cast<const uint4 *>(b)[i]));
+    }
+    for (uint64_t i = n4 * 4 + t; i < n; i += stride) {
         m = max(m, a[i]);
         if (b) m = max(m, b[i]);
     }
@@ -413,8 +455,12 @@ void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N
     AG_HIP(hipMemsetAsync(degree, 0, (size_t)N * 4, s));
     AG_HIP(hipMemsetAsync(edges_idx, 0, (size_t)N * 4, s));
     if (n) {
-        hipLaunchKernelGGL(k_segment_bounds, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                           sorted_keys, n, edges_idx, degree);
+        if ((reinterpret_cast<uintptr_t>(sorted_keys) & 15u) == 0)
+            hipLaunchKernelGGL(k_segment_bounds4, dim3(ceil_div(ceil_div(n, 4), 256)), dim3(256), 0,
+                               s, sorted_keys, n, edges_idx, degree);
+        else
+            hipLaunchKernelGGL(k_segment_bounds, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                               sorted_keys, n, edges_idx, degree);
         hipLaunchKernelGGL(k_segment_degree, dim3(ceil_div(N, 256)), dim3(256), 0, s, edges_idx,
                            degree, N);
         AG_HIP(hipGetLastError());
@@ -428,8 +474,10 @@ static uint32_t device_max_impl(ag_ctx *ctx, const uint32_t *a, const uint32_t *
     std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
     uint32_t *out = static_cast<uint32_t *>(scratch(ctx, kScrMax, 4));
     AG_HIP(hipMemsetAsync(out, 0, 4, ctx->stream));
-    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock * 8), 4096);
-    hipLaunchKernelGGL(k_max2, dim3(blocks), dim3(kBlock), 0, ctx->stream, a, b, n, out);
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock * 16), 4096);
+    const uint32_t vec = (reinterpret_cast<uintptr_t>(a) & 15u) == 0 &&
+                         (!b || (reinterpret_cast<uintptr_t>(b) & 15u) == 0);
+    hipLaunchKernelGGL(k_max2, dim3(blocks), dim3(kBlock), 0, ctx->stream, a, b, n, vec, out);
     AG_HIP(hipGetLastError());
     uint32_t m = 0;
     AG_HIP(hipMemcpyAsync(&m, out, 4, hipMemcpyDeviceToHost, ctx->stream));
