@@ -99,22 +99,44 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32
 }
 
 // ---- radix sort -----------------------------------------------------------------
+// Per-tile digit histogram.  Each wave counts into its own LDS copy (fewer
+// same-address atomics); with `vec` (keys 16-byte aligned) a full tile is read
+// as 16-byte loads, four keys each.
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restrict__ keys,
                                                        uint32_t n, uint32_t shift,
-                                                       uint32_t mask,
+                                                       uint32_t mask, uint32_t vec,
                                                        uint32_t *__restrict__ hist)
 {
-    __shared__ uint32_t h[kRadix];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[kWaves][kRadix];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) h[i][threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
+    if (vec && base + kTile <= n) {
+        const uint4 *k4 = reinterpret_cast<const uint4 *>(keys + base);
+        uint4 q[kItems / 4];
 #pragma unroll
-    for (int i = 0; i < kItems; i++) {
-        uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & mask], 1u);
+        for (int i = 0; i < kItems / 4; i++) q[i] = k4[i * kBlock + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < kItems / 4; i++) {
+            atomicAdd(&h[w][(q[i].x >> shift) & mask], 1u);
+            atomicAdd(&h[w][(q[i].y >> shift) & mask], 1u);
+            atomicAdd(&h[w][(q[i].z >> shift) & mask], 1u);
+            atomicAdd(&h[w][(q[i].w >> shift) & mask], 1u);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kItems; i++) {
+            const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+            if (idx < n) atomicAdd(&h[w][(keys[idx] >> shift) & mask], 1u);
+        }
     }
     __syncthreads();
-    hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = h[threadIdx.x];
+    uint32_t t = 0;
+#pragma unroll
+    for (int i = 0; i < kWaves; i++) t += h[i][threadIdx.x];
+    hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = t;
 }
 
 // Phase 1: per chunk of `tpc` tiles, per digit sums.
@@ -429,8 +451,9 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint32_t *ok = to_out ? kout : tk;
         uint32_t *ov = to_out ? vout : tv;
+        const uint32_t vec = (reinterpret_cast<uintptr_t>(ck) & 15u) == 0;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ck, n, shift,
-                           (1u << nbits) - 1u, hist);
+                           (1u << nbits) - 1u, vec, hist);
         hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
                            csum);
         hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum, nchunks);
