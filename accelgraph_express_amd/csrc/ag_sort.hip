@@ -216,6 +216,19 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_apply(uint32_t *__restric
 // running count.  The tile is then reordered by digit in LDS (wave 0 < 1 < 2 <
 // 3 inside a digit, i.e. input order) and written out run by run, so that keys
 // of one digit leave as consecutive words instead of one scattered word each.
+// Workgroups are dispatched round-robin over the 8 XCDs.  Giving each XCD a
+// contiguous range of tiles puts the neighbouring runs of one digit (tile t's
+// and tile t+1's) into the same L2, where they merge into whole lines before
+// they are written back.  A bijection on [0, n): XCD x < n % 8 takes n / 8 + 1
+// tiles, the others n / 8.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
+{
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    const uint32_t x = b % kXcds, j = b / kXcds;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
@@ -231,7 +244,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     for (int i = 0; i < kWaves; i++) wcnt[i][threadIdx.x] = 0;
     __syncthreads();
 
-    const uint64_t tile0 = (uint64_t)blockIdx.x * kTile;
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const uint64_t tile0 = (uint64_t)tile * kTile;
     const uint64_t base = tile0 + (uint64_t)w * (kTile / kWaves);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t k[kItems], v[kItems], r[kItems];
@@ -271,7 +285,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     __syncthreads();
     uint32_t loc = incl - tot;
     for (int i = 0; i < w; i++) loc += wsum[i];
-    gbase[d0] = hist[(uint64_t)blockIdx.x * kRadix + d0] - loc;
+    gbase[d0] = hist[(uint64_t)tile * kRadix + d0] - loc;
     {
         uint32_t run = loc;
 #pragma unroll
