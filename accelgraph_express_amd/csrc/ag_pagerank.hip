@@ -755,19 +755,26 @@ __global__ void k_add_base(uint32_t *__restrict__ x, uint32_t n, uint32_t base)
 }
 
 // CSR rows (hub + mid): one wave per row, relabelled sources in original order.
-__global__ void k_fill_csr_rows(const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
-                                const uint32_t *__restrict__ inv_idx,
-                                const uint32_t *__restrict__ inv_dst,
-                                const uint32_t *__restrict__ rank, uint32_t nrows,
-                                uint32_t *__restrict__ cols)
+// CSR rows: one thread per edge slot e < roff[nrows]; its row is the last r
+// with roff[r] <= e (a binary search over the row offsets, which stay in
+// cache).  One wave per row made the largest hubs (~1.6e5 in-edges at R-MAT
+// scale 22) a serial tail of ~2 ms.
+__global__ void k_fill_csr_edges(const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
+                                 const uint32_t *__restrict__ inv_idx,
+                                 const uint32_t *__restrict__ inv_dst,
+                                 const uint32_t *__restrict__ rank, uint32_t nrows, uint32_t total,
+                                 uint32_t *__restrict__ cols)
 {
-    const uint32_t row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= nrows) return;
-    const uint32_t v = order[row];
-    const uint32_t b = roff[row], n = roff[row + 1] - b;
-    const uint32_t s = inv_idx[v];
-    for (uint32_t k = lane; k < n; k += 64) cols[b + k] = rank[inv_dst[s + k]];
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    uint32_t lo = 0, hi = nrows; // roff[lo] <= e < roff[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (roff[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t v = order[lo];
+    cols[e] = rank[inv_dst[inv_idx[v] + (e - roff[lo])]];
 }
 
 // SELL rows: one thread per row.
@@ -885,11 +892,11 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     const uint64_t ncols = (uint64_t)csr_edges + sell_slots;
     p->cols.alloc((size_t)ncols * 4);
     AG_HIP(hipMemsetAsync(p->cols.p, 0, (size_t)ncols * 4, s));
-    if (p->n_nonlow)
-        hipLaunchKernelGGL(k_fill_csr_rows, dim3(ceil_div(p->n_nonlow, 4)), dim3(256), 0, s,
+    if (p->n_nonlow && csr_edges)
+        hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(csr_edges, 256)), dim3(256), 0, s,
                            p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
-                           p->rank.as<uint32_t>(), p->n_nonlow, p->cols.as<uint32_t>());
+                           p->rank.as<uint32_t>(), p->n_nonlow, csr_edges, p->cols.as<uint32_t>());
     if (nlow)
         hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
                            p->order.as<uint32_t>(), rdeg.as<uint32_t>(),
