@@ -66,6 +66,11 @@ constexpr uint32_t kMid = 0x80008000u; // low halves of two coarse columns: inte
 #define AG_PPR_COARSE_FLOOR 0.01
 #endif
 constexpr double kCoarseFloor = AG_PPR_COARSE_FLOOR;
+// Largest relative error of a coarse read (the midpoint of a BF16 truncation
+// interval): half a BF16 ulp, 2^-8 of the value.
+constexpr double kMidRel = 0x1p-8;
+// E * 2^-134 for E < 2^32 edges: the midpoint reads of zero contributions.
+constexpr double kMidAbs = 0x1p-102;
 // Fewest remaining sweeps (estimated at 0.6x residual per sweep for the
 // slowest running column) for which a draining batch packs its columns.
 constexpr uint32_t kCompactMinTail = 3;
@@ -706,6 +711,7 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t np
             const double R = (w_r[0] + w_r[1]) + (w_r[2] + w_r[3]);
             const double D = (w_d[0] + w_d[1]) + (w_d[2] + w_d[3]);
             const double prev = col.residual;
+            const double d_read = col.dangling; // dangling mass of the iterate this sweep read
             // R is |T(x) - x| for the x this sweep read; after an extrapolated
             // step the stored iterate is not T(x), so R certifies nothing and
             // the column cannot stop on it
@@ -716,6 +722,19 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t np
             col.dangling = D;
             col.t = (float)((1.0 - (double)a.damp) + (double)a.damp * D);
             if (col.iters >= col.max_iter) {
+                col.done = 1;
+            } else if (col.coarse && !stepped && (!a.extrap || col.iters <= 4) &&
+                       R + kMidRel * fmax(0.0, 1.000001 - d_read) + kMidAbs <= col.tol) {
+                // Certified on a coarse sweep.  A midpoint read is within 2^-8
+                // of each contribution plus 2^-134 (the midpoint read of a
+                // zero), so the coarse map T_c is within delta = d (2^-8 M +
+                // E 2^-134) of T in L1, M the mass of x on rows with out-edges
+                // (1 - dangling mass while x >= 0: no extrapolated step yet).
+                // With R + delta / d <= tol the returned T_c(x) satisfies
+                // ||T_c(x) - x*||_1 <= d / (1 - d) tol, the bound of a plain
+                // stop.  It fires when x sits on rows without out-edges
+                // (M ~ 0), e.g. a query whose seeds are isolated, which would
+                // otherwise force one FP32-read sweep.
                 col.done = 1;
             } else if (col.coarse && (R <= col.tol || R <= kCoarseFloor ||
                                        (R > 0.97 * prev && R <= 8.0 * kCoarseFloor))) {

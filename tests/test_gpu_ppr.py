@@ -164,3 +164,33 @@ def test_single_bf16_finishes_in_fp32(ctx):
     star, _, _ = oracle.ppr_f64(csr, seeds, 1e-13, 5000)
     assert st.residual <= 1e-6
     assert np.abs(x - star).sum() <= 0.85 / 0.15 * 1e-6 + FP32_L1
+
+
+def test_bf16_coarse_certified_stop(ctx):
+    """A BF16 column may stop on a coarse sweep when its residual plus the
+    coarse-read noise bound (2^-8 of the mass on rows with out-edges) meets the
+    tolerance.  A query whose seeds have no out-edges is x = s from the start:
+    it stops after one sweep.  Its result is s up to the midpoint reads of zero
+    contributions (2^-134 each, so sums below 1e-30), and the other columns of
+    the batch are unchanged (alone == mixed)."""
+    g, csr, n = _graph(ctx)
+    od = np.asarray(csr["vertices.out_degree"])
+    sinks = np.flatnonzero(od == 0).astype(np.uint32)
+    assert sinks.size >= 8
+    qs = _queries(n, 10, 21)
+    qs.insert(3, (sinks[:1], 1e-6, 1000))
+    qs.insert(7, (sinks[1:5], 1e-6, 1000))
+    X, st, _ = ag.ppr_batch(g, qs, precision="bf16", cols_per_wave=64)
+    for i in (3, 7):
+        seeds = qs[i][0]
+        s = np.zeros(n, np.float32)
+        np.add.at(s, seeds.astype(np.int64), np.float32(1.0 / len(seeds)))
+        assert st[i].iterations == 1, i
+        np.testing.assert_allclose(X[i], s, rtol=0, atol=1e-30)
+        assert X[i][seeds].min() > 0.0 and np.abs(X[i] - s).sum() < 1e-28
+    for i in (0, 5, 11):
+        Xa, sa, _ = ag.ppr_batch(g, [qs[i]], precision="bf16", cols_per_wave=64)
+        assert sa[0].iterations == st[i].iterations
+        np.testing.assert_array_equal(Xa[0], X[i])
+        ref, _, _ = oracle.ppr_f64(csr, qs[i][0], 1e-13, 5000)
+        assert np.abs(X[i].astype(np.float64) - ref).sum() <= 0.85 / 0.15 * 1e-6 + FP32_L1
