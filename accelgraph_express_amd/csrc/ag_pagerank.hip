@@ -754,11 +754,10 @@ __global__ void k_add_base(uint32_t *__restrict__ x, uint32_t n, uint32_t base)
     if (i < n) x[i] += base;
 }
 
-// CSR rows (hub + mid): one wave per row, relabelled sources in original order.
-// CSR rows: one thread per edge slot e < roff[nrows]; its row is the last r
-// with roff[r] <= e (a binary search over the row offsets, which stay in
-// cache).  One wave per row made the largest hubs (~1.6e5 in-edges at R-MAT
-// scale 22) a serial tail of ~2 ms.
+// CSR rows (hub + mid), relabelled sources in original order.  One thread per
+// edge slot e < roff[nrows]; its row is the last r with roff[r] <= e (a binary
+// search over the row offsets, which stay in cache).  One wave per row made
+// the largest hubs (~1.6e5 in-edges at R-MAT scale 22) a serial tail of ~2 ms.
 __global__ void k_fill_csr_edges(const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
                                  const uint32_t *__restrict__ inv_idx,
                                  const uint32_t *__restrict__ inv_dst,
