@@ -33,6 +33,7 @@
 #include <mutex>
 #include <vector>
 
+#include "ag_extrap.h"
 #include "ag_plan.h"
 
 namespace ag {
@@ -580,19 +581,7 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
         if (E_ <= a.epsilon && !stepped) {
             st->done = 1;
         } else if (a.extrap) {
-            // the batch's rule: a steady residual ratio rho on two consecutive
-            // plain sweeps (within 2%), at least three plain sweeps apart
-            if (stepped) {
-                st->since = 0;
-                st->rho = 0.0;
-            } else {
-                const double prev = st->prev_err;
-                const double rho = it > 0 && prev > 0.0 ? E_ / prev : 0.0;
-                st->since += 1;
-                if (st->since >= 3 && rho >= 0.2 && rho <= 0.95 && fabs(rho - st->rho) <= 0.02 * rho)
-                    st->w = (float)(rho / (1.0 - rho));
-                st->rho = rho;
-            }
+            st->w = extrapolation_weight(E_, it > 0 ? st->prev_err : 0.0, stepped, st->since, st->rho);
         }
     } else {
         st->dangling = D_;

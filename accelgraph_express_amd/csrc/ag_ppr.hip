@@ -44,6 +44,7 @@
 #include <cstring>
 #include <map>
 
+#include "ag_extrap.h"
 #include "ag_ppr.h"
 
 namespace ag {
@@ -661,28 +662,6 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows(PprArgs a)
 // exact FP32 contributions, and it is done at the tolerance there.
 constexpr int kRedBlock = 256;
 
-// Aitken-type extrapolation (Kamvar, Haveliwala, Manning, Golub, "Extrapolation
-// methods for accelerating PageRank computations", WWW 2003): once a column's
-// residual falls by the same ratio rho on consecutive sweeps (within 2%), its
-// error is dominated by one mode e_k = rho^k v, and x + rho / (1 - rho) (T(x)
-// - x) removes it.  On R-MAT graphs the ratio settles at 0.3-0.7 after a few
-// sweeps.  At least three plain sweeps separate two extrapolated steps, and
-// the stop rule only ever reads a plain sweep's residual, so the contraction
-// bound ||x - x*||_1 <= d / (1 - d) * tol holds as without it.
-__device__ __forceinline__ void extrapolation_rule(PprCol &col, double R, double prev, bool stepped)
-{
-    if (stepped) {
-        col.since = 0;
-        col.rho = 0.0;
-        return;
-    }
-    const double rho = prev > 0.0 && prev < INFINITY ? R / prev : 0.0;
-    col.since += 1;
-    if (col.since >= 3 && rho >= 0.2 && rho <= 0.95 && fabs(rho - col.rho) <= 0.02 * rho)
-        col.w = (float)(rho / (1.0 - rho));
-    col.rho = rho;
-}
-
 __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t nparts)
 {
     __shared__ double w_r[kRedBlock / 64], w_d[kRedBlock / 64];
@@ -743,7 +722,7 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t np
             } else if (R <= col.tol && !stepped) {
                 col.done = 1;
             }
-            if (!col.done && a.extrap) extrapolation_rule(col, R, prev, stepped);
+            if (!col.done && a.extrap) col.w = extrapolation_weight(R, prev, stepped, col.since, col.rho);
         }
     }
 }
