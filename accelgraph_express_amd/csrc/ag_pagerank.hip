@@ -765,7 +765,7 @@ __global__ void k_fill_csr_edges(const uint32_t *__restrict__ order, const uint3
     cols[e] = rank[inv_dst[inv_idx[v] + (e - roff[lo])]];
 }
 
-// SELL rows: one thread per row.
+// SELL rows: one thread per row, four edges' loads in flight at a time.
 __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint32_t *__restrict__ rdeg,
                                  const uint32_t *__restrict__ inv_idx,
                                  const uint32_t *__restrict__ inv_dst,
@@ -780,7 +780,17 @@ __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint3
     const uint32_t n = rdeg[row];
     const uint32_t s = inv_idx[v];
     uint32_t *p = cols + slice_ptr[t / 64] + (t % 64);
-    for (uint32_t k = 0; k < n; k++) p[k * 64] = rank[inv_dst[s + k]];
+    uint32_t k = 0;
+    for (; k + 4 <= n; k += 4) {
+        uint32_t d[4], r[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) d[j] = inv_dst[s + k + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) r[j] = rank[d[j]];
+#pragma unroll
+        for (int j = 0; j < 4; j++) p[(k + j) * 64] = r[j];
+    }
+    for (; k < n; k++) p[k * 64] = rank[inv_dst[s + k]];
 }
 
 } // namespace
