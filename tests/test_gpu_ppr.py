@@ -194,3 +194,19 @@ def test_bf16_coarse_certified_stop(ctx):
         np.testing.assert_array_equal(Xa[0], X[i])
         ref, _, _ = oracle.ppr_f64(csr, qs[i][0], 1e-13, 5000)
         assert np.abs(X[i].astype(np.float64) - ref).sum() <= 0.85 / 0.15 * 1e-6 + FP32_L1
+
+
+@pytest.mark.parametrize("damp", [0.5, 0.95])
+def test_damping_factor_single_and_batch(ctx, damp):
+    """ag_ppr_single_opts and ag_ppr_batch honour a damping factor other than
+    0.85: within the FP32 bound of the FP64 oracle run with the same d and stop
+    rule, iteration counts within one sweep."""
+    g, csr, n = _graph(ctx, scale=12)
+    qs = _queries(n, 6, 13)
+    X, st, _ = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=8, damp=damp)
+    for i, (seeds, tol, mi) in enumerate(qs):
+        ref, it, _ = oracle.ppr_f64(csr, seeds, tol, mi, damp=damp)
+        x, s1 = ag.ppr_single(g, seeds, tol, damp=damp)
+        assert abs(int(st[i].iterations) - it) <= 1 and abs(int(s1.iterations) - it) <= 1, i
+        assert np.abs(X[i] - ref).sum() <= FP32_L1, i
+        assert np.abs(x - ref).sum() <= FP32_L1, i
