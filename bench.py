@@ -231,11 +231,21 @@ def main() -> int:
         threads = oracle.omp_max_threads()
         log(f"CPU baseline: oracle OpenMP CSR build + {iters}-sweep pull PR, {threads} threads")
         csr_s, csr = oracle.csr_build_omp(src, dst, N, threads)
-        secs, cpu_ranks, it_cpu, _ = oracle.pagerank_pull_omp(csr, iters, 0.0, threads)
+        # one warm run, then the median of five (the loop takes ~0.1-0.2 s)
+        oracle.pagerank_pull_omp(csr, iters, 0.0, threads)
+        runs = [oracle.pagerank_pull_omp(csr, iters, 0.0, threads) for _ in range(5)]
+        runs.sort(key=lambda r: r[0])
+        secs, cpu_ranks, it_cpu, _ = runs[2]
+        secs1, _, _, _ = oracle.pagerank_pull_omp(csr, iters, 0.0, 1)
         cpu = {"value": round(E * iters / secs / 1e9, 4), "unit": "GTEPS", "cores": threads,
                "kind": "port",
-               "sample": f"full {iters}-sweep FP32 pull PageRank on the same R-MAT s{scale} graph "
-                         f"({secs:.2f} s loop; OpenMP CSR build {csr_s:.2f} s not included)"}
+               "sample": f"full {iters}-sweep FP32 pull PageRank on the same R-MAT s{scale} graph, "
+                         f"median of 5 after a warm run ({secs:.2f} s loop; OpenMP CSR build "
+                         f"{csr_s:.2f} s not included)",
+               "cpu_model": _cpu_model(),
+               "single_thread": {"value": round(E * iters / secs1 / 1e9, 4), "unit": "GTEPS",
+                                 "loop_s": round(secs1, 2)},
+               "csr_build_s": round(csr_s, 3)}
         # both against the same sweeps in FP64: the reference's sequential FP32
         # row sums drift from it on the largest hub rows (~1.6e5 in-edges)
         f64, _, _ = oracle.pagerank_pull_f64(csr, iters, 0.0)
@@ -274,7 +284,10 @@ def main() -> int:
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_pr_sweep",
                      "sweep_us": round(sweep_ms * 1e3, 2),
-                     "algorithmic_bytes_per_sweep": bytes_sweep},
+                     "algorithmic_bytes_per_sweep": bytes_sweep,
+                     # diagnostic, not a roofline: every gather counted as a
+                     # 4-byte read, most of which hit the LDS table or L2
+                     "gather_inclusive_GBps": round((8 * E + 20 * N) / (sweep_ms / 1e3) / 1e9, 1)},
         "cpu_baseline": cpu,
         "parity": parity,
         "ppr": ppr,
@@ -396,6 +409,17 @@ def ppr_leg(ag, g, N, np, nq=64, tol=1e-4, seed=4):
             f"sweeps mean {x['iterations_mean']} max {x['iterations_max']}; "
             f"max L1 vs converged {x['max_l1_vs_converged']}")
     return res
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _standard_error_bound(src, dst, n, x, damp=0.85):
