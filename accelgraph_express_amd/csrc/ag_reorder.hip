@@ -114,7 +114,7 @@ void check_ids(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t E
 {
     if (E == 0) return;
     if (N == 0) fail(AG_ERR_INVALID, "edges given for a graph of 0 vertices");
-    const uint32_t m = std::max(device_max1(ctx, src, E), device_max1(ctx, dst, E));
+    const uint32_t m = device_max2(ctx, src, dst, E); // one pass, one wait
     if (m >= N) fail(AG_ERR_INVALID, "vertex id %u >= num_vertices %u", m, N);
 }
 
