@@ -143,3 +143,23 @@ def test_serving_alone_equals_in_mix(ctx):
         l1 = np.abs(x.astype(np.float64) - _fixed_point(csr, seeds)).sum()
         assert l1 <= D / (1 - D) * tol + FP32_L1
     srv.close()
+
+
+def test_max_iter_and_null_options(ctx):
+    """A column that reaches max_iter stops there whatever the extrapolation
+    state (its last sweep may have stepped); its result stays finite with mass
+    1, alone == mixed.  ag_ppr_single_opts refuses a null options pointer."""
+    g, csr, n = _graph(ctx)
+    qs = [(s, 1e-12, 6) for s, _, _ in _queries(n, 5, 17)]
+    X, st, _ = ag.ppr_batch(g, qs, precision="fp32", cols_per_wave=8, extrapolate=True)
+    for i in range(5):
+        assert st[i].iterations == 6 and st[i].status == L.AG_OK
+        assert np.isfinite(X[i]).all() and abs(X[i].astype(np.float64).sum() - 1.0) < 1e-4
+        Xa, sa, _ = ag.ppr_batch(g, [qs[i]], precision="fp32", cols_per_wave=8, extrapolate=True)
+        np.testing.assert_array_equal(Xa[0], X[i])
+    x, s1 = ag.ppr_single(g, qs[0][0], 1e-12, max_iter=6, extrapolate=True)
+    assert s1.iterations == 6 and np.isfinite(x).all()
+    q, keep = ag.graph._query(qs[0][0], 1e-4, 100)
+    st1 = L.PprStat()
+    rc = L.load().ag_ppr_single_opts(g.ctx.h, g.h, C.byref(q), None, None, C.byref(st1))
+    assert rc == L.AG_ERR_INVALID
