@@ -604,9 +604,8 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows(PprArgs a)
 #pragma unroll
                 for (int i = 0; i < VW; i++) acc[i] = 0.0f;
                 if (amask) {
-                    // RU chunk partials per step (16 float4 loads in flight per
-                    // lane): the largest hubs' chains of ~150 partials per slot
-                    // set this kernel's time.  Sums stay in chunk order.
+                    // RU chunk partials per step, 16 float4 loads in flight per
+                    // lane; sums stay in chunk order.
                     constexpr int RU = 64 / VW;
                     uint32_t k = f0 + slot;
                     for (; k + (RU - 1) * kSlots < f1; k += RU * kSlots) {
