@@ -70,6 +70,7 @@ def _fixpoint_residual(src, dst, n, ranks):
 @pytest.mark.parametrize("eps", [0.0, 1e-4])
 def test_skewed_bins(ctx, eps):
     """R-MAT scale 16 with hub, mid, low and zero rows all populated."""
+    saved = ag.get_tuning()
     ag.set_tuning(low_max=24, hub_min=512, hub_chunk=256)
     try:
         src, dst = ag.gen_rmat(77, 16, 16)
@@ -82,7 +83,7 @@ def test_skewed_bins(ctx, eps):
         cap = 60 if eps == 0.0 else 500
         st, (pr, it, err, psum, act, errs, work) = _case(g, csr, cap, eps)
     finally:
-        ag.set_tuning(low_max=64, hub_min=4096, hub_chunk=4096)
+        ag.set_tuning(**saved)
     if eps == 0.0:
         assert st.iterations == it == cap
         np.testing.assert_array_equal(st.active_per_iter, act)

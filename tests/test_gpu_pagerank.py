@@ -126,6 +126,7 @@ def test_edge_cases(ctx):
 
 def test_skewed_bins_all_paths(ctx):
     """R-MAT scale 16: hub, mid, low and zero bins all populated."""
+    saved = ag.get_tuning()
     ag.set_tuning(low_max=24, hub_min=512, hub_chunk=256)
     try:
         src, dst = ag.gen_rmat(77, 16, 16)
@@ -134,7 +135,7 @@ def test_skewed_bins_all_paths(ctx):
         assert min(info["n_hub"], info["n_mid"], info["n_low"], info["n_zero"]) > 0
         assert info["hub_chunks"] > info["n_hub"]
     finally:
-        ag.set_tuning(low_max=64, hub_min=4096, hub_chunk=4096)
+        ag.set_tuning(**saved)
 
 
 def test_rmat_s22_20_sweeps(ctx):
@@ -156,11 +157,12 @@ def test_hot_table_sizes(ctx, hot_bytes):
     not the arithmetic: ranks are identical for every table size."""
     src, dst = ag.gen_rmat(21, 16, 16)
     ref = ag.Graph.build(ctx, src, dst, 1 << 16).pagerank_pull(20, 0.0).pageRanks
+    saved = ag.get_tuning()
     ag.set_tuning(hot_bytes=hot_bytes)
     try:
         got = ag.Graph.build(ctx, src, dst, 1 << 16).pagerank_pull(20, 0.0).pageRanks
     finally:
-        ag.set_tuning(hot_bytes=159744)
+        ag.set_tuning(**saved)
     np.testing.assert_array_equal(got, ref)
 
 

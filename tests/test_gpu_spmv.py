@@ -62,6 +62,7 @@ def test_c1_uniform_exact(ctx):
 
 
 def test_skewed_bins(ctx):
+    saved = ag.get_tuning()
     ag.set_tuning(low_max=24, hub_min=512, hub_chunk=256)
     try:
         src, dst = ag.gen_rmat(77, 16, 16)
@@ -71,7 +72,7 @@ def test_skewed_bins(ctx):
         info = g.plan_info()
         assert min(info["n_hub"], info["n_mid"], info["n_low"], info["n_zero"]) > 0
     finally:
-        ag.set_tuning(low_max=64, hub_min=4096, hub_chunk=4096)
+        ag.set_tuning(**saved)
     y64 = _y64(src, dst, x, n)
     ref = oracle.spmv_pull_f32(csr, x)
     nz = y64 > 0

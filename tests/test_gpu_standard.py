@@ -38,6 +38,7 @@ def test_uniform_65536_fixed_20_sweeps(ctx):
 
 
 def test_skewed_bins_fixed_sweeps(ctx):
+    saved = ag.get_tuning()
     ag.set_tuning(low_max=24, hub_min=512, hub_chunk=256)
     try:
         src, dst = ag.gen_rmat(77, 16, 16)
@@ -48,7 +49,7 @@ def test_skewed_bins_fixed_sweeps(ctx):
         assert min(info["n_hub"], info["n_mid"], info["n_low"], info["n_zero"]) > 0
         st = g.pagerank_pull(iterations=30, epsilon=0.0, mode="standard")
     finally:
-        ag.set_tuning(low_max=64, hub_min=4096, hub_chunk=4096)
+        ag.set_tuning(**saved)
     ref, it, res = oracle.pagerank_standard_f64(oracle.build_csr(src, dst, n), 30, 0.0)
     assert _rel(st.pageRanks, ref).max() <= 1e-5
     assert abs(st.pageRanks.astype(np.float64).sum() - 1.0) <= 1e-6
