@@ -176,7 +176,8 @@ uint32_t device_max1(ag_ctx *ctx, const uint32_t *a, uint32_t n);
 struct Tuning {
     uint32_t low_max = 64;
     uint32_t hub_min = 6144;   // s22: rows of 4,096-6,143 in-edges do better as 64-lane rows
-    uint32_t hub_chunk = 2048; // s22: ~240 us per sweep against 264 us at 4096 (balance)
+    // AG_TUNE_AUTO: chosen per graph from its hub rows (build_plan)
+    uint32_t hub_chunk = AG_TUNE_AUTO;
     uint32_t hot_bytes = 160 * 1024 - 2048; // LDS table of the hottest contributions
 };
 Tuning &tuning();

@@ -312,7 +312,8 @@ def csr_file_info(path: str) -> dict:
 
 def set_tuning(low_max: int = 0, hub_min: int = 0, hub_chunk: int = 0,
                hot_bytes: int = 0) -> None:
-    """0 keeps a knob; hot_bytes=1 disables the LDS table of hot contributions."""
+    """0 keeps a knob; hot_bytes=1 disables the LDS table of hot contributions;
+    hub_chunk=0xFFFFFFFF (AG_TUNE_AUTO, the default) picks it per graph."""
     t = L.Tuning(low_max, hub_min, hub_chunk, hot_bytes)
     L.check(L.load().ag_set_tuning(C.byref(t)))
 

@@ -257,10 +257,13 @@ typedef struct ag_tuning
 {
     uint32_t low_max;     /* in-degree <= low_max: thread per vertex */
     uint32_t hub_min;     /* in-degree >= hub_min: workgroup chunks */
-    uint32_t hub_chunk;   /* edges per hub workgroup chunk */
+    uint32_t hub_chunk;   /* edges per hub workgroup chunk; AG_TUNE_AUTO (the default):
+                             4,096 when the hub rows make at least 8 such chunks per
+                             compute unit, else 2,048 */
     uint32_t hot_bytes;   /* LDS bytes per sweep workgroup caching the hottest
                              contributions (multiple of 16; 1 disables the table) */
 } ag_tuning;
+#define AG_TUNE_AUTO 0xFFFFFFFFu
 int ag_set_tuning(const ag_tuning *t);
 int ag_get_tuning(ag_tuning *t);
 
