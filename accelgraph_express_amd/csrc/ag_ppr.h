@@ -135,6 +135,10 @@ class PprBatch {
     uint32_t used_count_ = 0, hi_ = 0; // hi_: one past the highest used slot
     float damp_;
     bool extrap_;
+    // forward CSR of the graph (vertex ids): admissions run their first sweep
+    // as a push over the seeds' out-edges; null without one or with
+    // extrapolated steps (pull sweep)
+    const uint32_t *fwd_off_ = nullptr, *fwd_dst_ = nullptr;
     int cur_ = 0;          // contribution buffer read by the next sweep
     uint32_t graph_lpr_ = 0; // lanes per row the captured graph was built for
     uint32_t blk_ = 64;      // BF16 split block (columns per highs/lows block)

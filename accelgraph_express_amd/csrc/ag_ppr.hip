@@ -654,13 +654,54 @@ __global__ __launch_bounds__(kBlock) void k_ppr_rows(PprArgs a)
     }
 }
 
+constexpr int kRedBlock = 256;
+
+// Closes one sweep of a running column: residual R of the iterate it read,
+// dangling mass D of the iterate it stored (k_ppr_reduce and the admission
+// push share it).
+__device__ void close_sweep(PprCol &col, double R, double D, float damp, uint32_t extrap)
+{
+    const double prev = col.residual;
+    const double d_read = col.dangling; // dangling mass of the iterate this sweep read
+    // R is |T(x) - x| for the x this sweep read; after an extrapolated
+    // step the stored iterate is not T(x), so R certifies nothing and
+    // the column cannot stop on it
+    const bool stepped = col.w != 0.0f;
+    col.w = 0.0f;
+    col.iters += 1;
+    col.residual = R;
+    col.dangling = D;
+    col.t = (float)((1.0 - (double)damp) + (double)damp * D);
+    if (col.iters >= col.max_iter) {
+        col.done = 1;
+    } else if (col.coarse && !stepped && (!extrap || col.iters <= 4) &&
+               R + kMidRel * fmax(0.0, 1.000001 - d_read) + kMidAbs <= col.tol) {
+        // Certified on a coarse sweep.  A midpoint read is within 2^-8
+        // of each contribution plus 2^-134 (the midpoint read of a
+        // zero), so the coarse map T_c is within delta = d (2^-8 M +
+        // E 2^-134) of T in L1, M the mass of x on rows with out-edges
+        // (1 - dangling mass while x >= 0: no extrapolated step yet).
+        // With R + delta / d <= tol the returned T_c(x) satisfies
+        // ||T_c(x) - x*||_1 <= d / (1 - d) tol, the bound of a plain
+        // stop.  It fires when x sits on rows without out-edges
+        // (M ~ 0), e.g. a query whose seeds are isolated, which would
+        // otherwise force one FP32-read sweep.
+        col.done = 1;
+    } else if (col.coarse && (R <= col.tol || R <= kCoarseFloor ||
+                               (R > 0.97 * prev && R <= 8.0 * kCoarseFloor))) {
+        col.coarse = 0; // storage floor reached: read full FP32 from now on
+        col.fine_at = col.iters;
+    } else if (R <= col.tol && !stepped) {
+        col.done = 1;
+    }
+    if (!col.done && extrap) col.w = extrapolation_weight(R, prev, stepped, col.since, col.rho);
+}
+
 // One workgroup per column slot: residual and dangling sums over the partial
 // rows of both kernels in a fixed order (thread t takes rows t, t + 256, ...,
 // then a fixed tree), then convergence.  A coarse (BF16) column whose residual
 // stalls or meets its tolerance turns fine: from the next sweep on it reads
 // exact FP32 contributions, and it is done at the tolerance there.
-constexpr int kRedBlock = 256;
-
 __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t nparts)
 {
     __shared__ double w_r[kRedBlock / 64], w_d[kRedBlock / 64];
@@ -688,40 +729,7 @@ __global__ __launch_bounds__(kRedBlock) void k_ppr_reduce(PprArgs a, uint32_t np
         if (!col.done) {
             const double R = (w_r[0] + w_r[1]) + (w_r[2] + w_r[3]);
             const double D = (w_d[0] + w_d[1]) + (w_d[2] + w_d[3]);
-            const double prev = col.residual;
-            const double d_read = col.dangling; // dangling mass of the iterate this sweep read
-            // R is |T(x) - x| for the x this sweep read; after an extrapolated
-            // step the stored iterate is not T(x), so R certifies nothing and
-            // the column cannot stop on it
-            const bool stepped = col.w != 0.0f;
-            col.w = 0.0f;
-            col.iters += 1;
-            col.residual = R;
-            col.dangling = D;
-            col.t = (float)((1.0 - (double)a.damp) + (double)a.damp * D);
-            if (col.iters >= col.max_iter) {
-                col.done = 1;
-            } else if (col.coarse && !stepped && (!a.extrap || col.iters <= 4) &&
-                       R + kMidRel * fmax(0.0, 1.000001 - d_read) + kMidAbs <= col.tol) {
-                // Certified on a coarse sweep.  A midpoint read is within 2^-8
-                // of each contribution plus 2^-134 (the midpoint read of a
-                // zero), so the coarse map T_c is within delta = d (2^-8 M +
-                // E 2^-134) of T in L1, M the mass of x on rows with out-edges
-                // (1 - dangling mass while x >= 0: no extrapolated step yet).
-                // With R + delta / d <= tol the returned T_c(x) satisfies
-                // ||T_c(x) - x*||_1 <= d / (1 - d) tol, the bound of a plain
-                // stop.  It fires when x sits on rows without out-edges
-                // (M ~ 0), e.g. a query whose seeds are isolated, which would
-                // otherwise force one FP32-read sweep.
-                col.done = 1;
-            } else if (col.coarse && (R <= col.tol || R <= kCoarseFloor ||
-                                       (R > 0.97 * prev && R <= 8.0 * kCoarseFloor))) {
-                col.coarse = 0; // storage floor reached: read full FP32 from now on
-                col.fine_at = col.iters;
-            } else if (R <= col.tol && !stepped) {
-                col.done = 1;
-            }
-            if (!col.done && a.extrap) col.w = extrapolation_weight(R, prev, stepped, col.since, col.rho);
+            close_sweep(col, R, D, a.damp, a.extrap);
         }
     }
 }
@@ -739,6 +747,26 @@ __global__ void k_ppr_clear_cols(float *X, void *C0, void *C1, uint32_t N, uint3
     X[k] = 0.0f;
     st(C0, fmt, cap, blk, row, adm[q].slot, 0.0f);
     st(C1, fmt, cap, blk, row, adm[q].slot, 0.0f);
+}
+
+// Initial column state of an admission: x0 = s, teleport factor from the
+// dangling seed mass d0.
+__device__ PprCol fresh_column(const PprAdmit &d, double d0, float damp, uint32_t fmt)
+{
+    PprCol col;
+    col.t = (float)((1.0 - (double)damp) + (double)damp * d0);
+    col.done = 0;
+    col.iters = 0;
+    col.max_iter = d.max_iter;
+    col.tol = d.tol;
+    col.residual = INFINITY;
+    col.dangling = d0;
+    col.coarse = fmt != AG_PREC_FP32 ? 1u : 0u;
+    col.fine_at = 0;
+    col.w = 0.0f;
+    col.since = 0;
+    col.rho = 0.0;
+    return col;
 }
 
 // One workgroup per admitted slot: x0 = s on the seed rows, contributions
@@ -759,20 +787,104 @@ __global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uin
         double d0 = 0.0;
         for (uint32_t q = d.e0; q < d.e1; q++)
             if (outdeg[sv[q]] == 0) d0 += (double)sw[q];
-        PprCol col;
-        col.t = (float)((1.0 - (double)damp) + (double)damp * d0);
-        col.done = 0;
-        col.iters = 0;
-        col.max_iter = d.max_iter;
-        col.tol = d.tol;
-        col.residual = INFINITY;
-        col.dangling = d0;
-        col.coarse = fmt != AG_PREC_FP32 ? 1u : 0u;
-        col.fine_at = 0;
-        col.w = 0.0f;
-        col.since = 0;
-        col.rho = 0.0;
-        cs[d.slot] = col;
+        cs[d.slot] = fresh_column(d, d0, damp, fmt);
+    }
+}
+
+// First sweep of the admitted columns by push, one workgroup per admitted
+// slot (X and both contribution buffers of the slot cleared).  x1 = T(x0) for
+// x0 = s is nonzero only on the seeds and their out-neighbours, so walking the
+// seeds' out-edges (forward CSR) replaces a sweep over all E in-edges:
+//  1. acc[v] = sum of s_u / outdeg(u) over the edges u -> v, seeds in table
+//     order with a barrier between seeds.  The adds of one seed carry equal
+//     values, so every sum is formed in one order, fixed by the query: the
+//     column's iterate does not depend on its batch.
+//  2. Each touched row is claimed once through the sign bit of its X word:
+//     x1 = d acc + s_v t, contribution x1 / outdeg, and the residual and
+//     dangling sums in 2^-61 fixed point (integer adds, hence independent of
+//     the claim order).  The claimed word holds -x1 until step 3.
+//  3. Sign bits cleared; the column state closes sweep 1 as k_ppr_reduce does.
+// The contributions are exact FP32 (no midpoint reads), for BF16 columns too.
+constexpr int kPushBlock = 1024;
+constexpr double kFix = 0x1p61; // sums <= 2 (residual) and <= 1 (dangling) fit in 64 bits
+
+__device__ __forceinline__ void push_claim(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap,
+                                           uint32_t blk, uint32_t fmt, uint32_t slot, uint32_t v,
+                                           float s, float t, float damp, unsigned long long *s_r,
+                                           unsigned long long *s_d)
+{
+    uint32_t *word = reinterpret_cast<uint32_t *>(X) + ((uint64_t)v * cap + slot);
+    const uint32_t old = atomicOr(word, 0x80000000u);
+    if (old & 0x80000000u) return;
+    const float nw = damp * __uint_as_float(old) + s * t;
+    atomicExch(word, __float_as_uint(nw) | 0x80000000u);
+    const uint32_t od = outdeg[v];
+    if (od) st(Cin, fmt, cap, blk, v, slot, nw / (float)od);
+    atomicAdd(s_r, (unsigned long long)(fabs((double)nw - (double)s) * kFix));
+    if (od == 0) atomicAdd(s_d, (unsigned long long)((double)nw * kFix));
+}
+
+__global__ __launch_bounds__(kPushBlock) void k_ppr_admit_push(
+    float *X, void *Cin, const uint32_t *outdeg, const uint32_t *order, const uint32_t *rank,
+    const uint32_t *fwd_off, const uint32_t *fwd_dst, uint32_t cap, uint32_t blk, uint32_t fmt,
+    const PprAdmit *adm, const uint32_t *sv, const float *sw, PprCol *cs, float damp, uint32_t extrap)
+{
+    __shared__ unsigned long long s_r, s_d;
+    __shared__ float s_t;
+    __shared__ double s_d0;
+    const PprAdmit d = adm[blockIdx.x];
+    const uint32_t slot = d.slot;
+    if (threadIdx.x == 0) {
+        double d0 = 0.0;
+        for (uint32_t q = d.e0; q < d.e1; q++)
+            if (outdeg[sv[q]] == 0) d0 += (double)sw[q];
+        s_d0 = d0;
+        s_t = (float)((1.0 - (double)damp) + (double)damp * d0);
+        s_r = 0;
+        s_d = 0;
+    }
+    __syncthreads();
+    // 1. gathered sums, one seed at a time
+    for (uint32_t q = d.e0; q < d.e1; q++) {
+        const uint32_t u = sv[q];
+        const uint32_t od = outdeg[u];
+        if (od == 0) continue; // uniform: every thread reads the same seed
+        const float c = sw[q] / (float)od;
+        const uint32_t vid = order[u];
+        const uint32_t j1 = fwd_off[vid + 1];
+        for (uint32_t j = fwd_off[vid] + threadIdx.x; j < j1; j += kPushBlock)
+            atomicAdd(&X[(uint64_t)rank[fwd_dst[j]] * cap + slot], c);
+        __threadfence();
+        __syncthreads();
+    }
+    // 2. seeds (their s_v), then the other touched rows (s_v = 0)
+    const float t = s_t;
+    for (uint32_t q = d.e0 + threadIdx.x; q < d.e1; q += kPushBlock)
+        push_claim(X, Cin, outdeg, cap, blk, fmt, slot, sv[q], sw[q], t, damp, &s_r, &s_d);
+    __threadfence();
+    __syncthreads();
+    for (uint32_t q = d.e0; q < d.e1; q++) {
+        const uint32_t vid = order[sv[q]];
+        const uint32_t j1 = fwd_off[vid + 1];
+        for (uint32_t j = fwd_off[vid] + threadIdx.x; j < j1; j += kPushBlock)
+            push_claim(X, Cin, outdeg, cap, blk, fmt, slot, rank[fwd_dst[j]], 0.0f, t, damp, &s_r, &s_d);
+    }
+    __threadfence();
+    __syncthreads();
+    // 3. sign bits cleared
+    for (uint32_t q = d.e0 + threadIdx.x; q < d.e1; q += kPushBlock)
+        atomicAnd(reinterpret_cast<uint32_t *>(X) + ((uint64_t)sv[q] * cap + slot), 0x7fffffffu);
+    for (uint32_t q = d.e0; q < d.e1; q++) {
+        const uint32_t vid = order[sv[q]];
+        const uint32_t j1 = fwd_off[vid + 1];
+        for (uint32_t j = fwd_off[vid] + threadIdx.x; j < j1; j += kPushBlock)
+            atomicAnd(reinterpret_cast<uint32_t *>(X) + ((uint64_t)rank[fwd_dst[j]] * cap + slot),
+                      0x7fffffffu);
+    }
+    if (threadIdx.x == 0) {
+        PprCol col = fresh_column(d, s_d0, damp, fmt);
+        close_sweep(col, (double)s_r / kFix, (double)s_d / kFix, damp, extrap);
+        cs[slot] = col;
     }
 }
 
@@ -1020,6 +1132,13 @@ PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_
     ensure_ppr_chunks(ctx, p_);
     N_ = p_->N;
     if (N_ == 0) fail(AG_ERR_INVALID, "graph has no vertices");
+    // Extrapolating columns keep the pull first sweep: with the exact first
+    // iterate of the push their residual ratios settle differently, and on
+    // R-MAT s22 the slowest of 64 columns took up to 3 more sweeps.
+    if (g->has_full_fwd && !extrapolate) {
+        fwd_off_ = g->fwd_off.as<uint32_t>();
+        fwd_dst_ = g->fwd_dst.as<uint32_t>();
+    }
     set_cpw(cpw);
     hipStream_t s = ctx->stream;
     {
@@ -1307,10 +1426,17 @@ void PprBatch::flush()
         hipLaunchKernelGGL(k_ppr_clear_cols, dim3(ceil_div((uint64_t)N_ * adm.size(), 256)), dim3(256),
                            0, s, X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, blk_,
                            desc_.as<PprAdmit>(), (uint32_t)adm.size(), fmt_);
-        hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)adm.size()), dim3(256), 0, s,
-                           X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, blk_, fmt_,
-                           desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(), seed_w_.as<float>(),
-                           cols_.as<PprCol>(), damp_);
+        if (fwd_off_)
+            hipLaunchKernelGGL(k_ppr_admit_push, dim3((uint32_t)adm.size()), dim3(kPushBlock), 0, s,
+                               X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(),
+                               p_->order.as<uint32_t>(), p_->rank.as<uint32_t>(), fwd_off_, fwd_dst_,
+                               cap_, blk_, fmt_, desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(),
+                               seed_w_.as<float>(), cols_.as<PprCol>(), damp_, extrap_ ? 1u : 0u);
+        else
+            hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)adm.size()), dim3(256), 0, s,
+                               X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, blk_, fmt_,
+                               desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(), seed_w_.as<float>(),
+                               cols_.as<PprCol>(), damp_);
         AG_HIP(hipGetLastError());
         AG_HIP(hipStreamSynchronize(s)); // adm is a temporary
         pending_.clear();
