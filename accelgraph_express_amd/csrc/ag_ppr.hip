@@ -806,6 +806,11 @@ __global__ void k_ppr_seed_cols(float *X, void *Cin, const uint32_t *outdeg, uin
 //  3. Sign bits cleared; the column state closes sweep 1 as k_ppr_reduce does.
 // The contributions are exact FP32 (no midpoint reads), for BF16 columns too.
 constexpr int kPushBlock = 1024;
+// A step per seed (~10 us with its barrier): queries of more seeds take the
+// pull first sweep, whose cost does not grow with the seed count.  The choice
+// depends on the query alone, so a column's result still does not depend on
+// its batch.
+constexpr uint32_t kPushMaxSeeds = 64;
 constexpr double kFix = 0x1p61; // sums <= 2 (residual) and <= 1 (dangling) fit in 64 bits
 
 __device__ __forceinline__ void push_claim(float *X, void *Cin, const uint32_t *outdeg, uint32_t cap,
@@ -1411,32 +1416,37 @@ void PprBatch::flush()
     }
     upload_seed_table();
     if (!pending_.empty()) {
-        // entry ranges of each admitted slot in the table just uploaded
+        // entry ranges of each admitted slot in the table just uploaded:
+        // columns whose first sweep is a push first, then the others
         adm.clear();
+        std::vector<PprAdmit> pulled;
         uint32_t off = 0;
         for (uint32_t k = 0; k < cap_; k++) {
             if (!used_[k]) continue;
             const uint32_t n = (uint32_t)q_[k].labels.size();
             if (std::find(pending_.begin(), pending_.end(), k) != pending_.end())
-                adm.push_back({k, off, off + n, q_[k].max_iter, q_[k].tol});
+                (fwd_off_ && n <= kPushMaxSeeds ? adm : pulled)
+                    .push_back({k, off, off + n, q_[k].max_iter, q_[k].tol});
             off += n;
         }
+        const uint32_t npush = (uint32_t)adm.size();
+        adm.insert(adm.end(), pulled.begin(), pulled.end());
         AG_HIP(hipMemcpyAsync(desc_.p, adm.data(), adm.size() * sizeof(PprAdmit),
                               hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_ppr_clear_cols, dim3(ceil_div((uint64_t)N_ * adm.size(), 256)), dim3(256),
                            0, s, X_.as<float>(), C_[0].p, C_[1].p, N_, cap_, blk_,
                            desc_.as<PprAdmit>(), (uint32_t)adm.size(), fmt_);
-        if (fwd_off_)
-            hipLaunchKernelGGL(k_ppr_admit_push, dim3((uint32_t)adm.size()), dim3(kPushBlock), 0, s,
+        if (npush)
+            hipLaunchKernelGGL(k_ppr_admit_push, dim3(npush), dim3(kPushBlock), 0, s,
                                X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(),
                                p_->order.as<uint32_t>(), p_->rank.as<uint32_t>(), fwd_off_, fwd_dst_,
                                cap_, blk_, fmt_, desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(),
                                seed_w_.as<float>(), cols_.as<PprCol>(), damp_, extrap_ ? 1u : 0u);
-        else
-            hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)adm.size()), dim3(256), 0, s,
+        if (!pulled.empty())
+            hipLaunchKernelGGL(k_ppr_seed_cols, dim3((uint32_t)pulled.size()), dim3(256), 0, s,
                                X_.as<float>(), C_[cur_].p, p_->outdeg.as<uint32_t>(), cap_, blk_, fmt_,
-                               desc_.as<PprAdmit>(), seed_v_.as<uint32_t>(), seed_w_.as<float>(),
-                               cols_.as<PprCol>(), damp_);
+                               desc_.as<PprAdmit>() + npush, seed_v_.as<uint32_t>(),
+                               seed_w_.as<float>(), cols_.as<PprCol>(), damp_);
         AG_HIP(hipGetLastError());
         AG_HIP(hipStreamSynchronize(s)); // adm is a temporary
         pending_.clear();
