@@ -126,6 +126,8 @@ class PprBatch {
   private:
     void flush();
     void choose_layout();
+    // split block the layout rule asks for with `hi` slots in use
+    uint32_t layout_for(uint32_t hi) const;
     void upload_seed_table();
     void launch_sweep(bool from0);
     uint32_t lanes_per_row() const;

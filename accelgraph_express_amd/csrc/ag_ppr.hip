@@ -900,6 +900,11 @@ __global__ void k_ppr_stop_cols(PprCol *cs, const PprAdmit *adm, uint32_t n)
     if (q < n) cs[adm[q].slot].done = 1;
 }
 
+
+// Re-lays every row of a split-BF16 buffer from `from`-column blocks to
+// `to`-column blocks, in place: a workgroup stages kRelayRows rows in LDS,
+// then writes each 32-bit word of the new layout from its two halves.
+constexpr uint32_t kRelayBlock = 256, kRewriteBytes = 16384;
 // Packs running columns into lower slots of the same batch: iterate and both
 // contribution buffers (both halves) copied, so rows a sweep never rewrites
 // (no in-edges, not a seed) keep the moved column's values, not those of the
@@ -928,38 +933,62 @@ __global__ void k_ppr_move_cols(float *X, void *C0, void *C1, uint32_t cap, uint
     }
 }
 
-// Re-lays every row of a split-BF16 buffer from `from`-column blocks to
-// `to`-column blocks, in place: a workgroup stages kRelayRows rows in LDS,
-// then writes each 32-bit word of the new layout from its two halves.
-constexpr uint32_t kRelayBlock = 256, kRelayBytes = 4096;
-__global__ __launch_bounds__(kRelayBlock) void k_ppr_relayout(void *C, uint32_t N, uint32_t cap,
-                                                             uint32_t from, uint32_t to)
+// Destination slot -> source slot of a row rewrite (identity for a plain
+// layout switch; a compaction maps the packed slots to the columns they take).
+struct SlotMap {
+    uint8_t src[kMaxCap];
+};
+
+inline SlotMap identity_map()
 {
-    __shared__ uint16_t h[kRelayBytes / 2];
-    const uint32_t rb = cap * 4;           // row bytes
-    const uint32_t rows = kRelayBytes / rb; // rows per workgroup (cap <= 64: >= 16)
+    SlotMap m;
+    for (uint32_t c = 0; c < kMaxCap; c++) m.src[c] = (uint8_t)c;
+    return m;
+}
+
+// Rows rewritten in place through LDS, kRewriteBytes of rows per workgroup
+// with 16-byte loads and stores.  split = 0: 32-bit words (FP32 iterate or
+// contributions) move with the slot map.  split = 1: split-BF16 rows are
+// re-laid from `from`-column blocks to `to`-column blocks while the halves
+// move with the slot map.  A table of source positions per output position,
+// built once per workgroup, keeps divisions out of the copy loop.
+__global__ __launch_bounds__(kRelayBlock) void k_ppr_rewrite_rows(void *A, uint32_t N, uint32_t cap,
+                                                                 uint32_t split, uint32_t from,
+                                                                 uint32_t to, SlotMap m)
+{
+    __shared__ uint4 h4[kRewriteBytes / 16];
+    __shared__ uint16_t tab[2 * kMaxCap];
+    const uint32_t rows = kRewriteBytes / (cap * 4);
     const uint64_t r0 = (uint64_t)blockIdx.x * rows;
     const uint32_t nr = (uint32_t)min<uint64_t>(rows, N - r0);
-    uint32_t *g = reinterpret_cast<uint32_t *>(static_cast<char *>(C) + r0 * rb);
-    uint32_t *lw = reinterpret_cast<uint32_t *>(h);
-    const uint32_t words = nr * cap; // 32-bit words staged
-    for (uint32_t i = threadIdx.x; i < words; i += kRelayBlock) lw[i] = g[i];
+    uint4 *g = reinterpret_cast<uint4 *>(static_cast<char *>(A) + r0 * cap * 4);
+    const uint32_t nq = nr * cap / 4; // cap is a multiple of 8
+    for (uint32_t i = threadIdx.x; i < nq; i += kRelayBlock) h4[i] = g[i];
+    const uint32_t j = threadIdx.x;
+    if (split && j < 2 * cap) {
+        // 16-bit position j of the new layout -> column and half -> old position
+        const uint32_t b = (j / (2 * to)) * to; // first column of j's block
+        const uint32_t wb = min(to, cap - b), r = j - 2 * b;
+        const bool hi = r < wb;
+        const uint32_t col = m.src[hi ? b + r : b + r - wb];
+        tab[j] = (uint16_t)((hi ? split_hi(cap, from, col) : split_lo(cap, from, col)) / 2);
+    } else if (!split && j < cap) {
+        tab[j] = m.src[j];
+    }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < words; i += kRelayBlock) {
-        const uint32_t row = i / cap, w = i - row * cap;
-        uint32_t out = 0;
+    const uint32_t *h32 = reinterpret_cast<const uint32_t *>(h4);
+    const uint16_t *h16 = reinterpret_cast<const uint16_t *>(h4);
+    for (uint32_t i = threadIdx.x; i < nq; i += kRelayBlock) {
+        const uint32_t w0 = 4 * i, row = w0 / cap, c0 = w0 - row * cap; // 4 words of one row
+        uint32_t o[4];
 #pragma unroll
-        for (uint32_t k = 0; k < 2; k++) {
-            // 16-bit index j of the new layout -> column and half
-            const uint32_t j = 2 * w + k;
-            const uint32_t b = (j / (2 * to)) * to; // first column of j's block
-            const uint32_t wb = min(to, cap - b), r = j - 2 * b;
-            const bool hi = r < wb;
-            const uint32_t col = hi ? b + r : b + r - wb;
-            const uint32_t src = (hi ? split_hi(cap, from, col) : split_lo(cap, from, col)) / 2;
-            out |= (uint32_t)h[row * (rb / 2) + src] << (16 * k);
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t c = c0 + k;
+            o[k] = split ? (uint32_t)h16[row * 2 * cap + tab[2 * c]] |
+                               ((uint32_t)h16[row * 2 * cap + tab[2 * c + 1]] << 16)
+                         : h32[row * cap + tab[c]];
         }
-        g[i] = out;
+        g[i] = make_uint4(o[0], o[1], o[2], o[3]);
     }
 }
 
@@ -1329,8 +1358,7 @@ bool PprBatch::cancel(uint64_t tag)
 void PprBatch::compact()
 {
     if (!pending_.empty() || used_count_ == 0 || used_count_ == hi_) return;
-    // Moving columns costs about one wide sweep, plus one per buffer when the
-    // split layout switches with it (s22: 1.0 ms + 2 x 0.9 ms against 2.4 ms
+    // Moving columns costs about half a wide sweep (s22: ~1 ms against 2.4 ms
     // fine sweeps that narrow to 0.9 ms): not worth it for the last two sweeps.
     if (tail_ < kCompactMinTail) return;
     const uint32_t vw = vec_width(fmt_);
@@ -1348,11 +1376,28 @@ void PprBatch::compact()
         if (!used_[k]) dst.push_back(k);
     std::vector<PprAdmit> mv;
     for (size_t i = 0; i < src.size(); i++) mv.push_back({src[i], dst[i], 0, 0, 0.0});
+    const uint32_t to = layout_for(used_count_);
     hipStream_t s = ctx_->stream;
     AG_HIP(hipMemcpyAsync(desc_.p, mv.data(), mv.size() * sizeof(PprAdmit), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_ppr_move_cols, dim3(ceil_div((uint64_t)N_ * mv.size(), 256)), dim3(256), 0, s,
-                       X_.as<float>(), C_[0].p, C_[1].p, cap_, blk_, fmt_, desc_.as<PprAdmit>(),
-                       (uint32_t)mv.size(), N_);
+    if (to != blk_) {
+        // The packed width switches the split layout: one in-place rewrite
+        // per buffer moves the columns and re-lays the rows in the same pass
+        // (rows without in-edges are never rewritten by a sweep, so both
+        // contribution buffers are rewritten whole).
+        SlotMap m = identity_map();
+        for (size_t i = 0; i < src.size(); i++) m.src[dst[i]] = (uint8_t)src[i];
+        const uint32_t grid = ceil_div(N_, kRewriteBytes / (cap_ * 4));
+        hipLaunchKernelGGL(k_ppr_rewrite_rows, dim3(grid), dim3(kRelayBlock), 0, s, X_.p, N_, cap_, 0u,
+                           blk_, blk_, m);
+        for (int b = 0; b < 2; b++)
+            hipLaunchKernelGGL(k_ppr_rewrite_rows, dim3(grid), dim3(kRelayBlock), 0, s, C_[b].p, N_, cap_,
+                               1u, blk_, to, m);
+        blk_ = to;
+    } else {
+        hipLaunchKernelGGL(k_ppr_move_cols, dim3(ceil_div((uint64_t)N_ * mv.size(), 256)), dim3(256), 0, s,
+                           X_.as<float>(), C_[0].p, C_[1].p, cap_, blk_, fmt_, desc_.as<PprAdmit>(),
+                           (uint32_t)mv.size(), N_);
+    }
     hipLaunchKernelGGL(k_ppr_move_state, dim3(1), dim3(kMaxCap), 0, s, cols_.as<PprCol>(),
                        desc_.as<PprAdmit>(), (uint32_t)mv.size());
     AG_HIP(hipGetLastError());
@@ -1377,19 +1422,24 @@ void PprBatch::compact()
 // once at most 32 are (fine sweeps of the packed columns read one line); in
 // between the current layout stays.  Running columns are re-laid in place in
 // both contribution buffers (rows no sweep rewrites keep their values).
+uint32_t PprBatch::layout_for(uint32_t hi) const
+{
+    if (fmt_ != AG_PREC_BF16 || cap_ <= 32) return blk_; // one block either way
+    if (blk_ == 64 && hi <= 32) return 32;
+    if (blk_ == 32 && hi > 48) return 64;
+    return blk_;
+}
+
 void PprBatch::choose_layout()
 {
-    if (fmt_ != AG_PREC_BF16 || cap_ <= 32) return; // one block either way
-    uint32_t want = blk_;
-    if (blk_ == 64 && hi_ <= 32) want = 32;
-    if (blk_ == 32 && hi_ > 48) want = 64;
+    const uint32_t want = layout_for(hi_);
     if (want == blk_) return;
     if (used_count_ > pending_.size()) { // running columns hold data
         hipStream_t s = ctx_->stream;
-        const uint32_t rows = kRelayBytes / (cap_ * 4);
+        const uint32_t rows = kRewriteBytes / (cap_ * 4);
         for (int b = 0; b < 2; b++)
-            hipLaunchKernelGGL(k_ppr_relayout, dim3(ceil_div(N_, rows)), dim3(kRelayBlock), 0, s,
-                               C_[b].p, N_, cap_, blk_, want);
+            hipLaunchKernelGGL(k_ppr_rewrite_rows, dim3(ceil_div(N_, rows)), dim3(kRelayBlock), 0, s,
+                               C_[b].p, N_, cap_, 1u, blk_, want, identity_map());
         AG_HIP(hipGetLastError());
     }
     blk_ = want;
