@@ -29,6 +29,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -956,16 +957,34 @@ Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms)
     return g->plan;
 }
 
+// Persistent sweep workgroups: as many per CU as the LDS hot table allows (<= 2).
+static uint32_t sweep_grid(const ag_ctx *ctx, const Plan *p)
+{
+    const uint32_t per_cu =
+        std::min<uint32_t>(2, std::max<uint32_t>(1, kLdsBytes / (p->hot_bytes + kStaticLds)));
+    return std::max<uint32_t>(1, std::min<uint32_t>(p->nvblocks, (uint32_t)ctx->num_cus * per_cu));
+}
+
+// Per-call device buffers of the sweep loop, as raw pointers: either the
+// plan's cached Work buffers or a carve-up of the allocation-free scratch.
+struct WorkPtrs {
+    uint32_t grid = 0;
+    float *pr = nullptr;
+    void *c0 = nullptr, *c1 = nullptr;
+    double *blk_err = nullptr, *blk_dang = nullptr;
+    uint32_t *blk_act = nullptr;
+    SweepState *st = nullptr;
+    double *err_iter = nullptr;
+    uint32_t *act_iter = nullptr, *fmt = nullptr;
+    double *sum_part = nullptr; // ceil(N / kBlock) + 1 doubles
+};
+
 static void ensure_work(ag_ctx *ctx, Plan *p, uint32_t max_iter)
 {
     Work &w = p->work;
     const uint32_t N = p->N;
     if (!w.pr.p) {
-        // persistent workgroups: as many per CU as the LDS hot table allows (<= 2)
-        const uint32_t per_cu =
-            std::min<uint32_t>(2, std::max<uint32_t>(1, kLdsBytes / (p->hot_bytes + kStaticLds)));
-        const uint32_t grid = std::max<uint32_t>(
-            1, std::min<uint32_t>(p->nvblocks, (uint32_t)ctx->num_cus * per_cu));
+        const uint32_t grid = sweep_grid(ctx, p);
         w.grid = grid;
         w.pr.alloc((size_t)N * 4);
         w.c0.alloc((size_t)N * 4);
@@ -985,9 +1004,27 @@ static void ensure_work(ag_ctx *ctx, Plan *p, uint32_t max_iter)
     }
 }
 
-static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_iter)
+static WorkPtrs work_ptrs(const Work &w)
 {
-    const Work &w = p->work;
+    WorkPtrs r;
+    r.grid = w.grid;
+    r.pr = w.pr.as<float>();
+    r.c0 = w.c0.p;
+    r.c1 = w.c1.p;
+    r.blk_err = w.blk_err.as<double>();
+    r.blk_dang = w.blk_dang.as<double>();
+    r.blk_act = w.blk_act.as<uint32_t>();
+    r.st = w.state.as<SweepState>();
+    r.err_iter = w.err_iter.as<double>();
+    r.act_iter = w.act_iter.as<uint32_t>();
+    r.fmt = w.fmt.as<uint32_t>();
+    r.sum_part = w.sum_part.as<double>();
+    return r;
+}
+
+static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options &o,
+                           uint32_t max_iter)
+{
     SweepArgs a;
     std::memset(&a, 0, sizeof(a));
     a.N = p->N;
@@ -1034,28 +1071,130 @@ static SweepArgs make_args(const Plan *p, const ag_pr_options &o, uint32_t max_i
     a.hub_partial = p->hub_partial.as<float>();
     a.nparts = w.grid;
     a.hot_bytes = p->hot_bytes;
-    a.pr = w.pr.as<float>();
-    a.c0 = w.c0.p;
-    a.c1 = w.c1.p;
-    a.blk_err = w.blk_err.as<double>();
-    a.blk_dang = w.blk_dang.as<double>();
-    a.blk_act = w.blk_act.as<uint32_t>();
-    a.st = w.state.as<SweepState>();
-    a.err_iter = w.err_iter.as<double>();
-    a.act_iter = w.act_iter.as<uint32_t>();
-    a.fmt = w.fmt.as<uint32_t>();
+    a.pr = w.pr;
+    a.c0 = w.c0;
+    a.c1 = w.c1;
+    a.blk_err = w.blk_err;
+    a.blk_dang = w.blk_dang;
+    a.blk_act = w.blk_act;
+    a.st = w.st;
+    a.err_iter = w.err_iter;
+    a.act_iter = w.act_iter;
+    a.fmt = w.fmt;
     return a;
 }
 
+// Enqueues sweeps [it0, it1): the sweep kernel(s) of each sweep's storage
+// format, then the finalize kernel.  `sev` (two events per sweep, or null)
+// brackets the sweep kernels alone.
+static void enqueue_sweeps(hipStream_t s, const SweepArgs &a, const ag_pr_options &o,
+                           uint32_t grid, uint32_t hot_bytes, uint32_t it0, uint32_t it1,
+                           hipEvent_t *sev)
+{
+    const bool dd = o.mode == AG_PR_DATA_DRIVEN;
+    for (uint32_t it = it0; it < it1; it++) {
+        if (sev) AG_HIP(hipEventRecord(sev[2 * it], s));
+        if (dd)
+            hipLaunchKernelGGL((k_pr_sweep<float, true>), dim3(grid), dim3(kSweepBlock), hot_bytes,
+                               s, a, it);
+        else if (o.precision == AG_PREC_BF16)
+            hipLaunchKernelGGL((k_pr_sweep<bf16_t, false>), dim3(grid), dim3(kSweepBlock),
+                               hot_bytes, s, a, it);
+        else if (o.precision == AG_PREC_FP16)
+            hipLaunchKernelGGL((k_pr_sweep<__half, false>), dim3(grid), dim3(kSweepBlock),
+                               hot_bytes, s, a, it);
+        if (!dd && (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32)))
+            hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(grid), dim3(kSweepBlock),
+                               hot_bytes, s, a, it);
+        if (sev) AG_HIP(hipEventRecord(sev[2 * it + 1], s));
+        hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
+    }
+    AG_HIP(hipGetLastError());
+}
+
+// Sweep loops are captured into hipGraphs and replayed unless AG_NO_GRAPH=1
+// is set in the environment (read once).
+static bool loop_graphs_enabled()
+{
+    static const bool on = [] {
+        const char *e = std::getenv("AG_NO_GRAPH");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
+// Runs `enqueue` on stream s through a captured graph cached on the plan
+// under `key` (captured on first use, replayed after).  The captured work
+// holds kernel launches only: no memset nodes.
+template <typename F>
+static void run_captured(Plan *p, hipStream_t s, const std::vector<unsigned char> &key, F &&enqueue)
+{
+    for (size_t i = 0; i < p->loop_graphs.size(); i++) {
+        if (p->loop_graphs[i].key == key) {
+            AG_HIP(hipGraphLaunch(p->loop_graphs[i].exec, s));
+            return;
+        }
+    }
+    LoopGraph lg;
+    lg.key = key;
+    AG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue();
+    } catch (...) {
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(s, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        throw;
+    }
+    AG_HIP(hipStreamEndCapture(s, &lg.graph));
+    const hipError_t e = hipGraphInstantiate(&lg.exec, lg.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(lg.graph);
+        fail(AG_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    if (p->loop_graphs.size() >= Plan::kLoopGraphs) {
+        LoopGraph &old = p->loop_graphs.front();
+        (void)hipGraphExecDestroy(old.exec);
+        (void)hipGraphDestroy(old.graph);
+        p->loop_graphs.erase(p->loop_graphs.begin());
+    }
+    p->loop_graphs.push_back(std::move(lg));
+    AG_HIP(hipGraphLaunch(p->loop_graphs.back().exec, s));
+}
+
+template <typename T> static void key_add(std::vector<unsigned char> &k, const T &v)
+{
+    const unsigned char *b = reinterpret_cast<const unsigned char *>(&v);
+    k.insert(k.end(), b, b + sizeof(T));
+}
+
 // Runs the sweep loop; ranks land in d_out (device, caller order).
+//
+// Three ways to run it:
+//   * eager, with HIP events (AG_PR_FLAG_TIME_SWEEPS, or AG_NO_GRAPH=1, or a
+//     personalized query, whose seed table changes per call);
+//   * replayed from hipGraphs cached on the plan: one graph for init + all
+//     sweeps + finish when the sweep count is fixed (epsilon <= 0), else one
+//     per group of 16 sweeps with the convergence check on the host between;
+//   * on caller-provided per-call buffers (`fixed_work`, from the
+//     allocation-free entry point): the graph's plan must exist already and no
+//     event, graph or buffer is created.  Timing then comes from the host clock.
 static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
                               const SeedTable *seeds, float *d_out, uint32_t *active_iter,
-                              double *err_iter, ag_pr_result *res)
+                              double *err_iter, ag_pr_result *res,
+                              const WorkPtrs *fixed_work = nullptr)
 {
+    const bool no_alloc = fixed_work != nullptr;
     hipStream_t s = ctx->stream;
     ag_pr_result r;
     std::memset(&r, 0, sizeof(r));
-    Plan *p = ensure_plan(ctx, g, &r.plan_ms);
+    Plan *p = nullptr;
+    if (no_alloc) {
+        p = g->plan;
+        if (!p) fail(AG_ERR_INVALID, "no gather layout: call ag_pagerank_prepare first");
+    } else {
+        p = ensure_plan(ctx, g, &r.plan_ms);
+    }
     std::lock_guard<std::mutex> lk(p->mu);
     const uint32_t N = p->N;
     const uint32_t max_iter = o.iterations;
@@ -1075,8 +1214,14 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         if (res) *res = r;
         return;
     }
-    ensure_work(ctx, p, max_iter);
-    SweepArgs a = make_args(p, o, max_iter);
+    WorkPtrs w;
+    if (no_alloc) {
+        w = *fixed_work;
+    } else {
+        ensure_work(ctx, p, max_iter);
+        w = work_ptrs(p->work);
+    }
+    SweepArgs a = make_args(p, w, o, max_iter);
     if (seeds) {
         a.seed_bits = seeds->bits;
         a.seed_v = seeds->v;
@@ -1087,59 +1232,87 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     const float x0 = ref_scale ? a.base : (float)(1.0 / (double)N);
     const double dang0 = (double)p->n_dangling * (1.0 / (double)N);
     const float tele0 = (float)((1.0 - (double)o.damp) / N + (double)o.damp * dang0 / N);
-    hipEvent_t ev[4];
-    for (auto &e : ev) AG_HIP(hipEventCreate(&e));
-    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } eg{ev};
+    const bool timed = !no_alloc && (o.flags & AG_PR_FLAG_TIME_SWEEPS) != 0;
+    const bool use_graph = !no_alloc && !timed && !seeds && loop_graphs_enabled();
+    const bool can_stop = o.epsilon > 0.0;
+    const uint32_t group = can_stop ? 16u : max_iter;
+    const uint32_t grid = w.grid;
+    const uint32_t nbf = ceil_div(N, kBlock);
+    double *parts = w.sum_part;
 
-    AG_HIP(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_pr_init, dim3(ceil_div((uint64_t)std::max(N, max_iter + 2), 256)),
-                       dim3(256), 0, s, a, x0, tele0, o.precision);
-    AG_HIP(hipGetLastError());
-    AG_HIP(hipEventRecord(ev[1], s));
-    const uint32_t grid = p->work.grid;
-    // optional per-sweep events around the sweep kernels (finalize excluded)
-    const bool timed = (o.flags & AG_PR_FLAG_TIME_SWEEPS) != 0;
+    auto enqueue_init = [&] {
+        hipLaunchKernelGGL(k_pr_init, dim3(ceil_div((uint64_t)std::max(N, max_iter + 2), 256)),
+                           dim3(256), 0, s, a, x0, tele0, o.precision);
+        AG_HIP(hipGetLastError());
+    };
+    auto enqueue_finish = [&] {
+        hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr, p->order.as<uint32_t>(),
+                           N, ref_scale ? 1u : 0u, d_out, parts);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
+        AG_HIP(hipGetLastError());
+    };
+    std::vector<unsigned char> key0;
+    if (use_graph) {
+        key_add(key0, a);
+        key_add(key0, x0);
+        key_add(key0, tele0);
+        key_add(key0, d_out);
+        key_add(key0, grid);
+        key_add(key0, p->hot_bytes);
+        key_add(key0, o.precision);
+        key_add(key0, o.flags);
+    }
+    auto seg_key = [&](uint32_t seg) {
+        std::vector<unsigned char> k = key0;
+        key_add(k, seg);
+        return k;
+    };
+
+    // Host-side events and timers (none in the allocation-free mode).
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (!no_alloc)
+        for (auto &e : ev) AG_HIP(hipEventCreate(&e));
+    struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 4; i++) if (e[i]) (void)hipEventDestroy(e[i]); } } eg{ev};
     std::vector<hipEvent_t> sev(timed ? 2 * (size_t)max_iter : 0);
     for (auto &e : sev) AG_HIP(hipEventCreate(&e));
     struct SevGuard { std::vector<hipEvent_t> &v; ~SevGuard() { for (auto e : v) (void)hipEventDestroy(e); } } sg{sev};
-    const bool can_stop = o.epsilon > 0.0;
-    const uint32_t group = can_stop ? 16u : max_iter;
-    for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
-        const uint32_t it1 = std::min(max_iter, it0 + group);
-        for (uint32_t it = it0; it < it1; it++) {
-            if (timed) AG_HIP(hipEventRecord(sev[2 * it], s));
-            if (dd)
-                hipLaunchKernelGGL((k_pr_sweep<float, true>), dim3(grid), dim3(kSweepBlock),
-                                   p->hot_bytes, s, a, it);
-            else if (o.precision == AG_PREC_BF16)
-                hipLaunchKernelGGL((k_pr_sweep<bf16_t, false>), dim3(grid), dim3(kSweepBlock),
-                                   p->hot_bytes, s, a, it);
-            else if (o.precision == AG_PREC_FP16)
-                hipLaunchKernelGGL((k_pr_sweep<__half, false>), dim3(grid), dim3(kSweepBlock),
-                                   p->hot_bytes, s, a, it);
-            if (!dd && (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32)))
-                hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(grid), dim3(kSweepBlock),
-                                   p->hot_bytes, s, a, it);
-            if (timed) AG_HIP(hipEventRecord(sev[2 * it + 1], s));
-            hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
+    const auto h0 = std::chrono::steady_clock::now();
+
+    if (ev[0]) AG_HIP(hipEventRecord(ev[0], s));
+    if (use_graph && !can_stop) {
+        // one replayable graph: init, every sweep, finish
+        run_captured(p, s, seg_key(0), [&] {
+            enqueue_init();
+            enqueue_sweeps(s, a, o, grid, p->hot_bytes, 0, max_iter, nullptr);
+            enqueue_finish();
+        });
+        if (ev[1]) AG_HIP(hipEventRecord(ev[1], s));
+        if (ev[2]) AG_HIP(hipEventRecord(ev[2], s));
+    } else {
+        if (!use_graph) enqueue_init();
+        if (ev[1]) AG_HIP(hipEventRecord(ev[1], s));
+        for (uint32_t it0 = 0; it0 < max_iter; it0 += group) {
+            const uint32_t it1 = std::min(max_iter, it0 + group);
+            if (use_graph) {
+                run_captured(p, s, seg_key(1 + it0 / group), [&] {
+                    if (it0 == 0) enqueue_init();
+                    enqueue_sweeps(s, a, o, grid, p->hot_bytes, it0, it1, nullptr);
+                });
+            } else {
+                enqueue_sweeps(s, a, o, grid, p->hot_bytes, it0, it1, timed ? sev.data() : nullptr);
+            }
+            if (can_stop && it1 < max_iter) {
+                uint32_t done = 0;
+                AG_HIP(hipMemcpyAsync(&done, &a.st->done, 4, hipMemcpyDeviceToHost, s));
+                AG_HIP(hipStreamSynchronize(s));
+                if (done) break;
+            }
         }
-        AG_HIP(hipGetLastError());
-        if (can_stop && it1 < max_iter) {
-            uint32_t done = 0;
-            AG_HIP(hipMemcpyAsync(&done, &a.st->done, 4, hipMemcpyDeviceToHost, s));
-            AG_HIP(hipStreamSynchronize(s));
-            if (done) break;
-        }
+        if (ev[2]) AG_HIP(hipEventRecord(ev[2], s));
+        if (use_graph) run_captured(p, s, seg_key(0xffffffffu), enqueue_finish);
+        else enqueue_finish();
     }
-    AG_HIP(hipEventRecord(ev[2], s));
-    const uint32_t nbf = ceil_div(N, kBlock);
-    double *parts = p->work.sum_part.as<double>();
-    hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, p->work.pr.as<float>(),
-                       p->order.as<uint32_t>(), N, ref_scale ? 1u : 0u, d_out,
-                       parts);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
-    AG_HIP(hipGetLastError());
-    AG_HIP(hipEventRecord(ev[3], s));
+    if (ev[3]) AG_HIP(hipEventRecord(ev[3], s));
     SweepState st;
     AG_HIP(hipMemcpyAsync(&st, a.st, sizeof(st), hipMemcpyDeviceToHost, s));
     AG_HIP(hipMemcpyAsync(&r.pr_sum, parts + nbf, 8, hipMemcpyDeviceToHost, s));
@@ -1151,13 +1324,24 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         AG_HIP(hipMemcpyAsync(err_iter, a.err_iter, (size_t)st.sweeps * 8,
                               hipMemcpyDeviceToHost, s));
     AG_HIP(hipStreamSynchronize(s));
-    float ms = 0.f;
-    AG_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
-    r.loop_ms = ms;
-    AG_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
-    r.total_ms = ms;
+    if (no_alloc) {
+        const double ms = std::chrono::duration<double, std::milli>(
+                              std::chrono::steady_clock::now() - h0).count();
+        r.loop_ms = ms;
+        r.total_ms = ms;
+    } else {
+        float ms = 0.f;
+        // a single replayed graph holds init and finish too: loop_ms then
+        // spans the whole replay
+        AG_HIP(hipEventElapsedTime(&ms, (use_graph && !can_stop) ? ev[0] : ev[1],
+                                   (use_graph && !can_stop) ? ev[3] : ev[2]));
+        r.loop_ms = ms;
+        AG_HIP(hipEventElapsedTime(&ms, ev[0], ev[3]));
+        r.total_ms = ms;
+    }
     r.sweep_kernel_ms = 0.0;
     for (uint32_t it = 0; timed && it < st.sweeps && it < max_iter; it++) {
+        float ms = 0.f;
         AG_HIP(hipEventElapsedTime(&ms, sev[2 * it], sev[2 * it + 1]));
         r.sweep_kernel_ms += ms;
     }
@@ -1205,11 +1389,12 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
         return;
     }
     ensure_work(ctx, p, iterations);
+    const WorkPtrs w = work_ptrs(p->work);
     ag_pr_options o;
     ag_pr_options_default(&o);
     o.mode = kModeSpmv;
     o.epsilon = 0.0;
-    SweepArgs a = make_args(p, o, iterations);
+    SweepArgs a = make_args(p, w, o, iterations);
     a.c1 = a.c0; // every sweep reads the same input
     a.skip_zero = 0;
     hipEvent_t ev[4];
@@ -1220,15 +1405,15 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
                        dim3(256), 0, s, a, d_x, p->order.as<uint32_t>());
     AG_HIP(hipEventRecord(ev[1], s));
     for (uint32_t it = 0; it < iterations; it++) {
-        hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(p->work.grid), dim3(kSweepBlock),
+        hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(w.grid), dim3(kSweepBlock),
                            p->hot_bytes, s, a, it);
         hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
     }
     AG_HIP(hipGetLastError());
     AG_HIP(hipEventRecord(ev[2], s));
     const uint32_t nbf = ceil_div(N, kBlock);
-    double *parts = p->work.sum_part.as<double>();
-    hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, p->work.pr.as<float>(),
+    double *parts = w.sum_part;
+    hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr,
                        p->order.as<uint32_t>(), N, 0u, d_y, parts);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
     AG_HIP(hipGetLastError());

@@ -20,6 +20,14 @@ struct Work {
     DevBuf pr, c0, c1, blk_err, blk_act, blk_dang, state, err_iter, act_iter, fmt, sum_part, out;
 };
 
+// A captured run of the PageRank sweep loop (init, sweeps, finish), replayed
+// while its launch arguments (`key`) stay the same.
+struct LoopGraph {
+    std::vector<unsigned char> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 // Buffers of a personalized-PageRank batch that a finished batch leaves to
 // the next one on the same graph (see PprBatch).
 constexpr int kKeepBufs = 11;
@@ -45,7 +53,20 @@ struct Plan {
     // buffers of the last finished PprBatch (columns, state, partials,
     // staging), lent to the next batch on this graph
     DevBuf ppr_keep[kKeepBufs];
+    // captured sweep loops, most recent last (at most kLoopGraphs)
+    static constexpr size_t kLoopGraphs = 8;
+    std::vector<LoopGraph> loop_graphs;
     std::mutex mu;
+    Plan() = default;
+    Plan(const Plan &) = delete;
+    Plan &operator=(const Plan &) = delete;
+    ~Plan()
+    {
+        for (LoopGraph &lg : loop_graphs) {
+            if (lg.exec) (void)hipGraphExecDestroy(lg.exec);
+            if (lg.graph) (void)hipGraphDestroy(lg.graph);
+        }
+    }
     uint64_t bytes() const
     {
         return order.bytes + rank.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +

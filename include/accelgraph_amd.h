@@ -165,7 +165,9 @@ typedef struct ag_pr_result
     double error_total;       /* REFERENCE: mean |delta|; DATA_DRIVEN: |delta|/N summed over
                                  the last worklist; STANDARD: L1 residual */
     double pr_sum;            /* sum of returned ranks (FP64) */
-    double loop_ms;           /* device time of the sweep loop (HIP events) */
+    double loop_ms;           /* device time of the sweep loop (HIP events); when the whole
+                                 fixed-count loop is replayed as one hipGraph (epsilon <= 0,
+                                 no AG_PR_FLAG_TIME_SWEEPS), the replay incl. init + normalise */
     double total_ms;          /* device time init + loop + normalise */
     double plan_ms;           /* gather-layout build time (0 if cached) */
     double sweep_kernel_ms;   /* AG_PR_FLAG_TIME_SWEEPS: device time of the sweep kernels
@@ -177,9 +179,13 @@ typedef struct ag_pr_result
 int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *ranks,
                      uint32_t *active_iter, double *err_iter, ag_pr_result *res);
 
-/* Same, writing ranks to a device buffer (N floats). */
+/* Same, writing ranks to a device buffer (N floats).  The sweep loop (init,
+ * sweeps, normalise) is captured into hipGraphs cached on the graph and
+ * replayed on later calls with the same arguments; AG_NO_GRAPH=1 in the
+ * environment runs it eagerly.  AG_PR_FLAG_TIME_SWEEPS always runs eagerly. */
 int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
                             float *d_ranks, ag_pr_result *res);
+
 
 /* ---- locality relabelling (the reference's -l/-L/-O options, README.md:362-411) ----
  * labels[v] = new label of vertex v: the EdgeList label_array the reference's
