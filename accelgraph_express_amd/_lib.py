@@ -137,6 +137,8 @@ _SIGS = {
                                    C.POINTER(PrResult)]),
     "ag_pagerank_pull_device": (C.c_int, [vp, vp, C.POINTER(PrOptions), vp,
                                           C.POINTER(PrResult)]),
+    "ag_pagerank_pull_device_noalloc": (C.c_int, [vp, vp, C.POINTER(PrOptions), vp,
+                                                  C.POINTER(PrResult)]),
     "ag_pagerank_prepare": (C.c_int, [vp, vp, f64p]),
     "ag_spmv_pull": (C.c_int, [vp, vp, f32p, f32p, C.c_uint32, C.POINTER(SpmvResult)]),
     "ag_reorder_labels": (C.c_int, [vp, u32p, u32p, C.c_uint32, C.c_uint32, C.c_uint32,

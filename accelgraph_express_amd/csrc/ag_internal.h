@@ -157,6 +157,9 @@ inline uint32_t bits_for(uint64_t limit) // bits needed for keys in [0, limit)
 
 ag_ctx *default_ctx();
 void check_ctx(ag_ctx *ctx);
+// Device address of the allocation-free entry point's static scratch on
+// `device` (the current device), looked up once per device (ag_pagerank.hip).
+void noalloc_scratch_init(int device);
 
 // ---- sort / scan primitives (ag_sort.hip) ------------------------------------
 // Exclusive scan of n u32 values into out[0..n]; out[n] = total.  in == out allowed.

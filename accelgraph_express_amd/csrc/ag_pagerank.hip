@@ -1429,6 +1429,88 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
     if (res) *res = r;
 }
 
+// ---- allocation-free entry point: static device scratch ----------------------
+// ag_pagerank_pull_device_noalloc takes its per-call buffers (ranks in label
+// order, both contribution buffers, per-workgroup partials, sweep state,
+// per-sweep arrays, reduction partials) from `scratch`, a static device array
+// of kNoAllocBytes that every device the library runs on holds until the
+// process ends.  In host code the array's name is only a host-side stand-in:
+// its device address, per device, comes from hipGetSymbolAddress at context
+// creation (noalloc_scratch_init) and only that address is used afterwards.
+constexpr size_t kNoAllocBytes = (size_t)16 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+
+} // namespace ag
+
+static __device__ __align__(256) unsigned char scratch[ag::kNoAllocBytes];
+
+namespace ag {
+
+namespace {
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+} // namespace
+
+// Called by context creation with `device` current, once per device.
+void noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return; // the entry point then refuses
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void *ptr = nullptr;
+    AG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch)));
+    if (!ptr) fail(AG_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+    g_noalloc_ptr[device] = ptr;
+}
+
+static unsigned char *noalloc_scratch_ptr(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return static_cast<unsigned char *>(g_noalloc_ptr[device]);
+}
+
+// Carves the per-call buffers of the sweep loop out of the device's scratch
+// (256-byte aligned pieces); refuses sizes beyond kNoAllocBytes.
+static WorkPtrs noalloc_work(ag_ctx *ctx, const Plan *p, uint32_t max_iter)
+{
+    const uint32_t grid = sweep_grid(ctx, p);
+    const size_t N = p->N;
+    const size_t sizes[] = {N * 4, N * 4, N * 4, (size_t)grid * 8, (size_t)grid * 8,
+                            (size_t)grid * 4, sizeof(SweepState), ((size_t)max_iter + 1) * 8,
+                            ((size_t)max_iter + 1) * 4, ((size_t)max_iter + 3) * 4,
+                            ((size_t)ceil_div(N, kBlock) + 1) * 8 + 8};
+    size_t need = 0;
+    for (size_t b : sizes) need += (b + 255) & ~(size_t)255;
+    if (need > kNoAllocBytes)
+        fail(AG_ERR_INVALID,
+             "allocation-free PageRank needs %zu bytes of scratch for N=%zu and %u sweeps; the "
+             "static scratch holds %zu (about 1.3 M vertices at 20 sweeps)",
+             need, N, max_iter, kNoAllocBytes);
+    unsigned char *base = noalloc_scratch_ptr(ctx->device);
+    if (!base) fail(AG_ERR_INVALID, "no allocation-free scratch on device %d", ctx->device);
+    size_t off = 0;
+    void *q[11];
+    for (int i = 0; i < 11; i++) {
+        q[i] = base + off;
+        off += (sizes[i] + 255) & ~(size_t)255;
+    }
+    WorkPtrs w;
+    w.grid = grid;
+    w.pr = static_cast<float *>(q[0]);
+    w.c0 = q[1];
+    w.c1 = q[2];
+    w.blk_err = static_cast<double *>(q[3]);
+    w.blk_dang = static_cast<double *>(q[4]);
+    w.blk_act = static_cast<uint32_t *>(q[5]);
+    w.st = static_cast<SweepState *>(q[6]);
+    w.err_iter = static_cast<double *>(q[7]);
+    w.act_iter = static_cast<uint32_t *>(q[8]);
+    w.fmt = static_cast<uint32_t *>(q[9]);
+    w.sum_part = static_cast<double *>(q[10]);
+    return w;
+}
+
 } // namespace ag
 
 using namespace ag;
@@ -1534,6 +1616,28 @@ int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, 
         if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
         if (g->N && !d_ranks) fail(AG_ERR_INVALID, "null output");
         run_pagerank(ctx, g, *opt, d_ranks, nullptr, nullptr, res);
+    });
+}
+
+int ag_pagerank_pull_device_noalloc(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
+                                    float *d_ranks, ag_pr_result *res)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
+        if (g->N && !d_ranks) fail(AG_ERR_INVALID, "null output");
+        if (opt->flags & AG_PR_FLAG_TIME_SWEEPS)
+            fail(AG_ERR_INVALID, "the allocation-free entry point does not time sweeps "
+                                 "(HIP events are created per call)");
+        if (!g->plan)
+            fail(AG_ERR_INVALID, "the allocation-free entry point needs the graph's gather "
+                                 "layout: call ag_pagerank_prepare first");
+        if (g->N == 0) {
+            run_pagerank_impl(ctx, g, *opt, nullptr, d_ranks, nullptr, nullptr, res);
+            return;
+        }
+        const WorkPtrs w = noalloc_work(ctx, g->plan, opt->iterations);
+        run_pagerank_impl(ctx, g, *opt, nullptr, d_ranks, nullptr, nullptr, res, &w);
     });
 }
 

@@ -60,6 +60,7 @@ static ag_ctx *make_ctx(int device)
     c->num_cus = prop.multiProcessorCount;
     c->arch = prop.gcnArchName;
     AG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    noalloc_scratch_init(device);
     return c.release();
 }
 

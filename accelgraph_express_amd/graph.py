@@ -267,6 +267,24 @@ class Graph:
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))
         return r
 
+    def pagerank_pull_device_noalloc(self, d_ranks_ptr: int, iterations: int = 20,
+                                     epsilon: float = 0.0, mode="reference", precision="fp32",
+                                     finish_fp32=False) -> L.PrResult:
+        """ag_pagerank_pull_device_noalloc: the same loop with no device
+        allocation (per-call buffers in the library's static 16 MiB scratch;
+        the gather layout must exist, see prepare())."""
+        lib = L.load()
+        o = L.PrOptions()
+        lib.ag_pr_options_default(C.byref(o))
+        o.iterations, o.epsilon = iterations, epsilon
+        o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
+        o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
+        o.flags = L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0
+        r = L.PrResult()
+        L.check(lib.ag_pagerank_pull_device_noalloc(self.ctx.h, self.h, C.byref(o),
+                                                    C.c_void_p(d_ranks_ptr), C.byref(r)))
+        return r
+
 
 REORDER_MODES = {"none": 0, "out_degree": 1, "in_degree": 2, "inout_degree": 3, "dbg_out": 4,
                  "dbg_in": 5, "hubsort_out": 6, "hubsort_in": 7, "hubcluster_out": 8,

@@ -186,6 +186,18 @@ int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *
 int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
                             float *d_ranks, ag_pr_result *res);
 
+/* Allocation-free form of ag_pagerank_pull_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc / hipFree, creates no event or graph, and takes its
+ * per-call buffers from a static 16 MiB device array that each GPU the library
+ * runs on holds until the process ends (about 1.3 M vertices at 20 sweeps;
+ * larger graphs are refused with AG_ERR_INVALID).  The graph's gather layout
+ * must exist (ag_pagerank_prepare).  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU race.  loop_ms / total_ms come from
+ * the host clock; AG_PR_FLAG_TIME_SWEEPS is refused. */
+int ag_pagerank_pull_device_noalloc(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt,
+                                    float *d_ranks, ag_pr_result *res);
 
 /* ---- locality relabelling (the reference's -l/-L/-O options, README.md:362-411) ----
  * labels[v] = new label of vertex v: the EdgeList label_array the reference's
