@@ -154,6 +154,7 @@ _SIGS = {
     "ag_ppr_options_default": (None, [C.POINTER(PprOptions)]),
     "ag_ppr_batch": (C.c_int, [vp, vp, C.POINTER(PprQuery), C.c_uint32, C.POINTER(PprOptions),
                                f32p, C.POINTER(PprStat), f64p]),
+    "ag_ppr_batch_bytes": (C.c_int, [vp, vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     "ag_ppr_autotune": (C.c_int, [vp, vp, C.c_uint32, u32p]),
     "ag_ppr_single": (C.c_int, [vp, vp, C.POINTER(PprQuery), C.c_uint32, f32p,
                                 C.POINTER(PprStat)]),

@@ -53,13 +53,17 @@ class PprBatch {
     // one row access of a wave covers, {4, 8, 16, 32, 64} (BF16: at least 8).
     // extrapolate: columns take Aitken-type extrapolated steps once their
     // residual falls geometrically (fewer sweeps; same stop rule and bound).
+    // exact_keep: buffers left on the graph by an earlier batch are lent only
+    // when their size is exactly this batch's (others are freed), so the
+    // batch holds exactly bytes_for(capacity) (the serving loop's budget).
     PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t cpw, uint32_t fmt,
-             float damp, bool extrapolate = false);
+             float damp, bool extrapolate = false, bool exact_keep = false);
     ~PprBatch();
     PprBatch(const PprBatch &) = delete;
     PprBatch &operator=(const PprBatch &) = delete;
 
-    // Device bytes of a batch of `capacity` columns on g (builds g's plan).
+    // Device bytes of a batch of `capacity` columns on g (builds g's plan):
+    // the buffers the constructor allocates plus 1 MiB for the seed tables.
     static uint64_t bytes_for(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t fmt);
     uint32_t capacity() const { return cap_; }
     uint32_t active() const { return used_count_; }
@@ -179,7 +183,12 @@ std::vector<PprBatch::Finished> PprBatch::retire(F out, bool may_compact, bool d
     return fin;
 }
 
-uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt);
+// Columns-per-wave for g, timed on a batch of `capacity` columns (default
+// 64; the result is cached on the graph only for 64).
+uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt, uint32_t capacity = 64,
+                      bool exact_keep = false);
+// Frees the column buffers a finished batch left on the graph's plan.
+void ppr_release_kept(ag_graph *g);
 void ensure_ppr_chunks(ag_ctx *ctx, Plan *p);
 // Host copy of the plan's rank array (vertex id -> label), made on first use.
 const std::vector<uint32_t> &rank_host(ag_ctx *ctx, Plan *p);

@@ -1142,19 +1142,44 @@ const std::vector<uint32_t> &rank_host(ag_ctx *ctx, Plan *p)
     return p->rank_host;
 }
 
+// Bytes of each buffer a batch of `cap` columns allocates (the order of
+// PprBatch's `mine` table): X, C0, C1, column state, batch state, residual
+// and dangling partials, chunk partials, seed bitmap, output staging,
+// admission descriptors.
+static void batch_need(const ag_ctx *ctx, const Plan *p, uint32_t cap, size_t need[kKeepBufs])
+{
+    const size_t N = p->N;
+    const size_t nparts = (size_t)ctx->num_cus * 16 * 2;
+    const size_t v[kKeepBufs] = {
+        N * cap * 4, N * cap * 4, N * cap * 4, sizeof(PprCol) * kMaxCap, sizeof(PprBatchState),
+        nparts * kMaxCap * 8, nparts * kMaxCap * 8,
+        (size_t)std::max<uint32_t>(p->ppr_nch, 1) * cap * 4, ((N + 31) / 32) * 4, N * 4,
+        sizeof(PprAdmit) * kMaxCap};
+    for (int i = 0; i < kKeepBufs; i++) need[i] = v[i];
+}
+
 uint64_t PprBatch::bytes_for(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t fmt)
 {
     if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
         fail(AG_ERR_INVALID, "batched columns store FP32 or BF16 (precision %u)", fmt);
     Plan *p = ensure_plan(ctx, g, nullptr);
     ensure_ppr_chunks(ctx, p);
-    const uint64_t N = p->N;
-    // X + two contribution buffers of 4 bytes per column (FP32, or split BF16)
-    return N * cap * 12ull + N / 8 + (uint64_t)p->ppr_nch * cap * 4 + (1u << 20);
+    size_t need[kKeepBufs];
+    batch_need(ctx, p, cap, need);
+    uint64_t b = 1u << 20; // seed tables (labels, weights, rows without in-edges)
+    for (size_t x : need) b += x;
+    return b;
+}
+
+void ppr_release_kept(ag_graph *g)
+{
+    if (!g || !g->plan) return;
+    std::lock_guard<std::mutex> lk(g->plan->mu);
+    for (DevBuf &k : g->plan->ppr_keep) k.release();
 }
 
 PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt, float damp,
-                   bool extrapolate)
+                   bool extrapolate, bool exact_keep)
     : ctx_(ctx), cap_(cap), fmt_(fmt), damp_(damp), extrap_(extrapolate)
 {
     if (cap == 0 || cap > kMaxCap || cap % 8) fail(AG_ERR_INVALID, "batch capacity must be 8..64, multiple of 8");
@@ -1181,18 +1206,17 @@ PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_
         // column buffers alone are ~12 bytes per vertex and column).  Every
         // admitted column is cleared before use, and every other buffer is
         // (re)written before it is read.
-        const size_t nparts = (size_t)ctx->num_cus * 16 * 2;
-        const size_t need[kKeepBufs] = {
-            (size_t)N_ * cap * 4, (size_t)N_ * cap * 4, (size_t)N_ * cap * 4,
-            sizeof(PprCol) * kMaxCap, sizeof(PprBatchState), nparts * kMaxCap * 8,
-            nparts * kMaxCap * 8, (size_t)std::max<uint32_t>(p_->ppr_nch, 1) * cap * 4,
-            (size_t)((N_ + 31) / 32) * 4, (size_t)N_ * 4, sizeof(PprAdmit) * kMaxCap};
+        size_t need[kKeepBufs];
+        batch_need(ctx, p_, cap, need);
         DevBuf *mine[kKeepBufs] = {&X_,        &C_[0], &C_[1], &cols_,      &bstate_, &blk_res_,
                                    &blk_dang_, &part_, &seed_bits_, &out_, &desc_};
         {
             std::lock_guard<std::mutex> lk(p_->mu);
-            for (int i = 0; i < kKeepBufs; i++)
-                if (p_->ppr_keep[i].bytes >= need[i]) *mine[i] = std::move(p_->ppr_keep[i]);
+            for (int i = 0; i < kKeepBufs; i++) {
+                DevBuf &k = p_->ppr_keep[i];
+                if (exact_keep ? k.bytes == need[i] : k.bytes >= need[i]) *mine[i] = std::move(k);
+                else if (exact_keep) k.release();
+            }
         }
         for (int i = 0; i < kKeepBufs; i++)
             if (!mine[i]->p) mine[i]->alloc(need[i]);
@@ -1656,15 +1680,19 @@ void PprBatch::extract(uint32_t slot, float *host_out)
     AG_HIP(hipStreamSynchronize(s));
 }
 
-uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt)
+uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt, uint32_t capacity, bool exact_keep)
 {
     if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
         fail(AG_ERR_INVALID, "batched columns store FP32 or BF16 (precision %u)", fmt);
-    if (g->ppr_cpw[fmt]) return g->ppr_cpw[fmt];
+    if (capacity == 0 || capacity > kMaxCap || capacity % 8)
+        fail(AG_ERR_INVALID, "autotune capacity must be 8..64, multiple of 8");
+    const bool cache = capacity == kMaxCap;
+    if (cache && g->ppr_cpw[fmt]) return g->ppr_cpw[fmt];
     Plan *p = ensure_plan(ctx, g, nullptr);
     const uint32_t N = p->N;
-    PprBatch b(ctx, g, kMaxCap, 64, fmt, 0.85f);
-    for (uint32_t q = 0; q < kMaxCap; q++) {
+    // a full batch of single-seed queries at the capacity the caller will run
+    PprBatch b(ctx, g, capacity, 64, fmt, 0.85f, false, exact_keep);
+    for (uint32_t q = 0; q < capacity; q++) {
         const uint32_t seed = (uint32_t)(((uint64_t)q * 2654435761u) % N);
         b.admit(b.make_seeds(&seed, 1, 0.0, 1u << 30), q);
     }
@@ -1680,7 +1708,7 @@ uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt)
             best = cpw;
         }
     }
-    g->ppr_cpw[fmt] = best;
+    if (cache) g->ppr_cpw[fmt] = best;
     return best;
 }
 
@@ -1698,6 +1726,18 @@ void ag_ppr_options_default(ag_ppr_options *o)
     o->cols_per_wave = 0;
     o->damp = 0.85f;
     o->use_graph_replay = 1;
+}
+
+int ag_ppr_batch_bytes(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t precision,
+                       uint64_t *bytes)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g || !bytes) fail(AG_ERR_INVALID, "null argument");
+        if (capacity == 0 || capacity > kMaxCap || capacity % 8)
+            fail(AG_ERR_INVALID, "batch capacity must be 8..64, multiple of 8");
+        *bytes = PprBatch::bytes_for(ctx, g, capacity, precision);
+    });
 }
 
 int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cpw)

@@ -28,6 +28,7 @@
 #include <cstring>
 #include <fstream>
 
+#include "ag_csr_check.h"
 #include "ag_internal.h"
 
 namespace ag {
@@ -151,25 +152,36 @@ int ag_graph_load(ag_ctx *ctx, const char *path, ag_graph **out)
         std::unique_ptr<ag_graph> g(new ag_graph());
         g->N = h.n;
         g->E = h.e;
+        // Every section is read and checked on the host before any upload: the
+        // checksum guards against corruption only (anyone can recompute it),
+        // so the per-vertex ranges are checked too (ag_csr_check.h).
         uint64_t sum = kFnvBasis;
-        std::vector<uint32_t> host;
+        std::vector<std::vector<uint32_t>> host(kSections);
         for (int i = 0; i < kSections; i++) {
             if (!(h.mask & (1u << i))) continue;
             const uint64_t len = section_len(i, h.n, h.e);
-            host.resize(len);
-            if (len && !f.read(reinterpret_cast<char *>(host.data()), (std::streamsize)(len * 4)))
+            host[i].resize(len);
+            if (len && !f.read(reinterpret_cast<char *>(host[i].data()), (std::streamsize)(len * 4)))
                 fail(AG_ERR_IO, "%s: read failed", path);
-            sum = fnv1a(sum, host.data(), len * 4);
-            if ((i == 4 || i == 9) && host[h.n] != h.e)
-                fail(AG_ERR_INVALID, "%s: offsets end at %u, not E = %u", path, host[h.n], h.e);
-            DevBuf &b = section_buf(g.get(), i);
-            b.alloc(len * 4);
-            if (len) {
-                AG_HIP(hipMemcpyAsync(b.p, host.data(), len * 4, hipMemcpyHostToDevice, ctx->stream));
-                AG_HIP(hipStreamSynchronize(ctx->stream)); // `host` is reused
-            }
+            sum = fnv1a(sum, host[i].data(), len * 4);
         }
         if (sum != h.checksum) fail(AG_ERR_INVALID, "%s: checksum mismatch (corrupt file)", path);
+        auto sec = [&](int i) { return (h.mask & (1u << i)) ? host[i].data() : nullptr; };
+        for (int d = 0; d < 2; d++) {
+            const char *why = csr_direction_check(h.n, h.e, sec(5 * d + 2), sec(5 * d + 3),
+                                                  sec(5 * d + 4));
+            if (why) fail(AG_ERR_INVALID, "%s: %s CSR: %s", path, d ? "inverse" : "forward", why);
+        }
+        for (int i = 0; i < kSections; i++) {
+            if (!(h.mask & (1u << i))) continue;
+            const uint64_t len = section_len(i, h.n, h.e);
+            DevBuf &b = section_buf(g.get(), i);
+            b.alloc(len * 4);
+            if (len)
+                AG_HIP(hipMemcpyAsync(b.p, host[i].data(), len * 4, hipMemcpyHostToDevice,
+                                      ctx->stream));
+        }
+        AG_HIP(hipStreamSynchronize(ctx->stream));
         // vertex ids must index the vertex arrays
         for (int i : {0, 1, 5, 6}) {
             if (!(h.mask & (1u << i)) || h.e == 0) continue;

@@ -82,9 +82,13 @@ using namespace ag;
 
 namespace {
 
+// Frees a graph's batch: its column buffers are returned to the device (not
+// kept on the graph for a later batch), so the budget counts only memory
+// that is actually held.
 void release_batch(ag_server *s, GraphEntry &ge)
 {
     ge.batch.reset();
+    ppr_release_kept(ge.g);
     s->reserved -= ge.reserved;
     ge.reserved = 0;
 }
@@ -113,10 +117,13 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
     s->alloc_attempts++;
     if (s->cfg.inject_fail_alloc && s->alloc_attempts == s->cfg.inject_fail_alloc)
         return false; // injected allocation failure: the query stays queued
-    if (!ge.cpw) ge.cpw = s->cfg.cols_per_wave ? s->cfg.cols_per_wave
-                                               : ppr_autotune(s->ctx, ge.g, s->cfg.precision);
+    // The autotune batch has the admitted capacity and the batch below takes
+    // its buffers over, so tuning stays inside the reservation.
+    if (!ge.cpw) ge.cpw = s->cfg.cols_per_wave
+                              ? s->cfg.cols_per_wave
+                              : ppr_autotune(s->ctx, ge.g, s->cfg.precision, cap, true);
     ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp,
-                                s->cfg.extrapolate != 0));
+                                s->cfg.extrapolate != 0, true));
     ge.reserved = PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision);
     s->reserved += ge.reserved;
     return true;
@@ -243,7 +250,17 @@ int ag_serve_create(ag_ctx *ctx, const ag_serve_config *cfg, ag_server **out)
 
 int ag_serve_destroy(ag_server *s)
 {
-    return guard([&] { delete s; });
+    return guard([&] {
+        if (!s) return;
+        {
+            // the server's column buffers go back to the device, not to the graphs
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->ctx) AG_HIP(hipSetDevice(s->ctx->device));
+            for (auto &ge : s->graphs)
+                if (ge.batch) release_batch(s, ge);
+        }
+        delete s;
+    });
 }
 
 int ag_serve_add_graph(ag_server *s, ag_graph *g, uint32_t *id)

@@ -418,6 +418,15 @@ def ppr_batch(graph: Graph, queries, precision="bf16", cols_per_wave: int = 0,
     return X, list(stats), ms.value
 
 
+def ppr_batch_bytes(graph: Graph, capacity: int, precision="bf16") -> int:
+    """Device bytes a batch of `capacity` columns holds on the graph (the
+    serving loop's per-batch reservation)."""
+    b = C.c_uint64()
+    prec = _PRECS[precision] if isinstance(precision, str) else int(precision)
+    L.check(L.load().ag_ppr_batch_bytes(graph.ctx.h, graph.h, capacity, prec, C.byref(b)))
+    return b.value
+
+
 def ppr_autotune(graph: Graph, precision="bf16") -> int:
     c = C.c_uint32()
     prec = _PRECS[precision] if isinstance(precision, str) else int(precision)

@@ -341,6 +341,10 @@ int ag_ppr_single_opts(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, const ag
 /* Chooses cols_per_wave for g by timing {4, 8, 16, 32, 64} (BF16: from 8) on a
  * full 64-column batch; result cached per graph and precision. */
 int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cols_per_wave);
+/* Device bytes a batch of `capacity` columns (8..64, multiple of 8) holds on g:
+ * what the serving loop reserves against its HBM budget per admitted batch. */
+int ag_ppr_batch_bytes(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t precision,
+                       uint64_t *bytes);
 
 /* ---- serving loop -------------------------------------------------------- */
 typedef struct ag_server ag_server;

@@ -78,3 +78,39 @@ def test_out_of_range_vertex_refused(ctx, tmp_path):
     ag.csr_file_info(p)  # the container itself is well formed
     with pytest.raises(ag.AGError):
         ag.Graph.load(ctx, p)
+
+
+@pytest.mark.parametrize("defect", ["idx_past_E", "offsets_decrease", "degree_vs_offsets",
+                                    "inverse_idx_past_E"])
+def test_inconsistent_ranges_refused_before_upload(ctx, tmp_path, defect):
+    """A file with a correct checksum whose per-vertex ranges leave the edge
+    array is refused on the host (the gather and push kernels would read and
+    write outside it)."""
+    rng = np.random.RandomState(11)
+    n, e = 200, 3000
+    src, dst = rng.randint(0, n, e), rng.randint(0, n, e)
+    csr = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in oracle.build_csr(src, dst, n).items()}
+    deg = csr["vertices.out_degree"]
+    v = int(np.argmax(deg > 0))
+    if defect == "idx_past_E":
+        csr["vertices.edges_idx"][v] = e - 1
+    elif defect == "offsets_decrease":
+        off = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint32)
+        off[v + 1], off[v + 2] = off[v + 2] + 1, off[v + 1]
+        csr["offsets"] = off
+    elif defect == "degree_vs_offsets":
+        # degrees still sum to E, but no longer match the offsets
+        csr["offsets"] = np.concatenate([[0], np.cumsum(deg)]).astype(np.uint32)
+        w = int(np.nonzero(deg > 0)[0][1])
+        d2 = deg.copy()
+        d2[v] += 1
+        d2[w] -= 1
+        csr["vertices.out_degree"] = d2
+    else:
+        csr["inverse_vertices.edges_idx"][int(np.argmax(csr["inverse_vertices.out_degree"]))] = e
+    p = str(tmp_path / f"{defect}.agcsr")
+    write_agcsr(p, csr, n, e)
+    ag.csr_file_info(p)  # checksum and size are consistent
+    with pytest.raises(ag.AGError) as ei:
+        ag.Graph.load(ctx, p)
+    assert "CSR" in str(ei.value)

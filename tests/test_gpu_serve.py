@@ -95,11 +95,19 @@ def test_bf16_serving_alone_equals_in_mix(ctx):
 def test_hbm_budget_admission(ctx):
     graphs = _setup(ctx)
     n = graphs[0][2]
-    # room for one 8-column batch only (FP32: iterate + two contribution buffers)
-    budget = 8 * n * 12 + n * 4 + (1 << 20) + 100
+    # room for one 8-column batch only, as the library counts it (for either
+    # graph: their chunk tables differ in size)
+    s2, d2 = ag.gen_rmat(7, 12, 16)
+    g2 = ag.Graph.build(ctx, s2, d2, n)
+    one = max(ag.ppr_batch_bytes(graphs[0][0], 8, "fp32"), ag.ppr_batch_bytes(g2, 8, "fp32"))
+    assert one > 8 * n * 12
+    budget = one + 100
+    assert budget < 2 * min(ag.ppr_batch_bytes(graphs[0][0], 8, "fp32"),
+                            ag.ppr_batch_bytes(g2, 8, "fp32"))
     srv = ag.Server(ctx, hbm_budget_bytes=budget, max_columns=8, precision="fp32",
                     cols_per_wave=8)
     gid = srv.add_graph(graphs[0][0])
+    plan_bytes0 = graphs[0][0].plan_info()["plan_bytes"]
     rng = np.random.RandomState(4)
     tickets = [srv.submit(gid, rng.randint(0, n, 3).astype(np.uint32), 1e-5) for _ in range(20)]
     srv.step()
@@ -110,13 +118,14 @@ def test_hbm_budget_admission(ctx):
     assert 0 < held <= budget  # the drained batch stays resident for the next query
     # a query on another graph of the same size evicts the idle batch instead
     # of waiting forever (the budget holds one batch)
-    s2, d2 = ag.gen_rmat(7, 12, 16)
-    g2 = ag.Graph.build(ctx, s2, d2, n)
     gid2 = srv.add_graph(g2)
     t2 = srv.submit(gid2, [1, 2, 3], 1e-5)
     res2 = _drain(srv, [t2], {t2: n})
     assert res2[t2][0] == L.AG_OK
     assert 0 < srv.reserved_bytes <= budget
+    # the evicted graph's column buffers were freed, not parked on the graph:
+    # its plan holds only its gather layout again
+    assert graphs[0][0].plan_info()["plan_bytes"] == plan_bytes0
     ref, _, _ = oracle.ppr_f64(oracle.build_csr(s2, d2, n), np.array([1, 2, 3], np.uint32), 1e-5)
     assert np.abs(res2[t2][1] - ref).sum() <= 2e-5
     # a graph whose column state never fits stays queued, and nothing breaks

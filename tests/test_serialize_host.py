@@ -25,8 +25,13 @@ def fnv1a64(b: bytes) -> int:
 
 
 def write_agcsr(path, csr, n, e, mask=MASK_FULL):
-    off = np.concatenate([[0], np.cumsum(csr["vertices.out_degree"])]).astype(np.uint32)
-    ioff = np.concatenate([[0], np.cumsum(csr["inverse_vertices.out_degree"])]).astype(np.uint32)
+    # offsets follow the degrees unless the caller gives them (malformed files)
+    off = csr.get("offsets")
+    if off is None:
+        off = np.concatenate([[0], np.cumsum(csr["vertices.out_degree"])]).astype(np.uint32)
+    ioff = csr.get("inverse_offsets")
+    if ioff is None:
+        ioff = np.concatenate([[0], np.cumsum(csr["inverse_vertices.out_degree"])]).astype(np.uint32)
     secs = [csr["sorted_edges_array.edges_array_src"], csr["sorted_edges_array.edges_array_dest"],
             csr["vertices.out_degree"], csr["vertices.edges_idx"], off,
             csr["inverse_sorted_edges_array.edges_array_src"],
