@@ -37,6 +37,9 @@ AG_PREC_BF16 = 1
 AG_PREC_FP16 = 2
 AG_PR_FLAG_FINISH_FP32 = 0x1
 AG_PR_FLAG_TIME_SWEEPS = 0x2
+AG_LAYOUT_TABLES = 1
+AG_LAYOUT_ROWS = 2
+AG_LAYOUT_AUTO = 3
 
 u32p = C.POINTER(C.c_uint32)
 f32p = C.POINTER(C.c_float)
@@ -72,12 +75,16 @@ class PrResult(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("n_hub", C.c_uint32), ("n_mid", C.c_uint32), ("n_low", C.c_uint32),
                 ("n_zero", C.c_uint32), ("hub_chunks", C.c_uint32), ("low_max", C.c_uint32),
-                ("hub_min", C.c_uint32), ("sell_slots", C.c_uint64), ("plan_bytes", C.c_uint64)]
+                ("hub_min", C.c_uint32), ("sell_slots", C.c_uint64), ("plan_bytes", C.c_uint64),
+                ("hub_chunk", C.c_uint32), ("layout", C.c_uint32), ("tables", C.c_uint32),
+                ("table_workgroups", C.c_uint32), ("subrows", C.c_uint64),
+                ("cold_edges", C.c_uint64)]
 
 
 class Tuning(C.Structure):
     _fields_ = [("low_max", C.c_uint32), ("hub_min", C.c_uint32), ("hub_chunk", C.c_uint32),
-                ("hot_bytes", C.c_uint32)]
+                ("hot_bytes", C.c_uint32), ("layout", C.c_uint32), ("max_tables", C.c_uint32),
+                ("lane_max", C.c_uint32)]
 
 
 class PprQuery(C.Structure):

@@ -182,6 +182,9 @@ struct Tuning {
     // AG_TUNE_AUTO: chosen per graph from its hub rows (build_plan)
     uint32_t hub_chunk = AG_TUNE_AUTO;
     uint32_t hot_bytes = 160 * 1024 - 2048; // LDS table of the hottest contributions
+    uint32_t layout = AG_LAYOUT_AUTO;       // gather layout of the pull sweep
+    uint32_t max_tables = 16;               // LDS tables of the partitioned layout
+    uint32_t lane_max = 16;                 // sub-rows up to this many edges: one lane each
 };
 Tuning &tuning();
 

@@ -121,6 +121,20 @@ struct SweepArgs {
     const uint32_t *seed_v;
     const float *seed_w;
     uint32_t nseeds;
+    // partitioned-table layout (TabPlan; see k_pr_tables / k_pr_combine)
+    uint32_t tab_T, tab_nt, tab_NT, tab_cold_base4, tab_nwaves, tab_lane_max4;
+    uint32_t tab_cold_unit0, tab_nunits; // units [cold_unit0, nunits) are the cold table's
+    uint32_t tab_row_blocks;             // combine blocks over the non-hub rows
+    const uint16_t *tab_idx16;
+    const uint32_t *tab_idx32;
+    const uint32_t *tab_sub_start;
+    float *tab_part;
+    const uint4 *tab_units;
+    const uint32_t *tab_wg_begin;
+    const uint4 *tab_items;
+    const uint4 *tab_hub_items;
+    const uint32_t *tab_rmask;
+    const uint32_t *tab_rbase;
 };
 
 namespace {
@@ -501,6 +515,339 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_sweep(SweepArgs a, uint32_t 
     }
 }
 
+// ---- partitioned-table sweep ------------------------------------------------
+//
+// k_pr_tables: each persistent workgroup walks its range of work units; a
+// unit belongs to one table, and the workgroup (re)loads that table's
+// contributions into LDS (converted to FP32, the sentinel slot T set to 0)
+// when the table changes.  Unit kinds, chosen so that every sub-row's index
+// words are in flight at once (one memory round trip per sub-row):
+//   kUnitLane: up to 4096 consecutive sub-rows, one lane each, kLaneRows per
+//              lane (sub-rows longer than lane_max entries are skipped: their
+//              own units cover them);
+//   kUnitG4/G16/G64: sub-rows from the items list, 4 / 16 / 64 lanes each
+//              (up to 64 / 256 / 1024 entries in one batch of loads);
+//   kUnitHub: hub-row chunks, one wave each, partial to hub_partial.
+// A lane sums its sub-row sequentially in edge order (4 indices per load);
+// lane groups keep per-lane partials over a fixed stride and reduce them
+// pairwise, then by shuffles.  Every partial sum's order is fixed by the
+// layout alone.  DATA_DRIVEN partials carry the OR of the marks in the sign bit.
+constexpr uint32_t kUnitLane = 0, kUnitG4 = 1, kUnitG16 = 2, kUnitG64 = 3, kUnitHub = 4;
+constexpr uint32_t kMaxTabNT = 32; // LDS tables + the cold table (row mask bits)
+constexpr uint32_t kLaneRows = 4;                           // sub-rows per lane of a lane unit
+constexpr uint32_t kLaneUnitRows = kSweepBlock * kLaneRows; // sub-rows per lane unit
+constexpr int kTabDepth = 4;                    // index loads in flight per lane
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Index words at position p (units of 4 entries): four u16 table-local
+// indices packed in two words (LDS tables), or four u32 labels (cold).
+template <bool COLD> struct IdxVec { typedef u32x2 T; };
+template <> struct IdxVec<true> { typedef u32x4 T; };
+
+template <bool COLD>
+__device__ __forceinline__ typename IdxVec<COLD>::T tab_raw(const SweepArgs &a, uint32_t p)
+{
+    if constexpr (COLD)
+        return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.tab_idx32) +
+                                          (p - a.tab_cold_base4));
+    else
+        return __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(a.tab_idx16) + p);
+}
+
+template <bool COLD>
+__device__ __forceinline__ uint32_t tab_idx(const typename IdxVec<COLD>::T &v, int q)
+{
+    if constexpr (COLD)
+        return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+    else
+        return q == 0 ? (v.x & 0xffffu) : q == 1 ? (v.x >> 16) : q == 2 ? (v.y & 0xffffu) : (v.y >> 16);
+}
+
+template <typename CT, bool COLD>
+__device__ __forceinline__ float tab_gather(const CT *__restrict__ cin, const float *tab, uint32_t u)
+{
+    return COLD ? load_c(cin, u) : tab[u];
+}
+
+// Adds the four entries of one index word, in order.
+template <typename CT, bool DD, bool COLD>
+__device__ __forceinline__ void tab_add4(float &s, uint32_t &fl, const CT *__restrict__ cin,
+                                         const float *tab, const typename IdxVec<COLD>::T &v)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc<DD>(s, fl, tab_gather<CT, COLD>(cin, tab, tab_idx<COLD>(v, q)));
+}
+
+// Sum of positions [p0, p1) with stride `step` (units of 4 entries), kTabDepth
+// partials per lane, combined pairwise.
+template <typename CT, bool DD, bool COLD>
+__device__ __forceinline__ float tab_strided_sum(const SweepArgs &a, const CT *__restrict__ cin,
+                                                 const float *tab, uint32_t p0, uint32_t p1,
+                                                 uint32_t step, uint32_t &fl)
+{
+    float sp[kTabDepth];
+#pragma unroll
+    for (int t = 0; t < kTabDepth; t++) sp[t] = 0.f;
+    for (uint32_t p = p0; p < p1; p += kTabDepth * step) {
+        typename IdxVec<COLD>::T v[kTabDepth];
+#pragma unroll
+        for (int t = 0; t < kTabDepth; t++)
+            if (p + t * step < p1) v[t] = tab_raw<COLD>(a, p + t * step);
+#pragma unroll
+        for (int t = 0; t < kTabDepth; t++)
+            if (p + t * step < p1) tab_add4<CT, DD, COLD>(sp[t], fl, cin, tab, v[t]);
+    }
+#pragma unroll
+    for (int w = 1; w < kTabDepth; w <<= 1)
+#pragma unroll
+        for (int t = 0; t + w < kTabDepth; t += 2 * w) sp[t] += sp[t + w];
+    return sp[0];
+}
+
+__device__ __forceinline__ float tab_mark(float s, uint32_t fl)
+{
+    return __uint_as_float(__float_as_uint(s) | (fl & kDdFlag));
+}
+
+template <typename CT, bool DD, bool COLD>
+__device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const float *tab,
+                         uint32_t kind, uint32_t first, uint32_t count, uint32_t dd0)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (kind == kUnitLane) {
+        // up to kLaneRows sub-rows per lane (first + (j*16 + w)*64 + lane); the
+        // first kTabDepth index words of all of them are in flight together
+        uint32_t p0[kLaneRows], p1[kLaneRows];
+        bool mine[kLaneRows];
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            const uint32_t k = (j * kSweepWaves + (uint32_t)w) * 64 + lane;
+            p0[j] = p1[j] = 0;
+            if (k < count) {
+                p0[j] = a.tab_sub_start[first + k];
+                p1[j] = a.tab_sub_start[first + k + 1];
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            mine[j] = p1[j] > p0[j] && p1[j] - p0[j] <= a.tab_lane_max4;
+            if (!mine[j]) p1[j] = p0[j]; // long sub-rows belong to their own units
+        }
+        typename IdxVec<COLD>::T v[kLaneRows][kTabDepth];
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++)
+#pragma unroll
+            for (int t = 0; t < kTabDepth; t++)
+                if (p0[j] + t < p1[j]) v[j][t] = tab_raw<COLD>(a, p0[j] + t);
+        float s[kLaneRows];
+        uint32_t fl[kLaneRows];
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            s[j] = 0.0f;
+            fl[j] = dd0;
+#pragma unroll
+            for (int t = 0; t < kTabDepth; t++)
+                if (p0[j] + t < p1[j]) tab_add4<CT, DD, COLD>(s[j], fl[j], cin, tab, v[j][t]);
+        }
+        // the rest of the longer lane sub-rows, sequentially in edge order
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            for (uint32_t p = p0[j] + kTabDepth; p < p1[j]; p += kTabDepth) {
+                typename IdxVec<COLD>::T x[kTabDepth];
+#pragma unroll
+                for (int t = 0; t < kTabDepth; t++)
+                    if (p + t < p1[j]) x[t] = tab_raw<COLD>(a, p + t);
+#pragma unroll
+                for (int t = 0; t < kTabDepth; t++)
+                    if (p + t < p1[j]) tab_add4<CT, DD, COLD>(s[j], fl[j], cin, tab, x[t]);
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            const uint32_t k = (j * kSweepWaves + (uint32_t)w) * 64 + lane;
+            if (mine[j]) a.tab_part[first + k] = DD ? tab_mark(s[j], fl[j]) : s[j];
+        }
+        return;
+    }
+    if (kind == kUnitHub) {
+        // one wave per chunk: item = (first position, end position, slot)
+        if ((uint32_t)w >= count) return;
+        const uint4 hi = a.tab_hub_items[first + w];
+        uint32_t fl = 0u;
+        float s = tab_strided_sum<CT, DD, COLD>(a, cin, tab, hi.x + lane, hi.y, 64, fl);
+        s = wave_sum(s);
+        if (DD) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
+        }
+        if (lane == 0) a.hub_partial[hi.z] = DD ? tab_mark(s, fl) : s;
+        return;
+    }
+    // long sub-rows, G lanes each: item = (sub-row, first position, end position)
+    const uint32_t G = kind == kUnitG4 ? 4u : kind == kUnitG16 ? 16u : 64u;
+    const uint32_t per_wave = 64u / G;
+    const uint32_t gl = (uint32_t)lane & (G - 1);
+    const uint32_t k = (uint32_t)w * per_wave + (uint32_t)lane / G;
+    const bool valid = k < count;
+    uint32_t fl = dd0;
+    float s = 0.0f;
+    uint4 it = make_uint4(0u, 0u, 0u, 0u);
+    if (valid) {
+        it = a.tab_items[first + k];
+        s = tab_strided_sum<CT, DD, COLD>(a, cin, tab, it.y + gl, it.z, G, fl);
+    }
+    for (uint32_t off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, (int)off, 64);
+    if (DD)
+        for (uint32_t off = G / 2; off > 0; off >>= 1) fl |= __shfl_xor(fl, (int)off, 64);
+    if (valid && gl == 0) a.tab_part[it.x] = DD ? tab_mark(s, fl) : s;
+}
+
+// Loads table contributions [base, base + n) into LDS as FP32: every thread
+// issues all of its 16-byte loads before the first LDS store (one global
+// latency per table load).  base is a multiple of 8 entries.
+constexpr uint32_t kTabEntries = 40440; // sources per LDS table (multiple of 8)
+template <typename CT>
+__device__ __forceinline__ void tab_load(float *tab, const CT *__restrict__ cin, uint32_t base,
+                                         uint32_t n)
+{
+    constexpr uint32_t per = 16 / sizeof(CT); // entries per 16 bytes
+    constexpr uint32_t kU = (kTabEntries / per + kSweepBlock - 1) / kSweepBlock;
+    const uint32_t nvec = n / per;
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(cin + base);
+    for (uint32_t i0 = threadIdx.x; i0 < nvec; i0 += kU * kSweepBlock) {
+        u32x4 v[kU];
+#pragma unroll
+        for (uint32_t k = 0; k < kU; k++)
+            if (i0 + k * kSweepBlock < nvec) v[k] = src[i0 + k * kSweepBlock];
+#pragma unroll
+        for (uint32_t k = 0; k < kU; k++) {
+            const uint32_t i = i0 + k * kSweepBlock;
+            if (i >= nvec) continue;
+            if (sizeof(CT) == 4) {
+                *reinterpret_cast<u32x4 *>(tab + i * per) = v[k];
+            } else {
+                const CT *h = reinterpret_cast<const CT *>(&v[k]);
+#pragma unroll
+                for (uint32_t q = 0; q < per; q++) tab[i * per + q] = load_c(h, q);
+            }
+        }
+    }
+    for (uint32_t i = nvec * per + threadIdx.x; i < n; i += kSweepBlock) tab[i] = load_c(cin, base + i);
+}
+
+template <typename CT, bool DD>
+__global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t it)
+{
+    extern __shared__ __align__(16) unsigned char tab_lds[];
+    float *tab = reinterpret_cast<float *>(tab_lds);
+    SweepState *st = a.st;
+    if (st->done) return;
+    if (a.fmt[it] != FmtOf<CT>::v) return;
+    const CT *cin = static_cast<const CT *>((it & 1) ? a.c1 : a.c0);
+    const uint32_t dd0 = (DD && it == 0) ? kDdFlag : 0u;
+    // the workgroup's static range of LDS units, then cold units (global
+    // gathers: latency-bound, with costs the static split cannot predict)
+    // taken one at a time from the sweep's counter (st->ticket: zeroed by
+    // init and, for the next sweep, by the combine kernel)
+    __shared__ uint32_t s_next;
+    uint32_t cur = 0xffffffffu;
+    const uint32_t u0 = a.tab_wg_begin[blockIdx.x], u1 = a.tab_wg_begin[blockIdx.x + 1];
+    for (uint32_t ui = u0; ui < u1; ui++) {
+        const uint4 un = a.tab_units[ui];
+        const uint32_t t = un.x >> 8, kind = un.x & 0xffu;
+        if (t != cur) {
+            // load table t: labels [t*T, min((t+1)*T, N)), then the zero sentinel
+            __syncthreads();
+            const uint32_t base = t * a.tab_T;
+            tab_load<CT>(tab, cin, base, min(a.tab_T, a.N - base));
+            if (threadIdx.x == 0) tab[a.tab_T] = 0.0f;
+            __syncthreads();
+            cur = t;
+        }
+        tab_unit<CT, DD, false>(a, cin, tab, kind, un.y, un.z, dd0);
+    }
+    for (;;) {
+        __syncthreads(); // s_next is free again
+        if (threadIdx.x == 0) s_next = atomicAdd(&st->ticket, 1u);
+        __syncthreads();
+        const uint32_t ui = a.tab_cold_unit0 + s_next;
+        if (ui >= a.tab_nunits) break;
+        const uint4 un = a.tab_units[ui];
+        tab_unit<CT, DD, true>(a, cin, tab, un.x & 0xffu, un.y, un.z, dd0);
+    }
+}
+
+// k_pr_combine: one lane per non-hub row with in-edges (labels n_hub ..
+// n_nonzero), one 64-row wave per hardware wave (full occupancy hides the
+// two dependent loads): the row's sub-row partials are added in table order
+// (table t's slot: the wave's base for t plus the count of lower lanes with a
+// sub-row in t), then the vertex is updated as in the row layout.  Blocks past
+// the row blocks update the zero-in-degree rows (unless skipped); hub rows
+// are the finalize kernel's.  Each block writes its residual partials.
+constexpr int kCombBlock = 256;
+template <bool DD>
+__global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t it)
+{
+    __shared__ double s_err[kCombBlock / 64], s_dang[kCombBlock / 64];
+    __shared__ uint32_t s_act[kCombBlock / 64];
+    SweepState *st = a.st;
+    if (st->done) return;
+    const uint32_t wfmt = a.fmt[it + 1];
+    const float rscale = a.fmt[it] == 2u ? a.f16_inv : 1.0f;
+    void *cout = (it & 1) ? a.c0 : a.c1;
+    const float tele = st->tele;
+    const uint32_t dd0 = (DD && it == 0) ? kDdFlag : 0u;
+    double err = 0.0, dang = 0.0;
+    uint32_t act = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->ticket = 0; // the next sweep's cold units
+    if (blockIdx.x < a.tab_row_blocks) {
+        const uint32_t wv = blockIdx.x * (kCombBlock / 64) + w;
+        if (wv < a.tab_nwaves) {
+            const uint64_t below = (1ull << lane) - 1ull;
+            const uint32_t r = a.n_hub + wv * 64 + lane;
+            const bool valid = r < a.n_nonzero;
+            const uint32_t m = valid ? a.tab_rmask[r - a.n_hub] : 0u;
+            const uint32_t *rb = a.tab_rbase + (size_t)wv * a.tab_NT;
+            // every partial load is issued before the first add
+            float pv[kMaxTabNT];
+#pragma unroll
+            for (uint32_t t = 0; t < kMaxTabNT; t++) {
+                pv[t] = 0.0f;
+                if (t < a.tab_NT) {
+                    const uint64_t b = __ballot((m >> t) & 1u);
+                    if ((m >> t) & 1u) pv[t] = a.tab_part[rb[t] + (uint32_t)__popcll(b & below)];
+                }
+            }
+            float s = 0.0f;
+            uint32_t fl = dd0;
+#pragma unroll
+            for (uint32_t t = 0; t < kMaxTabNT; t++)
+                if (t < a.tab_NT && ((m >> t) & 1u)) acc<DD>(s, fl, pv[t]);
+            if (valid) update_vertex(a, r, s * rscale, tele, cout, wfmt, err, act, dang, fl);
+        }
+    } else if (!a.skip_zero) {
+        const uint32_t i = (blockIdx.x - a.tab_row_blocks) * kCombBlock + threadIdx.x;
+        if (i < a.N - a.n_nonzero)
+            update_vertex(a, a.n_nonzero + i, 0.0f, tele, cout, wfmt, err, act, dang, dd0);
+    }
+    err = wave_sum(err);
+    dang = wave_sum(dang);
+    act = wave_sum(act);
+    if (lane == 0) { s_err[w] = err; s_dang[w] = dang; s_act[w] = act; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double e = 0.0, d = 0.0;
+        uint32_t c = 0;
+        for (int i = 0; i < kCombBlock / 64; i++) { e += s_err[i]; d += s_dang[i]; c += s_act[i]; }
+        a.blk_err[blockIdx.x] = e;
+        a.blk_dang[blockIdx.x] = d;
+        a.blk_act[blockIdx.x] = c;
+    }
+}
+
 // One workgroup, launched after each sweep (the kernel boundary orders it
 // after every sweep workgroup, so no device-scope fence is needed): sums each
 // hub row's chunk partials in chunk order and updates the hub vertices, then
@@ -611,6 +958,17 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
         }
     }
     if (i <= a.max_iter + 1) a.fmt[i] = fmt0;
+    if (i == a.N) {
+        // the zero after each contribution vector (read by the cold table's
+        // padding): in the FP32 view, and in the 16-bit view when the run
+        // starts in BF16 / FP16
+        static_cast<float *>(a.c0)[a.N] = 0.0f;
+        static_cast<float *>(a.c1)[a.N] = 0.0f;
+        if (fmt0 != kFmtF32) {
+            static_cast<uint16_t *>(a.c0)[a.N] = 0;
+            static_cast<uint16_t *>(a.c1)[a.N] = 0;
+        }
+    }
     if (i >= a.N) return;
     float x = x0;
     if (a.mode == kModePPR) {
@@ -641,6 +999,7 @@ __global__ void k_spmv_init(SweepArgs a, const float *__restrict__ x,
     }
     if (i <= a.max_iter + 1) a.fmt[i] = kFmtF32;
     if (i < a.N) static_cast<float *>(a.c0)[i] = x[order[i]];
+    if (i == a.N) static_cast<float *>(a.c0)[a.N] = 0.0f; // read by the cold table's padding
 }
 
 // Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
@@ -794,7 +1153,223 @@ __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint3
     for (; k < n; k++) p[k * 64] = rank[inv_dst[s + k]];
 }
 
+// ---- partitioned-table layout construction ----
+
+// Edges per table (sources u / T), for tables 0 .. kTabHist-2; the last bin
+// collects the rest.
+constexpr int kTabHist = 64;
+__global__ __launch_bounds__(256) void k_tab_hist(const uint32_t *__restrict__ ecol, uint32_t E,
+                                                  uint32_t T, uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t h[kTabHist];
+    if (threadIdx.x < kTabHist) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < E; e += gridDim.x * 256)
+        atomicAdd(&h[min(ecol[e] / T, (uint32_t)kTabHist - 1)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kTabHist && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// key = table of the edge's source (nt: cold), value = edge position.
+__global__ void k_tab_keys(const uint32_t *__restrict__ ecol, uint32_t E, uint32_t T, uint32_t nt,
+                           uint32_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    key[e] = min(ecol[e] / T, nt);
+    val[e] = e;
+}
+
+// Row of each table-sorted edge (last r with roff[r] <= e, r < nrows).
+__global__ void k_tab_rows(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ roff,
+                           uint32_t nrows, uint32_t E, uint32_t *__restrict__ srow)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    const uint32_t e = perm[i];
+    uint32_t lo = 0, hi = nrows;
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (roff[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    srow[i] = lo;
+}
+
+// 1 where a sub-row (table, row) starts in the table-sorted edge order.
+__global__ void k_tab_flags(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ srow,
+                            uint32_t E, uint32_t *__restrict__ flag)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    flag[i] = (i == 0 || skey[i] != skey[i - 1] || srow[i] != srow[i - 1]) ? 1u : 0u;
+}
+
+// Per sub-row: row, table, first table-sorted edge (sub_e0[S] = E).
+__global__ void k_tab_subrows(const uint32_t *__restrict__ flag, const uint32_t *__restrict__ fscan,
+                              const uint32_t *__restrict__ skey, const uint32_t *__restrict__ srow,
+                              uint32_t E, uint32_t *__restrict__ sub_row,
+                              uint32_t *__restrict__ sub_tab, uint32_t *__restrict__ sub_e0)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    if (flag[i]) {
+        const uint32_t sid = fscan[i];
+        sub_row[sid] = srow[i];
+        sub_tab[sid] = skey[i];
+        sub_e0[sid] = i;
+    }
+    if (i == E - 1) sub_e0[fscan[E]] = E;
+}
+
+// Padded length of each sub-row in units of 4 entries.
+__global__ void k_tab_len4(const uint32_t *__restrict__ sub_e0, uint32_t S, uint32_t *__restrict__ len4)
+{
+    const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sid < S) len4[sid] = (sub_e0[sid + 1] - sub_e0[sid] + 3u) / 4u;
+}
+
+__global__ void k_fill_u16(uint16_t *__restrict__ p, uint64_t n, uint16_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v)
+{
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Source index of every table-sorted edge into its sub-row's slot.
+__global__ void k_tab_fill_idx(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ perm,
+                               const uint32_t *__restrict__ ecol, const uint32_t *__restrict__ fscan,
+                               const uint32_t *__restrict__ sub_e0,
+                               const uint32_t *__restrict__ sub_start, uint32_t E, uint32_t T,
+                               uint32_t nt, uint32_t cold_base4, uint16_t *__restrict__ idx16,
+                               uint32_t *__restrict__ idx32)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E) return;
+    const uint32_t sid = fscan[i + 1] - 1u;
+    const uint32_t t = skey[i];
+    const uint32_t u = ecol[perm[i]];
+    const uint64_t pos = (uint64_t)sub_start[sid] * 4u + (i - sub_e0[sid]);
+    if (t < nt)
+        idx16[pos] = (uint16_t)(u - t * T);
+    else
+        idx32[pos - (uint64_t)cold_base4 * 4u] = u;
+}
+
+// First sub-row of every table that has one.
+__global__ void k_tab_bounds(const uint32_t *__restrict__ sub_tab, uint32_t S, uint32_t *__restrict__ sb)
+{
+    const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sid >= S) return;
+    if (sid == 0 || sub_tab[sid] != sub_tab[sid - 1]) sb[sub_tab[sid]] = sid;
+}
+
+// Class of a sub-row (its unit kind): hub row, else by padded length: one
+// lane (up to lane_max entries), 4 lanes (up to 64), 16 lanes (up to 256),
+// 64 lanes.  Key = table * 8 + class, value = sub-row.
+__global__ void k_tab_classes(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sub_tab,
+                              const uint32_t *__restrict__ len4, uint32_t S, uint32_t n_hub,
+                              uint32_t lane_max4, uint32_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sid >= S) return;
+    const uint32_t l4 = len4[sid];
+    uint32_t c;
+    if (sub_row[sid] < n_hub) c = 4;
+    else if (l4 <= lane_max4) c = kUnitLane;
+    else if (l4 <= 16) c = kUnitG4;
+    else if (l4 <= 64) c = kUnitG16;
+    else c = kUnitG64;
+    key[sid] = sub_tab[sid] * 8u + c;
+    val[sid] = sid;
+}
+
+// Table bits of every non-hub row.
+__global__ void k_tab_masks(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sub_tab,
+                            uint32_t S, uint32_t n_hub, uint32_t *__restrict__ rmask)
+{
+    const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sid >= S) return;
+    const uint32_t r = sub_row[sid];
+    if (r >= n_hub) atomicOr(&rmask[r - n_hub], 1u << sub_tab[sid]);
+}
+
+// rbase[w * NT + t] = first sub-row of table t whose row is >= the first row of
+// 64-row wave w (rows ascend inside a table).
+__global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sb,
+                            uint32_t NT, uint32_t nwaves, uint32_t n_hub,
+                            uint32_t *__restrict__ rbase)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nwaves * NT) return;
+    const uint32_t w = i / NT, t = i % NT;
+    const uint32_t row0 = n_hub + w * 64u;
+    uint32_t lo = sb[t], hi = sb[t + 1]; // first index in [lo, hi) with sub_row >= row0
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (sub_row[mid] < row0) lo = mid + 1;
+        else hi = mid;
+    }
+    rbase[i] = lo;
+}
+
+// Cost of each work unit in padded index positions (lane units: per wave,
+// 64 lanes times the largest per-lane total over its kLaneRows sub-rows; long
+// and hub units: their positions), for the static split over workgroups.
+__global__ void k_tab_unit_cost(const uint4 *__restrict__ units, uint32_t nunits,
+                                const uint32_t *__restrict__ sub_start,
+                                const uint4 *__restrict__ litems,
+                                const uint4 *__restrict__ hub_items, uint32_t lane_max4,
+                                uint32_t *__restrict__ cost)
+{
+    const uint32_t ui = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ui >= nunits) return;
+    const uint4 un = units[ui];
+    const uint32_t kind = un.x & 0xffu;
+    uint64_t c = 0;
+    if (kind == kUnitLane) {
+        for (uint32_t w = 0; w < kSweepWaves; w++) {
+            uint32_t mx = 0;
+            for (uint32_t l = 0; l < 64; l++) {
+                uint32_t tot = 0;
+                for (uint32_t j = 0; j < kLaneRows; j++) {
+                    const uint32_t k = (j * kSweepWaves + w) * 64 + l;
+                    if (k >= un.z) continue;
+                    const uint32_t l4 = sub_start[un.y + k + 1] - sub_start[un.y + k];
+                    if (l4 <= lane_max4) tot += l4;
+                }
+                mx = max(mx, tot);
+            }
+            c += 64ull * mx;
+        }
+    } else if (kind == kUnitHub) {
+        for (uint32_t k = 0; k < un.z; k++) c += hub_items[un.y + k].y - hub_items[un.y + k].x;
+    } else {
+        for (uint32_t k = 0; k < un.z; k++) c += litems[un.y + k].z - litems[un.y + k].y;
+    }
+    cost[ui] = (uint32_t)min<uint64_t>(c, 0xffffffffull);
+}
+
+// Long-sub-row items (sub-row, first position, end position) of one
+// (table, class) range of the class-sorted sub-rows.
+__global__ void k_tab_litems(const uint32_t *__restrict__ sorted, uint32_t n,
+                             const uint32_t *__restrict__ sub_start, uint4 *__restrict__ out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t sid = sorted[k];
+    out[k] = make_uint4(sid, sub_start[sid], sub_start[sid + 1], 0u);
+}
+
 } // namespace
+
+static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn);
 
 static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
 {
@@ -808,6 +1383,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->hub_min = tn.hub_min;
     p->hub_chunk = tn.hub_chunk;
     p->hot_bytes = tn.hot_bytes;
+    p->lane_max = tn.lane_max;
     if (p->hot_bytes) {
         static bool attr_set = false;
         if (!attr_set) {
@@ -816,6 +1392,13 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
                                  (const void *)k_pr_sweep<__half, false>,
                                  (const void *)k_pr_sweep<float, true>};
             for (const void *k : ks)
+                AG_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(kLdsBytes - kStaticLds)));
+            const void *kt[4] = {(const void *)k_pr_tables<float, false>,
+                                 (const void *)k_pr_tables<bf16_t, false>,
+                                 (const void *)k_pr_tables<__half, false>,
+                                 (const void *)k_pr_tables<float, true>};
+            for (const void *k : kt)
                 AG_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)(kLdsBytes - kStaticLds)));
             attr_set = true;
@@ -943,7 +1526,309 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->nb_zero = ceil_div(N - p->n_nonzero, kZeroRowsPerUnit);
     p->nvblocks = p->nb_hub + p->nb_w + p->nb_h + p->nb_q + p->nb_low + p->nb_zero;
     AG_HIP(hipStreamSynchronize(s));
+    // partitioned tables when forced, or (auto) when one LDS table cannot
+    // hold every source
+    if (tn.layout == AG_LAYOUT_TABLES || (tn.layout == AG_LAYOUT_AUTO && N > kTabEntries))
+        build_tables(ctx, g, p.get(), tn);
     return p.release();
+}
+
+// Builds the partitioned-table layout on top of the row plan (labels, row
+// offsets).  Per-edge work runs on the device; per-table and per-unit
+// bookkeeping on the host from small downloads.
+static constexpr uint32_t kMinTabEdges = 65536; // an LDS table needs this many edges
+// Fixed cost of a unit, in index positions: each wave of a unit pays about two
+// dependent memory round trips whatever its length.
+static constexpr uint32_t kUnitOverhead = 4096;
+
+static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t N = p->N, E = p->E, nrows = p->n_nonzero, n_hub = p->n_hub;
+    if (E == 0 || nrows == 0) return; // no gather: the row layout serves
+    std::unique_ptr<TabPlan> tp(new TabPlan());
+    const uint32_t T = kTabEntries; // LDS holds T + 1 floats
+    static_assert((kTabEntries + 1) * 4 <= kLdsBytes - kStaticLds, "LDS table too large");
+    tp->T = T;
+    auto dl = [&](void *dst, const void *src, size_t bytes) {
+        if (bytes) AG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    };
+    // 1. relabelled sources of every row with in-edges, label order
+    DevBuf ecol((size_t)E * 4);
+    hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(E, 256)), dim3(256), 0, s,
+                       p->order.as<uint32_t>(), p->roff.as<uint32_t>(), g->inv_idx.as<uint32_t>(),
+                       g->inv_dst.as<uint32_t>(), p->rank.as<uint32_t>(), nrows, E,
+                       ecol.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    // 2. edges per table; LDS tables while each holds enough edges
+    {
+        DevBuf hist(kTabHist * 4);
+        AG_HIP(hipMemsetAsync(hist.p, 0, kTabHist * 4, s));
+        hipLaunchKernelGGL(k_tab_hist, dim3(std::min<uint32_t>(ceil_div(E, 256), ctx->num_cus * 8)),
+                           dim3(256), 0, s, ecol.as<uint32_t>(), E, T, hist.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        uint32_t h[kTabHist];
+        dl(h, hist.p, sizeof(h));
+        AG_HIP(hipStreamSynchronize(s));
+        const uint32_t min_edges = std::max<uint32_t>(1, std::min<uint32_t>(kMinTabEdges, E / 64));
+        uint32_t nt = 0;
+        while (nt < tn.max_tables && nt < (uint32_t)kTabHist - 1 && (uint64_t)nt * T < N &&
+               h[nt] >= min_edges)
+            nt++;
+        tp->nt = nt;
+        tp->NT = nt + 1;
+    }
+    const uint32_t nt = tp->nt, NT = tp->NT;
+    // 3. stable sort of the edges by table: each table's edges stay in (row, edge) order
+    DevBuf skey((size_t)E * 4), perm((size_t)E * 4);
+    {
+        DevBuf key((size_t)E * 4), val((size_t)E * 4);
+        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol.as<uint32_t>(),
+                           E, T, nt, key.as<uint32_t>(), val.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        radix_sort_pairs(ctx, key.as<uint32_t>(), val.as<uint32_t>(), skey.as<uint32_t>(),
+                         perm.as<uint32_t>(), E, bits_for(NT));
+        AG_HIP(hipStreamSynchronize(s));
+    }
+    // 4. sub-rows: starts where (table, row) changes
+    DevBuf fscan(((size_t)E + 1) * 4);
+    uint32_t S = 0;
+    DevBuf sub_row, sub_tab, sub_e0;
+    {
+        DevBuf srow((size_t)E * 4), flag((size_t)E * 4);
+        hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm.as<uint32_t>(),
+                           p->roff.as<uint32_t>(), nrows, E, srow.as<uint32_t>());
+        hipLaunchKernelGGL(k_tab_flags, dim3(ceil_div(E, 256)), dim3(256), 0, s,
+                           skey.as<uint32_t>(), srow.as<uint32_t>(), E, flag.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        scan_exclusive_u32(ctx, flag.as<uint32_t>(), fscan.as<uint32_t>(), E);
+        dl(&S, fscan.as<uint32_t>() + E, 4);
+        AG_HIP(hipStreamSynchronize(s));
+        sub_row.alloc((size_t)S * 4);
+        sub_tab.alloc((size_t)S * 4);
+        sub_e0.alloc(((size_t)S + 1) * 4);
+        hipLaunchKernelGGL(k_tab_subrows, dim3(ceil_div(E, 256)), dim3(256), 0, s,
+                           flag.as<uint32_t>(), fscan.as<uint32_t>(), skey.as<uint32_t>(),
+                           srow.as<uint32_t>(), E, sub_row.as<uint32_t>(), sub_tab.as<uint32_t>(),
+                           sub_e0.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        AG_HIP(hipStreamSynchronize(s));
+    }
+    tp->S = S;
+    // 5. padded lengths and starts (units of 4 entries, all tables in one range)
+    DevBuf len4((size_t)S * 4);
+    tp->sub_start.alloc(((size_t)S + 1) * 4);
+    hipLaunchKernelGGL(k_tab_len4, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_e0.as<uint32_t>(), S,
+                       len4.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    scan_exclusive_u32(ctx, len4.as<uint32_t>(), tp->sub_start.as<uint32_t>(), S);
+    // 6. first sub-row of each table
+    DevBuf sbd(((size_t)NT + 1) * 4);
+    AG_HIP(hipMemsetAsync(sbd.p, 0xff, ((size_t)NT + 1) * 4, s));
+    hipLaunchKernelGGL(k_tab_bounds, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_tab.as<uint32_t>(),
+                       S, sbd.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    tp->SB.assign(NT + 1, 0);
+    dl(tp->SB.data(), sbd.p, (size_t)NT * 4);
+    uint32_t total4 = 0;
+    dl(&total4, tp->sub_start.as<uint32_t>() + S, 4);
+    AG_HIP(hipStreamSynchronize(s));
+    tp->SB[NT] = S;
+    for (int t = (int)NT - 1; t >= 0; t--)
+        if (tp->SB[t] == 0xffffffffu) tp->SB[t] = tp->SB[t + 1]; // empty table
+    AG_HIP(hipMemcpyAsync(sbd.p, tp->SB.data(), ((size_t)NT + 1) * 4, hipMemcpyHostToDevice, s));
+    uint32_t cold_base4 = total4;
+    if (tp->SB[nt] < S) dl(&cold_base4, tp->sub_start.as<uint32_t>() + tp->SB[nt], 4);
+    AG_HIP(hipStreamSynchronize(s));
+    tp->cold_base4 = cold_base4;
+    tp->idx16_len = (uint64_t)cold_base4 * 4;
+    tp->idx32_len = (uint64_t)(total4 - cold_base4) * 4;
+    // 7. source indices, padded with the sentinels
+    tp->idx16.alloc(std::max<uint64_t>(tp->idx16_len, 4) * 2);
+    tp->idx32.alloc(std::max<uint64_t>(tp->idx32_len, 4) * 4);
+    const uint32_t fill_grid = ctx->num_cus * 8;
+    hipLaunchKernelGGL(k_fill_u16, dim3(fill_grid), dim3(256), 0, s, tp->idx16.as<uint16_t>(),
+                       (uint64_t)tp->idx16.bytes / 2, (uint16_t)T);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, tp->idx32.as<uint32_t>(),
+                       (uint64_t)tp->idx32.bytes / 4, N);
+    hipLaunchKernelGGL(k_tab_fill_idx, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey.as<uint32_t>(),
+                       perm.as<uint32_t>(), ecol.as<uint32_t>(), fscan.as<uint32_t>(),
+                       sub_e0.as<uint32_t>(), tp->sub_start.as<uint32_t>(), E, T, nt, cold_base4,
+                       tp->idx16.as<uint16_t>(), tp->idx32.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    AG_HIP(hipStreamSynchronize(s));
+    ecol.release();
+    skey.release();
+    perm.release();
+    fscan.release();
+    sub_e0.release();
+    // 8. sub-rows by (table, class): the long ones' lists and the hub sub-rows
+    const uint32_t lane_max4 = tn.lane_max / 4;
+    const uint32_t nkeys = NT * 8;
+    std::vector<uint32_t> ccnt(nkeys), cfirst(nkeys);
+    DevBuf sorted_sub((size_t)S * 4);
+    {
+        DevBuf key2((size_t)S * 4), val2((size_t)S * 4), sk2((size_t)S * 4), cnt(nkeys * 4),
+            fst(nkeys * 4);
+        hipLaunchKernelGGL(k_tab_classes, dim3(ceil_div(S, 256)), dim3(256), 0, s,
+                           sub_row.as<uint32_t>(), sub_tab.as<uint32_t>(), len4.as<uint32_t>(), S,
+                           n_hub, lane_max4, key2.as<uint32_t>(), val2.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        radix_sort_pairs(ctx, key2.as<uint32_t>(), val2.as<uint32_t>(), sk2.as<uint32_t>(),
+                         sorted_sub.as<uint32_t>(), S, bits_for(nkeys));
+        vertex_map(ctx, sk2.as<uint32_t>(), S, nkeys, cnt.as<uint32_t>(), fst.as<uint32_t>(), nullptr);
+        dl(ccnt.data(), cnt.p, (size_t)nkeys * 4);
+        dl(cfirst.data(), fst.p, (size_t)nkeys * 4);
+        AG_HIP(hipStreamSynchronize(s));
+    }
+    // 9. hub chunks: slots in (row, table, chunk) order, processed by table
+    tp->hub_chunk = std::max<uint32_t>(4, p->hub_chunk / 4 * 4);
+    const uint32_t chunk4 = tp->hub_chunk / 4;
+    std::vector<uint32_t> H(NT);
+    std::vector<std::vector<uint32_t>> hrow(NT), hlen(NT), hstart(NT);
+    for (uint32_t t = 0; t < NT; t++) {
+        H[t] = ccnt[t * 8 + 4];
+        hrow[t].resize(H[t]);
+        hlen[t].resize(H[t]);
+        hstart[t].resize(H[t]);
+        // hub rows come first among a table's sub-rows (rows ascend)
+        dl(hrow[t].data(), sub_row.as<uint32_t>() + tp->SB[t], (size_t)H[t] * 4);
+        dl(hlen[t].data(), len4.as<uint32_t>() + tp->SB[t], (size_t)H[t] * 4);
+        dl(hstart[t].data(), tp->sub_start.as<uint32_t>() + tp->SB[t], (size_t)H[t] * 4);
+    }
+    AG_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> first(n_hub + 1, 0);
+    std::vector<std::vector<uint32_t>> slot_of(NT); // first slot of each hub sub-row
+    {
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> per_row(n_hub); // (table, k)
+        for (uint32_t t = 0; t < NT; t++) {
+            slot_of[t].resize(H[t]);
+            for (uint32_t k = 0; k < H[t]; k++) {
+                if (hrow[t][k] >= n_hub) fail(AG_ERR_INVALID, "table layout: hub sub-row order");
+                per_row[hrow[t][k]].push_back({t, k});
+            }
+        }
+        uint32_t slot = 0;
+        for (uint32_t h = 0; h < n_hub; h++) {
+            first[h] = slot;
+            for (auto &tk : per_row[h]) { // tables ascend
+                slot_of[tk.first][tk.second] = slot;
+                slot += ceil_div(hlen[tk.first][tk.second], chunk4);
+            }
+        }
+        first[n_hub] = slot;
+        tp->hub_chunks = slot;
+    }
+    std::vector<uint4> hub_items;
+    hub_items.reserve(tp->hub_chunks);
+    // 10. work units, table by table; long-sub-row items in unit order
+    std::vector<uint4> units;
+    uint32_t nlong = 0;
+    for (uint32_t t = 0; t < NT; t++)
+        for (uint32_t c = 1; c <= 3; c++) nlong += ccnt[t * 8 + c];
+    tp->items.alloc((size_t)std::max<uint32_t>(nlong, 1) * sizeof(uint4));
+    uint32_t lpos = 0;
+    for (uint32_t t = 0; t < NT; t++) {
+        const uint32_t tt = t << 8;
+        const uint32_t h0 = (uint32_t)hub_items.size();
+        for (uint32_t k = 0; k < H[t]; k++) {
+            const uint32_t nch = ceil_div(hlen[t][k], chunk4);
+            const uint32_t end = hstart[t][k] + hlen[t][k];
+            for (uint32_t j = 0; j < nch; j++) {
+                const uint32_t b = hstart[t][k] + j * chunk4;
+                hub_items.push_back(make_uint4(b, std::min(end, b + chunk4), slot_of[t][k] + j,
+                                               tp->SB[t] + k));
+            }
+        }
+        const uint32_t h1 = (uint32_t)hub_items.size();
+        for (uint32_t c = h0; c < h1; c += kSweepWaves)
+            units.push_back(make_uint4(tt | kUnitHub, c, std::min<uint32_t>(kSweepWaves, h1 - c), 0u));
+        // sub-rows per unit: 16 / 4 / 1 per wave
+        const uint32_t per_unit[4] = {0, 16 * kSweepWaves, 4 * kSweepWaves, kSweepWaves};
+        for (uint32_t c = 3; c >= 1; c--) {
+            const uint32_t n = ccnt[t * 8 + c], f = cfirst[t * 8 + c];
+            if (n)
+                hipLaunchKernelGGL(k_tab_litems, dim3(ceil_div(n, 256)), dim3(256), 0, s,
+                                   sorted_sub.as<uint32_t>() + f, n, tp->sub_start.as<uint32_t>(),
+                                   tp->items.as<uint4>() + lpos);
+            for (uint32_t k = 0; k < n; k += per_unit[c])
+                units.push_back(make_uint4(tt | c, lpos + k, std::min<uint32_t>(per_unit[c], n - k), 0u));
+            lpos += n;
+        }
+        // cold lane units are smaller: the kernel takes them one at a time
+        const uint32_t lrows = t < nt ? kLaneUnitRows : kSweepBlock;
+        for (uint32_t k = tp->SB[t] + H[t]; k < tp->SB[t + 1]; k += lrows)
+            units.push_back(make_uint4(tt | kUnitLane, k,
+                                       std::min<uint32_t>(lrows, tp->SB[t + 1] - k), 0u));
+    }
+    tp->nunits = (uint32_t)units.size();
+    tp->hub_items.alloc(std::max<size_t>(hub_items.size(), 1) * sizeof(uint4));
+    if (!hub_items.empty())
+        AG_HIP(hipMemcpyAsync(tp->hub_items.p, hub_items.data(), hub_items.size() * sizeof(uint4),
+                              hipMemcpyHostToDevice, s));
+    tp->units.alloc(std::max<size_t>(units.size(), 1) * sizeof(uint4));
+    if (!units.empty())
+        AG_HIP(hipMemcpyAsync(tp->units.p, units.data(), units.size() * sizeof(uint4),
+                              hipMemcpyHostToDevice, s));
+    tp->chunk_first.alloc(((size_t)n_hub + 1) * 4);
+    AG_HIP(hipMemcpyAsync(tp->chunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
+    tp->hub_partial.alloc(std::max<size_t>(tp->hub_chunks, 1) * 4);
+    tp->part.alloc(std::max<size_t>(S, 1) * 4);
+    // 11. split the units over one workgroup per CU by their cost
+    std::vector<uint32_t> cost(units.size());
+    if (!units.empty()) {
+        DevBuf dcost(units.size() * 4);
+        hipLaunchKernelGGL(k_tab_unit_cost, dim3(ceil_div(units.size(), 256)), dim3(256), 0, s,
+                           tp->units.as<uint4>(), (uint32_t)units.size(),
+                           tp->sub_start.as<uint32_t>(), tp->items.as<uint4>(),
+                           tp->hub_items.as<uint4>(), lane_max4, dcost.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        dl(cost.data(), dcost.p, units.size() * 4);
+        AG_HIP(hipStreamSynchronize(s));
+    }
+    // the cold table's units come last; the kernel deals them round-robin,
+    // and the LDS units [0, cold_unit0) are split into contiguous ranges
+    uint32_t cold_unit0 = (uint32_t)units.size();
+    while (cold_unit0 > 0 && (units[cold_unit0 - 1].x >> 8) == nt) cold_unit0--;
+    tp->cold_unit0 = cold_unit0;
+    const uint32_t grid = std::max<uint32_t>(1, (uint32_t)ctx->num_cus);
+    std::vector<uint64_t> pre(cold_unit0 + 1, 0);
+    for (size_t u = 0; u < cold_unit0; u++) pre[u + 1] = pre[u] + cost[u] + kUnitOverhead;
+    std::vector<uint32_t> wgb(grid + 1);
+    for (uint32_t b = 0; b <= grid; b++) {
+        const uint64_t target = pre.back() * b / grid;
+        wgb[b] = (uint32_t)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+        wgb[b] = std::min<uint32_t>(wgb[b], cold_unit0);
+    }
+    wgb[0] = 0;
+    wgb[grid] = cold_unit0;
+    tp->grid = grid;
+    tp->wg_begin.alloc(((size_t)grid + 1) * 4);
+    AG_HIP(hipMemcpyAsync(tp->wg_begin.p, wgb.data(), wgb.size() * 4, hipMemcpyHostToDevice, s));
+    // 12. row masks and per-wave slot bases of the combine kernel
+    const uint32_t nnh = nrows - std::min(n_hub, nrows);
+    tp->nwaves = ceil_div(nnh, 64);
+    tp->row_blocks = ceil_div(tp->nwaves, kCombBlock / 64);
+    tp->rmask.alloc((size_t)std::max<uint32_t>(nnh, 1) * 4);
+    AG_HIP(hipMemsetAsync(tp->rmask.p, 0, (size_t)std::max<uint32_t>(nnh, 1) * 4, s));
+    hipLaunchKernelGGL(k_tab_masks, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row.as<uint32_t>(),
+                       sub_tab.as<uint32_t>(), S, n_hub, tp->rmask.as<uint32_t>());
+    tp->rbase.alloc((size_t)std::max<uint32_t>(tp->nwaves * NT, 1) * 4);
+    if (tp->nwaves)
+        hipLaunchKernelGGL(k_tab_bases, dim3(ceil_div((uint64_t)tp->nwaves * NT, 256)), dim3(256), 0,
+                           s, sub_row.as<uint32_t>(), sbd.as<uint32_t>(), NT, tp->nwaves, n_hub,
+                           tp->rbase.as<uint32_t>());
+    AG_HIP(hipGetLastError());
+    tp->tab_edges.assign(NT, 0);
+    {
+        std::vector<uint32_t> st(NT + 1);
+        for (uint32_t t = 0; t <= NT; t++)
+            dl(&st[t], tp->sub_start.as<uint32_t>() + tp->SB[t], 4);
+        AG_HIP(hipStreamSynchronize(s));
+        for (uint32_t t = 0; t < NT; t++) tp->tab_edges[t] = (uint64_t)(st[t + 1] - st[t]) * 4;
+    }
+    AG_HIP(hipStreamSynchronize(s));
+    p->tab = std::move(tp);
 }
 
 Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms)
@@ -963,6 +1848,22 @@ static uint32_t sweep_grid(const ag_ctx *ctx, const Plan *p)
     const uint32_t per_cu =
         std::min<uint32_t>(2, std::max<uint32_t>(1, kLdsBytes / (p->hot_bytes + kStaticLds)));
     return std::max<uint32_t>(1, std::min<uint32_t>(p->nvblocks, (uint32_t)ctx->num_cus * per_cu));
+}
+
+// Combine-kernel blocks of one sweep: the row blocks, plus the zero-in-degree
+// rows' blocks unless those rows are skipped (at least one block).
+static uint32_t tab_comb_blocks(const Plan *p, bool skip_zero)
+{
+    const uint32_t z = skip_zero ? 0u : ceil_div(p->N - p->n_nonzero, kCombBlock);
+    return std::max<uint32_t>(1, p->tab->row_blocks + z);
+}
+
+// Residual-partial slots per sweep: the row-layout sweep grid, or the
+// combine kernel's blocks with the partitioned tables.
+static uint32_t part_slots(const ag_ctx *ctx, const Plan *p)
+{
+    const uint32_t n = sweep_grid(ctx, p);
+    return p->tab ? std::max(n, tab_comb_blocks(p, false)) : n;
 }
 
 // Per-call device buffers of the sweep loop, as raw pointers: either the
@@ -986,12 +1887,14 @@ static void ensure_work(ag_ctx *ctx, Plan *p, uint32_t max_iter)
     if (!w.pr.p) {
         const uint32_t grid = sweep_grid(ctx, p);
         w.grid = grid;
+        // one more slot than vertices: the zero the cold table's padding reads
         w.pr.alloc((size_t)N * 4);
-        w.c0.alloc((size_t)N * 4);
-        w.c1.alloc((size_t)N * 4);
-        w.blk_err.alloc((size_t)grid * 8);
-        w.blk_dang.alloc((size_t)grid * 8);
-        w.blk_act.alloc((size_t)grid * 4);
+        w.c0.alloc(((size_t)N + 1) * 4);
+        w.c1.alloc(((size_t)N + 1) * 4);
+        const uint32_t parts = part_slots(ctx, p);
+        w.blk_err.alloc((size_t)parts * 8);
+        w.blk_dang.alloc((size_t)parts * 8);
+        w.blk_act.alloc((size_t)parts * 4);
         w.state.alloc(sizeof(SweepState));
         w.sum_part.alloc((size_t)(ceil_div(N, kBlock) + 1) * 8 + 8);
         w.out.alloc((size_t)N * 4);
@@ -1070,6 +1973,34 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
     a.chunk_first = p->chunk_first.as<uint32_t>();
     a.hub_partial = p->hub_partial.as<float>();
     a.nparts = w.grid;
+    if (p->tab) {
+        // partitioned tables: hub rows are summed from their table chunks
+        const TabPlan &tp = *p->tab;
+        a.tab_T = tp.T;
+        a.tab_nt = tp.nt;
+        a.tab_NT = tp.NT;
+        a.tab_cold_base4 = tp.cold_base4;
+        a.tab_nwaves = tp.nwaves;
+        a.tab_lane_max4 = p->lane_max / 4;
+        a.tab_cold_unit0 = tp.cold_unit0;
+        a.tab_nunits = tp.nunits;
+        a.tab_idx16 = tp.idx16.as<uint16_t>();
+        a.tab_idx32 = tp.idx32.as<uint32_t>();
+        a.tab_sub_start = tp.sub_start.as<uint32_t>();
+        a.tab_part = tp.part.as<float>();
+        a.tab_units = tp.units.as<uint4>();
+        a.tab_wg_begin = tp.wg_begin.as<uint32_t>();
+        a.tab_items = tp.items.as<uint4>();
+        a.tab_hub_items = tp.hub_items.as<uint4>();
+        a.tab_rmask = tp.rmask.as<uint32_t>();
+        a.tab_rbase = tp.rbase.as<uint32_t>();
+        a.chunk_first = tp.chunk_first.as<uint32_t>();
+        a.hub_partial = tp.hub_partial.as<float>();
+        a.hub_chunk = tp.hub_chunk;
+        a.hub_chunks = tp.hub_chunks;
+        a.tab_row_blocks = tp.row_blocks;
+        a.nparts = tab_comb_blocks(p, a.skip_zero != 0);
+    }
     a.hot_bytes = p->hot_bytes;
     a.pr = w.pr;
     a.c0 = w.c0;
@@ -1084,28 +2015,61 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
     return a;
 }
 
-// Enqueues sweeps [it0, it1): the sweep kernel(s) of each sweep's storage
-// format, then the finalize kernel.  `sev` (two events per sweep, or null)
-// brackets the sweep kernels alone.
-static void enqueue_sweeps(hipStream_t s, const SweepArgs &a, const ag_pr_options &o,
-                           uint32_t grid, uint32_t hot_bytes, uint32_t it0, uint32_t it1,
+// Enqueues one sweep's gather kernels: with the row layout the sweep kernel
+// of each storage format the run can read (a mixed-precision run launches the
+// low-precision and the FP32 kernel; the one whose format does not match the
+// sweep exits at once); with the partitioned tables the table kernel(s), then
+// the combine kernel over a.nparts blocks (its residual partials).  `grid` is
+// the row-layout sweep grid.
+static void enqueue_gather(hipStream_t s, const Plan *p, const SweepArgs &a, uint32_t mode,
+                           uint32_t precision, uint32_t flags, uint32_t grid, uint32_t it)
+{
+    const bool dd = mode == AG_PR_DATA_DRIVEN;
+    const bool fp32 = !dd && (precision == AG_PREC_FP32 || (flags & AG_PR_FLAG_FINISH_FP32));
+    if (p->tab) {
+        const TabPlan &tp = *p->tab;
+        const uint32_t lds = (tp.T + 1) * 4;
+        if (dd)
+            hipLaunchKernelGGL((k_pr_tables<float, true>), dim3(tp.grid), dim3(kSweepBlock), lds, s,
+                               a, it);
+        else if (precision == AG_PREC_BF16)
+            hipLaunchKernelGGL((k_pr_tables<bf16_t, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
+                               s, a, it);
+        else if (precision == AG_PREC_FP16)
+            hipLaunchKernelGGL((k_pr_tables<__half, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
+                               s, a, it);
+        if (fp32)
+            hipLaunchKernelGGL((k_pr_tables<float, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
+                               s, a, it);
+        if (dd)
+            hipLaunchKernelGGL(k_pr_combine<true>, dim3(a.nparts), dim3(kCombBlock), 0, s, a, it);
+        else
+            hipLaunchKernelGGL(k_pr_combine<false>, dim3(a.nparts), dim3(kCombBlock), 0, s, a, it);
+        return;
+    }
+    if (dd)
+        hipLaunchKernelGGL((k_pr_sweep<float, true>), dim3(grid), dim3(kSweepBlock), p->hot_bytes,
+                           s, a, it);
+    else if (precision == AG_PREC_BF16)
+        hipLaunchKernelGGL((k_pr_sweep<bf16_t, false>), dim3(grid), dim3(kSweepBlock),
+                           p->hot_bytes, s, a, it);
+    else if (precision == AG_PREC_FP16)
+        hipLaunchKernelGGL((k_pr_sweep<__half, false>), dim3(grid), dim3(kSweepBlock),
+                           p->hot_bytes, s, a, it);
+    if (fp32)
+        hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(grid), dim3(kSweepBlock), p->hot_bytes,
+                           s, a, it);
+}
+
+// Enqueues sweeps [it0, it1): the gather kernels, then the finalize kernel.
+// `sev` (two events per sweep, or null) brackets the gather kernels.
+static void enqueue_sweeps(hipStream_t s, const Plan *p, const SweepArgs &a,
+                           const ag_pr_options &o, uint32_t grid, uint32_t it0, uint32_t it1,
                            hipEvent_t *sev)
 {
-    const bool dd = o.mode == AG_PR_DATA_DRIVEN;
     for (uint32_t it = it0; it < it1; it++) {
         if (sev) AG_HIP(hipEventRecord(sev[2 * it], s));
-        if (dd)
-            hipLaunchKernelGGL((k_pr_sweep<float, true>), dim3(grid), dim3(kSweepBlock), hot_bytes,
-                               s, a, it);
-        else if (o.precision == AG_PREC_BF16)
-            hipLaunchKernelGGL((k_pr_sweep<bf16_t, false>), dim3(grid), dim3(kSweepBlock),
-                               hot_bytes, s, a, it);
-        else if (o.precision == AG_PREC_FP16)
-            hipLaunchKernelGGL((k_pr_sweep<__half, false>), dim3(grid), dim3(kSweepBlock),
-                               hot_bytes, s, a, it);
-        if (!dd && (o.precision == AG_PREC_FP32 || (o.flags & AG_PR_FLAG_FINISH_FP32)))
-            hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(grid), dim3(kSweepBlock),
-                               hot_bytes, s, a, it);
+        enqueue_gather(s, p, a, o.mode, o.precision, o.flags, grid, it);
         if (sev) AG_HIP(hipEventRecord(sev[2 * it + 1], s));
         hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
     }
@@ -1241,7 +2205,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     double *parts = w.sum_part;
 
     auto enqueue_init = [&] {
-        hipLaunchKernelGGL(k_pr_init, dim3(ceil_div((uint64_t)std::max(N, max_iter + 2), 256)),
+        hipLaunchKernelGGL(k_pr_init, dim3(ceil_div((uint64_t)std::max(N + 1, max_iter + 2), 256)),
                            dim3(256), 0, s, a, x0, tele0, o.precision);
         AG_HIP(hipGetLastError());
     };
@@ -1283,7 +2247,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         // one replayable graph: init, every sweep, finish
         run_captured(p, s, seg_key(0), [&] {
             enqueue_init();
-            enqueue_sweeps(s, a, o, grid, p->hot_bytes, 0, max_iter, nullptr);
+            enqueue_sweeps(s, p, a, o, grid, 0, max_iter, nullptr);
             enqueue_finish();
         });
         if (ev[1]) AG_HIP(hipEventRecord(ev[1], s));
@@ -1296,10 +2260,10 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
             if (use_graph) {
                 run_captured(p, s, seg_key(1 + it0 / group), [&] {
                     if (it0 == 0) enqueue_init();
-                    enqueue_sweeps(s, a, o, grid, p->hot_bytes, it0, it1, nullptr);
+                    enqueue_sweeps(s, p, a, o, grid, it0, it1, nullptr);
                 });
             } else {
-                enqueue_sweeps(s, a, o, grid, p->hot_bytes, it0, it1, timed ? sev.data() : nullptr);
+                enqueue_sweeps(s, p, a, o, grid, it0, it1, timed ? sev.data() : nullptr);
             }
             if (can_stop && it1 < max_iter) {
                 uint32_t done = 0;
@@ -1397,16 +2361,16 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
     SweepArgs a = make_args(p, w, o, iterations);
     a.c1 = a.c0; // every sweep reads the same input
     a.skip_zero = 0;
+    if (p->tab) a.nparts = tab_comb_blocks(p, false);
     hipEvent_t ev[4];
     for (auto &e : ev) AG_HIP(hipEventCreate(&e));
     struct EvGuard { hipEvent_t *e; ~EvGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } eg{ev};
     AG_HIP(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_spmv_init, dim3(ceil_div((uint64_t)std::max(N, iterations + 2), 256)),
+    hipLaunchKernelGGL(k_spmv_init, dim3(ceil_div((uint64_t)std::max(N + 1, iterations + 2), 256)),
                        dim3(256), 0, s, a, d_x, p->order.as<uint32_t>());
     AG_HIP(hipEventRecord(ev[1], s));
     for (uint32_t it = 0; it < iterations; it++) {
-        hipLaunchKernelGGL((k_pr_sweep<float, false>), dim3(w.grid), dim3(kSweepBlock),
-                           p->hot_bytes, s, a, it);
+        enqueue_gather(s, p, a, kModeSpmv, AG_PREC_FP32, 0u, w.grid, it);
         hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
     }
     AG_HIP(hipGetLastError());
@@ -1475,9 +2439,10 @@ static unsigned char *noalloc_scratch_ptr(int device)
 static WorkPtrs noalloc_work(ag_ctx *ctx, const Plan *p, uint32_t max_iter)
 {
     const uint32_t grid = sweep_grid(ctx, p);
+    const size_t parts = part_slots(ctx, p);
     const size_t N = p->N;
-    const size_t sizes[] = {N * 4, N * 4, N * 4, (size_t)grid * 8, (size_t)grid * 8,
-                            (size_t)grid * 4, sizeof(SweepState), ((size_t)max_iter + 1) * 8,
+    const size_t sizes[] = {N * 4, (N + 1) * 4, (N + 1) * 4, parts * 8, parts * 8,
+                            parts * 4, sizeof(SweepState), ((size_t)max_iter + 1) * 8,
                             ((size_t)max_iter + 1) * 4, ((size_t)max_iter + 3) * 4,
                             ((size_t)ceil_div(N, kBlock) + 1) * 8 + 8};
     size_t need = 0;
@@ -1553,6 +2518,15 @@ int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info)
         info->hub_min = p->hub_min;
         info->sell_slots = p->sell_slots;
         info->plan_bytes = p->bytes();
+        info->hub_chunk = p->tab ? p->tab->hub_chunk : p->hub_chunk;
+        info->layout = p->tab ? AG_LAYOUT_TABLES : AG_LAYOUT_ROWS;
+        if (p->tab) {
+            info->hub_chunks = p->tab->hub_chunks;
+            info->tables = p->tab->nt;
+            info->table_workgroups = p->tab->grid;
+            info->subrows = p->tab->S;
+            info->cold_edges = p->tab->tab_edges.empty() ? 0 : p->tab->tab_edges.back();
+        }
     });
 }
 

@@ -28,6 +28,54 @@ struct LoopGraph {
     hipGraphExec_t exec = nullptr;
 };
 
+// Partitioned-table gather layout, the pull sweep's default (ag_pagerank.hip).
+// Source labels are cut into tables of T labels; the sweep's workgroups each
+// hold one table's contributions in LDS and sum, for every row, the row's
+// in-edges from that table (a "sub-row"), so that nearly every gather is an
+// LDS read.  Sources past the last LDS table form the cold table, gathered
+// from global memory.  A combine kernel then sums each row's sub-row partials
+// in table order and updates the vertex; hub rows' sub-rows are cut into
+// chunks that the finalize kernel sums.
+//
+// Sub-rows are numbered table by table, rows ascending inside a table (the
+// hub rows' sub-rows first).  A sub-row's source indices are contiguous and
+// padded with a sentinel to a multiple of 4: u16 table-local indices for LDS
+// tables (sentinel T reads the zero after the table), u32 labels for the cold
+// table (sentinel N reads the zero kept after the contribution vector).
+struct TabPlan {
+    uint32_t T = 0;          // sources per LDS table
+    uint32_t nt = 0;         // LDS tables; table nt is the cold table
+    uint32_t NT = 0;         // nt + 1
+    uint32_t S = 0;          // sub-rows
+    uint32_t cold_base4 = 0; // sub_start value at which the cold table's indices begin
+    uint32_t nunits = 0;     // work units of the table kernel
+    uint32_t cold_unit0 = 0; // units [cold_unit0, nunits) belong to the cold table
+    uint32_t grid = 0;       // workgroups of the table kernel (one per CU)
+    uint32_t hub_chunk = 0;  // edges per hub-row chunk (multiple of 4)
+    uint32_t hub_chunks = 0;
+    uint32_t nwaves = 0;     // 64-row waves of the combine kernel over the non-hub rows
+    uint32_t row_blocks = 0; // combine blocks over those waves
+    uint64_t idx16_len = 0, idx32_len = 0;
+    std::vector<uint32_t> SB;         // first sub-row of each table, SB[NT] = S (host copy)
+    std::vector<uint64_t> tab_edges;  // edges per table (host copy)
+    // idx16 / idx32: source indices; sub_start[S + 1]: start of each sub-row's
+    // indices in units of 4 entries; part[S]: sub-row partial sums; units,
+    // wg_begin: the table kernel's work units and each workgroup's range;
+    // items: sub-rows sorted by (table, class), the long ones' lists;
+    // hub_items: (sub-row, chunk, output slot) per hub chunk; chunk_first,
+    // hub_partial: hub rows' chunk slots, in (table, chunk) order per row;
+    // rmask: tables each non-hub row has a sub-row in; rbase[nwaves * NT]:
+    // first sub-row of each table at or after each 64-row wave's first row.
+    DevBuf idx16, idx32, sub_start, part, units, wg_begin, items, hub_items, chunk_first,
+        hub_partial, rmask, rbase;
+    uint64_t bytes() const
+    {
+        return idx16.bytes + idx32.bytes + sub_start.bytes + part.bytes + units.bytes +
+               wg_begin.bytes + items.bytes + hub_items.bytes + chunk_first.bytes +
+               hub_partial.bytes + rmask.bytes + rbase.bytes;
+    }
+};
+
 // Buffers of a personalized-PageRank batch that a finished batch leaves to
 // the next one on the same graph (see PprBatch).
 constexpr int kKeepBufs = 11;
@@ -36,7 +84,7 @@ struct Plan {
     uint32_t N = 0, E = 0;
     uint32_t n_hub = 0, n_w = 0, n_h = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
     uint32_t low_slices = 0, hub_chunks = 0;
-    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0, hot_bytes = 0;
+    uint32_t low_max = 0, hub_min = 0, hub_chunk = 0, hot_bytes = 0, lane_max = 16;
     uint64_t csr_edges = 0, sell_slots = 0;
     // order[i] = vertex with label i; rank[v] = label of vertex v
     DevBuf order, rank, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial;
@@ -56,6 +104,8 @@ struct Plan {
     // captured sweep loops, most recent last (at most kLoopGraphs)
     static constexpr size_t kLoopGraphs = 8;
     std::vector<LoopGraph> loop_graphs;
+    // partitioned-table layout (null: the row layout above serves the sweep)
+    std::unique_ptr<TabPlan> tab;
     std::mutex mu;
     Plan() = default;
     Plan(const Plan &) = delete;
@@ -71,7 +121,7 @@ struct Plan {
     {
         return order.bytes + rank.bytes + roff.bytes + cols.bytes + slice_ptr.bytes + outdeg.bytes +
                chunk_row.bytes + chunk_first.bytes + hub_partial.bytes + ppr_ch_row.bytes +
-               ppr_ch_first.bytes + ppr_multi.bytes + keep_bytes();
+               ppr_ch_first.bytes + ppr_multi.bytes + keep_bytes() + (tab ? tab->bytes() : 0);
     }
     uint64_t keep_bytes() const
     {

@@ -142,6 +142,14 @@ int ag_set_tuning(const ag_tuning *t)
         if (t->hub_min) n.hub_min = t->hub_min;
         if (t->hub_chunk) n.hub_chunk = t->hub_chunk;
         if (t->hot_bytes) n.hot_bytes = t->hot_bytes == 1 ? 0u : t->hot_bytes;
+        if (t->layout) n.layout = t->layout;
+        if (t->max_tables) n.max_tables = t->max_tables;
+        if (t->lane_max) n.lane_max = t->lane_max;
+        if (n.layout != AG_LAYOUT_TABLES && n.layout != AG_LAYOUT_ROWS && n.layout != AG_LAYOUT_AUTO)
+            fail(AG_ERR_INVALID, "layout must be AG_LAYOUT_AUTO, AG_LAYOUT_TABLES or AG_LAYOUT_ROWS");
+        if (n.max_tables < 1 || n.max_tables > 31) fail(AG_ERR_INVALID, "max_tables must be 1..31");
+        if (n.lane_max < 4 || n.lane_max > 64 || n.lane_max % 4)
+            fail(AG_ERR_INVALID, "lane_max must be a multiple of 4 in 4..64");
         if (n.hot_bytes % 16 || n.hot_bytes > 160 * 1024 - 2048)
             fail(AG_ERR_INVALID, "hot_bytes must be a multiple of 16 and <= 161792");
         if (n.low_max >= n.hub_min) fail(AG_ERR_INVALID, "low_max must be < hub_min");
@@ -158,6 +166,9 @@ int ag_get_tuning(ag_tuning *t)
         t->hub_min = g_tuning.hub_min;
         t->hub_chunk = g_tuning.hub_chunk;
         t->hot_bytes = g_tuning.hot_bytes;
+        t->layout = g_tuning.layout;
+        t->max_tables = g_tuning.max_tables;
+        t->lane_max = g_tuning.lane_max;
     });
 }
 

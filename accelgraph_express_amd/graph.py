@@ -328,19 +328,27 @@ def csr_file_info(path: str) -> dict:
     return {"num_vertices": n.value, "num_edges": e.value, "sections": m.value}
 
 
+LAYOUTS = {"tables": L.AG_LAYOUT_TABLES, "rows": L.AG_LAYOUT_ROWS, "auto": L.AG_LAYOUT_AUTO}
+
+
 def set_tuning(low_max: int = 0, hub_min: int = 0, hub_chunk: int = 0,
-               hot_bytes: int = 0) -> None:
+               hot_bytes: int = 0, layout=0, max_tables: int = 0, lane_max: int = 0) -> None:
     """0 keeps a knob; hot_bytes=1 disables the LDS table of hot contributions;
-    hub_chunk=0xFFFFFFFF (AG_TUNE_AUTO, the default) picks it per graph."""
-    t = L.Tuning(low_max, hub_min, hub_chunk, hot_bytes)
+    hub_chunk=0xFFFFFFFF (AG_TUNE_AUTO, the default) picks it per graph;
+    layout "auto" (default: tables when the sources outnumber one LDS table),
+    "tables" or "rows".  Applies to gather layouts built later."""
+    lay = LAYOUTS[layout] if isinstance(layout, str) else int(layout)
+    t = L.Tuning(low_max, hub_min, hub_chunk, hot_bytes, lay, max_tables, lane_max)
     L.check(L.load().ag_set_tuning(C.byref(t)))
 
 
 def get_tuning() -> dict:
     t = L.Tuning()
     L.check(L.load().ag_get_tuning(C.byref(t)))
+    lay = {v: k for k, v in LAYOUTS.items()}.get(t.layout, t.layout)
     return {"low_max": t.low_max, "hub_min": t.hub_min, "hub_chunk": t.hub_chunk,
-            "hot_bytes": t.hot_bytes}
+            "hot_bytes": t.hot_bytes, "layout": lay, "max_tables": t.max_tables,
+            "lane_max": t.lane_max}
 
 
 # ---- synthetic inputs ----

@@ -267,6 +267,14 @@ typedef struct ag_plan_info
     uint32_t low_max, hub_min;            /* bin thresholds (in-degree) */
     uint64_t sell_slots;                  /* padded slots in the low-degree slices */
     uint64_t plan_bytes;
+    uint32_t hub_chunk;                   /* edges per hub chunk chosen for this graph (the
+                                             hub rows' summation order depends on it) */
+    uint32_t layout;                      /* AG_LAYOUT_TABLES or AG_LAYOUT_ROWS */
+    uint32_t tables;                      /* partitioned layout: LDS tables (cold excluded) */
+    uint32_t table_workgroups;            /* partitioned layout: table-kernel workgroups */
+    uint64_t subrows;                     /* partitioned layout: (row, table) sub-rows */
+    uint64_t cold_edges;                  /* partitioned layout: edges gathered from global
+                                             memory (sources past the last LDS table) */
 } ag_plan_info;
 int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info);
 
@@ -280,8 +288,23 @@ typedef struct ag_tuning
                              compute unit, else 2,048 */
     uint32_t hot_bytes;   /* LDS bytes per sweep workgroup caching the hottest
                              contributions (multiple of 16; 1 disables the table) */
+    uint32_t layout;      /* gather layout of the pull sweep: AG_LAYOUT_AUTO (default),
+                             AG_LAYOUT_TABLES or AG_LAYOUT_ROWS; 0 keeps the current one */
+    uint32_t max_tables;  /* partitioned layout: at most this many LDS tables (1..31) */
+    uint32_t lane_max;    /* partitioned layout: sub-rows of up to this many edges are
+                             summed by one lane (multiple of 4, 4..64; default 16) */
 } ag_tuning;
 #define AG_TUNE_AUTO 0xFFFFFFFFu
+/* Partitioned tables: source labels are cut into LDS-sized tables (40,440
+ * sources each), every workgroup of the sweep holds one table and sums each
+ * row's in-edges from it; a combine kernel adds each row's partial sums in
+ * table order.  Rows: the degree-binned row layout with one LDS table of the
+ * hottest sources (its low-degree rows sum in the reference's edge order,
+ * bit-exact).  Auto: tables when the sources outnumber one table, else rows
+ * (whose LDS table then holds every source). */
+#define AG_LAYOUT_TABLES 1u
+#define AG_LAYOUT_ROWS   2u
+#define AG_LAYOUT_AUTO   3u
 int ag_set_tuning(const ag_tuning *t);
 int ag_get_tuning(ag_tuning *t);
 

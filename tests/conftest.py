@@ -20,3 +20,21 @@ def ctx():
     c = ag.Context(0)
     yield c
     c.close()
+
+
+class tuned:
+    """Context manager: library tuning knobs (ag.set_tuning) for gather
+    layouts built inside the block, restored afterwards."""
+
+    def __init__(self, **kw):
+        self.kw = kw
+
+    def __enter__(self):
+        import accelgraph_express_amd as ag
+        self.saved = ag.get_tuning()
+        ag.set_tuning(**self.kw)
+        return self
+
+    def __exit__(self, *exc):
+        import accelgraph_express_amd as ag
+        ag.set_tuning(**self.saved)

@@ -21,6 +21,7 @@ import pytest
 
 import accelgraph_express_amd as ag
 import oracle
+from conftest import tuned
 
 pytestmark = pytest.mark.gpu
 REL_TOL = 1e-5
@@ -95,7 +96,10 @@ def test_skewed_bins(ctx, eps):
     # early sweeps (every |delta| far from epsilon) agree exactly
     np.testing.assert_array_equal(st.active_per_iter[:10], act[:10])
     resid = _fixpoint_residual(src, dst, n, st.pageRanks)
-    assert resid.max() < eps * (1 + 1e-3) + 1e-6
+    # the stop is an epsilon-fixpoint in the GPU's FP32 arithmetic: allow 8 ulp
+    # of the largest un-normalised rank for the FP64 recomputation
+    ulp = 8 * 2.0 ** -24 * float(st.pageRanks.max()) * n
+    assert resid.max() < eps * (1 + 1e-3) + ulp
     l1 = np.abs(st.pageRanks.astype(np.float64) - pr).sum()
     print(f"sweeps gpu {st.iterations} oracle {it}; L1(gpu - oracle) {l1:.3e} "
           f"(bound {2 * eps / 0.15:.3e}); max residual {resid.max():.3e}")
@@ -104,10 +108,13 @@ def test_skewed_bins(ctx, eps):
 
 def test_c1_uniform_default_epsilon(ctx):
     """The CPU-runnable workload (65,536 vertices, 1,048,576 uniform edges) at
-    the reference CLI's epsilon 1e-4."""
+    the reference CLI's epsilon 1e-4, on the row layout, whose low rows sum in
+    the reference's edge order (the partitioned layout's C1 case is in
+    test_gpu_tables.py)."""
     src, dst = ag.gen_uniform(27491095, 65536, 1048576)
-    g = ag.Graph.build(ctx, src, dst, 65536)
-    g.prepare()
+    with tuned(layout="rows"):
+        g = ag.Graph.build(ctx, src, dst, 65536)
+        g.prepare()
     info = g.plan_info()
     assert info["n_mid"] == 0 and info["n_hub"] == 0  # in-degree ~ Poisson(16): all low rows
     csr = oracle.build_csr(src, dst, 65536)
