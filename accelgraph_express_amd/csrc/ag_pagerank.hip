@@ -218,7 +218,7 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
         if (flag & kDdFlag) {
             const float nw = a.base + a.damp * s;
             const double e = fabs((double)(nw - old));
-            err += e / (double)a.N;
+            err += e; // divided by N once, in k_pr_finalize
             if (e >= a.epsilon) {
                 a.pr[i] = nw;
                 keep = nw;
@@ -256,7 +256,7 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
     const uint32_t od = a.outdeg[i];
     if (a.mode == AG_PR_REFERENCE) {
         const double e = fabs((double)(nw - old));
-        err += e / (double)a.N;
+        err += e; // divided by N once, in k_pr_finalize
         act += (e >= a.epsilon) ? 1u : 0u;
     } else {
         err += fabs((double)nw - (double)old);
@@ -867,20 +867,43 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     double err = 0.0, dang = 0.0;
     uint32_t act = 0;
     const bool dd = a.mode == AG_PR_DATA_DRIVEN;
+    // loads in batches of kFinBatch, added in order (one memory round trip
+    // per batch instead of one per value)
+    constexpr uint32_t kFinBatch = 8;
     for (uint32_t row = threadIdx.x; row < a.n_hub; row += kFinBlock) {
         const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
         float sum = 0.0f;
         uint32_t fl = (dd && it == 0) ? kDdFlag : 0u;
-        if (dd)
-            for (uint32_t q = f0; q < f1; q++) acc<true>(sum, fl, a.hub_partial[q]);
-        else
-            for (uint32_t q = f0; q < f1; q++) sum += a.hub_partial[q];
+        for (uint32_t q0 = f0; q0 < f1; q0 += kFinBatch) {
+            float v[kFinBatch];
+#pragma unroll
+            for (uint32_t k = 0; k < kFinBatch; k++) v[k] = q0 + k < f1 ? a.hub_partial[q0 + k] : 0.0f;
+#pragma unroll
+            for (uint32_t k = 0; k < kFinBatch; k++) {
+                if (q0 + k >= f1) break;
+                if (dd) acc<true>(sum, fl, v[k]);
+                else sum += v[k];
+            }
+        }
         update_vertex(a, row, sum * rscale, tele, cout, wfmt, err, act, dang, fl);
     }
-    for (uint32_t b = threadIdx.x; b < a.nparts; b += kFinBlock) {
-        err += a.blk_err[b];
-        dang += a.blk_dang[b];
-        act += a.blk_act[b];
+    for (uint32_t b0 = threadIdx.x; b0 < a.nparts; b0 += kFinBatch * kFinBlock) {
+        double e[kFinBatch], d[kFinBatch];
+        uint32_t c[kFinBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFinBatch; k++) {
+            const uint32_t b = b0 + k * kFinBlock;
+            e[k] = b < a.nparts ? a.blk_err[b] : 0.0;
+            d[k] = b < a.nparts ? a.blk_dang[b] : 0.0;
+            c[k] = b < a.nparts ? a.blk_act[b] : 0u;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFinBatch; k++) {
+            if (b0 + k * kFinBlock >= a.nparts) break;
+            err += e[k];
+            dang += d[k];
+            act += c[k];
+        }
     }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     err = wave_sum(err);
@@ -892,6 +915,9 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     double E_ = 0.0, D_ = 0.0;
     uint32_t A_ = 0;
     for (int i = 0; i < kFinBlock / 64; i++) { E_ += s_err[i]; D_ += s_dang[i]; A_ += s_act[i]; }
+    // REFERENCE and DATA_DRIVEN: error_total is the mean |delta|
+    // (pageRank.c:1062-1063); the sum is divided by N once here
+    if (a.mode == AG_PR_REFERENCE || dd) E_ = E_ / (double)a.N;
     if (a.skip_zero && a.epsilon <= 0.0) A_ += a.N - a.n_nonzero;
     a.err_iter[it] = E_;
     a.act_iter[it] = A_;
