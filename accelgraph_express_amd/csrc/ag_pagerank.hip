@@ -133,8 +133,7 @@ struct SweepArgs {
     const uint32_t *tab_wg_begin;
     const uint4 *tab_items;
     const uint4 *tab_hub_items;
-    const uint32_t *tab_rmask;
-    const uint32_t *tab_rbase;
+    const uint32_t *tab_wtab;
 };
 
 namespace {
@@ -780,14 +779,18 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
 }
 
 // k_pr_combine: one lane per non-hub row with in-edges (labels n_hub ..
-// n_nonzero), one 64-row wave per hardware wave (full occupancy hides the
-// two dependent loads): the row's sub-row partials are added in table order
-// (table t's slot: the wave's base for t plus the count of lower lanes with a
-// sub-row in t), then the vertex is updated as in the row layout.  Blocks past
+// n_nonzero), one 64-row wave per hardware wave.  The wave's record for table
+// t (the lanes with a sub-row in t, and the slot of the first of them) is a
+// scalar load; a lane's slot is that base plus the count of lower lanes in
+// the mask.  The row's sub-row partials are added in table order, then the
+// vertex is updated as in the row layout.  Blocks past
 // the row blocks update the zero-in-degree rows (unless skipped); hub rows
 // are the finalize kernel's.  Each block writes its residual partials.
 constexpr int kCombBlock = 256;
-template <bool DD>
+// Combine records read through the constant address space: uniform loads of
+// memory the kernel never writes, so they become scalar loads.
+typedef __attribute__((address_space(4))) const uint32_t ConstU32;
+template <bool DD, uint32_t NTP>
 __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t it)
 {
     __shared__ double s_err[kCombBlock / 64], s_dang[kCombBlock / 64];
@@ -802,31 +805,33 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
     double err = 0.0, dang = 0.0;
     uint32_t act = 0;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->ticket = 0; // the next sweep's cold units
     if (blockIdx.x < a.tab_row_blocks) {
-        const uint32_t wv = blockIdx.x * (kCombBlock / 64) + w;
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(blockIdx.x * (kCombBlock / 64) + w);
         if (wv < a.tab_nwaves) {
-            const uint64_t below = (1ull << lane) - 1ull;
             const uint32_t r = a.n_hub + wv * 64 + lane;
-            const bool valid = r < a.n_nonzero;
-            const uint32_t m = valid ? a.tab_rmask[r - a.n_hub] : 0u;
-            const uint32_t *rb = a.tab_rbase + (size_t)wv * a.tab_NT;
-            // every partial load is issued before the first add
-            float pv[kMaxTabNT];
+            // the wave's record: NTP lane masks (two words each), NTP slot bases
+            const ConstU32 *rec = (const ConstU32 *)(a.tab_wtab + (size_t)wv * (3 * NTP));
+            uint64_t m[NTP];
 #pragma unroll
-            for (uint32_t t = 0; t < kMaxTabNT; t++) {
-                pv[t] = 0.0f;
-                if (t < a.tab_NT) {
-                    const uint64_t b = __ballot((m >> t) & 1u);
-                    if ((m >> t) & 1u) pv[t] = a.tab_part[rb[t] + (uint32_t)__popcll(b & below)];
-                }
+            for (uint32_t t = 0; t < NTP; t++) m[t] = ((uint64_t)rec[2 * t + 1] << 32) | rec[2 * t];
+            // every partial load is issued before the first add; a lane
+            // without a sub-row in table t reads the slot after the previous
+            // lane's (inside the table's range, or the slack slot after the
+            // last table) and takes 0 instead
+            float pv[NTP];
+#pragma unroll
+            for (uint32_t t = 0; t < NTP; t++) {
+                const uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[t] >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m[t], 0u));
+                const char *pt = reinterpret_cast<const char *>(a.tab_part + rec[2 * NTP + t]);
+                pv[t] = *reinterpret_cast<const float *>(pt + slot * 4u);
             }
             float s = 0.0f;
             uint32_t fl = dd0;
 #pragma unroll
-            for (uint32_t t = 0; t < kMaxTabNT; t++)
-                if (t < a.tab_NT && ((m >> t) & 1u)) acc<DD>(s, fl, pv[t]);
-            if (valid) update_vertex(a, r, s * rscale, tele, cout, wfmt, err, act, dang, fl);
+            for (uint32_t t = 0; t < NTP; t++)
+                acc<DD>(s, fl, __builtin_amdgcn_inverse_ballot_w64(m[t]) ? pv[t] : 0.0f);
+            if (r < a.n_nonzero) update_vertex(a, r, s * rscale, tele, cout, wfmt, err, act, dang, fl);
         }
     } else if (!a.skip_zero) {
         const uint32_t i = (blockIdx.x - a.tab_row_blocks) * kCombBlock + threadIdx.x;
@@ -845,6 +850,7 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
         a.blk_err[blockIdx.x] = e;
         a.blk_dang[blockIdx.x] = d;
         a.blk_act[blockIdx.x] = c;
+        if (blockIdx.x == 0) st->ticket = 0; // the next sweep's cold units
     }
 }
 
@@ -1316,21 +1322,25 @@ __global__ void k_tab_classes(const uint32_t *__restrict__ sub_row, const uint32
     val[sid] = sid;
 }
 
-// Table bits of every non-hub row.
-__global__ void k_tab_masks(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sub_tab,
-                            uint32_t S, uint32_t n_hub, uint32_t *__restrict__ rmask)
+// Lane bits of the combine records: the row of sub-row sid is lane
+// (row - n_hub) % 64 of wave (row - n_hub) / 64.
+__global__ void k_tab_wmask(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sub_tab,
+                            uint32_t S, uint32_t n_hub, uint32_t NTpad, uint32_t *__restrict__ wtab)
 {
     const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
     if (sid >= S) return;
     const uint32_t r = sub_row[sid];
-    if (r >= n_hub) atomicOr(&rmask[r - n_hub], 1u << sub_tab[sid]);
+    if (r < n_hub) return;
+    const uint32_t q = r - n_hub, lane = q & 63u;
+    uint32_t *rec = wtab + (size_t)(q >> 6) * 3u * NTpad;
+    atomicOr(&rec[2u * sub_tab[sid] + (lane >> 5)], 1u << (lane & 31u));
 }
 
-// rbase[w * NT + t] = first sub-row of table t whose row is >= the first row of
-// 64-row wave w (rows ascend inside a table).
+// Slot bases of the combine records: the first sub-row of table t whose row
+// is >= the first row of 64-row wave w (rows ascend inside a table).
 __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sb,
-                            uint32_t NT, uint32_t nwaves, uint32_t n_hub,
-                            uint32_t *__restrict__ rbase)
+                            uint32_t NT, uint32_t NTpad, uint32_t nwaves, uint32_t n_hub,
+                            uint32_t *__restrict__ wtab)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nwaves * NT) return;
@@ -1342,7 +1352,7 @@ __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t
         if (sub_row[mid] < row0) lo = mid + 1;
         else hi = mid;
     }
-    rbase[i] = lo;
+    wtab[(size_t)w * 3u * NTpad + 2u * NTpad + t] = lo;
 }
 
 // Cost of each work unit in padded index positions (lane units: per wave,
@@ -1799,7 +1809,9 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     tp->chunk_first.alloc(((size_t)n_hub + 1) * 4);
     AG_HIP(hipMemcpyAsync(tp->chunk_first.p, first.data(), first.size() * 4, hipMemcpyHostToDevice, s));
     tp->hub_partial.alloc(std::max<size_t>(tp->hub_chunks, 1) * 4);
-    tp->part.alloc(std::max<size_t>(S, 1) * 4);
+    // one slot of slack: the combine kernel's lanes without a sub-row in a
+    // table read (and discard) the slot after the wave's last one
+    tp->part.alloc(((size_t)S + 1) * 4);
     // 11. split the units over one workgroup per CU by their cost
     std::vector<uint32_t> cost(units.size());
     if (!units.empty()) {
@@ -1835,15 +1847,16 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     const uint32_t nnh = nrows - std::min(n_hub, nrows);
     tp->nwaves = ceil_div(nnh, 64);
     tp->row_blocks = ceil_div(tp->nwaves, kCombBlock / 64);
-    tp->rmask.alloc((size_t)std::max<uint32_t>(nnh, 1) * 4);
-    AG_HIP(hipMemsetAsync(tp->rmask.p, 0, (size_t)std::max<uint32_t>(nnh, 1) * 4, s));
-    hipLaunchKernelGGL(k_tab_masks, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row.as<uint32_t>(),
-                       sub_tab.as<uint32_t>(), S, n_hub, tp->rmask.as<uint32_t>());
-    tp->rbase.alloc((size_t)std::max<uint32_t>(tp->nwaves * NT, 1) * 4);
+    tp->NTpad = NT <= 4 ? 4u : NT <= 8 ? 8u : NT <= 17 ? 17u : kMaxTabNT;
+    const size_t wrec = ((size_t)tp->nwaves * 3 * tp->NTpad + 1) * 4;
+    tp->wtab.alloc(wrec);
+    AG_HIP(hipMemsetAsync(tp->wtab.p, 0, wrec, s));
+    hipLaunchKernelGGL(k_tab_wmask, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row.as<uint32_t>(),
+                       sub_tab.as<uint32_t>(), S, n_hub, tp->NTpad, tp->wtab.as<uint32_t>());
     if (tp->nwaves)
         hipLaunchKernelGGL(k_tab_bases, dim3(ceil_div((uint64_t)tp->nwaves * NT, 256)), dim3(256), 0,
-                           s, sub_row.as<uint32_t>(), sbd.as<uint32_t>(), NT, tp->nwaves, n_hub,
-                           tp->rbase.as<uint32_t>());
+                           s, sub_row.as<uint32_t>(), sbd.as<uint32_t>(), NT, tp->NTpad, tp->nwaves,
+                           n_hub, tp->wtab.as<uint32_t>());
     AG_HIP(hipGetLastError());
     tp->tab_edges.assign(NT, 0);
     {
@@ -2018,8 +2031,7 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
         a.tab_wg_begin = tp.wg_begin.as<uint32_t>();
         a.tab_items = tp.items.as<uint4>();
         a.tab_hub_items = tp.hub_items.as<uint4>();
-        a.tab_rmask = tp.rmask.as<uint32_t>();
-        a.tab_rbase = tp.rbase.as<uint32_t>();
+        a.tab_wtab = tp.wtab.as<uint32_t>();
         a.chunk_first = tp.chunk_first.as<uint32_t>();
         a.hub_partial = tp.hub_partial.as<float>();
         a.hub_chunk = tp.hub_chunk;
@@ -2067,10 +2079,17 @@ static void enqueue_gather(hipStream_t s, const Plan *p, const SweepArgs &a, uin
         if (fp32)
             hipLaunchKernelGGL((k_pr_tables<float, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
                                s, a, it);
-        if (dd)
-            hipLaunchKernelGGL(k_pr_combine<true>, dim3(a.nparts), dim3(kCombBlock), 0, s, a, it);
-        else
-            hipLaunchKernelGGL(k_pr_combine<false>, dim3(a.nparts), dim3(kCombBlock), 0, s, a, it);
+        // the combine kernel's table loops are unrolled to the records' NTpad
+        const void *kc;
+        switch (tp.NTpad) {
+        case 4: kc = dd ? (const void *)k_pr_combine<true, 4> : (const void *)k_pr_combine<false, 4>; break;
+        case 8: kc = dd ? (const void *)k_pr_combine<true, 8> : (const void *)k_pr_combine<false, 8>; break;
+        case 17: kc = dd ? (const void *)k_pr_combine<true, 17> : (const void *)k_pr_combine<false, 17>; break;
+        default:
+            kc = dd ? (const void *)k_pr_combine<true, kMaxTabNT> : (const void *)k_pr_combine<false, kMaxTabNT>;
+        }
+        void *kargs[] = {(void *)&a, (void *)&it};
+        AG_HIP(hipLaunchKernel(kc, dim3(a.nparts), dim3(kCombBlock), kargs, 0, s));
         return;
     }
     if (dd)

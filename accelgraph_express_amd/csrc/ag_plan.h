@@ -54,6 +54,7 @@ struct TabPlan {
     uint32_t hub_chunk = 0;  // edges per hub-row chunk (multiple of 4)
     uint32_t hub_chunks = 0;
     uint32_t nwaves = 0;     // 64-row waves of the combine kernel over the non-hub rows
+    uint32_t NTpad = 0;      // tables per combine record (NT rounded up to 4, 8, 17 or 32)
     uint32_t row_blocks = 0; // combine blocks over those waves
     uint64_t idx16_len = 0, idx32_len = 0;
     std::vector<uint32_t> SB;         // first sub-row of each table, SB[NT] = S (host copy)
@@ -64,15 +65,17 @@ struct TabPlan {
     // items: sub-rows sorted by (table, class), the long ones' lists;
     // hub_items: (sub-row, chunk, output slot) per hub chunk; chunk_first,
     // hub_partial: hub rows' chunk slots, in (table, chunk) order per row;
-    // rmask: tables each non-hub row has a sub-row in; rbase[nwaves * NT]:
-    // first sub-row of each table at or after each 64-row wave's first row.
+    // wtab: per 64-row wave of the non-hub rows, 3 * NTpad words: for each
+    // table the lanes (rows) with a sub-row in it (two words: lanes 0-31,
+    // 32-63), then for each table the first sub-row of the table at or after
+    // the wave's first row.  Tables NT .. NTpad - 1 have no lanes.
     DevBuf idx16, idx32, sub_start, part, units, wg_begin, items, hub_items, chunk_first,
-        hub_partial, rmask, rbase;
+        hub_partial, wtab;
     uint64_t bytes() const
     {
         return idx16.bytes + idx32.bytes + sub_start.bytes + part.bytes + units.bytes +
                wg_begin.bytes + items.bytes + hub_items.bytes + chunk_first.bytes +
-               hub_partial.bytes + rmask.bytes + rbase.bytes;
+               hub_partial.bytes + wtab.bytes;
     }
 };
 
