@@ -125,6 +125,8 @@ struct SweepArgs {
     uint32_t tab_T, tab_nt, tab_NT, tab_cold_base4, tab_nwaves, tab_lane_max4;
     uint32_t tab_cold_unit0, tab_nunits; // units [cold_unit0, nunits) are the cold table's
     uint32_t tab_row_blocks;             // combine blocks over the non-hub rows
+    uint32_t tab_hub_blocks;             // combine blocks over the hub rows (one wave each)
+    uint32_t fin_hubs;                   // k_pr_finalize sums the hub rows (row layout)
     const uint16_t *tab_idx16;
     const uint32_t *tab_idx32;
     const uint32_t *tab_sub_start;
@@ -536,6 +538,7 @@ constexpr uint32_t kMaxTabNT = 32; // LDS tables + the cold table (row mask bits
 constexpr uint32_t kLaneRows = 4;                           // sub-rows per lane of a lane unit
 constexpr uint32_t kLaneUnitRows = kSweepBlock * kLaneRows; // sub-rows per lane unit
 constexpr int kTabDepth = 4;                    // index loads in flight per lane
+constexpr uint32_t kLongItems = 2;              // long sub-rows per lane group of a unit
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -684,23 +687,59 @@ __device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const f
         if (lane == 0) a.hub_partial[hi.z] = DD ? tab_mark(s, fl) : s;
         return;
     }
-    // long sub-rows, G lanes each: item = (sub-row, first position, end position)
+    // long sub-rows, G lanes each, kLongItems per lane group: item = (sub-row,
+    // first position, end position).  The first kTabDepth words per lane of
+    // every item are in flight together; lane gl of a group adds positions
+    // gl + t*G (+ k*kTabDepth*G) into its partial t, the partials are combined
+    // pairwise and then over the group by shuffles.
     const uint32_t G = kind == kUnitG4 ? 4u : kind == kUnitG16 ? 16u : 64u;
     const uint32_t per_wave = 64u / G;
     const uint32_t gl = (uint32_t)lane & (G - 1);
-    const uint32_t k = (uint32_t)w * per_wave + (uint32_t)lane / G;
-    const bool valid = k < count;
-    uint32_t fl = dd0;
-    float s = 0.0f;
-    uint4 it = make_uint4(0u, 0u, 0u, 0u);
-    if (valid) {
-        it = a.tab_items[first + k];
-        s = tab_strided_sum<CT, DD, COLD>(a, cin, tab, it.y + gl, it.z, G, fl);
+    const uint32_t grp = (uint32_t)w * per_wave + (uint32_t)lane / G;
+    const uint32_t ngrp = kSweepWaves * per_wave;
+    uint4 it[kLongItems];
+    typename IdxVec<COLD>::T v[kLongItems][kTabDepth];
+#pragma unroll
+    for (uint32_t r = 0; r < kLongItems; r++) {
+        const uint32_t k = r * ngrp + grp;
+        it[r] = k < count ? a.tab_items[first + k] : make_uint4(0u, 0u, 0u, 0u);
     }
-    for (uint32_t off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, (int)off, 64);
-    if (DD)
-        for (uint32_t off = G / 2; off > 0; off >>= 1) fl |= __shfl_xor(fl, (int)off, 64);
-    if (valid && gl == 0) a.tab_part[it.x] = DD ? tab_mark(s, fl) : s;
+#pragma unroll
+    for (uint32_t r = 0; r < kLongItems; r++)
+#pragma unroll
+        for (uint32_t t = 0; t < kTabDepth; t++) {
+            const uint32_t p = it[r].y + gl + t * G;
+            if (p < it[r].z) v[r][t] = tab_raw<COLD>(a, p);
+        }
+#pragma unroll
+    for (uint32_t r = 0; r < kLongItems; r++) {
+        float sp[kTabDepth];
+        uint32_t fl = dd0;
+#pragma unroll
+        for (uint32_t t = 0; t < kTabDepth; t++) {
+            sp[t] = 0.0f;
+            if (it[r].y + gl + t * G < it[r].z) tab_add4<CT, DD, COLD>(sp[t], fl, cin, tab, v[r][t]);
+        }
+        // the rest of a sub-row longer than kTabDepth * G words
+        for (uint32_t p = it[r].y + gl + kTabDepth * G; p < it[r].z; p += kTabDepth * G) {
+            typename IdxVec<COLD>::T x[kTabDepth];
+#pragma unroll
+            for (uint32_t t = 0; t < kTabDepth; t++)
+                if (p + t * G < it[r].z) x[t] = tab_raw<COLD>(a, p + t * G);
+#pragma unroll
+            for (uint32_t t = 0; t < kTabDepth; t++)
+                if (p + t * G < it[r].z) tab_add4<CT, DD, COLD>(sp[t], fl, cin, tab, x[t]);
+        }
+#pragma unroll
+        for (int q = 1; q < kTabDepth; q <<= 1)
+#pragma unroll
+            for (int t = 0; t + q < kTabDepth; t += 2 * q) sp[t] += sp[t + q];
+        float sum = sp[0];
+        for (uint32_t off = G / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, (int)off, 64);
+        if (DD)
+            for (uint32_t off = G / 2; off > 0; off >>= 1) fl |= __shfl_xor(fl, (int)off, 64);
+        if (r * ngrp + grp < count && gl == 0) a.tab_part[it[r].x] = DD ? tab_mark(sum, fl) : sum;
+    }
 }
 
 // Loads table contributions [base, base + n) into LDS as FP32: every thread
@@ -783,9 +822,9 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
 // t (the lanes with a sub-row in t, and the slot of the first of them) is a
 // scalar load; a lane's slot is that base plus the count of lower lanes in
 // the mask.  The row's sub-row partials are added in table order, then the
-// vertex is updated as in the row layout.  Blocks past
-// the row blocks update the zero-in-degree rows (unless skipped); hub rows
-// are the finalize kernel's.  Each block writes its residual partials.
+// vertex is updated as in the row layout.  The next blocks take the hub rows
+// (one wave per row, over its chunk partials), the last ones the zero
+// in-degree rows (unless skipped).  Each block writes its residual partials.
 constexpr int kCombBlock = 256;
 // Combine records read through the constant address space: uniform loads of
 // memory the kernel never writes, so they become scalar loads.
@@ -833,8 +872,24 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
                 acc<DD>(s, fl, __builtin_amdgcn_inverse_ballot_w64(m[t]) ? pv[t] : 0.0f);
             if (r < a.n_nonzero) update_vertex(a, r, s * rscale, tele, cout, wfmt, err, act, dang, fl);
         }
+    } else if (blockIdx.x < a.tab_row_blocks + a.tab_hub_blocks) {
+        // hub rows, one wave each: lane k adds the row's chunk partials k,
+        // k + 64, ... in order, then a shuffle tree (a fixed order)
+        const uint32_t row = (blockIdx.x - a.tab_row_blocks) * (kCombBlock / 64) + (uint32_t)w;
+        if (row < a.n_hub) {
+            const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
+            float sp = 0.0f;
+            uint32_t fl = dd0;
+            for (uint32_t q = f0 + lane; q < f1; q += 64) acc<DD>(sp, fl, a.hub_partial[q]);
+            sp = wave_sum(sp);
+            if (DD) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
+            }
+            if (lane == 0) update_vertex(a, row, sp * rscale, tele, cout, wfmt, err, act, dang, fl);
+        }
     } else if (!a.skip_zero) {
-        const uint32_t i = (blockIdx.x - a.tab_row_blocks) * kCombBlock + threadIdx.x;
+        const uint32_t i = (blockIdx.x - a.tab_row_blocks - a.tab_hub_blocks) * kCombBlock + threadIdx.x;
         if (i < a.N - a.n_nonzero)
             update_vertex(a, a.n_nonzero + i, 0.0f, tele, cout, wfmt, err, act, dang, dd0);
     }
@@ -855,8 +910,9 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
 }
 
 // One workgroup, launched after each sweep (the kernel boundary orders it
-// after every sweep workgroup, so no device-scope fence is needed): sums each
-// hub row's chunk partials in chunk order and updates the hub vertices, then
+// after every sweep workgroup, so no device-scope fence is needed): in the
+// row layout sums each hub row's chunk partials in chunk order and updates the
+// hub vertices (the partitioned layout's combine kernel does that), then
 // reduces all residual / active / dangling partials in a fixed order and
 // decides convergence.
 constexpr int kFinBlock = 1024;
@@ -876,7 +932,7 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     // loads in batches of kFinBatch, added in order (one memory round trip
     // per batch instead of one per value)
     constexpr uint32_t kFinBatch = 8;
-    for (uint32_t row = threadIdx.x; row < a.n_hub; row += kFinBlock) {
+    for (uint32_t row = threadIdx.x; a.fin_hubs && row < a.n_hub; row += kFinBlock) {
         const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
         float sum = 0.0f;
         uint32_t fl = (dd && it == 0) ? kDdFlag : 0u;
@@ -1779,8 +1835,9 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
         const uint32_t h1 = (uint32_t)hub_items.size();
         for (uint32_t c = h0; c < h1; c += kSweepWaves)
             units.push_back(make_uint4(tt | kUnitHub, c, std::min<uint32_t>(kSweepWaves, h1 - c), 0u));
-        // sub-rows per unit: 16 / 4 / 1 per wave
-        const uint32_t per_unit[4] = {0, 16 * kSweepWaves, 4 * kSweepWaves, kSweepWaves};
+        // sub-rows per unit: kLongItems per lane group (16 / 4 / 1 groups per wave)
+        const uint32_t per_unit[4] = {0, 16 * kSweepWaves * kLongItems, 4 * kSweepWaves * kLongItems,
+                                      kSweepWaves * kLongItems};
         for (uint32_t c = 3; c >= 1; c--) {
             const uint32_t n = ccnt[t * 8 + c], f = cfirst[t * 8 + c];
             if (n)
@@ -1847,6 +1904,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     const uint32_t nnh = nrows - std::min(n_hub, nrows);
     tp->nwaves = ceil_div(nnh, 64);
     tp->row_blocks = ceil_div(tp->nwaves, kCombBlock / 64);
+    tp->hub_blocks = ceil_div(std::min(n_hub, nrows), kCombBlock / 64);
     tp->NTpad = NT <= 4 ? 4u : NT <= 8 ? 8u : NT <= 17 ? 17u : kMaxTabNT;
     const size_t wrec = ((size_t)tp->nwaves * 3 * tp->NTpad + 1) * 4;
     tp->wtab.alloc(wrec);
@@ -1894,7 +1952,7 @@ static uint32_t sweep_grid(const ag_ctx *ctx, const Plan *p)
 static uint32_t tab_comb_blocks(const Plan *p, bool skip_zero)
 {
     const uint32_t z = skip_zero ? 0u : ceil_div(p->N - p->n_nonzero, kCombBlock);
-    return std::max<uint32_t>(1, p->tab->row_blocks + z);
+    return std::max<uint32_t>(1, p->tab->row_blocks + p->tab->hub_blocks + z);
 }
 
 // Residual-partial slots per sweep: the row-layout sweep grid, or the
@@ -2012,9 +2070,13 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
     a.chunk_first = p->chunk_first.as<uint32_t>();
     a.hub_partial = p->hub_partial.as<float>();
     a.nparts = w.grid;
+    a.fin_hubs = 1;
     if (p->tab) {
-        // partitioned tables: hub rows are summed from their table chunks
+        // partitioned tables: hub rows are summed from their table chunks by
+        // the combine kernel
         const TabPlan &tp = *p->tab;
+        a.fin_hubs = 0;
+        a.tab_hub_blocks = tp.hub_blocks;
         a.tab_T = tp.T;
         a.tab_nt = tp.nt;
         a.tab_NT = tp.NT;

@@ -56,6 +56,7 @@ struct TabPlan {
     uint32_t nwaves = 0;     // 64-row waves of the combine kernel over the non-hub rows
     uint32_t NTpad = 0;      // tables per combine record (NT rounded up to 4, 8, 17 or 32)
     uint32_t row_blocks = 0; // combine blocks over those waves
+    uint32_t hub_blocks = 0; // combine blocks over the hub rows (one wave per row)
     uint64_t idx16_len = 0, idx32_len = 0;
     std::vector<uint32_t> SB;         // first sub-row of each table, SB[NT] = S (host copy)
     std::vector<uint64_t> tab_edges;  // edges per table (host copy)
