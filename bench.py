@@ -7,10 +7,12 @@ CSR (count-sort build, bit-exact with the oracle) and the gather layout are
 built once before timing and reported separately.
 
     value    = GTEPS = E * 20 / step seconds / 1e9
-    roofline = the sweep kernel: algorithmic bytes per sweep (4E + 24N + 4) /
-               mean duration of the sweep kernel launches (HIP events recorded
-               on the library's stream around them, in extra eager steps after
-               the timed ones) vs 8 TB/s
+    roofline = the sweep (k_pr_tables then k_pr_combine): algorithmic bytes
+               per sweep (4E + 24N + 4) / mean sweep duration (HIP events
+               recorded on the library's stream before the first and after the
+               second kernel, in extra eager steps after the timed ones) vs
+               8 TB/s; traffic = the two kernels' copy-calibrated FETCH_SIZE +
+               WRITE_SIZE from the committed rocprofv3 summary
     cpu_baseline = the oracle's OpenMP restatement of the reference loop
                (pageRank.c:1024-1085 + the schedule(dynamic,1024) gather of
                pageRank.c:2890-2907) on the host's cores, same graph; the same
@@ -276,7 +278,9 @@ def main() -> int:
                    **({"relabel": relabel} if relabel else {})},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": "k_pr_sweep",
+                     "traffic": traffic,
+                     "kernel": "k_pr_tables + k_pr_combine (one sweep; HIP events before the "
+                               "first and after the second)",
                      "sweep_us": round(sweep_ms * 1e3, 2),
                      "algorithmic_bytes_per_sweep": bytes_sweep,
                      # diagnostic, not a roofline: every gather counted as a
