@@ -8,10 +8,12 @@ pageRank.c:2810-2968), CPU only.
   loop stops at the first sweep whose change 0.15*0.85^(t+1) is below
   epsilon; vertex k ends at 0.15 * sum_{i<=min(t*,k)} 0.85^i.  Marking
   in-neighbours instead of out-neighbours, or updating below epsilon, breaks it.
-* epsilon = 0 reduces to the plain pull oracle (itself pinned against the
-  reference's own fixtures): every evaluated vertex updates, and a vertex off
-  the worklist has a constant gathered sum, so ranks and error totals agree bit
-  for bit.  A Gauss-Seidel slip (in-place contributions) breaks this.
+* epsilon = 0 reduces to the plain pull oracle (itself pinned by closed forms
+  -- cycle, two-vertex and star graphs -- and by an independent scipy power
+  iteration in FP64, in test_oracle_pagerank.py): every evaluated vertex
+  updates, and a vertex off the worklist has a constant gathered sum, so ranks
+  and error totals agree bit for bit.  A Gauss-Seidel slip (in-place
+  contributions) breaks this.
 * Stopping property on random graphs (FP64 recomputation): when the loop stops
   before its cap, every vertex's pull residual against the final ranks is below
   epsilon.
