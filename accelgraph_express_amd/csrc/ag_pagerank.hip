@@ -534,6 +534,9 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_sweep(SweepArgs a, uint32_t 
 // pairwise, then by shuffles.  Every partial sum's order is fixed by the
 // layout alone.  DATA_DRIVEN partials carry the OR of the marks in the sign bit.
 constexpr uint32_t kUnitLane = 0, kUnitG4 = 1, kUnitG16 = 2, kUnitG64 = 3, kUnitHub = 4;
+constexpr int kFinBlock = 1024; // threads of the finalize workgroup
+static_assert(kFinBlock == kSweepBlock, "the table kernel's workgroups run the finalize body");
+__device__ void finalize_body(const SweepArgs &a, uint32_t it);
 constexpr uint32_t kMaxTabNT = 32; // LDS tables + the cold table (row mask bits)
 constexpr uint32_t kLaneRows = 4;                           // sub-rows per lane of a lane unit
 constexpr uint32_t kLaneUnitRows = kSweepBlock * kLaneRows; // sub-rows per lane unit
@@ -776,7 +779,7 @@ __device__ __forceinline__ void tab_load(float *tab, const CT *__restrict__ cin,
 }
 
 template <typename CT, bool DD>
-__global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t it)
+__global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t it, uint32_t fin_prev)
 {
     extern __shared__ __align__(16) unsigned char tab_lds[];
     float *tab = reinterpret_cast<float *>(tab_lds);
@@ -788,7 +791,11 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
     // the workgroup's static range of LDS units, then cold units (global
     // gathers: latency-bound, with costs the static split cannot predict)
     // taken one at a time from the sweep's counter (st->ticket: zeroed by
-    // init and, for the next sweep, by the combine kernel)
+    // init and, for the next sweep, by the combine kernel).  With fin_prev,
+    // ticket 0 is the previous sweep's finalize (its combine kernel has
+    // ended: the kernel boundary orders the two), run by the first workgroup
+    // done with its static range; this sweep's combine kernel starts after
+    // it, so the state it sets (convergence, teleport, formats) is in place.
     __shared__ uint32_t s_next;
     uint32_t cur = 0xffffffffu;
     const uint32_t u0 = a.tab_wg_begin[blockIdx.x], u1 = a.tab_wg_begin[blockIdx.x + 1];
@@ -810,7 +817,12 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
         __syncthreads(); // s_next is free again
         if (threadIdx.x == 0) s_next = atomicAdd(&st->ticket, 1u);
         __syncthreads();
-        const uint32_t ui = a.tab_cold_unit0 + s_next;
+        const uint32_t tk = s_next;
+        if (fin_prev && tk == 0) {
+            finalize_body(a, it - 1);
+            continue;
+        }
+        const uint32_t ui = a.tab_cold_unit0 + tk - fin_prev;
         if (ui >= a.tab_nunits) break;
         const uint4 un = a.tab_units[ui];
         tab_unit<CT, DD, true>(a, cin, tab, un.x & 0xffu, un.y, un.z, dd0);
@@ -909,19 +921,21 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
     }
 }
 
-// One workgroup, launched after each sweep (the kernel boundary orders it
-// after every sweep workgroup, so no device-scope fence is needed): in the
-// row layout sums each hub row's chunk partials in chunk order and updates the
-// hub vertices (the partitioned layout's combine kernel does that), then
-// reduces all residual / active / dangling partials in a fixed order and
-// decides convergence.
-constexpr int kFinBlock = 1024;
-__global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t it)
+// The end of a sweep, in one workgroup that a kernel boundary orders after
+// every sweep workgroup (so no device-scope fence is needed): k_pr_finalize
+// after each row-layout sweep and after the last sweep of a partitioned-layout
+// batch, else inside the next sweep's table kernel.  In the row layout it sums
+// each hub row's chunk partials in chunk order and updates the hub vertices
+// (the partitioned layout's combine kernel does that); then it reduces all
+// residual / active / dangling partials in a fixed order and decides
+// convergence.
+// Runs in one kFinBlock-thread workgroup: k_pr_finalize, or the table-kernel
+// workgroup that takes the first ticket of the next sweep's cold phase.
+__device__ void finalize_body(const SweepArgs &a, uint32_t it)
 {
     __shared__ double s_err[kFinBlock / 64], s_dang[kFinBlock / 64];
     __shared__ uint32_t s_act[kFinBlock / 64];
     SweepState *st = a.st;
-    if (st->done) return;
     const uint32_t wfmt = a.fmt[it + 1];
     const float rscale = a.fmt[it] == 2u ? a.f16_inv : 1.0f;
     void *cout = (it & 1) ? a.c0 : a.c1;
@@ -1025,6 +1039,12 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
         st->iterations = it + 1;
         if (a.epsilon > 0.0 && E_ <= a.epsilon) st->done = 1;
     }
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t it)
+{
+    if (a.st->done) return;
+    finalize_body(a, it);
 }
 
 __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
@@ -2121,26 +2141,33 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
 // sweep exits at once); with the partitioned tables the table kernel(s), then
 // the combine kernel over a.nparts blocks (its residual partials).  `grid` is
 // the row-layout sweep grid.
+// Enqueues sweep `it`'s gather.  With the partitioned tables and fin_prev set,
+// the table kernel also runs sweep it - 1's finalize (so that sweep's
+// k_pr_finalize launch is left out); the row layout ignores fin_prev.
 static void enqueue_gather(hipStream_t s, const Plan *p, const SweepArgs &a, uint32_t mode,
-                           uint32_t precision, uint32_t flags, uint32_t grid, uint32_t it)
+                           uint32_t precision, uint32_t flags, uint32_t grid, uint32_t it,
+                           uint32_t fin_prev)
 {
     const bool dd = mode == AG_PR_DATA_DRIVEN;
     const bool fp32 = !dd && (precision == AG_PREC_FP32 || (flags & AG_PR_FLAG_FINISH_FP32));
     if (p->tab) {
         const TabPlan &tp = *p->tab;
         const uint32_t lds = (tp.T + 1) * 4;
+        // of two table kernels (low-precision run with the FP32 finish) only
+        // the one whose format the sweep reads passes its entry check and
+        // takes tickets, so the finalize runs once
         if (dd)
             hipLaunchKernelGGL((k_pr_tables<float, true>), dim3(tp.grid), dim3(kSweepBlock), lds, s,
-                               a, it);
+                               a, it, fin_prev);
         else if (precision == AG_PREC_BF16)
             hipLaunchKernelGGL((k_pr_tables<bf16_t, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
-                               s, a, it);
+                               s, a, it, fin_prev);
         else if (precision == AG_PREC_FP16)
             hipLaunchKernelGGL((k_pr_tables<__half, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
-                               s, a, it);
+                               s, a, it, fin_prev);
         if (fp32)
             hipLaunchKernelGGL((k_pr_tables<float, false>), dim3(tp.grid), dim3(kSweepBlock), lds,
-                               s, a, it);
+                               s, a, it, fin_prev);
         // the combine kernel's table loops are unrolled to the records' NTpad
         const void *kc;
         switch (tp.NTpad) {
@@ -2174,11 +2201,15 @@ static void enqueue_sweeps(hipStream_t s, const Plan *p, const SweepArgs &a,
                            const ag_pr_options &o, uint32_t grid, uint32_t it0, uint32_t it1,
                            hipEvent_t *sev)
 {
+    // partitioned tables: sweep it's finalize runs in sweep it + 1's table
+    // kernel, the batch's last one as its own kernel
+    const bool fused = p->tab != nullptr;
     for (uint32_t it = it0; it < it1; it++) {
         if (sev) AG_HIP(hipEventRecord(sev[2 * it], s));
-        enqueue_gather(s, p, a, o.mode, o.precision, o.flags, grid, it);
+        enqueue_gather(s, p, a, o.mode, o.precision, o.flags, grid, it, fused && it > it0 ? 1u : 0u);
         if (sev) AG_HIP(hipEventRecord(sev[2 * it + 1], s));
-        hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
+        if (!fused || it + 1 == it1)
+            hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
     }
     AG_HIP(hipGetLastError());
 }
@@ -2476,9 +2507,11 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
     hipLaunchKernelGGL(k_spmv_init, dim3(ceil_div((uint64_t)std::max(N + 1, iterations + 2), 256)),
                        dim3(256), 0, s, a, d_x, p->order.as<uint32_t>());
     AG_HIP(hipEventRecord(ev[1], s));
+    const bool fused = p->tab != nullptr; // as in enqueue_sweeps
     for (uint32_t it = 0; it < iterations; it++) {
-        enqueue_gather(s, p, a, kModeSpmv, AG_PREC_FP32, 0u, w.grid, it);
-        hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
+        enqueue_gather(s, p, a, kModeSpmv, AG_PREC_FP32, 0u, w.grid, it, fused && it > 0 ? 1u : 0u);
+        if (!fused || it + 1 == iterations)
+            hipLaunchKernelGGL(k_pr_finalize, dim3(1), dim3(kFinBlock), 0, s, a, it);
     }
     AG_HIP(hipGetLastError());
     AG_HIP(hipEventRecord(ev[2], s));
