@@ -78,7 +78,8 @@ class PlanInfo(C.Structure):
                 ("hub_min", C.c_uint32), ("sell_slots", C.c_uint64), ("plan_bytes", C.c_uint64),
                 ("hub_chunk", C.c_uint32), ("layout", C.c_uint32), ("tables", C.c_uint32),
                 ("table_workgroups", C.c_uint32), ("subrows", C.c_uint64),
-                ("cold_edges", C.c_uint64)]
+                ("cold_edges", C.c_uint64), ("lane_max", C.c_uint32), ("max_tables", C.c_uint32),
+                ("autotuned", C.c_uint32)]
 
 
 class Tuning(C.Structure):
@@ -156,6 +157,7 @@ _SIGS = {
     "ag_relabel_edges_device": (C.c_int, [vp, vp, C.c_uint32, vp, vp, C.c_uint32]),
     "ag_spmv_pull_device": (C.c_int, [vp, vp, vp, vp, C.c_uint32, C.POINTER(SpmvResult)]),
     "ag_pagerank_plan_info": (C.c_int, [vp, C.POINTER(PlanInfo)]),
+    "ag_pagerank_autotune": (C.c_int, [vp, vp, C.c_uint32, C.POINTER(Tuning), f64p]),
     "ag_set_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "ag_get_tuning": (C.c_int, [C.POINTER(Tuning)]),
     "ag_ppr_options_default": (None, [C.POINTER(PprOptions)]),

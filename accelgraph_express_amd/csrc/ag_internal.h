@@ -131,6 +131,18 @@ struct Plan;
 struct PprState;
 void plan_destroy(Plan *p);
 
+// ---- tuning ----------------------------------------------------------------
+struct Tuning {
+    uint32_t low_max = 64;
+    uint32_t hub_min = 6144;   // s22: rows of 4,096-6,143 in-edges do better as 64-lane rows
+    // AG_TUNE_AUTO: chosen per graph from its hub rows (build_plan)
+    uint32_t hub_chunk = AG_TUNE_AUTO;
+    uint32_t hot_bytes = 160 * 1024 - 2048; // LDS table of the hottest contributions
+    uint32_t layout = AG_LAYOUT_AUTO;       // gather layout of the pull sweep
+    uint32_t max_tables = 16;               // LDS tables of the partitioned layout
+    uint32_t lane_max = 16;                 // sub-rows up to this many edges: one lane each
+};
+
 } // namespace ag
 
 struct ag_graph {
@@ -140,6 +152,10 @@ struct ag_graph {
     ag::DevBuf fwd_src, fwd_dst, fwd_outdeg, fwd_idx, fwd_off;
     ag::DevBuf inv_src, inv_dst, inv_outdeg, inv_idx, inv_off;
     ag::Plan *plan = nullptr;
+    // the graph's own gather-layout tuning (set by ag_pagerank_autotune);
+    // without it a layout is built with the process-wide tuning
+    bool has_tune = false;
+    ag::Tuning tune;
     uint32_t ppr_cpw[3] = {0, 0, 0}; // autotuned columns-per-wave per precision
     ~ag_graph() { ag::plan_destroy(plan); }
     uint64_t bytes() const;
@@ -175,17 +191,6 @@ void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N
 uint32_t device_max2(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n);
 uint32_t device_max1(ag_ctx *ctx, const uint32_t *a, uint32_t n);
 
-// ---- tuning ----------------------------------------------------------------
-struct Tuning {
-    uint32_t low_max = 64;
-    uint32_t hub_min = 6144;   // s22: rows of 4,096-6,143 in-edges do better as 64-lane rows
-    // AG_TUNE_AUTO: chosen per graph from its hub rows (build_plan)
-    uint32_t hub_chunk = AG_TUNE_AUTO;
-    uint32_t hot_bytes = 160 * 1024 - 2048; // LDS table of the hottest contributions
-    uint32_t layout = AG_LAYOUT_AUTO;       // gather layout of the pull sweep
-    uint32_t max_tables = 16;               // LDS tables of the partitioned layout
-    uint32_t lane_max = 16;                 // sub-rows up to this many edges: one lane each
-};
 Tuning &tuning();
 
 } // namespace ag

@@ -1487,7 +1487,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
 {
     hipStream_t s = ctx->stream;
     const uint32_t N = g->N, E = g->E;
-    const Tuning tn = tuning();
+    const Tuning tn = g->has_tune ? g->tune : tuning();
     std::unique_ptr<Plan> p(new Plan());
     p->N = N;
     p->E = E;
@@ -1496,6 +1496,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->hub_chunk = tn.hub_chunk;
     p->hot_bytes = tn.hot_bytes;
     p->lane_max = tn.lane_max;
+    p->max_tables = tn.max_tables;
     if (p->hot_bytes) {
         static bool attr_set = false;
         if (!attr_set) {
@@ -2642,6 +2643,92 @@ int ag_pagerank_prepare(ag_ctx *ctx, ag_graph *g, double *ms)
     });
 }
 
+int ag_pagerank_autotune(ag_ctx *ctx, ag_graph *g, uint32_t sweeps, ag_tuning *best, double *ms)
+{
+    return guard([&] {
+        check_ctx(ctx);
+        if (!g) fail(AG_ERR_INVALID, "null graph");
+        if (sweeps == 0 || sweeps > 1000) fail(AG_ERR_INVALID, "sweeps must be in 1..1000");
+        const auto t0 = std::chrono::steady_clock::now();
+        Tuning cur = g->has_tune ? g->tune : tuning();
+        // a layout built from `tn` becomes g's plan (the previous one is kept
+        // by the caller or destroyed)
+        auto build = [&](const Tuning &tn) {
+            g->tune = tn;
+            g->has_tune = true;
+            g->plan = nullptr;
+            return ensure_plan(ctx, g, nullptr);
+        };
+        ag_pr_options o;
+        ag_pr_options_default(&o);
+        o.iterations = sweeps;
+        o.epsilon = 0.0;
+        DevBuf out((size_t)std::max<uint32_t>(g->N, 1) * 4);
+        // FP32 pull, `sweeps` sweeps: the first call captures the loop's
+        // graph, the faster of two replays is the candidate's time
+        auto measure = [&] {
+            ag_pr_result r;
+            double t = 1e300;
+            for (int k = 0; k < 3; k++) {
+                run_pagerank(ctx, g, o, out.as<float>(), nullptr, nullptr, &r);
+                if (k > 0) t = std::min(t, r.total_ms);
+            }
+            return t;
+        };
+        plan_destroy(g->plan);
+        Plan *best_plan = build(cur);
+        double best_ms = g->N && g->E ? measure() : 0.0;
+        Tuning best_t = cur;
+        if (g->N && g->E) {
+            struct Knob { uint32_t Tuning::*f; std::vector<uint32_t> vals; };
+            std::vector<Knob> knobs;
+            if (best_plan->tab) {
+                knobs.push_back({&Tuning::max_tables, {8, 12, 16, 20, 24, 31}});
+                knobs.push_back({&Tuning::lane_max, {8, 12, 16, 24}});
+            } else {
+                knobs.push_back({&Tuning::low_max, {16, 32, 64, 128}});
+            }
+            knobs.push_back({&Tuning::hub_min, {4096, 6144, 8192, 12288}});
+            knobs.push_back({&Tuning::hub_chunk, {AG_TUNE_AUTO, 1024, 2048, 4096}});
+            // one pass of coordinate search; a candidate replaces the best
+            // only when at least 1% faster (timing noise is below that)
+            for (const Knob &k : knobs)
+                for (uint32_t v : k.vals) {
+                    if (best_t.*k.f == v) continue;
+                    Tuning t = best_t;
+                    t.*k.f = v;
+                    Plan *cand = build(t);
+                    const double m = measure();
+                    if (m < 0.99 * best_ms) {
+                        plan_destroy(best_plan);
+                        best_plan = cand;
+                        best_ms = m;
+                        best_t = t;
+                    } else {
+                        plan_destroy(cand);
+                    }
+                    g->plan = nullptr;
+                }
+        }
+        g->plan = best_plan;
+        g->tune = best_t;
+        g->has_tune = true;
+        g->plan->autotuned = true;
+        if (best) {
+            std::memset(best, 0, sizeof(*best));
+            best->low_max = best_t.low_max;
+            best->hub_min = best_t.hub_min;
+            best->hub_chunk = best_t.hub_chunk;
+            best->hot_bytes = best_t.hot_bytes;
+            best->layout = best_t.layout;
+            best->max_tables = best_t.max_tables;
+            best->lane_max = best_t.lane_max;
+        }
+        if (ms)
+            *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
 int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info)
 {
     return guard([&] {
@@ -2660,6 +2747,9 @@ int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info)
         info->plan_bytes = p->bytes();
         info->hub_chunk = p->tab ? p->tab->hub_chunk : p->hub_chunk;
         info->layout = p->tab ? AG_LAYOUT_TABLES : AG_LAYOUT_ROWS;
+        info->lane_max = p->lane_max;
+        info->max_tables = p->max_tables;
+        info->autotuned = p->autotuned ? 1u : 0u;
         if (p->tab) {
             info->hub_chunks = p->tab->hub_chunks;
             info->tables = p->tab->nt;

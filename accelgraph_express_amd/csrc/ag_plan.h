@@ -89,6 +89,8 @@ struct Plan {
     uint32_t n_hub = 0, n_w = 0, n_h = 0, n_nonlow = 0, n_nonzero = 0, n_dangling = 0;
     uint32_t low_slices = 0, hub_chunks = 0;
     uint32_t low_max = 0, hub_min = 0, hub_chunk = 0, hot_bytes = 0, lane_max = 16;
+    uint32_t max_tables = 16;  // partitioned layout: LDS tables at most (tuning)
+    bool autotuned = false;    // chosen by ag_pagerank_autotune
     uint64_t csr_edges = 0, sell_slots = 0;
     // order[i] = vertex with label i; rank[v] = label of vertex v
     DevBuf order, rank, roff, cols, slice_ptr, outdeg, chunk_row, chunk_first, hub_partial;

@@ -197,6 +197,14 @@ class Graph:
         L.check(L.load().ag_pagerank_prepare(self.ctx.h, self.h, C.byref(ms)))
         return ms.value
 
+    def autotune(self, sweeps: int = 20) -> tuple:
+        """Times candidate gather layouts of this graph on the device and keeps
+        the fastest (ag_pagerank_autotune).  Returns (tuning dict, wall ms)."""
+        t = L.Tuning()
+        ms = C.c_double(0.0)
+        L.check(L.load().ag_pagerank_autotune(self.ctx.h, self.h, sweeps, C.byref(t), C.byref(ms)))
+        return {f: getattr(t, f) for f, _ in L.Tuning._fields_}, ms.value
+
     def plan_info(self) -> dict:
         info = L.PlanInfo()
         L.check(L.load().ag_pagerank_plan_info(self.h, C.byref(info)))

@@ -65,6 +65,8 @@ def main() -> int:
     ap.add_argument("--no-ppr", action="store_true", help="skip the batched-PPR leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the R-MAT s24 precision block")
     ap.add_argument("--no-c5", action="store_true", help="skip the serving block")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default gather-layout tuning (no per-graph search)")
     ap.add_argument("--no-dd", action="store_true",
                     help="skip the data-driven pull and SPMV lines")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"])
@@ -145,6 +147,14 @@ def main() -> int:
     info = g.plan_info()
     del d_src, d_dst
     log(f"CSR build {csr_ms:.1f} ms, gather layout {plan_ms:.1f} ms, plan {info}")
+    # per-graph layout search (timed candidates of the FP32 pull loop); its
+    # choice is the layout measured below and reported in config.plan
+    tune = None
+    if not args.no_autotune:
+        best, tune_ms = g.autotune(sweeps=iters)
+        info = g.plan_info()
+        tune = {"search_ms": round(tune_ms, 1), "tuning": best}
+        log(f"autotune: {tune_ms:.0f} ms, chose {best}")
     ser = None
     if args.serialize:
         path = os.path.join(args.serialize, f"rmat_s{scale}.agcsr")
@@ -251,7 +261,7 @@ def main() -> int:
     g.free()
     del out
     torch.cuda.empty_cache()
-    c3 = None if args.no_c3 else c3_leg(ag, np, torch, args.seed)
+    c3 = None if args.no_c3 else c3_leg(ag, np, torch, args.seed, not args.no_autotune)
     c5 = None if args.no_c5 else c5_leg(ag, np, args.seed, extrapolate=args.extrapolate)
 
     line = {
@@ -272,7 +282,8 @@ def main() -> int:
                    "step": "hipGraph replay of init + sweeps + normalise",
                    "eager_step_ms_with_sweep_events": round(min(eager), 4),
                    "csr_build_ms": round(csr_ms, 2), "gather_layout_ms": round(plan_ms, 2),
-                   "plan": info, **({"serialization": ser} if ser else {}),
+                   "plan": info, **({"autotune": tune} if tune else {}),
+                   **({"serialization": ser} if ser else {}),
                    **({"data_driven_pull": dd} if dd else {}),
                    **({"spmv_pull": spmv} if spmv else {}),
                    **({"relabel": relabel} if relabel else {})},
@@ -374,7 +385,7 @@ def cpu_leg(ag, np, src, dst, N, E, iters, ranks, ctx):
     return cpu, parity
 
 
-def c3_leg(ag, np, torch, seed):
+def c3_leg(ag, np, torch, seed, autotune=True):
     """R-MAT s24 STANDARD PageRank to L1 residual 1e-6: FP32, and BF16 / FP16
     storage finishing in FP32; time to tolerance and the FP64 error bound."""
     scale, tol = 24, 1e-6
@@ -390,6 +401,12 @@ def c3_leg(ag, np, torch, seed):
     del d_src, d_dst
     torch.cuda.empty_cache()
     plan_ms = g.prepare()
+    tune = None
+    if autotune:
+        best, tune_ms = g.autotune(sweeps=20)
+        tune = {"search_ms": round(tune_ms, 1), "tuning": best}
+        log(f"c3 autotune: {tune_ms:.0f} ms, chose {best}")
+    info = g.plan_info()
     runs, ranks = {}, {}
     out = torch.empty(N, dtype=torch.float32, device="cuda")
     for prec in ("fp32", "bf16", "fp16"):
@@ -418,6 +435,9 @@ def c3_leg(ag, np, torch, seed):
     log(f"c3: {runs}")
     return {"workload": f"rmat_s24_ef16_standard_pagerank_to_l1_residual_1e-6",
             "num_vertices": N, "num_edges": E, "gather_layout_ms": round(plan_ms, 1),
+            "plan": {k: info[k] for k in ("layout", "tables", "subrows", "cold_edges", "lane_max",
+                                          "max_tables", "hub_min", "hub_chunk", "autotuned")},
+            **({"autotune": tune} if tune else {}),
             "timing": "device-resident ranks, median of 3 after a warm run",
             "error_bound": "||x - x*||_1 <= ||x - T(x)||_1 / (1 - d), T once in FP64 (numpy)",
             "runs": runs}
@@ -651,7 +671,7 @@ def workload_c3(args):
     import numpy as np
     import torch
     import accelgraph_express_amd as ag
-    res = c3_leg(ag, np, torch, args.seed)
+    res = c3_leg(ag, np, torch, args.seed, not args.no_autotune)
     runs = res["runs"]
     line = {"metric": "time to L1 residual 1e-6, STANDARD PageRank (R-MAT s24)",
             "value": runs["bf16"]["ms"] / 1e3, "unit": "s", "n_gpus": 1, "steps": 3, "warmup": 1,

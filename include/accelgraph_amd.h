@@ -273,8 +273,12 @@ typedef struct ag_plan_info
     uint32_t tables;                      /* partitioned layout: LDS tables (cold excluded) */
     uint32_t table_workgroups;            /* partitioned layout: table-kernel workgroups */
     uint64_t subrows;                     /* partitioned layout: (row, table) sub-rows */
-    uint64_t cold_edges;                  /* partitioned layout: edges gathered from global
-                                             memory (sources past the last LDS table) */
+    uint64_t cold_edges;                  /* partitioned layout: padded index entries gathered
+                                             from global memory (sources past the last LDS
+                                             table) */
+    uint32_t lane_max;                    /* partitioned layout: one-lane sub-row limit */
+    uint32_t max_tables;                  /* partitioned layout: LDS table limit */
+    uint32_t autotuned;                   /* 1 when chosen by ag_pagerank_autotune */
 } ag_plan_info;
 int ag_pagerank_plan_info(const ag_graph *g, ag_plan_info *info);
 
@@ -307,6 +311,22 @@ typedef struct ag_tuning
 #define AG_LAYOUT_AUTO   3u
 int ag_set_tuning(const ag_tuning *t);
 int ag_get_tuning(ag_tuning *t);
+
+/* Chooses g's gather layout by timing candidates on this device (no
+ * counterpart in the reference, whose gather has no layout to tune): from g's
+ * current tuning (the process-wide one unless autotuned before), one pass of
+ * coordinate search over max_tables {8,12,16,20,24,31} and lane_max
+ * {8,12,16,24} (partitioned tables; low_max {16,32,64,128} for the row
+ * layout), hub_min {4096,6144,8192,12288} and hub_chunk {auto,1024,2048,4096}.
+ * Each candidate layout is built and timed over `sweeps` (1..1000) FP32 pull
+ * sweeps (the faster of two graph replays); one replaces the best only when
+ * at least 1% faster.  The fastest becomes g's layout and g's own tuning (later
+ * rebuilds use it), reported by ag_pagerank_plan_info; *best (optional)
+ * receives it and *ms (optional) the wall time of the search.  Builds the
+ * layout up to 19 times (about 20 ms each at 4 M vertices and 67 M edges) and
+ * holds two layouts at once; it must not overlap other calls on g, nor run
+ * while a server holds g (its batches point at the layout). */
+int ag_pagerank_autotune(ag_ctx *ctx, ag_graph *g, uint32_t sweeps, ag_tuning *best, double *ms);
 
 /* ---- batched personalized PageRank -------------------------------------- */
 typedef struct ag_ppr_query
