@@ -56,8 +56,11 @@ class PprBatch {
     // exact_keep: buffers left on the graph by an earlier batch are lent only
     // when their size is exactly this batch's (others are freed), so the
     // batch holds exactly bytes_for(capacity) (the serving loop's budget).
+    // `stream`: the stream all of the batch's work is queued on (null: the
+    // context's); the serving loop gives each graph's batch its own.
     PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t cpw, uint32_t fmt,
-             float damp, bool extrapolate = false, bool exact_keep = false);
+             float damp, bool extrapolate = false, bool exact_keep = false,
+             hipStream_t stream = nullptr);
     ~PprBatch();
     PprBatch(const PprBatch &) = delete;
     PprBatch &operator=(const PprBatch &) = delete;
@@ -67,6 +70,7 @@ class PprBatch {
     static uint64_t bytes_for(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t fmt);
     uint32_t capacity() const { return cap_; }
     uint32_t active() const { return used_count_; }
+    hipStream_t stream() const { return s_; }
     uint32_t free_slots() const { return cap_ - used_count_; }
     uint64_t bytes() const;
     uint32_t cpw() const { return cpw_; }
@@ -136,6 +140,7 @@ class PprBatch {
     void launch_sweep(bool from0);
     uint32_t lanes_per_row() const;
     ag_ctx *ctx_;
+    hipStream_t s_;
     Plan *p_;
     uint32_t N_, cap_, cpw_ = 0, fmt_;
     uint32_t used_count_ = 0, hi_ = 0; // hi_: one past the highest used slot

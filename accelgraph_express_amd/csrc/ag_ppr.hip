@@ -1179,8 +1179,9 @@ void ppr_release_kept(ag_graph *g)
 }
 
 PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_t fmt, float damp,
-                   bool extrapolate, bool exact_keep)
-    : ctx_(ctx), cap_(cap), fmt_(fmt), damp_(damp), extrap_(extrapolate)
+                   bool extrapolate, bool exact_keep, hipStream_t stream)
+    : ctx_(ctx), s_(stream ? stream : ctx->stream), cap_(cap), fmt_(fmt), damp_(damp),
+      extrap_(extrapolate)
 {
     if (cap == 0 || cap > kMaxCap || cap % 8) fail(AG_ERR_INVALID, "batch capacity must be 8..64, multiple of 8");
     if (fmt != AG_PREC_FP32 && fmt != AG_PREC_BF16)
@@ -1199,7 +1200,7 @@ PprBatch::PprBatch(ag_ctx *ctx, ag_graph *g, uint32_t cap, uint32_t cpw, uint32_
         fwd_dst_ = g->fwd_dst.as<uint32_t>();
     }
     set_cpw(cpw);
-    hipStream_t s = ctx->stream;
+    hipStream_t s = s_;
     {
         // The buffers of the last finished batch on this graph are lent to
         // this one, so a repeated batch call makes no hipMalloc / hipFree (the
@@ -1233,7 +1234,7 @@ PprBatch::~PprBatch()
 {
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
     if (graph_) (void)hipGraphDestroy(graph_);
-    (void)hipStreamSynchronize(ctx_->stream);
+    (void)hipStreamSynchronize(s_);
     // keep the larger buffers on the graph for the next batch
     DevBuf *mine[kKeepBufs] = {&X_,        &C_[0], &C_[1], &cols_,      &bstate_, &blk_res_,
                                &blk_dang_, &part_, &seed_bits_, &out_, &desc_};
@@ -1293,7 +1294,7 @@ PprSeedSet PprBatch::make_seeds(const uint32_t *seeds, uint32_t n, double tol, u
 
 void PprBatch::upload_seed_table()
 {
-    hipStream_t s = ctx_->stream;
+    hipStream_t s = s_;
     std::vector<uint32_t> v, c, z;
     std::vector<float> w;
     for (uint32_t k = 0; k < cap_; k++) {
@@ -1401,7 +1402,7 @@ void PprBatch::compact()
     std::vector<PprAdmit> mv;
     for (size_t i = 0; i < src.size(); i++) mv.push_back({src[i], dst[i], 0, 0, 0.0});
     const uint32_t to = layout_for(used_count_);
-    hipStream_t s = ctx_->stream;
+    hipStream_t s = s_;
     AG_HIP(hipMemcpyAsync(desc_.p, mv.data(), mv.size() * sizeof(PprAdmit), hipMemcpyHostToDevice, s));
     if (to != blk_) {
         // The packed width switches the split layout: one in-place rewrite
@@ -1459,7 +1460,7 @@ void PprBatch::choose_layout()
     const uint32_t want = layout_for(hi_);
     if (want == blk_) return;
     if (used_count_ > pending_.size()) { // running columns hold data
-        hipStream_t s = ctx_->stream;
+        hipStream_t s = s_;
         const uint32_t rows = kRewriteBytes / (cap_ * 4);
         for (int b = 0; b < 2; b++)
             hipLaunchKernelGGL(k_ppr_rewrite_rows, dim3(ceil_div(N_, rows)), dim3(kRelayBlock), 0, s,
@@ -1475,7 +1476,7 @@ void PprBatch::flush()
 {
     if (!dirty_) return;
     choose_layout();
-    hipStream_t s = ctx_->stream;
+    hipStream_t s = s_;
     std::vector<PprAdmit> adm;
     if (!stop_.empty()) {
         std::sort(stop_.begin(), stop_.end());
@@ -1568,7 +1569,7 @@ void PprBatch::launch_sweep(bool from0)
     const uint32_t g1 = (uint32_t)ctx_->num_cus * (uint32_t)nb;
     const uint32_t rows_per_block = (64 / lpr / kSlots) * kWaves;
     const uint32_t g2 = std::max<uint32_t>(1, std::min<uint32_t>(g1, ceil_div(p_->ppr_nmulti, rows_per_block)));
-    hipStream_t s = ctx_->stream;
+    hipStream_t s = s_;
     if (fmt_ == AG_PREC_FP32) launch_ct<float>(lpr, a, g1, g2, s);
     else launch_ct<bf16_t>(lpr, a, g1, g2, s);
     const uint32_t nparts = g1 + (a.nmulti ? g2 : 0u);
@@ -1580,7 +1581,7 @@ void PprBatch::sweep(uint32_t n, bool replay, bool timed)
 {
     flush();
     if (used_count_ == 0 || n == 0) return;
-    hipStream_t s = ctx_->stream;
+    hipStream_t s = s_;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (timed) {
         AG_HIP(hipEventCreate(&e0));
@@ -1636,8 +1637,8 @@ std::vector<PprBatch::Done> PprBatch::poll()
     flush();
     std::vector<PprCol> cs(hi_);
     AG_HIP(hipMemcpyAsync(cs.data(), cols_.p, hi_ * sizeof(PprCol), hipMemcpyDeviceToHost,
-                          ctx_->stream));
-    AG_HIP(hipStreamSynchronize(ctx_->stream));
+                          s_));
+    AG_HIP(hipStreamSynchronize(s_));
     uint32_t hint = 8, tail = 0;
     bool known = false;
     for (uint32_t k = 0; k < hi_; k++) {
@@ -1663,7 +1664,7 @@ void PprBatch::extract_to(uint32_t slot, float *dst)
 {
     if (!used(slot)) fail(AG_ERR_INVALID, "slot %u not in use", slot);
     if (!dst) return;
-    hipLaunchKernelGGL(k_ppr_extract_vo, dim3(ceil_div(N_, 256)), dim3(256), 0, ctx_->stream,
+    hipLaunchKernelGGL(k_ppr_extract_vo, dim3(ceil_div(N_, 256)), dim3(256), 0, s_,
                        X_.as<float>(), p_->rank.as<uint32_t>(), N_, cap_, slot, dst);
     AG_HIP(hipGetLastError());
 }
@@ -1672,7 +1673,7 @@ void PprBatch::extract(uint32_t slot, float *host_out)
 {
     if (!used(slot)) fail(AG_ERR_INVALID, "slot %u not in use", slot);
     if (!host_out) return;
-    hipStream_t s = ctx_->stream;
+    hipStream_t s = s_;
     hipLaunchKernelGGL(k_ppr_extract, dim3(ceil_div(N_, 256)), dim3(256), 0, s, X_.as<float>(),
                        p_->order.as<uint32_t>(), N_, cap_, slot, out_.as<float>());
     AG_HIP(hipGetLastError());

@@ -51,7 +51,14 @@ struct GraphEntry {
     std::unique_ptr<ag::PprBatch> batch;
     uint64_t reserved = 0;
     uint32_t cpw = 0;
+    uint32_t stream = 0; // index into the server's streams
 };
+
+// Streams of the server's batches: graph i's batch runs on stream i % 4, so
+// the sweeps that one step queues for several graphs overlap on the device
+// (a small graph's sweeps need not wait behind a large one's).  Default
+// priority, like every stream of the library.
+constexpr uint32_t kServeStreams = 4;
 
 } // namespace
 
@@ -68,9 +75,15 @@ struct ag_server {
     std::vector<std::unique_ptr<ResultBuf>> bufs;
     std::unordered_map<uint32_t, std::vector<ResultBuf *>> free_bufs;
     std::mutex mu;
+    hipStream_t streams[kServeStreams] = {};
     ~ag_server()
     {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        for (hipStream_t st : streams)
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+            }
         for (auto &b : bufs) {
             if (b->ready) (void)hipEventDestroy(b->ready);
             if (b->host) (void)hipHostFree(b->host);
@@ -123,7 +136,7 @@ bool ensure_batch(ag_server *s, GraphEntry &ge)
                               ? s->cfg.cols_per_wave
                               : ppr_autotune(s->ctx, ge.g, s->cfg.precision, cap, true);
     ge.batch.reset(new PprBatch(s->ctx, ge.g, cap, ge.cpw, s->cfg.precision, s->cfg.damp,
-                                s->cfg.extrapolate != 0, true));
+                                s->cfg.extrapolate != 0, true, s->streams[ge.stream]));
     ge.reserved = PprBatch::bytes_for(s->ctx, ge.g, cap, s->cfg.precision);
     s->reserved += ge.reserved;
     return true;
@@ -208,7 +221,7 @@ void retire(ag_server *s, uint32_t gi)
     for (auto &f : fin) {
         auto qi = s->queries.find(f.tag);
         if (qi == s->queries.end() || qi->second.state != QState::Running) continue;
-        if (qi->second.res) AG_HIP(hipEventRecord(qi->second.res->ready, s->ctx->stream));
+        if (qi->second.res) AG_HIP(hipEventRecord(qi->second.res->ready, ge.batch->stream()));
         qi->second.stat.iterations = f.iters;
         qi->second.stat.residual = f.residual;
         qi->second.stat.fine_sweep = f.fine_at;
@@ -244,6 +257,8 @@ int ag_serve_create(ag_ctx *ctx, const ag_serve_config *cfg, ag_server **out)
         if (s->cfg.damp == 0.0f) s->cfg.damp = 0.85f;
         if (s->cfg.precision != AG_PREC_FP32 && s->cfg.precision != AG_PREC_BF16)
             fail(AG_ERR_INVALID, "served columns store FP32 or BF16 (precision %u)", s->cfg.precision);
+        AG_HIP(hipSetDevice(ctx->device));
+        for (hipStream_t &st : s->streams) AG_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         *out = s.release();
     });
 }
@@ -272,6 +287,7 @@ int ag_serve_add_graph(ag_server *s, ag_graph *g, uint32_t *id)
         ensure_plan(s->ctx, g, nullptr);
         GraphEntry ge;
         ge.g = g;
+        ge.stream = (uint32_t)(s->graphs.size() % kServeStreams);
         s->graphs.push_back(std::move(ge));
         if (id) *id = (uint32_t)(s->graphs.size() - 1);
     });
@@ -326,7 +342,8 @@ int ag_serve_step(ag_server *s, uint32_t *active)
         admit(s);
         for (uint32_t gi = 0; gi < s->graphs.size(); gi++) {
             GraphEntry &ge = s->graphs[gi];
-            // sweeps of all graphs are queued back to back; retire() waits.
+            // sweeps of all graphs are queued first, each graph's on its own
+            // stream so that they overlap; retire() waits per graph.
             // Two sweeps per step keep admissions prompt; a full batch with
             // queries waiting admits nothing before its first column
             // finishes, so it sweeps until then.
