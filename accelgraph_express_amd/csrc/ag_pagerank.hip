@@ -616,64 +616,105 @@ __device__ __forceinline__ float tab_mark(float s, uint32_t fl)
     return __uint_as_float(__float_as_uint(s) | (fl & kDdFlag));
 }
 
+// Lane sub-rows: sub-row first + (j*16 + w)*64 + lane of [first, end) for
+// j < kLaneRows.  lane_bounds loads their index-word bounds; lane_sums adds
+// each one sequentially in edge order (sub-rows longer than lane_max entries
+// are skipped: their own units cover them) with the first kTabDepth index
+// words of all of them in flight together.
+__device__ __forceinline__ void lane_bounds(const SweepArgs &a, uint32_t first, uint32_t end,
+                                            uint32_t (&p0)[kLaneRows], uint32_t (&p1)[kLaneRows])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneRows; j++) {
+        const uint32_t k = first + (j * kSweepWaves + (uint32_t)w) * 64 + lane;
+        p0[j] = p1[j] = 0;
+        if (k < end) {
+            p0[j] = a.tab_sub_start[k];
+            p1[j] = a.tab_sub_start[k + 1];
+        }
+    }
+}
+
+template <typename CT, bool DD, bool COLD>
+__device__ __forceinline__ void lane_sums(const SweepArgs &a, const CT *__restrict__ cin,
+                                          const float *tab, uint32_t first,
+                                          uint32_t (&p0)[kLaneRows], uint32_t (&p1)[kLaneRows],
+                                          uint32_t dd0)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    bool mine[kLaneRows];
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneRows; j++) {
+        mine[j] = p1[j] > p0[j] && p1[j] - p0[j] <= a.tab_lane_max4;
+        if (!mine[j]) p1[j] = p0[j]; // long sub-rows belong to their own units
+    }
+    typename IdxVec<COLD>::T v[kLaneRows][kTabDepth];
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneRows; j++)
+#pragma unroll
+        for (int t = 0; t < kTabDepth; t++)
+            if (p0[j] + t < p1[j]) v[j][t] = tab_raw<COLD>(a, p0[j] + t);
+    float s[kLaneRows];
+    uint32_t fl[kLaneRows];
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneRows; j++) {
+        s[j] = 0.0f;
+        fl[j] = dd0;
+#pragma unroll
+        for (int t = 0; t < kTabDepth; t++)
+            if (p0[j] + t < p1[j]) tab_add4<CT, DD, COLD>(s[j], fl[j], cin, tab, v[j][t]);
+    }
+    // the rest of the longer lane sub-rows, sequentially in edge order
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneRows; j++) {
+        for (uint32_t p = p0[j] + kTabDepth; p < p1[j]; p += kTabDepth) {
+            typename IdxVec<COLD>::T x[kTabDepth];
+#pragma unroll
+            for (int t = 0; t < kTabDepth; t++)
+                if (p + t < p1[j]) x[t] = tab_raw<COLD>(a, p + t);
+#pragma unroll
+            for (int t = 0; t < kTabDepth; t++)
+                if (p + t < p1[j]) tab_add4<CT, DD, COLD>(s[j], fl[j], cin, tab, x[t]);
+        }
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kLaneRows; j++) {
+        const uint32_t k = (j * kSweepWaves + (uint32_t)w) * 64 + lane;
+        if (mine[j]) a.tab_part[first + k] = DD ? tab_mark(s[j], fl[j]) : s[j];
+    }
+}
+
+// A run of an LDS table's lane units: sub-rows [first, end) in chunks of
+// kLaneUnitRows, each chunk's index-word bounds loaded while the previous
+// chunk's index words are in flight.
+template <typename CT, bool DD>
+__device__ void lane_run(const SweepArgs &a, const CT *__restrict__ cin, const float *tab,
+                         uint32_t first, uint32_t end, uint32_t dd0)
+{
+    uint32_t p0[kLaneRows], p1[kLaneRows];
+    lane_bounds(a, first, end, p0, p1);
+    for (uint32_t c = first; c < end; c += kLaneUnitRows) {
+        uint32_t q0[kLaneRows], q1[kLaneRows];
+        lane_bounds(a, c + kLaneUnitRows, end, q0, q1);
+        lane_sums<CT, DD, false>(a, cin, tab, c, p0, p1, dd0);
+#pragma unroll
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            p0[j] = q0[j];
+            p1[j] = q1[j];
+        }
+    }
+}
+
 template <typename CT, bool DD, bool COLD>
 __device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const float *tab,
                          uint32_t kind, uint32_t first, uint32_t count, uint32_t dd0)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (kind == kUnitLane) {
-        // up to kLaneRows sub-rows per lane (first + (j*16 + w)*64 + lane); the
-        // first kTabDepth index words of all of them are in flight together
         uint32_t p0[kLaneRows], p1[kLaneRows];
-        bool mine[kLaneRows];
-#pragma unroll
-        for (uint32_t j = 0; j < kLaneRows; j++) {
-            const uint32_t k = (j * kSweepWaves + (uint32_t)w) * 64 + lane;
-            p0[j] = p1[j] = 0;
-            if (k < count) {
-                p0[j] = a.tab_sub_start[first + k];
-                p1[j] = a.tab_sub_start[first + k + 1];
-            }
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kLaneRows; j++) {
-            mine[j] = p1[j] > p0[j] && p1[j] - p0[j] <= a.tab_lane_max4;
-            if (!mine[j]) p1[j] = p0[j]; // long sub-rows belong to their own units
-        }
-        typename IdxVec<COLD>::T v[kLaneRows][kTabDepth];
-#pragma unroll
-        for (uint32_t j = 0; j < kLaneRows; j++)
-#pragma unroll
-            for (int t = 0; t < kTabDepth; t++)
-                if (p0[j] + t < p1[j]) v[j][t] = tab_raw<COLD>(a, p0[j] + t);
-        float s[kLaneRows];
-        uint32_t fl[kLaneRows];
-#pragma unroll
-        for (uint32_t j = 0; j < kLaneRows; j++) {
-            s[j] = 0.0f;
-            fl[j] = dd0;
-#pragma unroll
-            for (int t = 0; t < kTabDepth; t++)
-                if (p0[j] + t < p1[j]) tab_add4<CT, DD, COLD>(s[j], fl[j], cin, tab, v[j][t]);
-        }
-        // the rest of the longer lane sub-rows, sequentially in edge order
-#pragma unroll
-        for (uint32_t j = 0; j < kLaneRows; j++) {
-            for (uint32_t p = p0[j] + kTabDepth; p < p1[j]; p += kTabDepth) {
-                typename IdxVec<COLD>::T x[kTabDepth];
-#pragma unroll
-                for (int t = 0; t < kTabDepth; t++)
-                    if (p + t < p1[j]) x[t] = tab_raw<COLD>(a, p + t);
-#pragma unroll
-                for (int t = 0; t < kTabDepth; t++)
-                    if (p + t < p1[j]) tab_add4<CT, DD, COLD>(s[j], fl[j], cin, tab, x[t]);
-            }
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < kLaneRows; j++) {
-            const uint32_t k = (j * kSweepWaves + (uint32_t)w) * 64 + lane;
-            if (mine[j]) a.tab_part[first + k] = DD ? tab_mark(s[j], fl[j]) : s[j];
-        }
+        lane_bounds(a, first, first + count, p0, p1);
+        lane_sums<CT, DD, COLD>(a, cin, tab, first, p0, p1, dd0);
         return;
     }
     if (kind == kUnitHub) {
@@ -796,7 +837,6 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
     // ended: the kernel boundary orders the two), run by the first workgroup
     // done with its static range; this sweep's combine kernel starts after
     // it, so the state it sets (convergence, teleport, formats) is in place.
-    __shared__ uint32_t s_next;
     uint32_t cur = 0xffffffffu;
     const uint32_t u0 = a.tab_wg_begin[blockIdx.x], u1 = a.tab_wg_begin[blockIdx.x + 1];
     for (uint32_t ui = u0; ui < u1; ui++) {
@@ -811,8 +851,21 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
             __syncthreads();
             cur = t;
         }
+        if (kind == kUnitLane) {
+            // the run of this table's lane units inside [ui, u1): consecutive
+            // units cover consecutive kLaneUnitRows-chunks of sub-rows up to
+            // un.w, the end of the table's sub-rows
+            const uint32_t n = min(u1 - ui, (un.w - un.y + kLaneUnitRows - 1) / kLaneUnitRows);
+            lane_run<CT, DD>(a, cin, tab, un.y, min(un.w, un.y + n * kLaneUnitRows), dd0);
+            ui += n - 1;
+            continue;
+        }
         tab_unit<CT, DD, false>(a, cin, tab, kind, un.y, un.z, dd0);
     }
+    // cold phase, one unit per ticket (taking the next ticket before a unit's
+    // work was slower: a workgroup then holds a unit it has not started
+    // while others run out of work)
+    __shared__ uint32_t s_next;
     for (;;) {
         __syncthreads(); // s_next is free again
         if (threadIdx.x == 0) s_next = atomicAdd(&st->ticket, 1u);
@@ -1873,7 +1926,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
         const uint32_t lrows = t < nt ? kLaneUnitRows : kSweepBlock;
         for (uint32_t k = tp->SB[t] + H[t]; k < tp->SB[t + 1]; k += lrows)
             units.push_back(make_uint4(tt | kUnitLane, k,
-                                       std::min<uint32_t>(lrows, tp->SB[t + 1] - k), 0u));
+                                       std::min<uint32_t>(lrows, tp->SB[t + 1] - k), tp->SB[t + 1]));
     }
     tp->nunits = (uint32_t)units.size();
     tp->hub_items.alloc(std::max<size_t>(hub_items.size(), 1) * sizeof(uint4));
