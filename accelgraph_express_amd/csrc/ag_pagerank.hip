@@ -706,48 +706,35 @@ __device__ void lane_run(const SweepArgs &a, const CT *__restrict__ cin, const f
     }
 }
 
-template <typename CT, bool DD, bool COLD>
-__device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const float *tab,
-                         uint32_t kind, uint32_t first, uint32_t count, uint32_t dd0)
+// Long sub-rows, G lanes each, kLongItems per lane group: item = (sub-row,
+// first position, end position); group grp of the workgroup takes items
+// first + r*ngrp + grp of [first, end).  long_items loads them (zero items
+// past `end`); long_sums has the first kTabDepth words per lane of every item
+// in flight together; lane gl of a group adds positions gl + t*G
+// (+ k*kTabDepth*G) into its partial t, the partials are combined pairwise
+// and then over the group by shuffles.
+__device__ __forceinline__ void long_items(const SweepArgs &a, uint32_t first, uint32_t end,
+                                           uint32_t G, uint4 (&it)[kLongItems])
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (kind == kUnitLane) {
-        uint32_t p0[kLaneRows], p1[kLaneRows];
-        lane_bounds(a, first, first + count, p0, p1);
-        lane_sums<CT, DD, COLD>(a, cin, tab, first, p0, p1, dd0);
-        return;
-    }
-    if (kind == kUnitHub) {
-        // one wave per chunk: item = (first position, end position, slot)
-        if ((uint32_t)w >= count) return;
-        const uint4 hi = a.tab_hub_items[first + w];
-        uint32_t fl = 0u;
-        float s = tab_strided_sum<CT, DD, COLD>(a, cin, tab, hi.x + lane, hi.y, 64, fl);
-        s = wave_sum(s);
-        if (DD) {
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
-        }
-        if (lane == 0) a.hub_partial[hi.z] = DD ? tab_mark(s, fl) : s;
-        return;
-    }
-    // long sub-rows, G lanes each, kLongItems per lane group: item = (sub-row,
-    // first position, end position).  The first kTabDepth words per lane of
-    // every item are in flight together; lane gl of a group adds positions
-    // gl + t*G (+ k*kTabDepth*G) into its partial t, the partials are combined
-    // pairwise and then over the group by shuffles.
-    const uint32_t G = kind == kUnitG4 ? 4u : kind == kUnitG16 ? 16u : 64u;
     const uint32_t per_wave = 64u / G;
-    const uint32_t gl = (uint32_t)lane & (G - 1);
     const uint32_t grp = (uint32_t)w * per_wave + (uint32_t)lane / G;
     const uint32_t ngrp = kSweepWaves * per_wave;
-    uint4 it[kLongItems];
-    typename IdxVec<COLD>::T v[kLongItems][kTabDepth];
 #pragma unroll
     for (uint32_t r = 0; r < kLongItems; r++) {
-        const uint32_t k = r * ngrp + grp;
-        it[r] = k < count ? a.tab_items[first + k] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t k = first + r * ngrp + grp;
+        it[r] = k < end ? a.tab_items[k] : make_uint4(0u, 0u, 0u, 0u);
     }
+}
+
+template <typename CT, bool DD, bool COLD>
+__device__ __forceinline__ void long_sums(const SweepArgs &a, const CT *__restrict__ cin,
+                                          const float *tab, uint32_t G,
+                                          const uint4 (&it)[kLongItems], uint32_t dd0)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t gl = (uint32_t)lane & (G - 1);
+    typename IdxVec<COLD>::T v[kLongItems][kTabDepth];
 #pragma unroll
     for (uint32_t r = 0; r < kLongItems; r++)
 #pragma unroll
@@ -782,7 +769,60 @@ __device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const f
         for (uint32_t off = G / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, (int)off, 64);
         if (DD)
             for (uint32_t off = G / 2; off > 0; off >>= 1) fl |= __shfl_xor(fl, (int)off, 64);
-        if (r * ngrp + grp < count && gl == 0) a.tab_part[it[r].x] = DD ? tab_mark(sum, fl) : sum;
+        // an item past the range is all zero: it.z == 0 stores nothing
+        if (it[r].z != 0u && gl == 0) a.tab_part[it[r].x] = DD ? tab_mark(sum, fl) : sum;
+    }
+}
+
+template <typename CT, bool DD, bool COLD>
+__device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const float *tab,
+                         uint32_t kind, uint32_t first, uint32_t count, uint32_t dd0)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (kind == kUnitLane) {
+        uint32_t p0[kLaneRows], p1[kLaneRows];
+        lane_bounds(a, first, first + count, p0, p1);
+        lane_sums<CT, DD, COLD>(a, cin, tab, first, p0, p1, dd0);
+        return;
+    }
+    if (kind == kUnitHub) {
+        // one wave per chunk: item = (first position, end position, slot)
+        if ((uint32_t)w >= count) return;
+        const uint4 hi = a.tab_hub_items[first + w];
+        uint32_t fl = 0u;
+        float s = tab_strided_sum<CT, DD, COLD>(a, cin, tab, hi.x + lane, hi.y, 64, fl);
+        s = wave_sum(s);
+        if (DD) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
+        }
+        if (lane == 0) a.hub_partial[hi.z] = DD ? tab_mark(s, fl) : s;
+        return;
+    }
+    // long sub-rows: G lanes each, kLongItems per lane group
+    const uint32_t G = kind == kUnitG4 ? 4u : kind == kUnitG16 ? 16u : 64u;
+    uint4 it[kLongItems];
+    long_items(a, first, first + count, G, it);
+    long_sums<CT, DD, COLD>(a, cin, tab, G, it, dd0);
+}
+
+// A run of an LDS table's long units of one class: items [first, end) in
+// chunks of one unit, each chunk's items loaded while the previous chunk's
+// index words are in flight.
+template <typename CT, bool DD>
+__device__ void long_run(const SweepArgs &a, const CT *__restrict__ cin, const float *tab,
+                         uint32_t kind, uint32_t first, uint32_t end, uint32_t dd0)
+{
+    const uint32_t G = kind == kUnitG4 ? 4u : kind == kUnitG16 ? 16u : 64u;
+    const uint32_t chunk = kSweepWaves * (64u / G) * kLongItems;
+    uint4 it[kLongItems];
+    long_items(a, first, end, G, it);
+    for (uint32_t c = first; c < end; c += chunk) {
+        uint4 nx[kLongItems];
+        long_items(a, c + chunk, end, G, nx);
+        long_sums<CT, DD, false>(a, cin, tab, G, it, dd0);
+#pragma unroll
+        for (uint32_t r = 0; r < kLongItems; r++) it[r] = nx[r];
     }
 }
 
@@ -857,6 +897,15 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
             // un.w, the end of the table's sub-rows
             const uint32_t n = min(u1 - ui, (un.w - un.y + kLaneUnitRows - 1) / kLaneUnitRows);
             lane_run<CT, DD>(a, cin, tab, un.y, min(un.w, un.y + n * kLaneUnitRows), dd0);
+            ui += n - 1;
+            continue;
+        }
+        if (kind != kUnitHub) {
+            // likewise a run of one class's long units (items up to un.w)
+            const uint32_t G = kind == kUnitG4 ? 4u : kind == kUnitG16 ? 16u : 64u;
+            const uint32_t chunk = kSweepWaves * (64u / G) * kLongItems;
+            const uint32_t n = min(u1 - ui, (un.w - un.y + chunk - 1) / chunk);
+            long_run<CT, DD>(a, cin, tab, kind, un.y, min(un.w, un.y + n * chunk), dd0);
             ui += n - 1;
             continue;
         }
@@ -1919,7 +1968,8 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
                                    sorted_sub.as<uint32_t>() + f, n, tp->sub_start.as<uint32_t>(),
                                    tp->items.as<uint4>() + lpos);
             for (uint32_t k = 0; k < n; k += per_unit[c])
-                units.push_back(make_uint4(tt | c, lpos + k, std::min<uint32_t>(per_unit[c], n - k), 0u));
+                units.push_back(make_uint4(tt | c, lpos + k, std::min<uint32_t>(per_unit[c], n - k),
+                                           lpos + n));
             lpos += n;
         }
         // cold lane units are smaller: the kernel takes them one at a time
