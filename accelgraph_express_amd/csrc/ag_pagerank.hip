@@ -943,7 +943,7 @@ constexpr int kCombBlock = 256;
 // Combine records read through the constant address space: uniform loads of
 // memory the kernel never writes, so they become scalar loads.
 typedef __attribute__((address_space(4))) const uint32_t ConstU32;
-template <bool DD, uint32_t NTP>
+template <bool DD, uint32_t NTP, uint32_t GRP>
 __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t it)
 {
     __shared__ double s_err[kCombBlock / 64], s_dang[kCombBlock / 64];
@@ -962,28 +962,33 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
         const uint32_t wv = __builtin_amdgcn_readfirstlane(blockIdx.x * (kCombBlock / 64) + w);
         if (wv < a.tab_nwaves) {
             const uint32_t r = a.n_hub + wv * 64 + lane;
-            // the wave's record: NTP lane masks (two words each), NTP slot bases
-            const ConstU32 *rec = (const ConstU32 *)(a.tab_wtab + (size_t)wv * (3 * NTP));
-            uint64_t m[NTP];
-#pragma unroll
-            for (uint32_t t = 0; t < NTP; t++) m[t] = ((uint64_t)rec[2 * t + 1] << 32) | rec[2 * t];
-            // every partial load is issued before the first add; a lane
-            // without a sub-row in table t reads the slot after the previous
-            // lane's (inside the table's range, or the slack slot after the
-            // last table) and takes 0 instead
-            float pv[NTP];
-#pragma unroll
-            for (uint32_t t = 0; t < NTP; t++) {
-                const uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(m[t] >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)m[t], 0u));
-                const char *pt = reinterpret_cast<const char *>(a.tab_part + rec[2 * NTP + t]);
-                pv[t] = *reinterpret_cast<const float *>(pt + slot * 4u);
-            }
+            // the wave's records, in groups of GRP tables: GRP lane masks (two
+            // words each), then GRP slot bases.  One group's records at a time
+            // (a loop, not unrolled: their scalar registers stay within
+            // budget); inside a group every partial load is issued before the
+            // first add.  A lane without a sub-row in table t reads the slot
+            // after the previous lane's (inside the table's range, or the
+            // slack slot after the last table) and takes 0 instead.
             float s = 0.0f;
             uint32_t fl = dd0;
+#pragma unroll 1
+            for (uint32_t g0 = 0; g0 < NTP; g0 += GRP) {
+                const ConstU32 *rec = (const ConstU32 *)(a.tab_wtab + (size_t)wv * (3 * NTP) + 3 * g0);
+                uint64_t m[GRP];
 #pragma unroll
-            for (uint32_t t = 0; t < NTP; t++)
-                acc<DD>(s, fl, __builtin_amdgcn_inverse_ballot_w64(m[t]) ? pv[t] : 0.0f);
+                for (uint32_t t = 0; t < GRP; t++) m[t] = ((uint64_t)rec[2 * t + 1] << 32) | rec[2 * t];
+                float pv[GRP];
+#pragma unroll
+                for (uint32_t t = 0; t < GRP; t++) {
+                    const uint32_t slot = __builtin_amdgcn_mbcnt_hi(
+                        (uint32_t)(m[t] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[t], 0u));
+                    const char *pt = reinterpret_cast<const char *>(a.tab_part + rec[2 * GRP + t]);
+                    pv[t] = *reinterpret_cast<const float *>(pt + slot * 4u);
+                }
+#pragma unroll
+                for (uint32_t t = 0; t < GRP; t++)
+                    acc<DD>(s, fl, __builtin_amdgcn_inverse_ballot_w64(m[t]) ? pv[t] : 0.0f);
+            }
             if (r < a.n_nonzero) update_vertex(a, r, s * rscale, tele, cout, wfmt, err, act, dang, fl);
         }
     } else if (blockIdx.x < a.tab_row_blocks + a.tab_hub_blocks) {
@@ -1503,21 +1508,23 @@ __global__ void k_tab_classes(const uint32_t *__restrict__ sub_row, const uint32
 // Lane bits of the combine records: the row of sub-row sid is lane
 // (row - n_hub) % 64 of wave (row - n_hub) / 64.
 __global__ void k_tab_wmask(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sub_tab,
-                            uint32_t S, uint32_t n_hub, uint32_t NTpad, uint32_t *__restrict__ wtab)
+                            uint32_t S, uint32_t n_hub, uint32_t NTpad, uint32_t grp,
+                            uint32_t *__restrict__ wtab)
 {
     const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
     if (sid >= S) return;
     const uint32_t r = sub_row[sid];
     if (r < n_hub) return;
     const uint32_t q = r - n_hub, lane = q & 63u;
-    uint32_t *rec = wtab + (size_t)(q >> 6) * 3u * NTpad;
-    atomicOr(&rec[2u * sub_tab[sid] + (lane >> 5)], 1u << (lane & 31u));
+    const uint32_t t = sub_tab[sid];
+    uint32_t *rec = wtab + (size_t)(q >> 6) * 3u * NTpad + 3u * (t / grp) * grp;
+    atomicOr(&rec[2u * (t % grp) + (lane >> 5)], 1u << (lane & 31u));
 }
 
 // Slot bases of the combine records: the first sub-row of table t whose row
 // is >= the first row of 64-row wave w (rows ascend inside a table).
 __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sb,
-                            uint32_t NT, uint32_t NTpad, uint32_t nwaves, uint32_t n_hub,
+                            uint32_t NT, uint32_t NTpad, uint32_t grp, uint32_t nwaves, uint32_t n_hub,
                             uint32_t *__restrict__ wtab)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1530,7 +1537,7 @@ __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t
         if (sub_row[mid] < row0) lo = mid + 1;
         else hi = mid;
     }
-    wtab[(size_t)w * 3u * NTpad + 2u * NTpad + t] = lo;
+    wtab[(size_t)w * 3u * NTpad + 3u * (t / grp) * grp + 2u * grp + t % grp] = lo;
 }
 
 // Cost of each work unit in padded index positions (lane units: per wave,
@@ -2029,16 +2036,20 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     tp->nwaves = ceil_div(nnh, 64);
     tp->row_blocks = ceil_div(tp->nwaves, kCombBlock / 64);
     tp->hub_blocks = ceil_div(std::min(n_hub, nrows), kCombBlock / 64);
-    tp->NTpad = NT <= 4 ? 4u : NT <= 8 ? 8u : NT <= 17 ? 17u : kMaxTabNT;
+    // tables per combine record (a multiple of the group the kernel loads at
+    // once): 4, 8, 12, 17, 2 x 12 or 2 x 17
+    tp->NTpad = NT <= 4 ? 4u : NT <= 8 ? 8u : NT <= 12 ? 12u : NT <= 17 ? 17u : NT <= 24 ? 24u : 34u;
+    tp->grp = tp->NTpad == 24 ? 12u : tp->NTpad == 34 ? 17u : tp->NTpad;
+    static_assert(kMaxTabNT <= 34, "combine records hold at most 34 tables");
     const size_t wrec = ((size_t)tp->nwaves * 3 * tp->NTpad + 1) * 4;
     tp->wtab.alloc(wrec);
     AG_HIP(hipMemsetAsync(tp->wtab.p, 0, wrec, s));
     hipLaunchKernelGGL(k_tab_wmask, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row.as<uint32_t>(),
-                       sub_tab.as<uint32_t>(), S, n_hub, tp->NTpad, tp->wtab.as<uint32_t>());
+                       sub_tab.as<uint32_t>(), S, n_hub, tp->NTpad, tp->grp, tp->wtab.as<uint32_t>());
     if (tp->nwaves)
         hipLaunchKernelGGL(k_tab_bases, dim3(ceil_div((uint64_t)tp->nwaves * NT, 256)), dim3(256), 0,
-                           s, sub_row.as<uint32_t>(), sbd.as<uint32_t>(), NT, tp->NTpad, tp->nwaves,
-                           n_hub, tp->wtab.as<uint32_t>());
+                           s, sub_row.as<uint32_t>(), sbd.as<uint32_t>(), NT, tp->NTpad, tp->grp,
+                           tp->nwaves, n_hub, tp->wtab.as<uint32_t>());
     AG_HIP(hipGetLastError());
     tp->tab_edges.assign(NT, 0);
     {
@@ -2275,11 +2286,12 @@ static void enqueue_gather(hipStream_t s, const Plan *p, const SweepArgs &a, uin
         // the combine kernel's table loops are unrolled to the records' NTpad
         const void *kc;
         switch (tp.NTpad) {
-        case 4: kc = dd ? (const void *)k_pr_combine<true, 4> : (const void *)k_pr_combine<false, 4>; break;
-        case 8: kc = dd ? (const void *)k_pr_combine<true, 8> : (const void *)k_pr_combine<false, 8>; break;
-        case 17: kc = dd ? (const void *)k_pr_combine<true, 17> : (const void *)k_pr_combine<false, 17>; break;
-        default:
-            kc = dd ? (const void *)k_pr_combine<true, kMaxTabNT> : (const void *)k_pr_combine<false, kMaxTabNT>;
+        case 4: kc = dd ? (const void *)k_pr_combine<true, 4, 4> : (const void *)k_pr_combine<false, 4, 4>; break;
+        case 8: kc = dd ? (const void *)k_pr_combine<true, 8, 8> : (const void *)k_pr_combine<false, 8, 8>; break;
+        case 12: kc = dd ? (const void *)k_pr_combine<true, 12, 12> : (const void *)k_pr_combine<false, 12, 12>; break;
+        case 17: kc = dd ? (const void *)k_pr_combine<true, 17, 17> : (const void *)k_pr_combine<false, 17, 17>; break;
+        case 24: kc = dd ? (const void *)k_pr_combine<true, 24, 12> : (const void *)k_pr_combine<false, 24, 12>; break;
+        default: kc = dd ? (const void *)k_pr_combine<true, 34, 17> : (const void *)k_pr_combine<false, 34, 17>;
         }
         void *kargs[] = {(void *)&a, (void *)&it};
         AG_HIP(hipLaunchKernel(kc, dim3(a.nparts), dim3(kCombBlock), kargs, 0, s));

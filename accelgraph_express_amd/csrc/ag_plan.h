@@ -54,7 +54,8 @@ struct TabPlan {
     uint32_t hub_chunk = 0;  // edges per hub-row chunk (multiple of 4)
     uint32_t hub_chunks = 0;
     uint32_t nwaves = 0;     // 64-row waves of the combine kernel over the non-hub rows
-    uint32_t NTpad = 0;      // tables per combine record (NT rounded up to 4, 8, 17 or 32)
+    uint32_t NTpad = 0;      // tables per combine record (NT rounded up to 4, 8, 12, 17, 24 or 34)
+    uint32_t grp = 0;        // tables per group of a record (the combine kernel loads one at a time)
     uint32_t row_blocks = 0; // combine blocks over those waves
     uint32_t hub_blocks = 0; // combine blocks over the hub rows (one wave per row)
     uint64_t idx16_len = 0, idx32_len = 0;
@@ -66,10 +67,11 @@ struct TabPlan {
     // items: sub-rows sorted by (table, class), the long ones' lists;
     // hub_items: (sub-row, chunk, output slot) per hub chunk; chunk_first,
     // hub_partial: hub rows' chunk slots, in (table, chunk) order per row;
-    // wtab: per 64-row wave of the non-hub rows, 3 * NTpad words: for each
-    // table the lanes (rows) with a sub-row in it (two words: lanes 0-31,
-    // 32-63), then for each table the first sub-row of the table at or after
-    // the wave's first row.  Tables NT .. NTpad - 1 have no lanes.
+    // wtab: per 64-row wave of the non-hub rows, 3 * NTpad words in groups of
+    // grp tables: for each table of the group the lanes (rows) with a
+    // sub-row in it (two words: lanes 0-31, 32-63), then for each table of the
+    // group the first sub-row of the table at or after the wave's first row.
+    // Tables NT .. NTpad - 1 have no lanes.
     DevBuf idx16, idx32, sub_start, part, units, wg_begin, items, hub_items, chunk_first,
         hub_partial, wtab;
     uint64_t bytes() const
