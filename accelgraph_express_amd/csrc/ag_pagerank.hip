@@ -537,7 +537,7 @@ constexpr uint32_t kUnitLane = 0, kUnitG4 = 1, kUnitG16 = 2, kUnitG64 = 3, kUnit
 constexpr int kFinBlock = 1024; // threads of the finalize workgroup
 static_assert(kFinBlock == kSweepBlock, "the table kernel's workgroups run the finalize body");
 __device__ void finalize_body(const SweepArgs &a, uint32_t it);
-constexpr uint32_t kMaxTabNT = 32; // LDS tables + the cold table (row mask bits)
+constexpr uint32_t kMaxTabNT = 63; // LDS tables + the cold table (tuning allows 62 LDS tables)
 constexpr uint32_t kLaneRows = 4;                           // sub-rows per lane of a lane unit
 constexpr uint32_t kLaneUnitRows = kSweepBlock * kLaneRows; // sub-rows per lane unit
 constexpr int kTabDepth = 4;                    // index loads in flight per lane
@@ -2037,10 +2037,11 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     tp->row_blocks = ceil_div(tp->nwaves, kCombBlock / 64);
     tp->hub_blocks = ceil_div(std::min(n_hub, nrows), kCombBlock / 64);
     // tables per combine record (a multiple of the group the kernel loads at
-    // once): 4, 8, 12, 17, 2 x 12 or 2 x 17
-    tp->NTpad = NT <= 4 ? 4u : NT <= 8 ? 8u : NT <= 12 ? 12u : NT <= 17 ? 17u : NT <= 24 ? 24u : 34u;
-    tp->grp = tp->NTpad == 24 ? 12u : tp->NTpad == 34 ? 17u : tp->NTpad;
-    static_assert(kMaxTabNT <= 34, "combine records hold at most 34 tables");
+    // once): 4, 8, 12, 17, 2 x 12, or 2, 3 or 4 x 17
+    tp->NTpad = NT <= 4 ? 4u : NT <= 8 ? 8u : NT <= 12 ? 12u : NT <= 17 ? 17u : NT <= 24 ? 24u
+              : NT <= 34 ? 34u : NT <= 51 ? 51u : 68u;
+    tp->grp = tp->NTpad == 24 ? 12u : tp->NTpad > 17 ? 17u : tp->NTpad;
+    static_assert(kMaxTabNT <= 68, "combine records hold at most 68 tables");
     const size_t wrec = ((size_t)tp->nwaves * 3 * tp->NTpad + 1) * 4;
     tp->wtab.alloc(wrec);
     AG_HIP(hipMemsetAsync(tp->wtab.p, 0, wrec, s));
@@ -2291,7 +2292,9 @@ static void enqueue_gather(hipStream_t s, const Plan *p, const SweepArgs &a, uin
         case 12: kc = dd ? (const void *)k_pr_combine<true, 12, 12> : (const void *)k_pr_combine<false, 12, 12>; break;
         case 17: kc = dd ? (const void *)k_pr_combine<true, 17, 17> : (const void *)k_pr_combine<false, 17, 17>; break;
         case 24: kc = dd ? (const void *)k_pr_combine<true, 24, 12> : (const void *)k_pr_combine<false, 24, 12>; break;
-        default: kc = dd ? (const void *)k_pr_combine<true, 34, 17> : (const void *)k_pr_combine<false, 34, 17>;
+        case 34: kc = dd ? (const void *)k_pr_combine<true, 34, 17> : (const void *)k_pr_combine<false, 34, 17>; break;
+        case 51: kc = dd ? (const void *)k_pr_combine<true, 51, 17> : (const void *)k_pr_combine<false, 51, 17>; break;
+        default: kc = dd ? (const void *)k_pr_combine<true, 68, 17> : (const void *)k_pr_combine<false, 68, 17>;
         }
         void *kargs[] = {(void *)&a, (void *)&it};
         AG_HIP(hipLaunchKernel(kc, dim3(a.nparts), dim3(kCombBlock), kargs, 0, s));
@@ -2798,7 +2801,7 @@ int ag_pagerank_autotune(ag_ctx *ctx, ag_graph *g, uint32_t sweeps, ag_tuning *b
             struct Knob { uint32_t Tuning::*f; std::vector<uint32_t> vals; };
             std::vector<Knob> knobs;
             if (best_plan->tab) {
-                knobs.push_back({&Tuning::max_tables, {8, 12, 16, 20, 24, 31}});
+                knobs.push_back({&Tuning::max_tables, {8, 12, 16, 20, 24, 31, 40, 48, 62}});
                 knobs.push_back({&Tuning::lane_max, {8, 12, 16, 24}});
             } else {
                 knobs.push_back({&Tuning::low_max, {16, 32, 64, 128}});

@@ -147,7 +147,7 @@ int ag_set_tuning(const ag_tuning *t)
         if (t->lane_max) n.lane_max = t->lane_max;
         if (n.layout != AG_LAYOUT_TABLES && n.layout != AG_LAYOUT_ROWS && n.layout != AG_LAYOUT_AUTO)
             fail(AG_ERR_INVALID, "layout must be AG_LAYOUT_AUTO, AG_LAYOUT_TABLES or AG_LAYOUT_ROWS");
-        if (n.max_tables < 1 || n.max_tables > 31) fail(AG_ERR_INVALID, "max_tables must be 1..31");
+        if (n.max_tables < 1 || n.max_tables > 62) fail(AG_ERR_INVALID, "max_tables must be 1..62");
         if (n.lane_max < 4 || n.lane_max > 64 || n.lane_max % 4)
             fail(AG_ERR_INVALID, "lane_max must be a multiple of 4 in 4..64");
         if (n.hot_bytes % 16 || n.hot_bytes > 160 * 1024 - 2048)

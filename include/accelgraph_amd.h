@@ -294,7 +294,7 @@ typedef struct ag_tuning
                              contributions (multiple of 16; 1 disables the table) */
     uint32_t layout;      /* gather layout of the pull sweep: AG_LAYOUT_AUTO (default),
                              AG_LAYOUT_TABLES or AG_LAYOUT_ROWS; 0 keeps the current one */
-    uint32_t max_tables;  /* partitioned layout: at most this many LDS tables (1..31) */
+    uint32_t max_tables;  /* partitioned layout: at most this many LDS tables (1..62) */
     uint32_t lane_max;    /* partitioned layout: sub-rows of up to this many edges are
                              summed by one lane (multiple of 4, 4..64; default 16) */
 } ag_tuning;
@@ -315,7 +315,7 @@ int ag_get_tuning(ag_tuning *t);
 /* Chooses g's gather layout by timing candidates on this device (no
  * counterpart in the reference, whose gather has no layout to tune): from g's
  * current tuning (the process-wide one unless autotuned before), one pass of
- * coordinate search over max_tables {8,12,16,20,24,31} and lane_max
+ * coordinate search over max_tables {8,12,16,20,24,31,40,48,62} and lane_max
  * {8,12,16,24} (partitioned tables; low_max {16,32,64,128} for the row
  * layout), hub_min {4096,6144,8192,12288} and hub_chunk {auto,1024,2048,4096}.
  * Each candidate layout is built and timed over `sweeps` (1..1000) FP32 pull
@@ -323,7 +323,7 @@ int ag_get_tuning(ag_tuning *t);
  * at least 1% faster.  The fastest becomes g's layout and g's own tuning (later
  * rebuilds use it), reported by ag_pagerank_plan_info; *best (optional)
  * receives it and *ms (optional) the wall time of the search.  Builds the
- * layout up to 19 times (about 20 ms each at 4 M vertices and 67 M edges) and
+ * layout up to 22 times (about 20 ms each at 4 M vertices and 67 M edges) and
  * holds two layouts at once; it must not overlap other calls on g, nor run
  * while a server holds g (its batches point at the layout). */
 int ag_pagerank_autotune(ag_ctx *ctx, ag_graph *g, uint32_t sweeps, ag_tuning *best, double *ms);

@@ -83,6 +83,33 @@ def test_reference_mode_parity(ctx, name, kw):
                   act_e[:k].astype(np.int64)).max() <= 8
 
 
+@pytest.mark.parametrize("max_tables", [20, 40])
+def test_many_tables_grouped_records(ctx, max_tables):
+    """More tables than one combine group holds (17): the records come in
+    groups loaded one after another and the partials are still added in table
+    order.  R-MAT scale 21 has enough sources for 40 tables of 40,440."""
+    src, dst = ag.gen_rmat(2021, 21, 16)
+    n = 1 << 21
+    print(f"case: s21 max_tables={max_tables}", flush=True)
+    with tuned(layout="tables", max_tables=max_tables):
+        g = ag.Graph.build(ctx, src, dst, n)
+        g.prepare()
+        info = g.plan_info()
+        st = g.pagerank_pull(iterations=20, epsilon=0.0)
+    print(f"plan: tables {info['tables']} subrows {info['subrows']} cold {info['cold_edges']}")
+    assert info["tables"] > 17
+    csr = oracle.build_csr(src, dst, n)
+    ref, it, err, psum, act, _ = oracle.pagerank_pull_f32(csr, 20, 0.0)
+    f64, _, _ = oracle.pagerank_pull_f64(csr, 20, 0.0)
+    _check(st.pageRanks, ref, f64)
+    np.testing.assert_array_equal(st.active_per_iter, act)
+    # the same sums as with the default table count, up to summation order
+    g.free()
+    g2 = ag.Graph.build(ctx, src, dst, n)
+    base = g2.pagerank_pull(iterations=20, epsilon=0.0).pageRanks
+    assert _rel(st.pageRanks, base.astype(np.float64)).max() <= 2 * REL_TOL
+
+
 def test_single_table_low_rows_bit_exact(ctx):
     """All sources in table 0 and every row short: each row is one sub-row
     summed by one lane in the reference's edge order."""
