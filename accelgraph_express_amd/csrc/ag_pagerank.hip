@@ -1219,19 +1219,22 @@ __global__ void k_spmv_init(SweepArgs a, const float *__restrict__ x,
 
 // Un-relabel (and, in REFERENCE mode, divide by N: pageRank.c:1079-1085) into
 // the caller's order; per-block FP64 partial sums of the returned ranks.
+// Output in vertex order: out[v] = pr[rank[v]] (coalesced writes; the
+// label-order ranks are read at random, from cache), optionally / N, and
+// per-block FP64 partial sums in vertex order.
 __global__ __launch_bounds__(kBlock) void k_pr_finish(const float *__restrict__ pr,
-                                                      const uint32_t *__restrict__ order,
+                                                      const uint32_t *__restrict__ rank,
                                                       uint32_t N, uint32_t divide,
                                                       float *__restrict__ out,
                                                       double *__restrict__ part)
 {
     __shared__ double s[kBlock / 64];
     double acc = 0.0;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < N) {
-        float x = pr[i];
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < N) {
+        float x = pr[rank[v]];
         if (divide) x = x / (float)N;
-        out[order[i]] = x;
+        out[v] = x;
         acc = (double)x;
     }
     acc = wave_sum(acc);
@@ -2467,7 +2470,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         AG_HIP(hipGetLastError());
     };
     auto enqueue_finish = [&] {
-        hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr, p->order.as<uint32_t>(),
+        hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr, p->rank.as<uint32_t>(),
                            N, ref_scale ? 1u : 0u, d_out, parts);
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
         AG_HIP(hipGetLastError());
@@ -2637,7 +2640,7 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
     const uint32_t nbf = ceil_div(N, kBlock);
     double *parts = w.sum_part;
     hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr,
-                       p->order.as<uint32_t>(), N, 0u, d_y, parts);
+                       p->rank.as<uint32_t>(), N, 0u, d_y, parts);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
     AG_HIP(hipGetLastError());
     AG_HIP(hipEventRecord(ev[3], s));
