@@ -468,6 +468,32 @@ def _ppr_bounds(np, torch, src_s, lengths, od_d, N, xs, seeds, damp=0.85):
     return out
 
 
+def _ppr_roofline(st, N, E, prec, ms):
+    """Batch-level roofline of a batched PPR run (the in-library time of the
+    whole batch, every kernel included).  Compulsory bytes: each sweep streams
+    the source indices, row offsets and out-degrees (4E + 8N + 4); each active
+    column reads and writes its FP32 iterate (8N) and writes and reads its
+    contributions once per sweep: 8N in FP32, 4N for a BF16 column's coarse
+    sweeps (high halves only).  A BF16 column is coarse until fine_sweep, or
+    for all of its sweeps when it stopped coarse (fine_sweep 0).  The
+    gather-inclusive rate counts every edge's gather of each active column
+    (2 bytes coarse, 4 fine) as a read, most of which hit L2 or MALL."""
+    sweeps = max(s.iterations for s in st)
+    b = sweeps * (4 * E + 8 * N + 4)
+    gath = 0
+    for s in st:
+        coarse = (s.fine_sweep or s.iterations) if prec == "bf16" else 0
+        b += coarse * 12 * N + (s.iterations - coarse) * 16 * N
+        gath += E * (2 * coarse + 4 * (s.iterations - coarse))
+    gbs = b / (ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "algorithmic_bytes_per_batch": int(b), "batch_ms_in_library": round(ms, 2),
+            "sweeps": sweeps,
+            "model": "sweeps*(4E+8N+4) + per column-sweep 16N (FP32) or 12N (BF16 coarse)",
+            "gather_inclusive_GBps": round((gath + b) / (ms / 1e3) / 1e9, 1)}
+
+
 def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
     """64 PPR queries, 1-16 uniform seeds each, per-column stop at L1 residual
     `tol`, all 64 as the columns of one batch vs one query at a time.
@@ -479,6 +505,7 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
     compares the paths at the loosest tolerance where the largest bound of
     both is <= 1e-4."""
     import torch
+    E = int(src.size)
     rng = np.random.RandomState(seed)
     seeds = [rng.randint(0, N, rng.randint(1, 17)).astype(np.uint32) for _ in range(nq)]
     cpw = {p: ag.ppr_autotune(g, p) for p in ("bf16", "fp32")}
@@ -562,6 +589,8 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
                                  "fp32": round(float(np.mean([s.iterations for s in st_32])), 2),
                                  "single": round(float(np.mean([s.iterations for s in st_1])), 2)},
              "bf16_fine_sweep_mean": round(float(np.mean([s.fine_sweep for s in st_bf])), 2),
+             "roofline_bf16": _ppr_roofline(st_bf, N, E, "bf16", ms_bf),
+             "roofline_fp32": _ppr_roofline(st_32, N, E, "fp32", ms_32),
              "fp64_error_bound": {"bf16": bounds("bf16", t), "fp32": bounds("fp32", t),
                                   "single": bounds("single", t)},
              "extrapolated": {
@@ -570,6 +599,7 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
                  "speedup_vs_single": round(t_1 / t_bx, 2),
                  "batch_ms": {"bf16_wall": round(t_bx * 1e3, 2),
                               "bf16_in_library": round(ms_bx, 2)},
+                 "roofline_bf16": _ppr_roofline(st_bx, N, E, "bf16", ms_bx),
                  "latency_ms_p50_bf16": round(_pct(lat_x, 50), 3),
                  "latency_ms_p99_bf16": round(_pct(lat_x, 99), 3),
                  "iterations_mean": {
@@ -598,16 +628,16 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
         cands = []
         if r["fp64_error_bound"]["bf16"]["max"] <= 1e-4:
             cands.append(("bf16", r["qps_batched_bf16"], r["latency_ms_p99_bf16"],
-                          r["fp64_error_bound"]["bf16"]["max"]))
+                          r["fp64_error_bound"]["bf16"]["max"], r["roofline_bf16"]))
         if r["extrapolated"]["fp64_error_bound"]["bf16"]["max"] <= 1e-4:
             x = r["extrapolated"]
             cands.append(("bf16 extrapolated", x["qps_batched_bf16"], x["latency_ms_p99_bf16"],
-                          x["fp64_error_bound"]["bf16"]["max"]))
+                          x["fp64_error_bound"]["bf16"]["max"], x["roofline_bf16"]))
         if not cands:
             continue
         best = max(cands, key=lambda c: c[1])
         head = {"residual_tol": r["residual_tol"], "batched": best[0], "qps_batched": best[1],
-                "latency_ms_p99": best[2], "max_error_batched": best[3],
+                "latency_ms_p99": best[2], "max_error_batched": best[3], "roofline": best[4],
                 "qps_single_fp32": r["qps_single_fp32"],
                 "max_error_single": r["fp64_error_bound"]["single"]["max"],
                 "speedup_vs_single_fp32": round(best[1] / r["qps_single_fp32"], 2)}
