@@ -202,17 +202,19 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 // contribution carries bit 31 (the sign bit; contributions are >= 0) so that
 // the next sweep's gathers find its out-neighbours.  That is the reference's
 // workListNext marking over forward edges, read from the pull side.
-__device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, float s,
-                                              float tele, void *cout, uint32_t wfmt,
-                                              double &err, uint32_t &act, double &dang,
-                                              uint32_t flag = 0u)
+// update_vertex_v takes the old rank pr[i] and the out-degree outdeg[i] as
+// arguments (a caller can load them before its gather); update_vertex loads
+// them itself.
+__device__ __forceinline__ void update_vertex_v(const SweepArgs &a, uint32_t i, float s,
+                                                float old, uint32_t od, float tele, void *cout,
+                                                uint32_t wfmt, double &err, uint32_t &act,
+                                                double &dang, uint32_t flag = 0u)
 {
 #pragma clang fp contract(off)
     if (a.mode == kModeSpmv) { // SPMV pull: y[i] = gathered sum
         a.pr[i] = s;
         return;
     }
-    const float old = a.pr[i];
     if (a.mode == AG_PR_DATA_DRIVEN) {
         float keep = old;
         uint32_t mark = 0u;
@@ -227,7 +229,6 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
                 mark = kDdFlag;
             }
         }
-        const uint32_t od = a.outdeg[i];
         const float c = od ? keep / (float)od : 0.0f;
         static_cast<uint32_t *>(cout)[i] = __float_as_uint(c) | mark;
         return;
@@ -254,7 +255,6 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
         if (w != 0.0f) xo = nw + w * (nw - old);
     }
     a.pr[i] = xo;
-    const uint32_t od = a.outdeg[i];
     if (a.mode == AG_PR_REFERENCE) {
         const double e = fabs((double)(nw - old));
         err += e; // divided by N once, in k_pr_finalize
@@ -265,6 +265,16 @@ __device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, fl
     }
     const float c = od ? xo / (float)od : 0.0f;
     store_c(cout, wfmt, i, c, a.f16_scale);
+}
+
+__device__ __forceinline__ void update_vertex(const SweepArgs &a, uint32_t i, float s,
+                                              float tele, void *cout, uint32_t wfmt,
+                                              double &err, uint32_t &act, double &dang,
+                                              uint32_t flag = 0u)
+{
+    const bool spmv = a.mode == kModeSpmv;
+    update_vertex_v(a, i, s, spmv ? 0.0f : a.pr[i], spmv ? 0u : a.outdeg[i], tele, cout, wfmt, err,
+                    act, dang, flag);
 }
 
 template <typename CT> struct FmtOf;
@@ -962,6 +972,10 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
         const uint32_t wv = __builtin_amdgcn_readfirstlane(blockIdx.x * (kCombBlock / 64) + w);
         if (wv < a.tab_nwaves) {
             const uint32_t r = a.n_hub + wv * 64 + lane;
+            // the update's loads go out with the partial loads below
+            const bool live = r < a.n_nonzero;
+            const float old = live && a.mode != kModeSpmv ? a.pr[r] : 0.0f;
+            const uint32_t od = live && a.mode != kModeSpmv ? a.outdeg[r] : 0u;
             // the wave's records, in groups of GRP tables: GRP lane masks (two
             // words each), then GRP slot bases.  One group's records at a time
             // (a loop, not unrolled: their scalar registers stay within
@@ -989,7 +1003,7 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
                 for (uint32_t t = 0; t < GRP; t++)
                     acc<DD>(s, fl, __builtin_amdgcn_inverse_ballot_w64(m[t]) ? pv[t] : 0.0f);
             }
-            if (r < a.n_nonzero) update_vertex(a, r, s * rscale, tele, cout, wfmt, err, act, dang, fl);
+            if (live) update_vertex_v(a, r, s * rscale, old, od, tele, cout, wfmt, err, act, dang, fl);
         }
     } else if (blockIdx.x < a.tab_row_blocks + a.tab_hub_blocks) {
         // hub rows, one wave each: lane k adds the row's chunk partials k,
