@@ -282,6 +282,9 @@ def main() -> int:
                    "step": "hipGraph replay of init + sweeps + normalise",
                    "eager_step_ms_with_sweep_events": round(min(eager), 4),
                    "csr_build_ms": round(csr_ms, 2), "gather_layout_ms": round(plan_ms, 2),
+                   # a single call on a graph whose gather layout is not built
+                   # yet pays the layout once (later calls reuse it)
+                   "first_call_GTEPS_incl_layout": round(E * iters / ((ms_step + plan_ms) / 1e3) / 1e9, 2),
                    "plan": info, **({"autotune": tune} if tune else {}),
                    **({"serialization": ser} if ser else {}),
                    **({"data_driven_pull": dd} if dd else {}),
