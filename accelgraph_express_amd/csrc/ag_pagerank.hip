@@ -1560,38 +1560,41 @@ __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t
 // Cost of each work unit in padded index positions (lane units: per wave,
 // 64 lanes times the largest per-lane total over its kLaneRows sub-rows; long
 // and hub units: their positions), for the static split over workgroups.
-__global__ void k_tab_unit_cost(const uint4 *__restrict__ units, uint32_t nunits,
+__global__ __launch_bounds__(kSweepBlock) void k_tab_unit_cost(const uint4 *__restrict__ units, uint32_t nunits,
                                 const uint32_t *__restrict__ sub_start,
                                 const uint4 *__restrict__ litems,
                                 const uint4 *__restrict__ hub_items, uint32_t lane_max4,
                                 uint32_t *__restrict__ cost)
 {
-    const uint32_t ui = blockIdx.x * blockDim.x + threadIdx.x;
+    // one workgroup per unit, thread (w, lane) as in the table kernel
+    __shared__ unsigned long long s_c;
+    const uint32_t ui = blockIdx.x;
     if (ui >= nunits) return;
+    if (threadIdx.x == 0) s_c = 0;
+    __syncthreads();
     const uint4 un = units[ui];
     const uint32_t kind = un.x & 0xffu;
-    uint64_t c = 0;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     if (kind == kUnitLane) {
-        for (uint32_t w = 0; w < kSweepWaves; w++) {
-            uint32_t mx = 0;
-            for (uint32_t l = 0; l < 64; l++) {
-                uint32_t tot = 0;
-                for (uint32_t j = 0; j < kLaneRows; j++) {
-                    const uint32_t k = (j * kSweepWaves + w) * 64 + l;
-                    if (k >= un.z) continue;
-                    const uint32_t l4 = sub_start[un.y + k + 1] - sub_start[un.y + k];
-                    if (l4 <= lane_max4) tot += l4;
-                }
-                mx = max(mx, tot);
-            }
-            c += 64ull * mx;
+        uint32_t tot = 0;
+        for (uint32_t j = 0; j < kLaneRows; j++) {
+            const uint32_t k = (j * kSweepWaves + w) * 64 + lane;
+            if (k >= un.z) continue;
+            const uint32_t l4 = sub_start[un.y + k + 1] - sub_start[un.y + k];
+            if (l4 <= lane_max4) tot += l4;
         }
-    } else if (kind == kUnitHub) {
-        for (uint32_t k = 0; k < un.z; k++) c += hub_items[un.y + k].y - hub_items[un.y + k].x;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot = max(tot, (uint32_t)__shfl_xor((int)tot, off, 64));
+        if (lane == 0) atomicAdd(&s_c, 64ull * tot);
     } else {
-        for (uint32_t k = 0; k < un.z; k++) c += litems[un.y + k].z - litems[un.y + k].y;
+        unsigned long long c = 0;
+        for (uint32_t k = threadIdx.x; k < un.z; k += blockDim.x)
+            c += kind == kUnitHub ? hub_items[un.y + k].y - hub_items[un.y + k].x
+                                  : litems[un.y + k].z - litems[un.y + k].y;
+        if (c) atomicAdd(&s_c, c);
     }
-    cost[ui] = (uint32_t)min<uint64_t>(c, 0xffffffffull);
+    __syncthreads();
+    if (threadIdx.x == 0) cost[ui] = (uint32_t)min<unsigned long long>(s_c, 0xffffffffull);
 }
 
 // Long-sub-row items (sub-row, first position, end position) of one
@@ -2021,7 +2024,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     std::vector<uint32_t> cost(units.size());
     if (!units.empty()) {
         DevBuf dcost(units.size() * 4);
-        hipLaunchKernelGGL(k_tab_unit_cost, dim3(ceil_div(units.size(), 256)), dim3(256), 0, s,
+        hipLaunchKernelGGL(k_tab_unit_cost, dim3((uint32_t)units.size()), dim3(kSweepBlock), 0, s,
                            tp->units.as<uint4>(), (uint32_t)units.size(),
                            tp->sub_start.as<uint32_t>(), tp->items.as<uint4>(),
                            tp->hub_items.as<uint4>(), lane_max4, dcost.as<uint32_t>());
