@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "ag_extrap.h"
@@ -187,6 +188,78 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Sums by DPP lane moves (in registers, no LDS), each in a fixed order:
+// quad_sum / row_sum leave the sum over the lane's group of 4 / 16 lanes in
+// every lane of the group; wave_sum63 leaves the sum over all 64 lanes in
+// lane 63 (row sums, then row 0+1 into row 1 and row 2+3 into row 3 by
+// row_bcast:15, then rows 0+1 into row 3 by row_bcast:31).  Lanes a DPP move
+// does not write (outside its row mask) add 0.  *_or: the same for OR.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, ROWMASK, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f32(float x)
+{
+    return __uint_as_float(dpp_u32<CTRL, ROWMASK>(__float_as_uint(x)));
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double x)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = dpp_u32<CTRL, ROWMASK>((uint32_t)b), hi = dpp_u32<CTRL, ROWMASK>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+constexpr int kDppQuad1 = 0xB1, kDppQuad2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128,
+              kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+template <int CTRL, int ROWMASK, typename T> __device__ __forceinline__ T dpp_move(T x)
+{
+    if constexpr (sizeof(T) == 8) return dpp_f64<CTRL, ROWMASK>(x);
+    else if constexpr (std::is_same<T, float>::value) return dpp_f32<CTRL, ROWMASK>(x);
+    else return dpp_u32<CTRL, ROWMASK>(x);
+}
+template <typename T> __device__ __forceinline__ T quad_sum(T x)
+{
+    x += dpp_move<kDppQuad1, 0xf>(x);
+    x += dpp_move<kDppQuad2, 0xf>(x);
+    return x;
+}
+template <typename T> __device__ __forceinline__ T row_sum(T x)
+{
+    x = quad_sum(x);
+    x += dpp_move<kDppRor4, 0xf>(x);
+    x += dpp_move<kDppRor8, 0xf>(x);
+    return x;
+}
+template <typename T> __device__ __forceinline__ T wave_sum63(T x)
+{
+    x = row_sum(x);
+    x += dpp_move<kDppBcast15, 0xa>(x);
+    x += dpp_move<kDppBcast31, 0xc>(x);
+    return x;
+}
+__device__ __forceinline__ uint32_t quad_or(uint32_t x)
+{
+    x |= dpp_u32<kDppQuad1, 0xf>(x);
+    x |= dpp_u32<kDppQuad2, 0xf>(x);
+    return x;
+}
+__device__ __forceinline__ uint32_t row_or(uint32_t x)
+{
+    x = quad_or(x);
+    x |= dpp_u32<kDppRor4, 0xf>(x);
+    x |= dpp_u32<kDppRor8, 0xf>(x);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_or63(uint32_t x)
+{
+    x = row_or(x);
+    x |= dpp_u32<kDppBcast15, 0xa>(x);
+    x |= dpp_u32<kDppBcast31, 0xc>(x);
     return x;
 }
 
@@ -1007,29 +1080,26 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
         }
     } else if (blockIdx.x < a.tab_row_blocks + a.tab_hub_blocks) {
         // hub rows, one wave each: lane k adds the row's chunk partials k,
-        // k + 64, ... in order, then a shuffle tree (a fixed order)
+        // k + 64, ... in order, then a DPP sum into lane 63 (a fixed order)
         const uint32_t row = (blockIdx.x - a.tab_row_blocks) * (kCombBlock / 64) + (uint32_t)w;
         if (row < a.n_hub) {
             const uint32_t f0 = a.chunk_first[row], f1 = a.chunk_first[row + 1];
             float sp = 0.0f;
             uint32_t fl = dd0;
             for (uint32_t q = f0 + lane; q < f1; q += 64) acc<DD>(sp, fl, a.hub_partial[q]);
-            sp = wave_sum(sp);
-            if (DD) {
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
-            }
-            if (lane == 0) update_vertex(a, row, sp * rscale, tele, cout, wfmt, err, act, dang, fl);
+            sp = wave_sum63(sp);
+            if (DD) fl = wave_or63(fl);
+            if (lane == 63) update_vertex(a, row, sp * rscale, tele, cout, wfmt, err, act, dang, fl);
         }
     } else if (!a.skip_zero) {
         const uint32_t i = (blockIdx.x - a.tab_row_blocks - a.tab_hub_blocks) * kCombBlock + threadIdx.x;
         if (i < a.N - a.n_nonzero)
             update_vertex(a, a.n_nonzero + i, 0.0f, tele, cout, wfmt, err, act, dang, dd0);
     }
-    err = wave_sum(err);
-    dang = wave_sum(dang);
-    act = wave_sum(act);
-    if (lane == 0) { s_err[w] = err; s_dang[w] = dang; s_act[w] = act; }
+    err = wave_sum63(err);
+    dang = wave_sum63(dang);
+    act = wave_sum63(act);
+    if (lane == 63) { s_err[w] = err; s_dang[w] = dang; s_act[w] = act; }
     __syncthreads();
     if (threadIdx.x == 0) {
         double e = 0.0, d = 0.0;
