@@ -325,7 +325,11 @@ int ag_get_tuning(ag_tuning *t);
  * receives it and *ms (optional) the wall time of the search.  Builds the
  * layout up to 22 times (about 20 ms each at 4 M vertices and 67 M edges) and
  * holds two layouts at once; it must not overlap other calls on g, nor run
- * while a server holds g (its batches point at the layout). */
+ * while a server holds g (its batches point at the layout).  A layout fixes
+ * the summation order of the ranks (every result stays within the parity
+ * bar); two searches may choose differently when candidates time within
+ * noise, so a run that must be bit-reproducible sets the reported tuning
+ * with ag_set_tuning instead of searching. */
 int ag_pagerank_autotune(ag_ctx *ctx, ag_graph *g, uint32_t sweeps, ag_tuning *best, double *ms);
 
 /* ---- batched personalized PageRank -------------------------------------- */
