@@ -125,6 +125,8 @@ struct SweepArgs {
     // partitioned-table layout (TabPlan; see k_pr_tables / k_pr_combine)
     uint32_t tab_T, tab_nt, tab_NT, tab_cold_base4, tab_nwaves, tab_lane_max4;
     uint32_t tab_cold_unit0, tab_nunits; // units [cold_unit0, nunits) are the cold table's
+    uint32_t tab_cold_lane_u0, tab_cold_lane_s0, tab_cold_lane_s1; // cold lane units: unit index
+                                         // of the first, its first sub-row, the cold table's end
     uint32_t tab_row_blocks;             // combine blocks over the non-hub rows
     uint32_t tab_hub_blocks;             // combine blocks over the hub rows (one wave each)
     uint32_t fin_hubs;                   // k_pr_finalize sums the hub rows (row layout)
@@ -1009,8 +1011,15 @@ __global__ __launch_bounds__(kSweepBlock) void k_pr_tables(SweepArgs a, uint32_t
         }
         const uint32_t ui = a.tab_cold_unit0 + tk - fin_prev;
         if (ui >= a.tab_nunits) break;
-        const uint4 un = a.tab_units[ui];
-        tab_unit<CT, DD, true>(a, cin, tab, un.x & 0xffu, un.y, un.z, dd0);
+        if (ui >= a.tab_cold_lane_u0) {
+            // cold lane units are consecutive kSweepBlock-sub-row ranges: no
+            // descriptor load on the ticket's critical path
+            const uint32_t f = a.tab_cold_lane_s0 + (ui - a.tab_cold_lane_u0) * kSweepBlock;
+            tab_unit<CT, DD, true>(a, cin, tab, kUnitLane, f, min(kSweepBlock, a.tab_cold_lane_s1 - f), dd0);
+        } else {
+            const uint4 un = a.tab_units[ui];
+            tab_unit<CT, DD, true>(a, cin, tab, un.x & 0xffu, un.y, un.z, dd0);
+        }
     }
 }
 
@@ -2076,6 +2085,17 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
                                        std::min<uint32_t>(lrows, tp->SB[t + 1] - k), tp->SB[t + 1]));
     }
     tp->nunits = (uint32_t)units.size();
+    // the cold table's lane units close the list: consecutive kSweepBlock
+    // ranges of sub-rows (the kernel computes them from the unit index)
+    tp->cold_lane_u0 = tp->nunits;
+    while (tp->cold_lane_u0 > 0 && units[tp->cold_lane_u0 - 1].x == ((nt << 8) | kUnitLane))
+        tp->cold_lane_u0--;
+    tp->cold_lane_s0 = tp->cold_lane_u0 < tp->nunits ? units[tp->cold_lane_u0].y : 0u;
+    tp->cold_lane_s1 = tp->SB[NT];
+    for (uint32_t u = tp->cold_lane_u0; u < tp->nunits; u++)
+        if (units[u].y != tp->cold_lane_s0 + (u - tp->cold_lane_u0) * kSweepBlock ||
+            units[u].z != std::min<uint32_t>(kSweepBlock, tp->cold_lane_s1 - units[u].y))
+            fail(AG_ERR_INVALID, "table layout: cold lane units are not consecutive ranges");
     tp->hub_items.alloc(std::max<size_t>(hub_items.size(), 1) * sizeof(uint4));
     if (!hub_items.empty())
         AG_HIP(hipMemcpyAsync(tp->hub_items.p, hub_items.data(), hub_items.size() * sizeof(uint4),
@@ -2310,6 +2330,9 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
         a.tab_nwaves = tp.nwaves;
         a.tab_lane_max4 = p->lane_max / 4;
         a.tab_cold_unit0 = tp.cold_unit0;
+        a.tab_cold_lane_u0 = tp.cold_lane_u0;
+        a.tab_cold_lane_s0 = tp.cold_lane_s0;
+        a.tab_cold_lane_s1 = tp.cold_lane_s1;
         a.tab_nunits = tp.nunits;
         a.tab_idx16 = tp.idx16.as<uint16_t>();
         a.tab_idx32 = tp.idx32.as<uint32_t>();

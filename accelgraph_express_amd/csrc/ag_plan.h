@@ -50,6 +50,8 @@ struct TabPlan {
     uint32_t cold_base4 = 0; // sub_start value at which the cold table's indices begin
     uint32_t nunits = 0;     // work units of the table kernel
     uint32_t cold_unit0 = 0; // units [cold_unit0, nunits) belong to the cold table
+    uint32_t cold_lane_u0 = 0, cold_lane_s0 = 0, cold_lane_s1 = 0; // cold lane units: first unit,
+                                                            // its first sub-row, the cold table's end
     uint32_t grid = 0;       // workgroups of the table kernel (one per CU)
     uint32_t hub_chunk = 0;  // edges per hub-row chunk (multiple of 4)
     uint32_t hub_chunks = 0;
