@@ -144,11 +144,31 @@ struct SweepArgs {
 namespace {
 
 __device__ __forceinline__ float load_c(const float *p, uint32_t i) { return p[i]; }
+
+// Element i of a device array addressed by a 32-bit byte offset from the
+// array's base: with the base in scalar registers a load or store then takes
+// one 32-bit offset per lane instead of a 64-bit address.  Callers keep every
+// such array below 4 GiB (the partitioned-table layout checks its arrays).
+template <typename T> __device__ __forceinline__ const T &at32(const T *base, uint32_t i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ T &at32(T *base, uint32_t i)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (uint32_t)sizeof(T));
+}
 __device__ __forceinline__ float load_c(const bf16_t *p, uint32_t i)
 {
     return __uint_as_float((uint32_t)p[i].x << 16);
 }
 __device__ __forceinline__ float load_c(const __half *p, uint32_t i) { return __half2float(p[i]); }
+// The same through a 32-bit byte offset (contribution buffers stay below 4 GiB).
+__device__ __forceinline__ float load_c32(const float *p, uint32_t i) { return at32(p, i); }
+__device__ __forceinline__ float load_c32(const bf16_t *p, uint32_t i)
+{
+    return __uint_as_float((uint32_t)at32(p, i).x << 16);
+}
+__device__ __forceinline__ float load_c32(const __half *p, uint32_t i) { return __half2float(at32(p, i)); }
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 {
@@ -640,10 +660,10 @@ template <bool COLD>
 __device__ __forceinline__ typename IdxVec<COLD>::T tab_raw(const SweepArgs &a, uint32_t p)
 {
     if constexpr (COLD)
-        return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.tab_idx32) +
-                                          (p - a.tab_cold_base4));
+        return __builtin_nontemporal_load(
+            &at32(reinterpret_cast<const u32x4 *>(a.tab_idx32), p - a.tab_cold_base4));
     else
-        return __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(a.tab_idx16) + p);
+        return __builtin_nontemporal_load(&at32(reinterpret_cast<const u32x2 *>(a.tab_idx16), p));
 }
 
 template <bool COLD>
@@ -658,7 +678,7 @@ __device__ __forceinline__ uint32_t tab_idx(const typename IdxVec<COLD>::T &v, i
 template <typename CT, bool COLD>
 __device__ __forceinline__ float tab_gather(const CT *__restrict__ cin, const float *tab, uint32_t u)
 {
-    return COLD ? load_c(cin, u) : tab[u];
+    return COLD ? load_c32(cin, u) : tab[u];
 }
 
 // Adds the four entries of one index word, in order.
@@ -715,8 +735,8 @@ __device__ __forceinline__ void lane_bounds(const SweepArgs &a, uint32_t first, 
         const uint32_t k = first + (j * kSweepWaves + (uint32_t)w) * 64 + lane;
         p0[j] = p1[j] = 0;
         if (k < end) {
-            p0[j] = a.tab_sub_start[k];
-            p1[j] = a.tab_sub_start[k + 1];
+            p0[j] = at32(a.tab_sub_start, k);
+            p1[j] = at32(a.tab_sub_start, k + 1);
         }
     }
 }
@@ -766,7 +786,7 @@ __device__ __forceinline__ void lane_sums(const SweepArgs &a, const CT *__restri
 #pragma unroll
     for (uint32_t j = 0; j < kLaneRows; j++) {
         const uint32_t k = (j * kSweepWaves + (uint32_t)w) * 64 + lane;
-        if (mine[j]) a.tab_part[first + k] = DD ? tab_mark(s[j], fl[j]) : s[j];
+        if (mine[j]) at32(a.tab_part, first + k) = DD ? tab_mark(s[j], fl[j]) : s[j];
     }
 }
 
@@ -808,7 +828,7 @@ __device__ __forceinline__ void long_items(const SweepArgs &a, uint32_t first, u
 #pragma unroll
     for (uint32_t r = 0; r < kLongItems; r++) {
         const uint32_t k = first + r * ngrp + grp;
-        it[r] = k < end ? a.tab_items[k] : make_uint4(0u, 0u, 0u, 0u);
+        it[r] = k < end ? at32(a.tab_items, k) : make_uint4(0u, 0u, 0u, 0u);
     }
 }
 
@@ -855,7 +875,7 @@ __device__ __forceinline__ void long_sums(const SweepArgs &a, const CT *__restri
         if (DD)
             for (uint32_t off = G / 2; off > 0; off >>= 1) fl |= __shfl_xor(fl, (int)off, 64);
         // an item past the range is all zero: it.z == 0 stores nothing
-        if (it[r].z != 0u && gl == 0) a.tab_part[it[r].x] = DD ? tab_mark(sum, fl) : sum;
+        if (it[r].z != 0u && gl == 0) at32(a.tab_part, it[r].x) = DD ? tab_mark(sum, fl) : sum;
     }
 }
 
@@ -873,7 +893,7 @@ __device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const f
     if (kind == kUnitHub) {
         // one wave per chunk: item = (first position, end position, slot)
         if ((uint32_t)w >= count) return;
-        const uint4 hi = a.tab_hub_items[first + w];
+        const uint4 hi = at32(a.tab_hub_items, first + w);
         uint32_t fl = 0u;
         float s = tab_strided_sum<CT, DD, COLD>(a, cin, tab, hi.x + lane, hi.y, 64, fl);
         s = wave_sum(s);
@@ -881,7 +901,7 @@ __device__ void tab_unit(const SweepArgs &a, const CT *__restrict__ cin, const f
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) fl |= __shfl_xor(fl, off, 64);
         }
-        if (lane == 0) a.hub_partial[hi.z] = DD ? tab_mark(s, fl) : s;
+        if (lane == 0) at32(a.hub_partial, hi.z) = DD ? tab_mark(s, fl) : s;
         return;
     }
     // long sub-rows: G lanes each, kLongItems per lane group
@@ -1970,6 +1990,11 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     uint32_t cold_base4 = total4;
     if (tp->SB[nt] < S) dl(&cold_base4, tp->sub_start.as<uint32_t>() + tp->SB[nt], 4);
     AG_HIP(hipStreamSynchronize(s));
+    // the table kernel addresses its arrays by 32-bit byte offsets (at32):
+    // larger layouts are left to the row layout
+    if ((uint64_t)total4 * 16 >= (1ull << 32) || (uint64_t)S * 16 >= (1ull << 32) ||
+        ((uint64_t)N + 1) * 4 >= (1ull << 32))
+        return;
     tp->cold_base4 = cold_base4;
     tp->idx16_len = (uint64_t)cold_base4 * 4;
     tp->idx32_len = (uint64_t)(total4 - cold_base4) * 4;
