@@ -102,7 +102,7 @@ struct ag_ctx {
     // grow on demand and are kept until the context is destroyed, so a build
     // does not pay hipMalloc / hipFree (which waits for the device) per step.
     // The mutex serialises the host threads that use it.
-    static constexpr int kScratchSlots = 16;
+    static constexpr int kScratchSlots = 24;
     std::recursive_mutex scratch_mu;
     ag::DevBuf scratch[kScratchSlots];
 };
@@ -113,17 +113,25 @@ namespace ag {
 enum ScratchSlot : int {
     kScrHist = 0, kScrCsum, kScrSortK, kScrSortV, kScrScan0, // kScrScan0 .. +3: scan levels
     kScrMax = 8, kScrPairA, kScrPairB, kScrInSrc, kScrInDst,
+    kScrLay0 = 13, // kScrLay0 .. +7: gather-layout build temporaries
 };
 // Device buffer of at least `bytes` in `slot`; growing waits for the stream
-// first, since the old buffer may still be read by queued work.
+// first, since the old buffer may still be read by queued work.  Sizes are
+// rounded up (1 MiB, plus 1 MiB of slack) so that an array of n + 1 words
+// after one of n words does not reallocate.
 inline void *scratch(ag_ctx *ctx, int slot, size_t bytes)
 {
     DevBuf &b = ctx->scratch[slot];
     if (b.bytes < bytes) {
         AG_HIP(hipStreamSynchronize(ctx->stream));
-        b.alloc(bytes);
+        constexpr size_t kMiB = (size_t)1 << 20;
+        b.alloc((bytes + kMiB - 1) / kMiB * kMiB + kMiB);
     }
     return b.p;
+}
+template <typename T> inline T *scratch_as(ag_ctx *ctx, int slot, size_t count)
+{
+    return static_cast<T *>(scratch(ctx, slot, count * sizeof(T)));
 }
 
 // Gather layout of a graph for the pull kernels (built lazily, cached).

@@ -1443,26 +1443,50 @@ __global__ void k_add_base(uint32_t *__restrict__ x, uint32_t n, uint32_t base)
     if (i < n) x[i] += base;
 }
 
-// CSR rows (hub + mid), relabelled sources in original order.  One thread per
-// edge slot e < roff[nrows]; its row is the last r with roff[r] <= e (a binary
-// search over the row offsets, which stay in cache).  One wave per row made
-// the largest hubs (~1.6e5 in-edges at R-MAT scale 22) a serial tail of ~2 ms.
-__global__ void k_fill_csr_edges(const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
-                                 const uint32_t *__restrict__ inv_idx,
-                                 const uint32_t *__restrict__ inv_dst,
-                                 const uint32_t *__restrict__ rank, uint32_t nrows, uint32_t total,
-                                 uint32_t *__restrict__ cols)
+// CSR rows, relabelled sources in original order: edge slots [first, total)
+// of rows [0, nrows) (every such row has an in-edge, total <= roff[nrows]).
+// One thread per slot; its row is the last r with roff[r] <= e.  A block's
+// 256 slots span at most 256 rows: thread 0 finds the block's first row by a
+// binary search over the row offsets, the block stages the offsets of the
+// rows it spans in LDS, and each thread searches those (8 steps).  One wave
+// per row made the largest hubs (~1.6e5 in-edges at R-MAT scale 22) a serial
+// tail of ~2 ms.
+constexpr uint32_t kFillBlock = 256;
+__global__ __launch_bounds__(kFillBlock) void k_fill_csr_edges(
+    const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
+    const uint32_t *__restrict__ inv_idx, const uint32_t *__restrict__ inv_dst,
+    const uint32_t *__restrict__ rank, uint32_t nrows, uint32_t first, uint32_t total,
+    uint32_t *__restrict__ cols)
 {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t s_off[kFillBlock + 1];
+    __shared__ uint32_t s_r0;
+    const uint32_t e0 = first + blockIdx.x * kFillBlock;
+    if (threadIdx.x == 0) {
+        uint32_t lo = 0, hi = nrows; // roff[lo] <= e0 < roff[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (roff[mid] <= e0) lo = mid;
+            else hi = mid;
+        }
+        s_r0 = lo;
+    }
+    __syncthreads();
+    const uint32_t r0 = s_r0;
+    // offsets of rows r0 .. r0 + 256 (clamped to nrows): the block's slots
+    // end before roff[min(r0 + 256, nrows)]
+    const uint32_t nr = min(kFillBlock, nrows - r0);
+    for (uint32_t i = threadIdx.x; i <= nr; i += kFillBlock) s_off[i] = roff[r0 + i];
+    __syncthreads();
+    const uint32_t e = e0 + threadIdx.x;
     if (e >= total) return;
-    uint32_t lo = 0, hi = nrows; // roff[lo] <= e < roff[hi]
+    uint32_t lo = 0, hi = nr; // s_off[lo] <= e < s_off[hi]
     while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (roff[mid] <= e) lo = mid;
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_off[mid] <= e) lo = mid;
         else hi = mid;
     }
-    const uint32_t v = order[lo];
-    cols[e] = rank[inv_dst[inv_idx[v] + (e - roff[lo])]];
+    const uint32_t v = order[r0 + lo];
+    cols[e] = rank[inv_dst[inv_idx[v] + (e - s_off[lo])]];
 }
 
 // SELL rows: one thread per row, four edges' loads in flight at a time.
@@ -1716,6 +1740,26 @@ __global__ void k_tab_litems(const uint32_t *__restrict__ sorted, uint32_t n,
     out[k] = make_uint4(sid, sub_start[sid], sub_start[sid + 1], 0u);
 }
 
+// out[i] = a[idx[i]], out[n + i] = b[idx[i]], out[2n + i] = c[idx[i]]
+__global__ void k_gather3(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                          const uint32_t *__restrict__ c, const uint32_t *__restrict__ idx,
+                          uint32_t n, uint32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = idx[i];
+    out[i] = a[k];
+    out[n + i] = b[k];
+    out[2 * (size_t)n + i] = c[k];
+}
+
+// out[i] = a[idx[i]] for i < n (one block)
+__global__ void k_gather1(const uint32_t *__restrict__ a, const uint32_t *__restrict__ idx,
+                          uint32_t n, uint32_t *__restrict__ out)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) out[i] = a[idx[i]];
+}
+
 } // namespace
 
 static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn);
@@ -1767,23 +1811,27 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     }
     const uint32_t maxdeg = device_max1(ctx, g->inv_outdeg.as<uint32_t>(), N);
     const uint32_t kbits = bits_for((uint64_t)maxdeg + 1);
-    DevBuf keys((size_t)N * 4), ids((size_t)N * 4), skeys((size_t)N * 4);
+    // temporaries in the context's scratch slots (build_tables, run at the
+    // end, leaves Lay6 / Lay7 alone)
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    uint32_t *keys = scratch_as<uint32_t>(ctx, kScrPairA, N);
+    uint32_t *ids = scratch_as<uint32_t>(ctx, kScrPairB, N);
+    uint32_t *skeys = scratch_as<uint32_t>(ctx, kScrLay0 + 7, N);
+    uint32_t *rdeg = scratch_as<uint32_t>(ctx, kScrLay0 + 6, N);
     p->rank.alloc((size_t)N * 4);
-    DevBuf rdeg((size_t)N * 4), counts(32);
+    DevBuf counts(32);
     const uint32_t nb = ceil_div(N, 256);
     hipLaunchKernelGGL(k_degree_keys, dim3(nb), dim3(256), 0, s, g->inv_outdeg.as<uint32_t>(), N,
-                       maxdeg, keys.as<uint32_t>(), ids.as<uint32_t>());
+                       maxdeg, keys, ids);
     AG_HIP(hipGetLastError());
-    radix_sort_pairs(ctx, keys.as<uint32_t>(), ids.as<uint32_t>(), skeys.as<uint32_t>(),
-                     p->order.as<uint32_t>(), N, kbits);
+    radix_sort_pairs(ctx, keys, ids, skeys, p->order.as<uint32_t>(), N, kbits);
     AG_HIP(hipMemsetAsync(counts.p, 0, 32, s));
     hipLaunchKernelGGL(k_relabel_vertex, dim3(std::min<uint32_t>(nb, (uint32_t)ctx->num_cus * 8)),
-                       dim3(256), 0, s, p->order.as<uint32_t>(),
-                       skeys.as<uint32_t>(), maxdeg, g->fwd_outdeg.as<uint32_t>(), N,
-                       p->rank.as<uint32_t>(), rdeg.as<uint32_t>(), p->outdeg.as<uint32_t>(),
-                       tn.low_max, tn.hub_min, counts.as<uint32_t>());
+                       dim3(256), 0, s, p->order.as<uint32_t>(), skeys, maxdeg,
+                       g->fwd_outdeg.as<uint32_t>(), N, p->rank.as<uint32_t>(), rdeg,
+                       p->outdeg.as<uint32_t>(), tn.low_max, tn.hub_min, counts.as<uint32_t>());
     AG_HIP(hipGetLastError());
-    scan_exclusive_u32(ctx, rdeg.as<uint32_t>(), p->roff.as<uint32_t>(), N);
+    scan_exclusive_u32(ctx, rdeg, p->roff.as<uint32_t>(), N);
     uint32_t cnt[8];
     AG_HIP(hipMemcpyAsync(cnt, counts.p, 32, hipMemcpyDeviceToHost, s));
     AG_HIP(hipStreamSynchronize(s));
@@ -1806,7 +1854,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     uint32_t sell_slots = 0;
     if (p->low_slices) {
         hipLaunchKernelGGL(k_slice_width, dim3(ceil_div(p->low_slices, 256)), dim3(256), 0, s,
-                           rdeg.as<uint32_t>(), p->n_nonlow, p->low_slices,
+                           rdeg, p->n_nonlow, p->low_slices,
                            p->slice_ptr.as<uint32_t>());
         scan_exclusive_u32(ctx, p->slice_ptr.as<uint32_t>(), p->slice_ptr.as<uint32_t>(),
                            p->low_slices);
@@ -1825,13 +1873,14 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->cols.alloc((size_t)ncols * 4);
     AG_HIP(hipMemsetAsync(p->cols.p, 0, (size_t)ncols * 4, s));
     if (p->n_nonlow && csr_edges)
-        hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(csr_edges, 256)), dim3(256), 0, s,
-                           p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
+        hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(csr_edges, kFillBlock)), dim3(kFillBlock),
+                           0, s, p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
-                           p->rank.as<uint32_t>(), p->n_nonlow, csr_edges, p->cols.as<uint32_t>());
+                           p->rank.as<uint32_t>(), p->n_nonlow, 0u, csr_edges,
+                           p->cols.as<uint32_t>());
     if (nlow)
         hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
-                           p->order.as<uint32_t>(), rdeg.as<uint32_t>(),
+                           p->order.as<uint32_t>(), rdeg,
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
                            p->rank.as<uint32_t>(), p->slice_ptr.as<uint32_t>(), p->n_nonlow, nlow,
                            p->cols.as<uint32_t>());
@@ -1840,7 +1889,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     // Hub chunk table (host: hub rows are few).
     std::vector<uint32_t> hdeg(p->n_hub);
     if (p->n_hub)
-        AG_HIP(hipMemcpyAsync(hdeg.data(), rdeg.p, (size_t)p->n_hub * 4, hipMemcpyDeviceToHost, s));
+        AG_HIP(hipMemcpyAsync(hdeg.data(), rdeg, (size_t)p->n_hub * 4, hipMemcpyDeviceToHost, s));
     AG_HIP(hipStreamSynchronize(s));
     if (tn.hub_chunk == AG_TUNE_AUTO) {
         // Hub chunks are the sweep's first work units; the persistent
@@ -1903,19 +1952,31 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     auto dl = [&](void *dst, const void *src, size_t bytes) {
         if (bytes) AG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
     };
-    // 1. relabelled sources of every row with in-edges, label order
-    DevBuf ecol((size_t)E * 4);
-    hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(E, 256)), dim3(256), 0, s,
-                       p->order.as<uint32_t>(), p->roff.as<uint32_t>(), g->inv_idx.as<uint32_t>(),
-                       g->inv_dst.as<uint32_t>(), p->rank.as<uint32_t>(), nrows, E,
-                       ecol.as<uint32_t>());
+    // Temporaries live in the context's scratch slots (kept across builds:
+    // freeing a large buffer waits for the device).  Slot use by step:
+    //   InSrc: ecol (1-7);  PairA/PairB: key/val (3), srow/flag (4),
+    //   key2/val2 (8);  InDst: fscan (4-7), sk2 (8);  Lay0/Lay1: skey/perm
+    //   (3-7);  Lay2/Lay3: sub_row/sub_tab (4-12);  Lay4: sub_e0 (4-7),
+    //   sorted_sub (8-10);  Lay5: len4 (5-9);  Lay6 is build_plan's.
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    // 1. relabelled sources of every row with in-edges, label order: the row
+    // layout's CSR part holds those of rows [0, n_nonlow) already
+    uint32_t *ecol = scratch_as<uint32_t>(ctx, kScrInSrc, E);
+    const uint32_t e_csr = (uint32_t)std::min<uint64_t>(p->csr_edges, E);
+    if (e_csr)
+        AG_HIP(hipMemcpyAsync(ecol, p->cols.p, (size_t)e_csr * 4, hipMemcpyDeviceToDevice, s));
+    if (e_csr < E)
+        hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(E - e_csr, kFillBlock)), dim3(kFillBlock),
+                           0, s, p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
+                           g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
+                           p->rank.as<uint32_t>(), nrows, e_csr, E, ecol);
     AG_HIP(hipGetLastError());
     // 2. edges per table; LDS tables while each holds enough edges
     {
         DevBuf hist(kTabHist * 4);
         AG_HIP(hipMemsetAsync(hist.p, 0, kTabHist * 4, s));
         hipLaunchKernelGGL(k_tab_hist, dim3(std::min<uint32_t>(ceil_div(E, 256), ctx->num_cus * 8)),
-                           dim3(256), 0, s, ecol.as<uint32_t>(), E, T, hist.as<uint32_t>());
+                           dim3(256), 0, s, ecol, E, T, hist.as<uint32_t>());
         AG_HIP(hipGetLastError());
         uint32_t h[kTabHist];
         dl(h, hist.p, sizeof(h));
@@ -1930,53 +1991,49 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     }
     const uint32_t nt = tp->nt, NT = tp->NT;
     // 3. stable sort of the edges by table: each table's edges stay in (row, edge) order
-    DevBuf skey((size_t)E * 4), perm((size_t)E * 4);
+    uint32_t *skey = scratch_as<uint32_t>(ctx, kScrLay0, E);
+    uint32_t *perm = scratch_as<uint32_t>(ctx, kScrLay0 + 1, E);
     {
-        DevBuf key((size_t)E * 4), val((size_t)E * 4);
-        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol.as<uint32_t>(),
-                           E, T, nt, key.as<uint32_t>(), val.as<uint32_t>());
+        uint32_t *key = scratch_as<uint32_t>(ctx, kScrPairA, E);
+        uint32_t *val = scratch_as<uint32_t>(ctx, kScrPairB, E);
+        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol, E, T, nt, key,
+                           val);
         AG_HIP(hipGetLastError());
-        radix_sort_pairs(ctx, key.as<uint32_t>(), val.as<uint32_t>(), skey.as<uint32_t>(),
-                         perm.as<uint32_t>(), E, bits_for(NT));
-        AG_HIP(hipStreamSynchronize(s));
+        radix_sort_pairs(ctx, key, val, skey, perm, E, bits_for(NT));
     }
     // 4. sub-rows: starts where (table, row) changes
-    DevBuf fscan(((size_t)E + 1) * 4);
+    uint32_t *fscan = scratch_as<uint32_t>(ctx, kScrInDst, (size_t)E + 1);
     uint32_t S = 0;
-    DevBuf sub_row, sub_tab, sub_e0;
+    uint32_t *sub_row, *sub_tab, *sub_e0;
     {
-        DevBuf srow((size_t)E * 4), flag((size_t)E * 4);
-        hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm.as<uint32_t>(),
-                           p->roff.as<uint32_t>(), nrows, E, srow.as<uint32_t>());
-        hipLaunchKernelGGL(k_tab_flags, dim3(ceil_div(E, 256)), dim3(256), 0, s,
-                           skey.as<uint32_t>(), srow.as<uint32_t>(), E, flag.as<uint32_t>());
+        uint32_t *srow = scratch_as<uint32_t>(ctx, kScrPairA, E);
+        uint32_t *flag = scratch_as<uint32_t>(ctx, kScrPairB, E);
+        hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm,
+                           p->roff.as<uint32_t>(), nrows, E, srow);
+        hipLaunchKernelGGL(k_tab_flags, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, srow, E, flag);
         AG_HIP(hipGetLastError());
-        scan_exclusive_u32(ctx, flag.as<uint32_t>(), fscan.as<uint32_t>(), E);
-        dl(&S, fscan.as<uint32_t>() + E, 4);
+        scan_exclusive_u32(ctx, flag, fscan, E);
+        dl(&S, fscan + E, 4);
         AG_HIP(hipStreamSynchronize(s));
-        sub_row.alloc((size_t)S * 4);
-        sub_tab.alloc((size_t)S * 4);
-        sub_e0.alloc(((size_t)S + 1) * 4);
-        hipLaunchKernelGGL(k_tab_subrows, dim3(ceil_div(E, 256)), dim3(256), 0, s,
-                           flag.as<uint32_t>(), fscan.as<uint32_t>(), skey.as<uint32_t>(),
-                           srow.as<uint32_t>(), E, sub_row.as<uint32_t>(), sub_tab.as<uint32_t>(),
-                           sub_e0.as<uint32_t>());
+        sub_row = scratch_as<uint32_t>(ctx, kScrLay0 + 2, S);
+        sub_tab = scratch_as<uint32_t>(ctx, kScrLay0 + 3, S);
+        sub_e0 = scratch_as<uint32_t>(ctx, kScrLay0 + 4, (size_t)S + 1);
+        hipLaunchKernelGGL(k_tab_subrows, dim3(ceil_div(E, 256)), dim3(256), 0, s, flag, fscan, skey,
+                           srow, E, sub_row, sub_tab, sub_e0);
         AG_HIP(hipGetLastError());
-        AG_HIP(hipStreamSynchronize(s));
     }
     tp->S = S;
     // 5. padded lengths and starts (units of 4 entries, all tables in one range)
-    DevBuf len4((size_t)S * 4);
+    uint32_t *len4 = scratch_as<uint32_t>(ctx, kScrLay0 + 5, S);
     tp->sub_start.alloc(((size_t)S + 1) * 4);
-    hipLaunchKernelGGL(k_tab_len4, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_e0.as<uint32_t>(), S,
-                       len4.as<uint32_t>());
+    hipLaunchKernelGGL(k_tab_len4, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_e0, S, len4);
     AG_HIP(hipGetLastError());
-    scan_exclusive_u32(ctx, len4.as<uint32_t>(), tp->sub_start.as<uint32_t>(), S);
+    scan_exclusive_u32(ctx, len4, tp->sub_start.as<uint32_t>(), S);
     // 6. first sub-row of each table
     DevBuf sbd(((size_t)NT + 1) * 4);
     AG_HIP(hipMemsetAsync(sbd.p, 0xff, ((size_t)NT + 1) * 4, s));
-    hipLaunchKernelGGL(k_tab_bounds, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_tab.as<uint32_t>(),
-                       S, sbd.as<uint32_t>());
+    hipLaunchKernelGGL(k_tab_bounds, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_tab, S,
+                       sbd.as<uint32_t>());
     AG_HIP(hipGetLastError());
     tp->SB.assign(NT + 1, 0);
     dl(tp->SB.data(), sbd.p, (size_t)NT * 4);
@@ -2006,32 +2063,25 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
                        (uint64_t)tp->idx16.bytes / 2, (uint16_t)T);
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, tp->idx32.as<uint32_t>(),
                        (uint64_t)tp->idx32.bytes / 4, N);
-    hipLaunchKernelGGL(k_tab_fill_idx, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey.as<uint32_t>(),
-                       perm.as<uint32_t>(), ecol.as<uint32_t>(), fscan.as<uint32_t>(),
-                       sub_e0.as<uint32_t>(), tp->sub_start.as<uint32_t>(), E, T, nt, cold_base4,
+    hipLaunchKernelGGL(k_tab_fill_idx, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, perm, ecol,
+                       fscan, sub_e0, tp->sub_start.as<uint32_t>(), E, T, nt, cold_base4,
                        tp->idx16.as<uint16_t>(), tp->idx32.as<uint32_t>());
     AG_HIP(hipGetLastError());
-    AG_HIP(hipStreamSynchronize(s));
-    ecol.release();
-    skey.release();
-    perm.release();
-    fscan.release();
-    sub_e0.release();
     // 8. sub-rows by (table, class): the long ones' lists and the hub sub-rows
     const uint32_t lane_max4 = tn.lane_max / 4;
     const uint32_t nkeys = NT * 8;
     std::vector<uint32_t> ccnt(nkeys), cfirst(nkeys);
-    DevBuf sorted_sub((size_t)S * 4);
+    uint32_t *sorted_sub = scratch_as<uint32_t>(ctx, kScrLay0 + 4, S); // sub_e0 is dead
     {
-        DevBuf key2((size_t)S * 4), val2((size_t)S * 4), sk2((size_t)S * 4), cnt(nkeys * 4),
-            fst(nkeys * 4);
-        hipLaunchKernelGGL(k_tab_classes, dim3(ceil_div(S, 256)), dim3(256), 0, s,
-                           sub_row.as<uint32_t>(), sub_tab.as<uint32_t>(), len4.as<uint32_t>(), S,
-                           n_hub, lane_max4, key2.as<uint32_t>(), val2.as<uint32_t>());
+        uint32_t *key2 = scratch_as<uint32_t>(ctx, kScrPairA, S);
+        uint32_t *val2 = scratch_as<uint32_t>(ctx, kScrPairB, S);
+        uint32_t *sk2 = scratch_as<uint32_t>(ctx, kScrInDst, S);
+        DevBuf cnt(nkeys * 4), fst(nkeys * 4);
+        hipLaunchKernelGGL(k_tab_classes, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row, sub_tab,
+                           len4, S, n_hub, lane_max4, key2, val2);
         AG_HIP(hipGetLastError());
-        radix_sort_pairs(ctx, key2.as<uint32_t>(), val2.as<uint32_t>(), sk2.as<uint32_t>(),
-                         sorted_sub.as<uint32_t>(), S, bits_for(nkeys));
-        vertex_map(ctx, sk2.as<uint32_t>(), S, nkeys, cnt.as<uint32_t>(), fst.as<uint32_t>(), nullptr);
+        radix_sort_pairs(ctx, key2, val2, sk2, sorted_sub, S, bits_for(nkeys));
+        vertex_map(ctx, sk2, S, nkeys, cnt.as<uint32_t>(), fst.as<uint32_t>(), nullptr);
         dl(ccnt.data(), cnt.p, (size_t)nkeys * 4);
         dl(cfirst.data(), fst.p, (size_t)nkeys * 4);
         AG_HIP(hipStreamSynchronize(s));
@@ -2041,17 +2091,38 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     const uint32_t chunk4 = tp->hub_chunk / 4;
     std::vector<uint32_t> H(NT);
     std::vector<std::vector<uint32_t>> hrow(NT), hlen(NT), hstart(NT);
-    for (uint32_t t = 0; t < NT; t++) {
-        H[t] = ccnt[t * 8 + 4];
-        hrow[t].resize(H[t]);
-        hlen[t].resize(H[t]);
-        hstart[t].resize(H[t]);
-        // hub rows come first among a table's sub-rows (rows ascend)
-        dl(hrow[t].data(), sub_row.as<uint32_t>() + tp->SB[t], (size_t)H[t] * 4);
-        dl(hlen[t].data(), len4.as<uint32_t>() + tp->SB[t], (size_t)H[t] * 4);
-        dl(hstart[t].data(), tp->sub_start.as<uint32_t>() + tp->SB[t], (size_t)H[t] * 4);
+    {
+        // hub rows come first among a table's sub-rows (rows ascend): gather
+        // (row, length, start) of sub-rows SB[t] .. SB[t] + H[t] - 1 of every
+        // table on the device, one download
+        std::vector<uint32_t> hidx;
+        for (uint32_t t = 0; t < NT; t++) {
+            H[t] = ccnt[t * 8 + 4];
+            for (uint32_t k = 0; k < H[t]; k++) hidx.push_back(tp->SB[t] + k);
+        }
+        const uint32_t nh = (uint32_t)hidx.size();
+        std::vector<uint32_t> hv((size_t)nh * 3);
+        if (nh) {
+            DevBuf didx((size_t)nh * 4), dout((size_t)nh * 12);
+            AG_HIP(hipMemcpyAsync(didx.p, hidx.data(), (size_t)nh * 4, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_gather3, dim3(ceil_div(nh, 256)), dim3(256), 0, s, sub_row, len4,
+                               tp->sub_start.as<uint32_t>(), didx.as<uint32_t>(), nh,
+                               dout.as<uint32_t>());
+            AG_HIP(hipGetLastError());
+            dl(hv.data(), dout.p, (size_t)nh * 12);
+            AG_HIP(hipStreamSynchronize(s));
+        }
+        for (uint32_t t = 0, q = 0; t < NT; t++) {
+            hrow[t].resize(H[t]);
+            hlen[t].resize(H[t]);
+            hstart[t].resize(H[t]);
+            for (uint32_t k = 0; k < H[t]; k++, q++) {
+                hrow[t][k] = hv[q];
+                hlen[t][k] = hv[nh + q];
+                hstart[t][k] = hv[2 * (size_t)nh + q];
+            }
+        }
     }
-    AG_HIP(hipStreamSynchronize(s));
     std::vector<uint32_t> first(n_hub + 1, 0);
     std::vector<std::vector<uint32_t>> slot_of(NT); // first slot of each hub sub-row
     {
@@ -2105,7 +2176,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
             const uint32_t n = ccnt[t * 8 + c], f = cfirst[t * 8 + c];
             if (n)
                 hipLaunchKernelGGL(k_tab_litems, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                                   sorted_sub.as<uint32_t>() + f, n, tp->sub_start.as<uint32_t>(),
+                                   sorted_sub + f, n, tp->sub_start.as<uint32_t>(),
                                    tp->items.as<uint4>() + lpos);
             for (uint32_t k = 0; k < n; k += per_unit[c])
                 units.push_back(make_uint4(tt | c, lpos + k, std::min<uint32_t>(per_unit[c], n - k),
@@ -2189,18 +2260,23 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     const size_t wrec = ((size_t)tp->nwaves * 3 * tp->NTpad + 1) * 4;
     tp->wtab.alloc(wrec);
     AG_HIP(hipMemsetAsync(tp->wtab.p, 0, wrec, s));
-    hipLaunchKernelGGL(k_tab_wmask, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row.as<uint32_t>(),
-                       sub_tab.as<uint32_t>(), S, n_hub, tp->NTpad, tp->grp, tp->wtab.as<uint32_t>());
+    hipLaunchKernelGGL(k_tab_wmask, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row, sub_tab, S,
+                       n_hub, tp->NTpad, tp->grp, tp->wtab.as<uint32_t>());
     if (tp->nwaves)
         hipLaunchKernelGGL(k_tab_bases, dim3(ceil_div((uint64_t)tp->nwaves * NT, 256)), dim3(256), 0,
-                           s, sub_row.as<uint32_t>(), sbd.as<uint32_t>(), NT, tp->NTpad, tp->grp,
-                           tp->nwaves, n_hub, tp->wtab.as<uint32_t>());
+                           s, sub_row, sbd.as<uint32_t>(), NT, tp->NTpad, tp->grp, tp->nwaves, n_hub,
+                           tp->wtab.as<uint32_t>());
     AG_HIP(hipGetLastError());
     tp->tab_edges.assign(NT, 0);
     {
+        // index positions per table: sub_start at each table's first sub-row
+        // (sbd holds SB[0 .. NT] on the device)
         std::vector<uint32_t> st(NT + 1);
-        for (uint32_t t = 0; t <= NT; t++)
-            dl(&st[t], tp->sub_start.as<uint32_t>() + tp->SB[t], 4);
+        DevBuf dst_((size_t)(NT + 1) * 4);
+        hipLaunchKernelGGL(k_gather1, dim3(1), dim3(256), 0, s, tp->sub_start.as<uint32_t>(),
+                           sbd.as<uint32_t>(), NT + 1, dst_.as<uint32_t>());
+        AG_HIP(hipGetLastError());
+        dl(st.data(), dst_.p, (size_t)(NT + 1) * 4);
         AG_HIP(hipStreamSynchronize(s));
         for (uint32_t t = 0; t < NT; t++) tp->tab_edges[t] = (uint64_t)(st[t + 1] - st[t]) * 4;
     }
