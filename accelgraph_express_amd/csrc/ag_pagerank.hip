@@ -1452,6 +1452,29 @@ __global__ void k_add_base(uint32_t *__restrict__ x, uint32_t n, uint32_t base)
 // per row made the largest hubs (~1.6e5 in-edges at R-MAT scale 22) a serial
 // tail of ~2 ms.
 constexpr uint32_t kFillBlock = 256;
+// The last row r < nrows with roff[r] <= e (binary search; roff[0] <= e).
+__device__ __forceinline__ uint32_t block_first_row(const uint32_t *__restrict__ roff,
+                                                    uint32_t nrows, uint32_t e)
+{
+    uint32_t lo = 0, hi = nrows; // roff[lo] <= e < roff[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (roff[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+// The last k < nr with s_off[k] <= e (s_off[0] <= e < s_off[nr]).
+__device__ __forceinline__ uint32_t staged_row(const uint32_t *s_off, uint32_t nr, uint32_t e)
+{
+    uint32_t lo = 0, hi = nr;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (s_off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
 __global__ __launch_bounds__(kFillBlock) void k_fill_csr_edges(
     const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
     const uint32_t *__restrict__ inv_idx, const uint32_t *__restrict__ inv_dst,
@@ -1461,15 +1484,7 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_csr_edges(
     __shared__ uint32_t s_off[kFillBlock + 1];
     __shared__ uint32_t s_r0;
     const uint32_t e0 = first + blockIdx.x * kFillBlock;
-    if (threadIdx.x == 0) {
-        uint32_t lo = 0, hi = nrows; // roff[lo] <= e0 < roff[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = lo + (hi - lo) / 2;
-            if (roff[mid] <= e0) lo = mid;
-            else hi = mid;
-        }
-        s_r0 = lo;
-    }
+    if (threadIdx.x == 0) s_r0 = block_first_row(roff, nrows, e0);
     __syncthreads();
     const uint32_t r0 = s_r0;
     // offsets of rows r0 .. r0 + 256 (clamped to nrows): the block's slots
@@ -1479,12 +1494,7 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_csr_edges(
     __syncthreads();
     const uint32_t e = e0 + threadIdx.x;
     if (e >= total) return;
-    uint32_t lo = 0, hi = nr; // s_off[lo] <= e < s_off[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (s_off[mid] <= e) lo = mid;
-        else hi = mid;
-    }
+    const uint32_t lo = staged_row(s_off, nr, e);
     const uint32_t v = order[r0 + lo];
     cols[e] = rank[inv_dst[inv_idx[v] + (e - s_off[lo])]];
 }
@@ -1545,19 +1555,30 @@ __global__ void k_tab_keys(const uint32_t *__restrict__ ecol, uint32_t E, uint32
 }
 
 // Row of each table-sorted edge (last r with roff[r] <= e, r < nrows).
-__global__ void k_tab_rows(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ roff,
-                           uint32_t nrows, uint32_t E, uint32_t *__restrict__ srow)
+__global__ void k_tab_rows(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ erow,
+                           uint32_t E, uint32_t *__restrict__ srow)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E) return;
-    const uint32_t e = perm[i];
-    uint32_t lo = 0, hi = nrows;
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (roff[mid] <= e) lo = mid;
-        else hi = mid;
-    }
-    srow[i] = lo;
+    if (i < E) srow[i] = erow[perm[i]];
+}
+
+// erow[e] = row of edge slot e < total (rows [0, nrows), each with an
+// in-edge, total <= roff[nrows]): the row search of k_fill_csr_edges.
+__global__ __launch_bounds__(kFillBlock) void k_edge_rows(const uint32_t *__restrict__ roff,
+                                                          uint32_t nrows, uint32_t total,
+                                                          uint32_t *__restrict__ erow)
+{
+    __shared__ uint32_t s_off[kFillBlock + 1];
+    __shared__ uint32_t s_r0;
+    const uint32_t e0 = blockIdx.x * kFillBlock;
+    if (threadIdx.x == 0) s_r0 = block_first_row(roff, nrows, e0);
+    __syncthreads();
+    const uint32_t r0 = s_r0;
+    const uint32_t nr = min(kFillBlock, nrows - r0);
+    for (uint32_t i = threadIdx.x; i <= nr; i += kFillBlock) s_off[i] = roff[r0 + i];
+    __syncthreads();
+    const uint32_t e = e0 + threadIdx.x;
+    if (e < total) erow[e] = r0 + staged_row(s_off, nr, e);
 }
 
 // 1 where a sub-row (table, row) starts in the table-sorted edge order.
@@ -1654,24 +1675,10 @@ __global__ void k_tab_classes(const uint32_t *__restrict__ sub_row, const uint32
     val[sid] = sid;
 }
 
-// Lane bits of the combine records: the row of sub-row sid is lane
-// (row - n_hub) % 64 of wave (row - n_hub) / 64.
-__global__ void k_tab_wmask(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sub_tab,
-                            uint32_t S, uint32_t n_hub, uint32_t NTpad, uint32_t grp,
-                            uint32_t *__restrict__ wtab)
-{
-    const uint32_t sid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sid >= S) return;
-    const uint32_t r = sub_row[sid];
-    if (r < n_hub) return;
-    const uint32_t q = r - n_hub, lane = q & 63u;
-    const uint32_t t = sub_tab[sid];
-    uint32_t *rec = wtab + (size_t)(q >> 6) * 3u * NTpad + 3u * (t / grp) * grp;
-    atomicOr(&rec[2u * (t % grp) + (lane >> 5)], 1u << (lane & 31u));
-}
-
-// Slot bases of the combine records: the first sub-row of table t whose row
-// is >= the first row of 64-row wave w (rows ascend inside a table).
+// Combine records, one thread per (64-row wave w, table t): the slot base
+// (the first sub-row of table t whose row is >= the wave's first row; rows
+// ascend inside a table) and the mask of the wave's rows with a sub-row in t
+// (the run of sub-rows from the base whose rows lie in the wave).
 __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t *__restrict__ sb,
                             uint32_t NT, uint32_t NTpad, uint32_t grp, uint32_t nwaves, uint32_t n_hub,
                             uint32_t *__restrict__ wtab)
@@ -1680,13 +1687,24 @@ __global__ void k_tab_bases(const uint32_t *__restrict__ sub_row, const uint32_t
     if (i >= nwaves * NT) return;
     const uint32_t w = i / NT, t = i % NT;
     const uint32_t row0 = n_hub + w * 64u;
-    uint32_t lo = sb[t], hi = sb[t + 1]; // first index in [lo, hi) with sub_row >= row0
+    const uint32_t end = sb[t + 1];
+    uint32_t lo = sb[t], hi = end; // first index in [lo, hi) with sub_row >= row0
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
         if (sub_row[mid] < row0) lo = mid + 1;
         else hi = mid;
     }
-    wtab[(size_t)w * 3u * NTpad + 3u * (t / grp) * grp + 2u * grp + t % grp] = lo;
+    // the wave's rows with a sub-row in t: the run from lo of rows < row0 + 64
+    uint64_t m = 0;
+    for (uint32_t j = lo; j < end; j++) {
+        const uint32_t r = sub_row[j];
+        if (r >= row0 + 64u) break;
+        m |= 1ull << (r - row0);
+    }
+    uint32_t *rec = wtab + (size_t)w * 3u * NTpad + 3u * (t / grp) * grp;
+    rec[2u * (t % grp)] = (uint32_t)m;
+    rec[2u * (t % grp) + 1u] = (uint32_t)(m >> 32);
+    rec[2u * grp + t % grp] = lo;
 }
 
 // Cost of each work unit in padded index positions (lane units: per wave,
@@ -2008,8 +2026,11 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     {
         uint32_t *srow = scratch_as<uint32_t>(ctx, kScrPairA, E);
         uint32_t *flag = scratch_as<uint32_t>(ctx, kScrPairB, E);
-        hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm,
-                           p->roff.as<uint32_t>(), nrows, E, srow);
+        // row of every label-order edge slot (in flag's slot), then of
+        // every table-sorted edge (perm ascends within a table)
+        hipLaunchKernelGGL(k_edge_rows, dim3(ceil_div(E, kFillBlock)), dim3(kFillBlock), 0, s,
+                           p->roff.as<uint32_t>(), nrows, E, flag);
+        hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm, flag, E, srow);
         hipLaunchKernelGGL(k_tab_flags, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, srow, E, flag);
         AG_HIP(hipGetLastError());
         scan_exclusive_u32(ctx, flag, fscan, E);
@@ -2260,8 +2281,6 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     const size_t wrec = ((size_t)tp->nwaves * 3 * tp->NTpad + 1) * 4;
     tp->wtab.alloc(wrec);
     AG_HIP(hipMemsetAsync(tp->wtab.p, 0, wrec, s));
-    hipLaunchKernelGGL(k_tab_wmask, dim3(ceil_div(S, 256)), dim3(256), 0, s, sub_row, sub_tab, S,
-                       n_hub, tp->NTpad, tp->grp, tp->wtab.as<uint32_t>());
     if (tp->nwaves)
         hipLaunchKernelGGL(k_tab_bases, dim3(ceil_div((uint64_t)tp->nwaves * NT, 256)), dim3(256), 0,
                            s, sub_row, sbd.as<uint32_t>(), NT, tp->NTpad, tp->grp, tp->nwaves, n_hub,
