@@ -1093,26 +1093,17 @@ __global__ __launch_bounds__(kCombBlock) void k_pr_combine(SweepArgs a, uint32_t
                 uint64_t m[GRP];
 #pragma unroll
                 for (uint32_t t = 0; t < GRP; t++) m[t] = ((uint64_t)rec[2 * t + 1] << 32) | rec[2 * t];
-                // the slot bases are read before any load is issued (the
-                // empty asm keeps the compiler from sinking their scalar loads
-                // into the masked blocks below)
-                uint32_t base[GRP];
-#pragma unroll
-                for (uint32_t t = 0; t < GRP; t++) base[t] = rec[2 * GRP + t];
-#pragma unroll
-                for (uint32_t t = 0; t < GRP; t++) asm volatile("" : "+s"(base[t]));
                 float pv[GRP];
 #pragma unroll
                 for (uint32_t t = 0; t < GRP; t++) {
                     const uint32_t slot = __builtin_amdgcn_mbcnt_hi(
                         (uint32_t)(m[t] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m[t], 0u));
-                    const char *pt = reinterpret_cast<const char *>(a.tab_part + base[t]);
-                    pv[t] = 0.0f;
-                    if (__builtin_amdgcn_inverse_ballot_w64(m[t]))
-                        pv[t] = *reinterpret_cast<const float *>(pt + slot * 4u);
+                    const char *pt = reinterpret_cast<const char *>(a.tab_part + rec[2 * GRP + t]);
+                    pv[t] = *reinterpret_cast<const float *>(pt + slot * 4u);
                 }
 #pragma unroll
-                for (uint32_t t = 0; t < GRP; t++) acc<DD>(s, fl, pv[t]);
+                for (uint32_t t = 0; t < GRP; t++)
+                    acc<DD>(s, fl, __builtin_amdgcn_inverse_ballot_w64(m[t]) ? pv[t] : 0.0f);
             }
             if (live) update_vertex_v(a, r, s * rscale, old, od, tele, cout, wfmt, err, act, dang, fl);
         }
