@@ -1257,7 +1257,14 @@ __device__ void finalize_body(const SweepArgs &a, uint32_t it)
         st->dangling = D_;
         st->tele = (float)((1.0 - (double)a.damp) * a.inv_n + (double)a.damp * D_ * a.inv_n);
         st->iterations = it + 1;
-        if (a.epsilon > 0.0 && E_ <= a.epsilon) st->done = 1;
+        // extrapolated steps as in personalized mode: stop on a plain sweep's
+        // residual only
+        const bool stepped = st->w != 0.0f;
+        st->w = 0.0f;
+        if (a.epsilon > 0.0 && E_ <= a.epsilon && !stepped)
+            st->done = 1;
+        else if (a.extrap)
+            st->w = extrapolation_weight(E_, it > 0 ? st->prev_err : 0.0, stepped, st->since, st->rho);
     }
 }
 
@@ -2660,8 +2667,8 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         fail(AG_ERR_INVALID, "data-driven pull stores FP32 contributions only");
     if ((o.mode == kModePPR) != (seeds != nullptr))
         fail(AG_ERR_INVALID, "personalized mode needs a seed table");
-    if ((o.flags & AG_PR_FLAG_EXTRAPOLATE) && o.mode != kModePPR)
-        fail(AG_ERR_INVALID, "extrapolated steps are offered for personalized PageRank only");
+    if ((o.flags & AG_PR_FLAG_EXTRAPOLATE) && o.mode != kModePPR && o.mode != AG_PR_STANDARD)
+        fail(AG_ERR_INVALID, "extrapolated steps are offered for STANDARD and personalized PageRank only");
     if (o.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", o.precision);
     if (!(o.damp > 0.0f && o.damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
     if (N == 0) {

@@ -212,7 +212,7 @@ class Graph:
 
     def pagerank_pull(self, iterations: int = 20, epsilon: float = 1e-4, mode="reference",
                       precision="fp32", finish_fp32: bool = False,
-                      damp: float = 0.85) -> PageRankStats:
+                      damp: float = 0.85, extrapolate: bool = False) -> PageRankStats:
         lib = L.load()
         o = L.PrOptions()
         lib.ag_pr_options_default(C.byref(o))
@@ -220,7 +220,8 @@ class Graph:
         o.epsilon = epsilon
         o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
         o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
-        o.flags = L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0
+        o.flags = (L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0) | \
+                  (L.AG_PR_FLAG_EXTRAPOLATE if extrapolate else 0)
         o.damp = damp
         n = self.num_vertices
         ranks = np.zeros(n, np.float32)
@@ -259,9 +260,10 @@ class Graph:
 
     def pagerank_pull_device(self, d_ranks_ptr: int, iterations: int = 20, epsilon: float = 0.0,
                              mode="reference", precision="fp32", finish_fp32=False,
-                             time_sweeps: bool = False) -> L.PrResult:
+                             time_sweeps: bool = False, extrapolate: bool = False) -> L.PrResult:
         """Ranks to a device buffer of N floats; time_sweeps fills
-        sweep_kernel_ms (HIP events around each sweep kernel)."""
+        sweep_kernel_ms (HIP events around each sweep kernel); extrapolate
+        (STANDARD mode) takes Aitken-type extrapolated steps."""
         lib = L.load()
         o = L.PrOptions()
         lib.ag_pr_options_default(C.byref(o))
@@ -269,7 +271,8 @@ class Graph:
         o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
         o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
         o.flags = (L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0) | \
-                  (L.AG_PR_FLAG_TIME_SWEEPS if time_sweeps else 0)
+                  (L.AG_PR_FLAG_TIME_SWEEPS if time_sweeps else 0) | \
+                  (L.AG_PR_FLAG_EXTRAPOLATE if extrapolate else 0)
         r = L.PrResult()
         L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))

@@ -140,7 +140,10 @@ int ag_vertex_map_host(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uin
 
 #define AG_PR_FLAG_FINISH_FP32 0x1u /* low precision: switch to FP32 storage when the residual stalls */
 #define AG_PR_FLAG_TIME_SWEEPS 0x2u /* time the sweep kernels alone (HIP events around each) */
-#define AG_PR_FLAG_EXTRAPOLATE 0x4u /* personalized PageRank only (ag_ppr_single_opts) */
+#define AG_PR_FLAG_EXTRAPOLATE 0x4u /* AG_PR_STANDARD (and personalized PageRank through
+                                       ag_ppr_single_opts): Aitken-type extrapolated steps
+                                       once the residual falls by a steady ratio; the run
+                                       stops only on a plain sweep's residual */
 
 typedef struct ag_pr_options
 {

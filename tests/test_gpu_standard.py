@@ -78,3 +78,34 @@ def test_converged_against_fp64(ctx, prec):
     print(f"{prec}: {st.sweeps} sweeps (FP32 from {st.fp32_switch_sweep}), L1 vs FP64 {l1:.2e}")
     assert st.error_total <= 1e-6
     assert l1 <= 0.85 / 0.15 * 1e-6 + 5e-6
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_extrapolated_converged_against_fp64(ctx, prec):
+    """Extrapolated steps (Kamvar et al., as in personalized mode): the run
+    still stops on a plain sweep's residual, so the contraction bound of
+    test_converged_against_fp64 holds; the FP32 run takes fewer sweeps than
+    plain power iteration on this graph."""
+    src, dst = ag.gen_rmat(5, 17, 16)
+    n = 1 << 17
+    print(f"case: STANDARD extrapolated R-MAT s17 {prec} to residual 1e-6", flush=True)
+    g = ag.Graph.build(ctx, src, dst, n)
+    kw = dict(iterations=300, epsilon=1e-6, mode="standard", precision=prec,
+              finish_fp32=prec != "fp32")
+    plain = g.pagerank_pull(**kw)
+    st = g.pagerank_pull(extrapolate=True, **kw)
+    star, _, _ = oracle.pagerank_standard_f64(oracle.build_csr(src, dst, n), 1000, 1e-13)
+    l1 = np.abs(st.pageRanks.astype(np.float64) - star).sum()
+    print(f"{prec}: {st.sweeps} sweeps extrapolated vs {plain.sweeps} plain, L1 vs FP64 {l1:.2e}")
+    assert st.error_total <= 1e-6
+    assert l1 <= 0.85 / 0.15 * 1e-6 + 5e-6
+    assert abs(st.pageRanks.astype(np.float64).sum() - 1.0) <= 1e-5
+    if prec == "fp32":
+        assert st.sweeps < plain.sweeps
+
+
+def test_extrapolate_refused_in_reference_mode(ctx):
+    src, dst = ag.gen_uniform(3, 1024, 8192)
+    g = ag.Graph.build(ctx, src, dst, 1024)
+    with pytest.raises(ag.AGError):
+        g.pagerank_pull(iterations=10, epsilon=0.0, mode="reference", extrapolate=True)
