@@ -114,6 +114,7 @@ enum ScratchSlot : int {
     kScrHist = 0, kScrCsum, kScrSortK, kScrSortV, kScrScan0, // kScrScan0 .. +3: scan levels
     kScrMax = 8, kScrPairA, kScrPairB, kScrInSrc, kScrInDst,
     kScrLay0 = 13, // kScrLay0 .. +7: gather-layout build temporaries
+    kScrBlk = 21,  // first row of each 256-slot block of an edge-slot fill
 };
 // Device buffer of at least `bytes` in `slot`; growing waits for the stream
 // first, since the old buffer may still be read by queued work.  Sizes are

@@ -1441,26 +1441,25 @@ __global__ void k_add_base(uint32_t *__restrict__ x, uint32_t n, uint32_t base)
     if (i < n) x[i] += base;
 }
 
-// CSR rows, relabelled sources in original order: edge slots [first, total)
-// of rows [0, nrows) (every such row has an in-edge, total <= roff[nrows]).
-// One thread per slot; its row is the last r with roff[r] <= e.  A block's
-// 256 slots span at most 256 rows: thread 0 finds the block's first row by a
-// binary search over the row offsets, the block stages the offsets of the
+// Edge-slot fills over rows [0, nrows) (every such row has an in-edge):
+// slots [first, total), total <= roff[nrows], one thread per slot, whose row
+// is the last r with roff[r] <= e.  Block b covers slots first + 256 b ..;
+// k_block_rows gives each block its first row (the row holding its first
+// slot: one writer per block), the block stages the offsets of the <= 256
 // rows it spans in LDS, and each thread searches those (8 steps).  One wave
 // per row made the largest hubs (~1.6e5 in-edges at R-MAT scale 22) a serial
-// tail of ~2 ms.
+// tail of ~2 ms; a binary search over all row offsets per block cost ~0.4 ms.
 constexpr uint32_t kFillBlock = 256;
-// The last row r < nrows with roff[r] <= e (binary search; roff[0] <= e).
-__device__ __forceinline__ uint32_t block_first_row(const uint32_t *__restrict__ roff,
-                                                    uint32_t nrows, uint32_t e)
+__global__ void k_block_rows(const uint32_t *__restrict__ roff, uint32_t nrows, uint32_t first,
+                             uint32_t total, uint32_t *__restrict__ brow)
 {
-    uint32_t lo = 0, hi = nrows; // roff[lo] <= e < roff[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = lo + (hi - lo) / 2;
-        if (roff[mid] <= e) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const uint32_t lo = max(roff[r], first), hi = min(roff[r + 1], total);
+    if (lo >= hi) return;
+    // blocks whose first slot lies in [lo, hi)
+    for (uint32_t b = (lo - first + kFillBlock - 1) / kFillBlock; first + b * kFillBlock < hi; b++)
+        brow[b] = r;
 }
 // The last k < nr with s_off[k] <= e (s_off[0] <= e < s_off[nr]).
 __device__ __forceinline__ uint32_t staged_row(const uint32_t *s_off, uint32_t nr, uint32_t e)
@@ -1475,16 +1474,13 @@ __device__ __forceinline__ uint32_t staged_row(const uint32_t *s_off, uint32_t n
 }
 __global__ __launch_bounds__(kFillBlock) void k_fill_csr_edges(
     const uint32_t *__restrict__ order, const uint32_t *__restrict__ roff,
-    const uint32_t *__restrict__ inv_idx, const uint32_t *__restrict__ inv_dst,
-    const uint32_t *__restrict__ rank, uint32_t nrows, uint32_t first, uint32_t total,
-    uint32_t *__restrict__ cols)
+    const uint32_t *__restrict__ brow, const uint32_t *__restrict__ inv_idx,
+    const uint32_t *__restrict__ inv_dst, const uint32_t *__restrict__ rank, uint32_t nrows,
+    uint32_t first, uint32_t total, uint32_t *__restrict__ cols)
 {
     __shared__ uint32_t s_off[kFillBlock + 1];
-    __shared__ uint32_t s_r0;
     const uint32_t e0 = first + blockIdx.x * kFillBlock;
-    if (threadIdx.x == 0) s_r0 = block_first_row(roff, nrows, e0);
-    __syncthreads();
-    const uint32_t r0 = s_r0;
+    const uint32_t r0 = brow[blockIdx.x];
     // offsets of rows r0 .. r0 + 256 (clamped to nrows): the block's slots
     // end before roff[min(r0 + 256, nrows)]
     const uint32_t nr = min(kFillBlock, nrows - r0);
@@ -1563,15 +1559,13 @@ __global__ void k_tab_rows(const uint32_t *__restrict__ perm, const uint32_t *__
 // erow[e] = row of edge slot e < total (rows [0, nrows), each with an
 // in-edge, total <= roff[nrows]): the row search of k_fill_csr_edges.
 __global__ __launch_bounds__(kFillBlock) void k_edge_rows(const uint32_t *__restrict__ roff,
+                                                          const uint32_t *__restrict__ brow,
                                                           uint32_t nrows, uint32_t total,
                                                           uint32_t *__restrict__ erow)
 {
     __shared__ uint32_t s_off[kFillBlock + 1];
-    __shared__ uint32_t s_r0;
     const uint32_t e0 = blockIdx.x * kFillBlock;
-    if (threadIdx.x == 0) s_r0 = block_first_row(roff, nrows, e0);
-    __syncthreads();
-    const uint32_t r0 = s_r0;
+    const uint32_t r0 = brow[blockIdx.x];
     const uint32_t nr = min(kFillBlock, nrows - r0);
     for (uint32_t i = threadIdx.x; i <= nr; i += kFillBlock) s_off[i] = roff[r0 + i];
     __syncthreads();
@@ -1888,12 +1882,16 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     const uint64_t ncols = (uint64_t)csr_edges + sell_slots;
     p->cols.alloc((size_t)ncols * 4);
     AG_HIP(hipMemsetAsync(p->cols.p, 0, (size_t)ncols * 4, s));
-    if (p->n_nonlow && csr_edges)
+    if (p->n_nonlow && csr_edges) {
+        uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(csr_edges, kFillBlock));
+        hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(p->n_nonlow, 256)), dim3(256), 0, s,
+                           p->roff.as<uint32_t>(), p->n_nonlow, 0u, csr_edges, brow);
         hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(csr_edges, kFillBlock)), dim3(kFillBlock),
-                           0, s, p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
+                           0, s, p->order.as<uint32_t>(), p->roff.as<uint32_t>(), brow,
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
                            p->rank.as<uint32_t>(), p->n_nonlow, 0u, csr_edges,
                            p->cols.as<uint32_t>());
+    }
     if (nlow)
         hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
                            p->order.as<uint32_t>(), rdeg,
@@ -1981,11 +1979,15 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     const uint32_t e_csr = (uint32_t)std::min<uint64_t>(p->csr_edges, E);
     if (e_csr)
         AG_HIP(hipMemcpyAsync(ecol, p->cols.p, (size_t)e_csr * 4, hipMemcpyDeviceToDevice, s));
-    if (e_csr < E)
+    if (e_csr < E) {
+        uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(E - e_csr, kFillBlock));
+        hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(nrows, 256)), dim3(256), 0, s,
+                           p->roff.as<uint32_t>(), nrows, e_csr, E, brow);
         hipLaunchKernelGGL(k_fill_csr_edges, dim3(ceil_div(E - e_csr, kFillBlock)), dim3(kFillBlock),
-                           0, s, p->order.as<uint32_t>(), p->roff.as<uint32_t>(),
+                           0, s, p->order.as<uint32_t>(), p->roff.as<uint32_t>(), brow,
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
                            p->rank.as<uint32_t>(), nrows, e_csr, E, ecol);
+    }
     AG_HIP(hipGetLastError());
     // 2. edges per table; LDS tables while each holds enough edges
     {
@@ -2026,8 +2028,11 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
         uint32_t *flag = scratch_as<uint32_t>(ctx, kScrPairB, E);
         // row of every label-order edge slot (in flag's slot), then of
         // every table-sorted edge (perm ascends within a table)
+        uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(E, kFillBlock));
+        hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(nrows, 256)), dim3(256), 0, s,
+                           p->roff.as<uint32_t>(), nrows, 0u, E, brow);
         hipLaunchKernelGGL(k_edge_rows, dim3(ceil_div(E, kFillBlock)), dim3(kFillBlock), 0, s,
-                           p->roff.as<uint32_t>(), nrows, E, flag);
+                           p->roff.as<uint32_t>(), brow, nrows, E, flag);
         hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm, flag, E, srow);
         hipLaunchKernelGGL(k_tab_flags, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, srow, E, flag);
         AG_HIP(hipGetLastError());
