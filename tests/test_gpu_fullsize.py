@@ -3,7 +3,8 @@ do not need the oracle to run to convergence at these sizes.
 
 * C3, R-MAT scale 24 (N = 16,777,216, E = 268,435,456): every CSR array equals
   the oracle's build bit for bit.  STANDARD PageRank to L1 residual 1e-6 in
-  FP32, BF16 and scaled FP16 (both finishing in FP32): total mass 1, and an
+  FP32, BF16 and scaled FP16 (both finishing in FP32), plain and with
+  extrapolated steps: total mass 1, and an
   FP64 error bound against the exact fixed point of 2e-5.  The map T(x) = (1-d)/N + d (A^T D^-1 x
   + dangling(x)/N) is a d-contraction in L1, so ||x - x*||_1 <= ||x - T(x)||_1
   / (1 - d), with T applied once in FP64 (numpy bincount over the edge list).
@@ -69,23 +70,29 @@ def test_c3_s24_standard_precisions(ctx, s24):
     src, dst, n = s24
     print(f"case: C3 STANDARD PageRank to 1e-6 N={n}", flush=True)
     g = ag.Graph.build(ctx, src, dst, n)
-    xs = {}
-    for prec, finish in (("fp32", False), ("bf16", True), ("fp16", True)):
-        st = g.pagerank_pull(iterations=200, epsilon=1e-6, mode="standard", precision=prec,
-                             finish_fp32=finish)
-        assert st.iterations < 200
-        x = st.pageRanks.astype(np.float64)
-        r = np.abs(x - _standard_T(src, dst, n, x)).sum()
-        bound = r / (1 - D)
-        print(f"{prec}: {st.iterations} sweeps, sum {x.sum():.9f}, ||x - T(x)||_1 {r:.3e}, "
-              f"error bound vs FP64 fixed point {bound:.3e}", flush=True)
-        assert abs(x.sum() - 1.0) <= 1e-5
-        assert bound <= 2e-5
-        if prec != "fp32":
-            assert st.fp32_switch_sweep > 0  # finished in FP32
-        xs[prec] = x
+    xs, sweeps = {}, {}
+    # plain power iteration, then with extrapolated steps: both stop on a
+    # plain sweep's residual, so the same bound holds
+    for ex in (False, True):
+        for prec, finish in (("fp32", False), ("bf16", True), ("fp16", True)):
+            st = g.pagerank_pull(iterations=200, epsilon=1e-6, mode="standard", precision=prec,
+                                 finish_fp32=finish, extrapolate=ex)
+            assert st.iterations < 200
+            x = st.pageRanks.astype(np.float64)
+            r = np.abs(x - _standard_T(src, dst, n, x)).sum()
+            bound = r / (1 - D)
+            name = prec + ("_extrap" if ex else "")
+            print(f"{name}: {st.iterations} sweeps, sum {x.sum():.9f}, ||x - T(x)||_1 {r:.3e}, "
+                  f"error bound vs FP64 fixed point {bound:.3e}", flush=True)
+            assert abs(x.sum() - 1.0) <= 1e-5
+            assert bound <= 2e-5
+            if prec != "fp32":
+                assert st.fp32_switch_sweep > 0  # finished in FP32
+            xs[name] = x
+            sweeps[name] = st.sweeps
     for prec in ("bf16", "fp16"):
         print(f"L1 {prec} vs fp32: {np.abs(xs[prec] - xs['fp32']).sum():.3e}")
+    assert sweeps["fp32_extrap"] < sweeps["fp32"]
     g.free()
 
 
