@@ -1363,15 +1363,31 @@ __global__ __launch_bounds__(kBlock) void k_pr_finish(const float *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-__global__ void k_sum_parts(const double *part, uint32_t n, double *out)
+// One workgroup: every thread adds its strided partials (kSumBatch loads in
+// flight at a time), then the waves' sums in order.
+constexpr uint32_t kSumBlock = 1024, kSumBatch = 8;
+__global__ __launch_bounds__(kSumBlock) void k_sum_parts(const double *part, uint32_t n, double *out)
 {
-    __shared__ double s[kBlock / 64];
+    __shared__ double s[kSumBlock / 64];
     double acc = 0.0;
-    for (uint32_t b = threadIdx.x; b < n; b += kBlock) acc += part[b];
+    for (uint32_t b0 = threadIdx.x; b0 < n; b0 += kSumBatch * kSumBlock) {
+        double v[kSumBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kSumBatch; k++) {
+            const uint32_t b = b0 + k * kSumBlock;
+            v[k] = b < n ? part[b] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kSumBatch; k++) acc += v[k];
+    }
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) *out = (s[0] + s[1]) + (s[2] + s[3]);
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (uint32_t i = 0; i < kSumBlock / 64; i++) t += s[i];
+        *out = t;
+    }
 }
 
 // ---- plan construction kernels ----
@@ -2714,7 +2730,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
     auto enqueue_finish = [&] {
         hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr, p->rank.as<uint32_t>(),
                            N, ref_scale ? 1u : 0u, d_out, parts);
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kSumBlock), 0, s, parts, nbf, parts + nbf);
         AG_HIP(hipGetLastError());
     };
     std::vector<unsigned char> key0;
@@ -2883,7 +2899,7 @@ static void run_spmv(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
     double *parts = w.sum_part;
     hipLaunchKernelGGL(k_pr_finish, dim3(nbf), dim3(kBlock), 0, s, w.pr,
                        p->rank.as<uint32_t>(), N, 0u, d_y, parts);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kBlock), 0, s, parts, nbf, parts + nbf);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kSumBlock), 0, s, parts, nbf, parts + nbf);
     AG_HIP(hipGetLastError());
     AG_HIP(hipEventRecord(ev[3], s));
     AG_HIP(hipMemcpyAsync(&r.y_sum, parts + nbf, 8, hipMemcpyDeviceToHost, s));
