@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -32,6 +33,17 @@ void set_last_error(int code, const char *msg);
             ::ag::fail(AG_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),     \
                        __FILE__, __LINE__);                                            \
     } while (0)
+
+// Names one library phase for a profiler (rocprofv3 --marker-trace shows it as
+// a range on the calling thread, around the host work and the kernel launches
+// it issues).  The phase ends when the scope does, also by an exception.
+// Without a profiler attached a push/pop pair costs a few tens of ns.
+struct PhaseRange {
+    explicit PhaseRange(const char *name) { roctxRangePushA(name); }
+    ~PhaseRange() { roctxRangePop(); }
+    PhaseRange(const PhaseRange &) = delete;
+    PhaseRange &operator=(const PhaseRange &) = delete;
+};
 
 // Every extern "C" entry point runs its body through guard(): exceptions become
 // status codes plus ag_last_error() text.

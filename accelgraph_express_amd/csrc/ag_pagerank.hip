@@ -2326,6 +2326,7 @@ Plan *ensure_plan(ag_ctx *ctx, ag_graph *g, double *ms)
 {
     if (ms) *ms = 0.0;
     if (g->plan) return g->plan;
+    PhaseRange phase_("pagerank.layout_build");
     auto t0 = std::chrono::steady_clock::now();
     g->plan = build_plan(ctx, g);
     auto t1 = std::chrono::steady_clock::now();
@@ -2619,6 +2620,7 @@ static void run_captured(Plan *p, hipStream_t s, const std::vector<unsigned char
     }
     LoopGraph lg;
     lg.key = key;
+    PhaseRange phase_("pagerank.graph_capture");
     AG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
         enqueue();
@@ -2784,6 +2786,7 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
                 enqueue_sweeps(s, p, a, o, grid, it0, it1, timed ? sev.data() : nullptr);
             }
             if (can_stop && it1 < max_iter) {
+                PhaseRange phase_("pagerank.stop_check");
                 uint32_t done = 0;
                 AG_HIP(hipMemcpyAsync(&done, &a.st->done, 4, hipMemcpyDeviceToHost, s));
                 AG_HIP(hipStreamSynchronize(s));
@@ -3016,6 +3019,7 @@ void ag_pr_options_default(ag_pr_options *o)
 int ag_pagerank_prepare(ag_ctx *ctx, ag_graph *g, double *ms)
 {
     return guard([&] {
+        PhaseRange phase_("ag_pagerank_prepare");
         check_ctx(ctx);
         if (!g) fail(AG_ERR_INVALID, "null graph");
         ensure_plan(ctx, g, ms);
@@ -3025,6 +3029,7 @@ int ag_pagerank_prepare(ag_ctx *ctx, ag_graph *g, double *ms)
 int ag_pagerank_autotune(ag_ctx *ctx, ag_graph *g, uint32_t sweeps, ag_tuning *best, double *ms)
 {
     return guard([&] {
+        PhaseRange phase_("ag_pagerank_autotune");
         check_ctx(ctx);
         if (!g) fail(AG_ERR_INVALID, "null graph");
         if (sweeps == 0 || sweeps > 1000) fail(AG_ERR_INVALID, "sweeps must be in 1..1000");
@@ -3143,6 +3148,7 @@ int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *
                      uint32_t *active_iter, double *err_iter, ag_pr_result *res)
 {
     return guard([&] {
+        PhaseRange phase_("ag_pagerank_pull");
         check_ctx(ctx);
         if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
         Plan *p = ensure_plan(ctx, g, nullptr);
@@ -3164,6 +3170,7 @@ int ag_spmv_pull(ag_ctx *ctx, ag_graph *g, const float *x, float *y, uint32_t it
                  ag_spmv_result *res)
 {
     return guard([&] {
+        PhaseRange phase_("ag_spmv_pull");
         check_ctx(ctx);
         if (!g) fail(AG_ERR_INVALID, "null graph");
         if (g->N && (!x || !y)) fail(AG_ERR_INVALID, "null vector");
@@ -3183,6 +3190,7 @@ int ag_spmv_pull_device(ag_ctx *ctx, ag_graph *g, const float *d_x, float *d_y,
                         uint32_t iterations, ag_spmv_result *res)
 {
     return guard([&] {
+        PhaseRange phase_("ag_spmv_pull_device");
         check_ctx(ctx);
         if (!g) fail(AG_ERR_INVALID, "null graph");
         if (g->N && (!d_x || !d_y)) fail(AG_ERR_INVALID, "null vector");
@@ -3195,6 +3203,7 @@ int ag_pagerank_pull_device(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, 
                             ag_pr_result *res)
 {
     return guard([&] {
+        PhaseRange phase_("ag_pagerank_pull_device");
         check_ctx(ctx);
         if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
         if (g->N && !d_ranks) fail(AG_ERR_INVALID, "null output");
@@ -3206,6 +3215,7 @@ int ag_pagerank_pull_device_noalloc(ag_ctx *ctx, ag_graph *g, const ag_pr_option
                                     float *d_ranks, ag_pr_result *res)
 {
     return guard([&] {
+        PhaseRange phase_("ag_pagerank_pull_device_noalloc");
         check_ctx(ctx);
         if (!g || !opt) fail(AG_ERR_INVALID, "null argument");
         if (g->N && !d_ranks) fail(AG_ERR_INVALID, "null output");

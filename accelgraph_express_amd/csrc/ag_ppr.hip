@@ -1744,6 +1744,7 @@ int ag_ppr_batch_bytes(ag_ctx *ctx, ag_graph *g, uint32_t capacity, uint32_t pre
 int ag_ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t precision, uint32_t *cpw)
 {
     return guard([&] {
+        PhaseRange phase_("ag_ppr_autotune");
         check_ctx(ctx);
         if (!g || !cpw) fail(AG_ERR_INVALID, "null argument");
         *cpw = ppr_autotune(ctx, g, precision);
@@ -1754,6 +1755,7 @@ int ag_ppr_batch(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, uint32_t nq,
                  const ag_ppr_options *opt, float *x, ag_ppr_stat *stats, double *elapsed_ms)
 {
     return guard([&] {
+        PhaseRange phase_("ag_ppr_batch");
         check_ctx(ctx);
         if (!g || (nq && !q)) fail(AG_ERR_INVALID, "null argument");
         ag_ppr_options o;
@@ -1807,6 +1809,7 @@ int ag_ppr_single_opts(ag_ctx *ctx, ag_graph *g, const ag_ppr_query *q, const ag
                        float *x, ag_ppr_stat *stat)
 {
     return guard([&] {
+        PhaseRange phase_("ag_ppr_single_opts");
         check_ctx(ctx);
         if (!opt) fail(AG_ERR_INVALID, "null argument");
         const uint32_t precision = opt->precision;

@@ -223,6 +223,7 @@ int ag_reorder_labels_device(ag_ctx *ctx, const uint32_t *d_src, const uint32_t 
                              uint32_t seed, uint32_t *d_labels)
 {
     return guard([&] {
+        PhaseRange phase_("ag_reorder_labels_device");
         check_ctx(ctx);
         if (num_edges && (!d_src || !d_dst)) fail(AG_ERR_INVALID, "null edge array");
         if (num_vertices && !d_labels) fail(AG_ERR_INVALID, "null labels");
@@ -234,6 +235,7 @@ int ag_reorder_labels(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uin
                       uint32_t num_vertices, uint32_t mode, uint32_t seed, uint32_t *labels)
 {
     return guard([&] {
+        PhaseRange phase_("ag_reorder_labels");
         check_ctx(ctx);
         if (num_edges && (!src || !dst)) fail(AG_ERR_INVALID, "null edge array");
         if (num_vertices && !labels) fail(AG_ERR_INVALID, "null labels");
@@ -257,6 +259,7 @@ int ag_relabel_edges_device(ag_ctx *ctx, const uint32_t *d_labels, uint32_t num_
                             uint32_t *d_src, uint32_t *d_dst, uint32_t num_edges)
 {
     return guard([&] {
+        PhaseRange phase_("ag_relabel_edges_device");
         check_ctx(ctx);
         if (num_edges && (!d_src || !d_dst)) fail(AG_ERR_INVALID, "null edge array");
         if (num_vertices && !d_labels) fail(AG_ERR_INVALID, "null labels");
@@ -268,6 +271,7 @@ int ag_relabel_edges(ag_ctx *ctx, const uint32_t *labels, uint32_t num_vertices,
                      uint32_t *dst, uint32_t num_edges)
 {
     return guard([&] {
+        PhaseRange phase_("ag_relabel_edges");
         check_ctx(ctx);
         if (num_edges && (!src || !dst)) fail(AG_ERR_INVALID, "null edge array");
         if (num_vertices && !labels) fail(AG_ERR_INVALID, "null labels");

@@ -176,6 +176,7 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
                  uint32_t flags, ag_graph **out)
 {
     return guard([&] {
+        PhaseRange phase_("ag_csr_build");
         check_ctx(ctx);
         if (!out) fail(AG_ERR_INVALID, "null out");
         if (E && (!src || !dst)) fail(AG_ERR_INVALID, "null edge arrays");
@@ -219,6 +220,7 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
             }
         };
 
+        std::unique_ptr<PhaseRange> half(new PhaseRange("csr.forward"));
         sort_into(ds, dd, g->fwd_src, g->fwd_dst);
         g->fwd_outdeg.alloc((size_t)N * 4);
         g->fwd_idx.alloc((size_t)N * 4);
@@ -227,7 +229,9 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
                    g->fwd_idx.as<uint32_t>(), g->fwd_off.as<uint32_t>());
         g->has_full_fwd = true;
 
+        half.reset();
         if (!(flags & AG_BUILD_NO_INVERSE)) {
+            PhaseRange inv_phase("csr.inverse");
             const bool from_input = (flags & AG_BUILD_INVERSE_FROM_INPUT) != 0;
             const uint32_t *ik = from_input ? dd : g->fwd_dst.as<uint32_t>();
             const uint32_t *iv = from_input ? ds : g->fwd_src.as<uint32_t>();
@@ -249,6 +253,7 @@ int ag_graph_from_csr(ag_ctx *ctx, uint32_t N, uint32_t E, const uint32_t *fwd_o
                       const uint32_t *inv_dest, ag_graph **out)
 {
     return guard([&] {
+        PhaseRange phase_("ag_graph_from_csr");
         check_ctx(ctx);
         if (!out) fail(AG_ERR_INVALID, "null out");
         if (N && (!fwd_out_degree || !inv_out_degree || !inv_edges_idx))
@@ -330,6 +335,7 @@ int ag_graph_download(ag_ctx *ctx, const ag_graph *g, uint32_t *out_degree, uint
                       uint32_t *inv_sorted_dest)
 {
     return guard([&] {
+        PhaseRange phase_("ag_graph_download");
         check_ctx(ctx);
         if (!g) fail(AG_ERR_INVALID, "null graph");
         hipStream_t s = ctx->stream;

@@ -91,6 +91,7 @@ extern "C" {
 int ag_graph_save(ag_ctx *ctx, const ag_graph *gc, const char *path)
 {
     return guard([&] {
+        PhaseRange phase_("ag_graph_save");
         check_ctx(ctx);
         if (!gc || !path) fail(AG_ERR_INVALID, "null argument");
         ag_graph *g = const_cast<ag_graph *>(gc); // only read
@@ -129,6 +130,7 @@ int ag_graph_save(ag_ctx *ctx, const ag_graph *gc, const char *path)
 int ag_graph_load(ag_ctx *ctx, const char *path, ag_graph **out)
 {
     return guard([&] {
+        PhaseRange phase_("ag_graph_load");
         check_ctx(ctx);
         if (!path || !out) fail(AG_ERR_INVALID, "null argument");
         std::ifstream f(path, std::ios::binary | std::ios::ate);

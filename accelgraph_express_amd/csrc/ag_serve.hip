@@ -297,6 +297,7 @@ int ag_serve_submit(ag_server *s, uint32_t gid, const uint32_t *seeds, uint32_t 
                     uint32_t max_iter, uint64_t *ticket)
 {
     return guard([&] {
+        PhaseRange phase_("ag_serve_submit");
         if (!s || !ticket) fail(AG_ERR_INVALID, "null argument");
         std::lock_guard<std::mutex> lk(s->mu);
         if (gid >= s->graphs.size()) fail(AG_ERR_NOT_FOUND, "unknown graph id %u", gid);
@@ -336,6 +337,7 @@ int ag_serve_cancel(ag_server *s, uint64_t ticket)
 int ag_serve_step(ag_server *s, uint32_t *active)
 {
     return guard([&] {
+        PhaseRange phase_("ag_serve_step");
         if (!s) fail(AG_ERR_INVALID, "null server");
         std::lock_guard<std::mutex> lk(s->mu);
         check_ctx(s->ctx);
