@@ -38,6 +38,7 @@ AG_PREC_FP16 = 2
 AG_PR_FLAG_FINISH_FP32 = 0x1
 AG_PR_FLAG_TIME_SWEEPS = 0x2
 AG_PR_FLAG_EXTRAPOLATE = 0x4
+AG_PR_FLAG_WARM_START = 0x8
 AG_LAYOUT_TABLES = 1
 AG_LAYOUT_ROWS = 2
 AG_LAYOUT_AUTO = 3

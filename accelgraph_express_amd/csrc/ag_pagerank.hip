@@ -1274,7 +1274,10 @@ __global__ __launch_bounds__(kFinBlock) void k_pr_finalize(SweepArgs a, uint32_t
     finalize_body(a, it);
 }
 
-__global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
+// Sweep set-up: state, per-sweep formats, the zero after each contribution
+// vector and (unless the caller's vector is the start, `warm`) the uniform or
+// seeded start in label space.
+__global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0, uint32_t warm)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -1304,7 +1307,7 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
             static_cast<uint16_t *>(a.c1)[a.N] = 0;
         }
     }
-    if (i >= a.N) return;
+    if (i >= a.N || warm) return;
     float x = x0;
     if (a.mode == kModePPR) {
         x = 0.0f;
@@ -1317,6 +1320,44 @@ __global__ void k_pr_init(SweepArgs a, float x0, float tele0, uint32_t fmt0)
     const float c = od ? x / (float)od : 0.0f;
     store_c(a.c0, fmt0, i, c, a.f16_scale);
     if (a.skip_zero && i >= a.n_nonzero) store_c(a.c1, fmt0, i, c, a.f16_scale);
+}
+
+// Warm start, after k_pr_init: vertex v's start xin[v] * xscale goes to its
+// label rank[v] (rank and contribution), and each block writes the dangling
+// mass of its vertices (FP64, fixed order) to part[block].  xscale is N in
+// REFERENCE and DATA_DRIVEN modes (the caller's vector is pr / N) and 1 in
+// STANDARD mode.
+__global__ __launch_bounds__(kBlock) void k_pr_init_from(SweepArgs a, const float *__restrict__ xin,
+                                                         const uint32_t *__restrict__ rank,
+                                                         float xscale, uint32_t fmt0,
+                                                         double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    __shared__ double s[kBlock / 64];
+    double dang = 0.0;
+    const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < a.N) {
+        const uint32_t l = rank[v];
+        const float x = xin[v] * xscale;
+        a.pr[l] = x;
+        const uint32_t od = a.outdeg[l];
+        store_c(a.c0, fmt0, l, od ? x / (float)od : 0.0f, a.f16_scale);
+        if (od == 0) dang = (double)x;
+    }
+    dang = wave_sum(dang);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = dang;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+// STANDARD warm start: the first sweep's teleport term from the start's
+// dangling mass D (summed by k_sum_parts), in the form k_pr_finalize uses
+// for later sweeps.
+__global__ void k_pr_warm_tele(SweepArgs a, const double *__restrict__ dsum)
+{
+    if (threadIdx.x != 0 || a.mode != AG_PR_STANDARD) return;
+    const double D = *dsum;
+    a.st->tele = (float)((1.0 - (double)a.damp) * a.inv_n + (double)a.damp * D * a.inv_n);
 }
 
 // SPMV pull set-up: resets the sweep state, marks every sweep FP32, and loads
@@ -2443,8 +2484,11 @@ static SweepArgs make_args(const Plan *p, const WorkPtrs &w, const ag_pr_options
     a.extrap = (o.flags & AG_PR_FLAG_EXTRAPOLATE) ? 1u : 0u;
     // DATA_DRIVEN zero rows never change rank; with epsilon > 0 they never take
     // the "updated" mark either, so both buffers can hold them from init.
-    a.skip_zero = ((o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ||
-                   (o.mode == AG_PR_DATA_DRIVEN && o.epsilon > 0.0))
+    // A warm start's zero rows do change (from the caller's value to base)
+    // in the first sweep, so they are swept.
+    a.skip_zero = !(o.flags & AG_PR_FLAG_WARM_START) &&
+                          ((o.mode == AG_PR_REFERENCE && o.precision == AG_PREC_FP32) ||
+                           (o.mode == AG_PR_DATA_DRIVEN && o.epsilon > 0.0))
                       ? 1u
                       : 0u;
     // STANDARD ranks are ~1/N: scale FP16 words by 2^ceil(log2 N)
@@ -2692,12 +2736,16 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
         fail(AG_ERR_INVALID, "personalized mode needs a seed table");
     if ((o.flags & AG_PR_FLAG_EXTRAPOLATE) && o.mode != kModePPR && o.mode != AG_PR_STANDARD)
         fail(AG_ERR_INVALID, "extrapolated steps are offered for STANDARD and personalized PageRank only");
+    const bool warm = (o.flags & AG_PR_FLAG_WARM_START) != 0;
+    if (warm && o.mode == kModePPR)
+        fail(AG_ERR_INVALID, "personalized PageRank starts at its seed vector: no warm start");
     if (o.precision > AG_PREC_FP16) fail(AG_ERR_INVALID, "unknown precision %u", o.precision);
     if (!(o.damp > 0.0f && o.damp < 1.0f)) fail(AG_ERR_INVALID, "damping must be in (0, 1)");
     if (N == 0) {
         if (res) *res = r;
         return;
     }
+    if (warm && !d_out) fail(AG_ERR_INVALID, "a warm start reads the start from the output buffer");
     WorkPtrs w;
     if (no_alloc) {
         w = *fixed_work;
@@ -2726,7 +2774,14 @@ static void run_pagerank_impl(ag_ctx *ctx, ag_graph *g, const ag_pr_options &o,
 
     auto enqueue_init = [&] {
         hipLaunchKernelGGL(k_pr_init, dim3(ceil_div((uint64_t)std::max(N + 1, max_iter + 2), 256)),
-                           dim3(256), 0, s, a, x0, tele0, o.precision);
+                           dim3(256), 0, s, a, x0, tele0, o.precision, warm ? 1u : 0u);
+        if (warm) {
+            hipLaunchKernelGGL(k_pr_init_from, dim3(nbf), dim3(kBlock), 0, s, a,
+                               (const float *)d_out, p->rank.as<uint32_t>(),
+                               ref_scale ? (float)N : 1.0f, o.precision, parts);
+            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kSumBlock), 0, s, parts, nbf, parts + nbf);
+            hipLaunchKernelGGL(k_pr_warm_tele, dim3(1), dim3(64), 0, s, a, (const double *)parts + nbf);
+        }
         AG_HIP(hipGetLastError());
     };
     auto enqueue_finish = [&] {
@@ -3157,6 +3212,11 @@ int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *
             return;
         }
         ensure_work(ctx, p, opt->iterations);
+        if (opt->flags & AG_PR_FLAG_WARM_START) {
+            if (!ranks) fail(AG_ERR_INVALID, "a warm start reads the start from `ranks`");
+            AG_HIP(hipMemcpyAsync(p->work.out.p, ranks, (size_t)g->N * 4, hipMemcpyHostToDevice,
+                                  ctx->stream));
+        }
         run_pagerank(ctx, g, *opt, p->work.out.as<float>(), active_iter, err_iter, res);
         if (ranks) {
             AG_HIP(hipMemcpyAsync(ranks, p->work.out.p, (size_t)g->N * 4, hipMemcpyDeviceToHost,

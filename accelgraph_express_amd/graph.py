@@ -212,7 +212,11 @@ class Graph:
 
     def pagerank_pull(self, iterations: int = 20, epsilon: float = 1e-4, mode="reference",
                       precision="fp32", finish_fp32: bool = False,
-                      damp: float = 0.85, extrapolate: bool = False) -> PageRankStats:
+                      damp: float = 0.85, extrapolate: bool = False,
+                      init=None) -> PageRankStats:
+        """Pull PageRank on the resident graph.  `init` (N floats, the scale of
+        the returned ranks, e.g. a previous call's pageRanks) replaces the
+        uniform start (AG_PR_FLAG_WARM_START; not in the reference)."""
         lib = L.load()
         o = L.PrOptions()
         lib.ag_pr_options_default(C.byref(o))
@@ -225,6 +229,12 @@ class Graph:
         o.damp = damp
         n = self.num_vertices
         ranks = np.zeros(n, np.float32)
+        if init is not None:
+            x0 = np.ascontiguousarray(init, np.float32)
+            if x0.size != n:
+                raise ValueError(f"init has {x0.size} entries, graph has {n} vertices")
+            ranks[:] = x0
+            o.flags |= L.AG_PR_FLAG_WARM_START
         act = np.zeros(max(iterations, 1), np.uint32)
         err = np.zeros(max(iterations, 1), np.float64)
         r = L.PrResult()
@@ -260,10 +270,12 @@ class Graph:
 
     def pagerank_pull_device(self, d_ranks_ptr: int, iterations: int = 20, epsilon: float = 0.0,
                              mode="reference", precision="fp32", finish_fp32=False,
-                             time_sweeps: bool = False, extrapolate: bool = False) -> L.PrResult:
+                             time_sweeps: bool = False, extrapolate: bool = False,
+                             warm_start: bool = False) -> L.PrResult:
         """Ranks to a device buffer of N floats; time_sweeps fills
         sweep_kernel_ms (HIP events around each sweep kernel); extrapolate
-        (STANDARD mode) takes Aitken-type extrapolated steps."""
+        (STANDARD mode) takes Aitken-type extrapolated steps; warm_start reads
+        the start from the same buffer first."""
         lib = L.load()
         o = L.PrOptions()
         lib.ag_pr_options_default(C.byref(o))
@@ -272,7 +284,8 @@ class Graph:
         o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
         o.flags = (L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0) | \
                   (L.AG_PR_FLAG_TIME_SWEEPS if time_sweeps else 0) | \
-                  (L.AG_PR_FLAG_EXTRAPOLATE if extrapolate else 0)
+                  (L.AG_PR_FLAG_EXTRAPOLATE if extrapolate else 0) | \
+                  (L.AG_PR_FLAG_WARM_START if warm_start else 0)
         r = L.PrResult()
         L.check(lib.ag_pagerank_pull_device(self.ctx.h, self.h, C.byref(o),
                                             C.c_void_p(d_ranks_ptr), C.byref(r)))
@@ -280,7 +293,7 @@ class Graph:
 
     def pagerank_pull_device_noalloc(self, d_ranks_ptr: int, iterations: int = 20,
                                      epsilon: float = 0.0, mode="reference", precision="fp32",
-                                     finish_fp32=False) -> L.PrResult:
+                                     finish_fp32=False, warm_start: bool = False) -> L.PrResult:
         """ag_pagerank_pull_device_noalloc: the same loop with no device
         allocation (per-call buffers in the library's static 16 MiB scratch;
         the gather layout must exist, see prepare())."""
@@ -290,7 +303,8 @@ class Graph:
         o.iterations, o.epsilon = iterations, epsilon
         o.mode = _MODES[mode] if isinstance(mode, str) else int(mode)
         o.precision = _PRECS[precision] if isinstance(precision, str) else int(precision)
-        o.flags = L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0
+        o.flags = (L.AG_PR_FLAG_FINISH_FP32 if finish_fp32 else 0) | \
+                  (L.AG_PR_FLAG_WARM_START if warm_start else 0)
         r = L.PrResult()
         L.check(lib.ag_pagerank_pull_device_noalloc(self.ctx.h, self.h, C.byref(o),
                                                     C.c_void_p(d_ranks_ptr), C.byref(r)))

@@ -144,6 +144,19 @@ int ag_vertex_map_host(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uin
                                        ag_ppr_single_opts): Aitken-type extrapolated steps
                                        once the residual falls by a steady ratio; the run
                                        stops only on a plain sweep's residual */
+#define AG_PR_FLAG_WARM_START  0x8u /* REFERENCE / STANDARD / DATA_DRIVEN: start from the rank
+                                       vector the caller passes in the output buffer (`ranks`
+                                       / `d_ranks`, N floats, read before it is overwritten)
+                                       instead of the reference's uniform start.  The vector
+                                       is in the output's scale (the ranks a previous call
+                                       returned): the sweeps read it times N in REFERENCE and
+                                       DATA_DRIVEN modes, as is in STANDARD mode, where the
+                                       first sweep's dangling mass is summed from it.  No
+                                       counterpart in the reference (pageRank.c:942-1108
+                                       always starts at base).  In REFERENCE mode with N a
+                                       power of two, k sweeps then a warm start with m more
+                                       equal k + m sweeps bit for bit; DATA_DRIVEN restarts
+                                       with every vertex on the worklist */
 
 typedef struct ag_pr_options
 {
@@ -177,7 +190,8 @@ typedef struct ag_pr_result
                                  alone, summed over sweeps (finalize launches excluded) */
 } ag_pr_result;
 
-/* Runs PageRank on a resident graph.  ranks (host, N floats) may be NULL;
+/* Runs PageRank on a resident graph.  ranks (host, N floats) may be NULL
+ * unless AG_PR_FLAG_WARM_START is set (it then holds the start on entry);
  * active_iter / err_iter (host, `iterations` entries) may be NULL. */
 int ag_pagerank_pull(ag_ctx *ctx, ag_graph *g, const ag_pr_options *opt, float *ranks,
                      uint32_t *active_iter, double *err_iter, ag_pr_result *res);

@@ -120,6 +120,18 @@ uint32_t oracle_pagerank_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
                                   double epsilon, float *pr, double *error_total,
                                   double *pr_sum, uint32_t *active_iter, double *err_iter)
 {
+    return oracle_pagerank_pull_f32_from(N, fwd_outdeg, inv_deg, inv_idx, inv_dest, iterations,
+                                         epsilon, NULL, pr, error_total, pr_sum, active_iter,
+                                         err_iter);
+}
+
+uint32_t oracle_pagerank_pull_f32_from(uint32_t N, const uint32_t *fwd_outdeg,
+                                       const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                       const uint32_t *inv_dest, uint32_t iterations,
+                                       double epsilon, const float *init, float *pr,
+                                       double *error_total, double *pr_sum,
+                                       uint32_t *active_iter, double *err_iter)
+{
     const float damp = 0.85f;
     const float base = 1.0f - damp;
     float *contrib = (float *)xcalloc(N, sizeof(float));
@@ -127,7 +139,8 @@ uint32_t oracle_pagerank_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
     double err_tot = 0.0;
     uint32_t it;
 
-    for (uint32_t v = 0; v < N; v++) { pr[v] = base; next[v] = 0.0f; }
+    /* pageRank.c starts at base; a warm start is the caller's pr / N, times N */
+    for (uint32_t v = 0; v < N; v++) { pr[v] = init ? init[v] * (float)N : base; next[v] = 0.0f; }
 
     for (it = 0; it < iterations; it++) {
         uint32_t active = 0;
@@ -207,12 +220,22 @@ uint32_t oracle_pagerank_standard_f64(uint32_t N, const uint32_t *fwd_outdeg,
                                       uint32_t iterations, double tol, double *x,
                                       double *residual)
 {
+    return oracle_pagerank_standard_f64_from(N, fwd_outdeg, inv_deg, inv_idx, inv_dest, damp,
+                                             iterations, tol, NULL, x, residual);
+}
+
+uint32_t oracle_pagerank_standard_f64_from(uint32_t N, const uint32_t *fwd_outdeg,
+                                           const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                           const uint32_t *inv_dest, double damp,
+                                           uint32_t iterations, double tol, const double *init,
+                                           double *x, double *residual)
+{
     double *contrib = (double *)xcalloc(N, sizeof(double));
     double *nx = (double *)xcalloc(N, sizeof(double));
     double res = 0.0;
     uint32_t it;
     if (N == 0) { if (residual) *residual = 0.0; free(contrib); free(nx); return 0; }
-    for (uint32_t v = 0; v < N; v++) x[v] = 1.0 / N;
+    for (uint32_t v = 0; v < N; v++) x[v] = init ? init[v] : 1.0 / N;
     for (it = 0; it < iterations; it++) {
         double dangling = 0.0;
         for (uint32_t v = 0; v < N; v++) {
@@ -285,6 +308,20 @@ uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
                                      double *pr_sum, uint32_t *active_iter, double *err_iter,
                                      uint32_t *work_iter)
 {
+    return oracle_pagerank_dd_pull_f32_from(N, fwd_outdeg, fwd_idx, fwd_dest, inv_deg, inv_idx,
+                                            inv_dest, iterations, epsilon, NULL, pr, error_total,
+                                            pr_sum, active_iter, err_iter, work_iter);
+}
+
+uint32_t oracle_pagerank_dd_pull_f32_from(uint32_t N, const uint32_t *fwd_outdeg,
+                                          const uint32_t *fwd_idx, const uint32_t *fwd_dest,
+                                          const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                          const uint32_t *inv_dest, uint32_t iterations,
+                                          double epsilon, const float *init, float *pr,
+                                          double *error_total, double *pr_sum,
+                                          uint32_t *active_iter, double *err_iter,
+                                          uint32_t *work_iter)
+{
     const float damp = 0.85f;
     const float base = 1.0f - damp;
     float *contrib = (float *)xcalloc(N ? N : 1, sizeof(float));
@@ -293,7 +330,7 @@ uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
     double err_tot = 0.0;
     uint32_t it;
 
-    for (uint32_t v = 0; v < N; v++) { pr[v] = base; cur[v] = 1; } /* :2862-2870 */
+    for (uint32_t v = 0; v < N; v++) { pr[v] = init ? init[v] * (float)N : base; cur[v] = 1; } /* :2862-2870 */
 
     for (it = 0; it < iterations; it++) {
         uint32_t active = 0, work = 0;

@@ -56,6 +56,16 @@ uint32_t oracle_pagerank_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
                                   double epsilon, float *ranks, double *error_total,
                                   double *pr_sum, uint32_t *active_iter, double *err_iter);
 
+/* Same, from a caller's start: init[v] (the scale of the returned ranks,
+ * pr / N) times N in FP32 replaces base.  init NULL: the reference's start.
+ * Not in the reference (a warm start of the library's pull). */
+uint32_t oracle_pagerank_pull_f32_from(uint32_t N, const uint32_t *fwd_outdeg,
+                                       const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                       const uint32_t *inv_dest, uint32_t iterations,
+                                       double epsilon, const float *init, float *ranks,
+                                       double *error_total, double *pr_sum,
+                                       uint32_t *active_iter, double *err_iter);
+
 /* Same operator in FP64 (the damping constants are the FP32 ones, promoted). */
 uint32_t oracle_pagerank_pull_f64(uint32_t N, const uint32_t *fwd_outdeg,
                                   const uint32_t *inv_deg, const uint32_t *inv_idx,
@@ -70,6 +80,13 @@ uint32_t oracle_pagerank_standard_f64(uint32_t N, const uint32_t *fwd_outdeg,
                                       const uint32_t *inv_dest, double damp,
                                       uint32_t iterations, double tol, double *x,
                                       double *residual);
+
+/* Same from x0 = init (N doubles; NULL: 1/N). */
+uint32_t oracle_pagerank_standard_f64_from(uint32_t N, const uint32_t *fwd_outdeg,
+                                           const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                           const uint32_t *inv_dest, double damp,
+                                           uint32_t iterations, double tol, const double *init,
+                                           double *x, double *residual);
 
 /* Personalized PageRank, FP64, one query: teleport vector s uniform over the
  * seed list (duplicates add mass); dangling mass returned to s.
@@ -95,6 +112,17 @@ uint32_t oracle_pagerank_dd_pull_f32(uint32_t N, const uint32_t *fwd_outdeg,
                                      double epsilon, float *ranks, double *error_total,
                                      double *pr_sum, uint32_t *active_iter, double *err_iter,
                                      uint32_t *work_iter);
+
+/* Same from the start init[v] * N (FP32; NULL: base), every vertex on the
+ * first worklist.  Not in the reference (a warm start of the library's pull). */
+uint32_t oracle_pagerank_dd_pull_f32_from(uint32_t N, const uint32_t *fwd_outdeg,
+                                          const uint32_t *fwd_idx, const uint32_t *fwd_dest,
+                                          const uint32_t *inv_deg, const uint32_t *inv_idx,
+                                          const uint32_t *inv_dest, uint32_t iterations,
+                                          double epsilon, const float *init, float *ranks,
+                                          double *error_total, double *pr_sum,
+                                          uint32_t *active_iter, double *err_iter,
+                                          uint32_t *work_iter);
 
 /* SPMVPullGraphCSR (SPMV.c:622-750), unweighted (graphConfig.h:7
  * WEIGHTED 0): the input vector of :699-706, x[v] = 1/out_degree(v) (FP32

@@ -39,6 +39,8 @@ def load() -> C.CDLL:
         "oracle_build_csr": (None, [U, U, u32p, u32p, U] + [u32p] * 10),
         "oracle_pagerank_pull_f32": (U, [U, u32p, u32p, u32p, u32p, U, D, f32p, f64p, f64p,
                                          u32p, f64p]),
+        "oracle_pagerank_pull_f32_from": (U, [U, u32p, u32p, u32p, u32p, U, D, f32p, f32p, f64p,
+                                              f64p, u32p, f64p]),
         "oracle_pagerank_pull_f64": (U, [U, u32p, u32p, u32p, u32p, U, D, f64p, f64p]),
         "oracle_spmv_input": (None, [U, u32p, f32p]),
         "oracle_reorder_labels": (I, [U, U, u32p, u32p, U, U, u32p]),
@@ -46,7 +48,11 @@ def load() -> C.CDLL:
         "oracle_spmv_sum": (D, [U, f32p]),
         "oracle_pagerank_dd_pull_f32": (U, [U] + [u32p] * 6 + [U, D, f32p, f64p, f64p, u32p,
                                                               f64p, u32p]),
+        "oracle_pagerank_dd_pull_f32_from": (U, [U] + [u32p] * 6 + [U, D, f32p, f32p, f64p, f64p,
+                                                                   u32p, f64p, u32p]),
         "oracle_pagerank_standard_f64": (U, [U, u32p, u32p, u32p, u32p, D, U, D, f64p, f64p]),
+        "oracle_pagerank_standard_f64_from": (U, [U, u32p, u32p, u32p, u32p, D, U, D, f64p, f64p,
+                                                  f64p]),
         "oracle_ppr_f64": (U, [U, u32p, u32p, u32p, u32p, D, u32p, U, D, U, f64p, f64p]),
         "oracle_csr_build_omp": (D, [U, U, u32p, u32p] + [u32p] * 8 + [I]),
         "oracle_pagerank_pull_omp": (D, [U, u32p, u32p, u32p, u32p, U, D, f32p, f64p, u32p, I]),
@@ -115,26 +121,38 @@ def _inv(csr):
             _u(csr["inverse_sorted_edges_array.edges_array_dest"]))
 
 
-def pagerank_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4):
-    """Returns (ranks f32, iterations, error_total, pr_sum, active[], err[])."""
+def _init32(init, n):
+    if init is None:
+        return None
+    x = np.ascontiguousarray(init, np.float32)
+    assert x.size == n
+    return x
+
+
+def pagerank_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4, init=None):
+    """Returns (ranks f32, iterations, error_total, pr_sum, active[], err[]).
+    init (N floats, the returned ranks' scale) replaces the reference's start."""
     n = csr["num_vertices"]
     od, idg, iidx, idst = _inv(csr)
+    x0 = _init32(init, n)
     pr = np.zeros(n, np.float32)
     et, ps = C.c_double(), C.c_double()
     act = np.zeros(max(iterations, 1), np.uint32)
     err = np.zeros(max(iterations, 1), np.float64)
-    it = load().oracle_pagerank_pull_f32(n, _p(od), _p(idg), _p(iidx), _p(idst), iterations,
-                                         epsilon, _p(pr, C.c_float), C.byref(et), C.byref(ps),
-                                         _p(act), _p(err, C.c_double))
+    it = load().oracle_pagerank_pull_f32_from(n, _p(od), _p(idg), _p(iidx), _p(idst), iterations,
+                                              epsilon, None if x0 is None else _p(x0, C.c_float),
+                                              _p(pr, C.c_float), C.byref(et), C.byref(ps),
+                                              _p(act), _p(err, C.c_double))
     sweeps = min(iterations, it + 1) if iterations else 0
     return pr, int(it), et.value, ps.value, act[:sweeps].copy(), err[:sweeps].copy()
 
 
-def pagerank_dd_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4):
+def pagerank_dd_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4, init=None):
     """Data-driven pull.  Returns (ranks f32, iterations, error_total, pr_sum,
     active[], err[], work[]) with per-sweep updated counts, error totals and
-    worklist sizes."""
+    worklist sizes.  init as in pagerank_pull_f32."""
     n = csr["num_vertices"]
+    x0 = _init32(init, n)
     od, idg, iidx, idst = _inv(csr)
     fidx = _u(csr["vertices.edges_idx"])
     fdst = _u(csr["sorted_edges_array.edges_array_dest"])
@@ -142,8 +160,10 @@ def pagerank_dd_pull_f32(csr: dict, iterations: int = 20, epsilon: float = 1e-4)
     et, ps = C.c_double(), C.c_double()
     k = max(iterations, 1)
     act, err, work = np.zeros(k, np.uint32), np.zeros(k, np.float64), np.zeros(k, np.uint32)
-    it = load().oracle_pagerank_dd_pull_f32(n, _p(od), _p(fidx), _p(fdst), _p(idg), _p(iidx),
-                                            _p(idst), iterations, epsilon, _p(pr, C.c_float),
+    it = load().oracle_pagerank_dd_pull_f32_from(n, _p(od), _p(fidx), _p(fdst), _p(idg), _p(iidx),
+                                            _p(idst), iterations, epsilon,
+                                            None if x0 is None else _p(x0, C.c_float),
+                                            _p(pr, C.c_float),
                                             C.byref(et), C.byref(ps), _p(act),
                                             _p(err, C.c_double), _p(work))
     sweeps = min(iterations, it + 1) if iterations else 0
@@ -196,13 +216,19 @@ def pagerank_pull_f64(csr: dict, iterations: int = 20, epsilon: float = 1e-4):
     return pr, int(it), et.value
 
 
-def pagerank_standard_f64(csr: dict, iterations: int, tol: float = 0.0, damp: float = 0.85):
+def pagerank_standard_f64(csr: dict, iterations: int, tol: float = 0.0, damp: float = 0.85,
+                          init=None):
+    """STANDARD PageRank in FP64 from x0 = 1/N, or from init (N values)."""
     n = csr["num_vertices"]
     od, idg, iidx, idst = _inv(csr)
+    x0 = None if init is None else np.ascontiguousarray(init, np.float64)
+    assert x0 is None or x0.size == n
     x = np.zeros(n, np.float64)
     res = C.c_double()
-    it = load().oracle_pagerank_standard_f64(n, _p(od), _p(idg), _p(iidx), _p(idst), damp,
-                                             iterations, tol, _p(x, C.c_double), C.byref(res))
+    it = load().oracle_pagerank_standard_f64_from(n, _p(od), _p(idg), _p(iidx), _p(idst), damp,
+                                                  iterations, tol,
+                                                  None if x0 is None else _p(x0, C.c_double),
+                                                  _p(x, C.c_double), C.byref(res))
     return x, int(it), res.value
 
 
