@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -127,6 +128,7 @@ enum ScratchSlot : int {
     kScrMax = 8, kScrPairA, kScrPairB, kScrInSrc, kScrInDst,
     kScrLay0 = 13, // kScrLay0 .. +7: gather-layout build temporaries
     kScrBlk = 21,  // first row of each 256-slot block of an edge-slot fill
+    kScrSort2K = 22, kScrSort2V = 23, // second ping-pong pair of radix_sort_pairs_late
 };
 // Device buffer of at least `bytes` in `slot`; growing waits for the stream
 // first, since the old buffer may still be read by queued work.  Sizes are
@@ -205,6 +207,12 @@ void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t
 // Inputs are not modified; outputs may not alias inputs.
 void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                       uint32_t *vout, uint32_t n, uint32_t key_bits);
+// The same, with the output arrays supplied by `out` (which may allocate
+// them) after the first pass is queued, so that a host-side allocation runs
+// while the device sorts.  The first pass writes to scratch.
+void radix_sort_pairs_late(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t n,
+                           uint32_t key_bits,
+                           const std::function<void(uint32_t **kout, uint32_t **vout)> &out);
 // Degree / edges_idx / offsets of a key-sorted array with keys < N.
 void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N,
                 uint32_t *degree, uint32_t *edges_idx, uint32_t *offsets /* N+1 or null */);

@@ -207,16 +207,22 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
         const uint32_t bits = bits_for(N);
         const bool by_src_dst = (mode == 1 || mode == 3);
 
+        // the sorted arrays are allocated once the sort's first pass is
+        // queued, so that hipMalloc's host time overlaps device work
         auto sort_into = [&](const uint32_t *k, const uint32_t *v, DevBuf &ko, DevBuf &vo) {
-            ko.alloc((size_t)E * 4);
-            vo.alloc((size_t)E * 4);
+            auto out = [&](uint32_t **kk, uint32_t **vv) {
+                ko.alloc((size_t)E * 4);
+                vo.alloc((size_t)E * 4);
+                *kk = ko.as<uint32_t>();
+                *vv = vo.as<uint32_t>();
+            };
             if (by_src_dst) {
                 uint32_t *ta = static_cast<uint32_t *>(scratch(ctx, kScrPairA, (size_t)E * 4));
                 uint32_t *tb = static_cast<uint32_t *>(scratch(ctx, kScrPairB, (size_t)E * 4));
                 radix_sort_pairs(ctx, v, k, tb, ta, E, bits);
-                radix_sort_pairs(ctx, ta, tb, ko.as<uint32_t>(), vo.as<uint32_t>(), E, bits);
+                radix_sort_pairs_late(ctx, ta, tb, E, bits, out);
             } else {
-                radix_sort_pairs(ctx, k, v, ko.as<uint32_t>(), vo.as<uint32_t>(), E, bits);
+                radix_sort_pairs_late(ctx, k, v, E, bits, out);
             }
         };
 

@@ -4,7 +4,8 @@
 // sort by source (sortRunAlgorithms, modes at README.md:434-439, called at
 // SSSP.c:548-553) followed by a vertex mapping that yields out_degree and
 // edges_idx (fields read at capienv.c:409-426).  Here the same permutation is
-// produced by stable LSD radix passes of 8-bit digits, each pass being:
+// produced by stable LSD radix passes of digits of at most 8 bits (the widths
+// balanced over the key: 22 bits are 8 + 7 + 7), each pass being:
 //
 //   k_radix_hist     per-tile digit histogram in LDS (tile = 4096 keys)
 //   k_hist_*         digit-major exclusive scan of the tile histograms
@@ -15,6 +16,8 @@
 // Every step is deterministic and atomic-free in global memory, so the sorted
 // arrays are bit-identical run to run and identical to a serial stable
 // counting sort.
+#include <algorithm>
+
 #include "ag_internal.h"
 
 namespace ag {
@@ -99,18 +102,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32
 }
 
 // ---- radix sort -----------------------------------------------------------------
-// Per-tile digit histogram.  Each wave counts into its own LDS copy (fewer
-// same-address atomics); with `vec` (keys 16-byte aligned) a full tile is read
-// as 16-byte loads, four keys each.
+// Per-tile digit histogram.  Each quarter of a wave counts into its own LDS
+// copy: an atomic add of 64 lanes into few buckets (a 7-bit digit has 128)
+// serialises on equal addresses, and the copies split those collisions four
+// ways.  With `vec` (keys 16-byte aligned) a full tile is read as 16-byte
+// loads, four keys each.
+constexpr int kHistCopies = kWaves * 4;
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restrict__ keys,
                                                        uint32_t n, uint32_t shift,
                                                        uint32_t mask, uint32_t vec,
                                                        uint32_t *__restrict__ hist)
 {
-    __shared__ uint32_t h[kWaves][kRadix];
-    const int w = threadIdx.x >> 6;
+    __shared__ uint32_t h[kHistCopies][kRadix];
+    const int w = (threadIdx.x >> 6) * 4 + (threadIdx.x & 3);
 #pragma unroll
-    for (int i = 0; i < kWaves; i++) h[i][threadIdx.x] = 0;
+    for (int i = 0; i < kHistCopies; i++) h[i][threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * kTile;
     if (vec && base + kTile <= n) {
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restric
     __syncthreads();
     uint32_t t = 0;
 #pragma unroll
-    for (int i = 0; i < kWaves; i++) t += h[i][threadIdx.x];
+    for (int i = 0; i < kHistCopies; i++) t += h[i][threadIdx.x];
     hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = t;
 }
 
@@ -208,12 +214,13 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_apply(uint32_t *__restric
     }
 }
 
-// Stable scatter.  Wave w of a tile owns the 1024 consecutive keys
-// [tile*4096 + w*1024, +1024) and walks them in 16 steps of 64.  In each step
-// the lanes holding equal digits are found with `nbits` ballots; the rank of a
-// lane is the wave's running count for its digit plus the number of equal-digit
-// lanes below it.  The lowest lane of each equal-digit group then advances the
-// running count.  The tile is then reordered by digit in LDS (wave 0 < 1 < 2 <
+// Stable scatter of NB-bit digits.  Wave w of a tile owns the 1024 consecutive
+// keys [tile*4096 + w*1024, +1024): it loads all of them (16 per lane) before
+// ranking any, so that the wave has its whole share of the tile in flight at
+// once, then walks them in 16 steps of 64.  In each step the lanes holding
+// equal digits are found with NB ballots; the rank of a lane is the wave's
+// running count for its digit plus the number of equal-digit lanes below it.
+// The lowest lane of each equal-digit group then advances the running count.  The tile is then reordered by digit in LDS (wave 0 < 1 < 2 <
 // 3 inside a digit, i.e. input order) and written out run by run, so that keys
 // of one digit leave as consecutive words instead of one scattered word each.
 // Workgroups are dispatched round-robin over the 8 XCDs.  Giving each XCD a
@@ -229,17 +236,19 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
+template <uint32_t NB>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
-    uint32_t nbits, const uint32_t *__restrict__ hist)
+    const uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t wcnt[kWaves][kRadix]; // per-wave digit counts, then in-tile positions
     __shared__ uint32_t gbase[kRadix];        // global position of the tile's first key of digit d
     __shared__ uint32_t wsum[kWaves];
     __shared__ uint32_t sk[kTile], sv[kTile];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t mask = (1u << nbits) - 1u;
+    constexpr uint32_t mask = (1u << NB) - 1u;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 #pragma unroll
     for (int i = 0; i < kWaves; i++) wcnt[i][threadIdx.x] = 0;
     __syncthreads();
@@ -248,20 +257,43 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint64_t tile0 = (uint64_t)tile * kTile;
     const uint64_t base = tile0 + (uint64_t)w * (kTile / kWaves);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // the wave's keys from a wave-uniform base: lane offsets stay 32-bit
+    const uint32_t *kw = kin + base;
+    const uint32_t *vw = vin + base;
     uint32_t k[kItems], v[kItems], r[kItems];
+    const bool full = tile0 + kTile <= n;
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < kItems; i++) {
+            k[i] = kw[i * 64 + lane];
+            v[i] = vw[i * 64 + lane];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kItems; i++) {
+            const bool valid = base + (uint64_t)i * 64 + lane < n;
+            k[i] = valid ? kw[i * 64 + lane] : 0u;
+            v[i] = valid ? vw[i * 64 + lane] : 0u;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
-        const uint64_t idx = base + (uint64_t)i * 64 + lane;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t val = valid ? vin[idx] : 0u;
-        const uint32_t d = (key >> shift) & mask;
-        uint64_t peers = __ballot(valid);
-        for (uint32_t b = 0; b < nbits; b++) {
-            const bool bit = (d >> b) & 1u;
+        const bool valid = full || base + (uint64_t)i * 64 + lane < n;
+        const uint32_t d = (k[i] >> shift) & mask;
+        // peers: the valid lanes whose digit equals this lane's; per bit b a
+        // lane keeps the ballot when its bit is set and its complement when
+        // not (x is 0 or all ones)
+        const uint64_t vb = __ballot(valid);
+        uint32_t plo = (uint32_t)vb, phi = (uint32_t)(vb >> 32);
+#pragma unroll
+        for (uint32_t b = 0; b < NB; b++) {
+            const uint32_t bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
+            const uint32_t x = bit - 1u;
+            plo &= (uint32_t)m ^ x;
+            phi &= (uint32_t)(m >> 32) ^ x;
         }
+        const uint64_t peers = ((uint64_t)phi << 32) | plo;
         uint32_t rank = 0;
         if (valid) {
             const uint32_t before = wcnt[w][d];
@@ -269,8 +301,6 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
             const int leader = __ffsll((unsigned long long)peers) - 1;
             if (lane == leader) wcnt[w][d] = before + (uint32_t)__popcll(peers);
         }
-        k[i] = key;
-        v[i] = val;
         r[i] = rank;
     }
     __syncthreads();
@@ -435,36 +465,34 @@ void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t
     scan_level(ctx, in, out, n, 0);
 }
 
-void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                      uint32_t *vout, uint32_t n, uint32_t key_bits)
+// Digit widths of the LSD passes over key_bits bits: as few passes of at most
+// 8 bits as cover the key, the widths within one bit of each other.
+static uint32_t digit_passes(uint32_t key_bits) { return (key_bits + 7) / 8; }
+static uint32_t digit_shift(uint32_t key_bits, uint32_t p, uint32_t *nbits)
 {
-    hipStream_t s = ctx->stream;
-    if (n == 0) return;
-    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
-    const uint32_t passes = (key_bits + 7) / 8;
-    if (passes == 0) {
-        AG_HIP(hipMemcpyAsync(kout, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-        AG_HIP(hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
-        return;
+    const uint32_t passes = digit_passes(key_bits);
+    const uint32_t base = key_bits / passes, extra = key_bits % passes;
+    *nbits = base + (p < extra ? 1u : 0u);
+    return p * base + std::min(p, extra);
+}
+
+struct SortPass {
+    ag_ctx *ctx;
+    uint32_t n, ntiles, tpc, nchunks;
+    uint32_t *hist, *csum;
+    SortPass(ag_ctx *c, uint32_t n_) : ctx(c), n(n_)
+    {
+        ntiles = ceil_div(n, kTile);
+        tpc = ceil_div(ntiles, 256);
+        nchunks = ceil_div(ntiles, tpc);
+        hist = static_cast<uint32_t *>(scratch(ctx, kScrHist, (size_t)ntiles * kRadix * 4));
+        csum = static_cast<uint32_t *>(scratch(ctx, kScrCsum, (size_t)nchunks * kRadix * 4));
     }
-    const uint32_t ntiles = ceil_div(n, kTile);
-    const uint32_t tpc = ceil_div(ntiles, 256);
-    const uint32_t nchunks = ceil_div(ntiles, tpc);
-    uint32_t *hist = static_cast<uint32_t *>(scratch(ctx, kScrHist, (size_t)ntiles * kRadix * 4));
-    uint32_t *csum = static_cast<uint32_t *>(scratch(ctx, kScrCsum, (size_t)nchunks * kRadix * 4));
-    uint32_t *tk = nullptr, *tv = nullptr;
-    if (passes > 1) {
-        tk = static_cast<uint32_t *>(scratch(ctx, kScrSortK, (size_t)n * 4));
-        tv = static_cast<uint32_t *>(scratch(ctx, kScrSortV, (size_t)n * 4));
-    }
-    // Ping-pong so that the last pass lands in (kout, vout).
-    const uint32_t *ck = kin, *cv = vin;
-    for (uint32_t p = 0; p < passes; p++) {
-        const uint32_t shift = 8 * p;
-        const uint32_t nbits = std::min<uint32_t>(8, key_bits - shift);
-        const bool to_out = ((passes - 1 - p) % 2) == 0;
-        uint32_t *ok = to_out ? kout : tk;
-        uint32_t *ov = to_out ? vout : tv;
+    // one stable pass: (ck, cv) -> (ok, ov) by the nbits-bit digit at shift
+    void run(const uint32_t *ck, const uint32_t *cv, uint32_t *ok, uint32_t *ov, uint32_t shift,
+             uint32_t nbits) const
+    {
+        hipStream_t s = ctx->stream;
         const uint32_t vec = (reinterpret_cast<uintptr_t>(ck) & 15u) == 0;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ck, n, shift,
                            (1u << nbits) - 1u, vec, hist);
@@ -473,11 +501,82 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
         hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum, nchunks);
         hipLaunchKernelGGL(k_hist_chunk_apply, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
                            csum);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, ck, cv, ok, ov, n,
-                           shift, nbits, hist);
+        switch (nbits) {
+#define AG_SCATTER(B)                                                                              \
+    case B:                                                                                        \
+        hipLaunchKernelGGL(k_radix_scatter<B>, dim3(ntiles), dim3(kBlock), 0, s, ck, cv, ok, ov, n, \
+                           shift, hist);                                                           \
+        break;
+            AG_SCATTER(1) AG_SCATTER(2) AG_SCATTER(3) AG_SCATTER(4)
+            AG_SCATTER(5) AG_SCATTER(6) AG_SCATTER(7) AG_SCATTER(8)
+#undef AG_SCATTER
+        default: fail(AG_ERR_INVALID, "radix digit of %u bits", nbits);
+        }
         AG_HIP(hipGetLastError());
+    }
+};
+
+void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                      uint32_t *vout, uint32_t n, uint32_t key_bits)
+{
+    hipStream_t s = ctx->stream;
+    if (n == 0) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    const uint32_t passes = digit_passes(key_bits);
+    if (passes == 0) {
+        AG_HIP(hipMemcpyAsync(kout, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        AG_HIP(hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    const SortPass sp(ctx, n);
+    uint32_t *tk = nullptr, *tv = nullptr;
+    if (passes > 1) {
+        tk = static_cast<uint32_t *>(scratch(ctx, kScrSortK, (size_t)n * 4));
+        tv = static_cast<uint32_t *>(scratch(ctx, kScrSortV, (size_t)n * 4));
+    }
+    // Ping-pong so that the last pass lands in (kout, vout).
+    const uint32_t *ck = kin, *cv = vin;
+    for (uint32_t p = 0; p < passes; p++) {
+        uint32_t nbits;
+        const uint32_t shift = digit_shift(key_bits, p, &nbits);
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t *ok = to_out ? kout : tk;
+        uint32_t *ov = to_out ? vout : tv;
+        sp.run(ck, cv, ok, ov, shift, nbits);
         ck = ok;
         cv = ov;
+    }
+}
+
+void radix_sort_pairs_late(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t n,
+                           uint32_t key_bits,
+                           const std::function<void(uint32_t **kout, uint32_t **vout)> &out)
+{
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    uint32_t *kout = nullptr, *vout = nullptr;
+    const uint32_t passes = digit_passes(key_bits);
+    if (n == 0 || passes <= 1) {
+        out(&kout, &vout);
+        radix_sort_pairs(ctx, kin, vin, kout, vout, n, key_bits);
+        return;
+    }
+    const SortPass sp(ctx, n);
+    uint32_t *bk[2] = {static_cast<uint32_t *>(scratch(ctx, kScrSortK, (size_t)n * 4)),
+                       static_cast<uint32_t *>(scratch(ctx, kScrSort2K, (size_t)n * 4))};
+    uint32_t *bv[2] = {static_cast<uint32_t *>(scratch(ctx, kScrSortV, (size_t)n * 4)),
+                       static_cast<uint32_t *>(scratch(ctx, kScrSort2V, (size_t)n * 4))};
+    uint32_t nbits;
+    uint32_t shift = digit_shift(key_bits, 0, &nbits);
+    sp.run(kin, vin, bk[0], bv[0], shift, nbits);
+    out(&kout, &vout); // the caller's allocation overlaps the first pass
+    int cur = 0;       // pair holding the last pass's output
+    for (uint32_t p = 1; p < passes; p++) {
+        shift = digit_shift(key_bits, p, &nbits);
+        const bool last = p + 1 == passes;
+        uint32_t *ok = last ? kout : bk[1 - cur];
+        uint32_t *ov = last ? vout : bv[1 - cur];
+        sp.run(bk[cur], bv[cur], ok, ov, shift, nbits);
+        cur = 1 - cur;
     }
 }
 
