@@ -115,7 +115,7 @@ struct ag_ctx {
     // grow on demand and are kept until the context is destroyed, so a build
     // does not pay hipMalloc / hipFree (which waits for the device) per step.
     // The mutex serialises the host threads that use it.
-    static constexpr int kScratchSlots = 24;
+    static constexpr int kScratchSlots = 25;
     std::recursive_mutex scratch_mu;
     ag::DevBuf scratch[kScratchSlots];
 };
@@ -129,6 +129,7 @@ enum ScratchSlot : int {
     kScrLay0 = 13, // kScrLay0 .. +7: gather-layout build temporaries
     kScrBlk = 21,  // first row of each 256-slot block of an edge-slot fill
     kScrSort2K = 22, kScrSort2V = 23, // second ping-pong pair of radix_sort_pairs_late
+    kScrTileMax = 24,                 // per-tile key maxima of a sort's first pass
 };
 // Device buffer of at least `bytes` in `slot`; growing waits for the stream
 // first, since the old buffer may still be read by queued work.  Sizes are
@@ -212,10 +213,14 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
 // while the device sorts.  The first pass writes to scratch.
 void radix_sort_pairs_late(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t n,
                            uint32_t key_bits,
-                           const std::function<void(uint32_t **kout, uint32_t **vout)> &out);
-// Degree / edges_idx / offsets of a key-sorted array with keys < N.
+                           const std::function<void(uint32_t **kout, uint32_t **vout)> &out,
+                           uint32_t *key_max = nullptr /* device word: atomicMax of the keys */);
+// Degree / edges_idx / offsets of a key-sorted array (keys >= N are skipped).
 void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N,
                 uint32_t *degree, uint32_t *edges_idx, uint32_t *offsets /* N+1 or null */);
+// atomicMax(*out, max over a and b), queued on the context's stream.
+void device_max_async(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n,
+                      uint32_t *out);
 // max over two arrays (returns host value); 0 for n == 0.
 uint32_t device_max2(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n);
 uint32_t device_max1(ag_ctx *ctx, const uint32_t *a, uint32_t n);

@@ -196,12 +196,19 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
             ds = static_cast<const uint32_t *>(in_src);
             dd = static_cast<const uint32_t *>(in_dst);
         }
-        const uint32_t maxid = device_max2(ctx, ds, dd, E);
+        // With num_vertices given, the ids are checked after the build: the
+        // first pass of each direction's sort takes the largest key on the
+        // way (dmax), and the vertex mapping skips ids >= N until then.
+        // Without it, N = max id + 1 is needed before the sort.
+        uint32_t *dmax = nullptr;
         if (N == 0) {
+            const uint32_t maxid = device_max2(ctx, ds, dd, E);
             if (E && maxid == 0xffffffffu) fail(AG_ERR_INVALID, "vertex id 0xffffffff");
             N = E ? maxid + 1 : 0;
-        } else if (E && maxid >= N) {
-            fail(AG_ERR_INVALID, "vertex id %u >= num_vertices %u", maxid, N);
+        } else if (E) {
+            dmax = scratch_as<uint32_t>(ctx, kScrMax, 1);
+            AG_HIP(hipMemsetAsync(dmax, 0, 4, s));
+            if (flags & AG_BUILD_NO_INVERSE) device_max_async(ctx, dd, nullptr, E, dmax);
         }
         g->N = N;
         const uint32_t bits = bits_for(N);
@@ -220,9 +227,9 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
                 uint32_t *ta = static_cast<uint32_t *>(scratch(ctx, kScrPairA, (size_t)E * 4));
                 uint32_t *tb = static_cast<uint32_t *>(scratch(ctx, kScrPairB, (size_t)E * 4));
                 radix_sort_pairs(ctx, v, k, tb, ta, E, bits);
-                radix_sort_pairs_late(ctx, ta, tb, E, bits, out);
+                radix_sort_pairs_late(ctx, ta, tb, E, bits, out, dmax);
             } else {
-                radix_sort_pairs_late(ctx, k, v, E, bits, out);
+                radix_sort_pairs_late(ctx, k, v, E, bits, out, dmax);
             }
         };
 
@@ -249,7 +256,10 @@ int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t
                        g->inv_idx.as<uint32_t>(), g->inv_off.as<uint32_t>());
             g->has_inverse_full = true;
         }
+        uint32_t maxid = 0;
+        if (dmax) AG_HIP(hipMemcpyAsync(&maxid, dmax, 4, hipMemcpyDeviceToHost, s));
         AG_HIP(hipStreamSynchronize(s));
+        if (dmax && maxid >= N) fail(AG_ERR_INVALID, "vertex id %u >= num_vertices %u", maxid, N);
         *out = g.release();
     });
 }

@@ -29,6 +29,7 @@ constexpr int kItems = 16;        // keys per thread per tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kRadix = 256;       // 8-bit digits
 constexpr int kWaves = kBlock / 64;
+static_assert(kWaves == 4, "k_radix_hist combines four wave maxima");
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane)
 {
@@ -105,16 +106,21 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const uint32_t *in, uint32
 // Per-tile digit histogram.  Each quarter of a wave counts into its own LDS
 // copy: an atomic add of 64 lanes into few buckets (a 7-bit digit has 128)
 // serialises on equal addresses, and the copies split those collisions four
-// ways.  With `vec` (keys 16-byte aligned) a full tile is read as 16-byte
-// loads, four keys each.
+// ways.  Copies are one word longer than the digit range, so that one digit's
+// counters in different copies sit in different LDS banks.  With `vec` (keys
+// 16-byte aligned) a full tile is read as 16-byte loads, four keys each.
+// tmax (may be null) receives the largest key of each tile.
 constexpr int kHistCopies = kWaves * 4;
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restrict__ keys,
                                                        uint32_t n, uint32_t shift,
                                                        uint32_t mask, uint32_t vec,
-                                                       uint32_t *__restrict__ hist)
+                                                       uint32_t *__restrict__ hist,
+                                                       uint32_t *__restrict__ tmax)
 {
-    __shared__ uint32_t h[kHistCopies][kRadix];
+    __shared__ uint32_t h[kHistCopies][kRadix + 1];
+    __shared__ uint32_t wmax[kWaves];
     const int w = (threadIdx.x >> 6) * 4 + (threadIdx.x & 3);
+    uint32_t m = 0;
 #pragma unroll
     for (int i = 0; i < kHistCopies; i++) h[i][threadIdx.x] = 0;
     __syncthreads();
@@ -130,19 +136,31 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t *__restric
             atomicAdd(&h[w][(q[i].y >> shift) & mask], 1u);
             atomicAdd(&h[w][(q[i].z >> shift) & mask], 1u);
             atomicAdd(&h[w][(q[i].w >> shift) & mask], 1u);
+            m = max(m, max(max(q[i].x, q[i].y), max(q[i].z, q[i].w)));
         }
     } else {
 #pragma unroll
         for (int i = 0; i < kItems; i++) {
             const uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
-            if (idx < n) atomicAdd(&h[w][(keys[idx] >> shift) & mask], 1u);
+            if (idx < n) {
+                const uint32_t key = keys[idx];
+                atomicAdd(&h[w][(key >> shift) & mask], 1u);
+                m = max(m, key);
+            }
         }
+    }
+    if (tmax) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
     }
     __syncthreads();
     uint32_t t = 0;
 #pragma unroll
     for (int i = 0; i < kHistCopies; i++) t += h[i][threadIdx.x];
     hist[(uint64_t)blockIdx.x * kRadix + threadIdx.x] = t;
+    if (tmax && threadIdx.x == 0)
+        tmax[blockIdx.x] = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
 }
 
 // Phase 1: per chunk of `tpc` tiles, per digit sums.
@@ -352,19 +370,21 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
 
 // ---- vertex mapping -------------------------------------------------------------
 // Segment starts -> edges_idx[v]; segment ends -> deg[v] = end (fixed up below).
-__global__ void k_segment_bounds(const uint32_t *__restrict__ keys, uint32_t n,
+// Keys >= N (a build whose ids are checked after the sort) are skipped.
+__global__ void k_segment_bounds(const uint32_t *__restrict__ keys, uint32_t n, uint32_t N,
                                  uint32_t *__restrict__ idx, uint32_t *__restrict__ end)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = keys[i];
+    if (v >= N) return;
     if (i == 0 || keys[i - 1] != v) idx[v] = (uint32_t)i;
     if (i == (uint64_t)n - 1 || keys[i + 1] != v) end[v] = (uint32_t)(i + 1);
 }
 
 // The same, four keys per thread from one 16-byte load (keys 16-byte aligned);
 // the neighbours across the group come from the adjacent words.
-__global__ void k_segment_bounds4(const uint32_t *__restrict__ keys, uint32_t n,
+__global__ void k_segment_bounds4(const uint32_t *__restrict__ keys, uint32_t n, uint32_t N,
                                   uint32_t *__restrict__ idx, uint32_t *__restrict__ end)
 {
     const uint64_t i0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
@@ -384,6 +404,7 @@ __global__ void k_segment_bounds4(const uint32_t *__restrict__ keys, uint32_t n,
     for (uint32_t j = 0; j < 4; j++) {
         if (j >= m) break;
         const uint32_t v = k[j];
+        if (v >= N) continue;
         const bool first = j == 0 ? (!has_prev || prev != v) : k[j - 1] != v;
         const bool last = j + 1 == m ? (!has_next || next != v) : k[j + 1] != v;
         if (first) idx[v] = (uint32_t)(i0 + j);
@@ -488,14 +509,19 @@ struct SortPass {
         hist = static_cast<uint32_t *>(scratch(ctx, kScrHist, (size_t)ntiles * kRadix * 4));
         csum = static_cast<uint32_t *>(scratch(ctx, kScrCsum, (size_t)nchunks * kRadix * 4));
     }
-    // one stable pass: (ck, cv) -> (ok, ov) by the nbits-bit digit at shift
+    // one stable pass: (ck, cv) -> (ok, ov) by the nbits-bit digit at shift;
+    // with key_max, also atomicMax(*key_max, largest key)
     void run(const uint32_t *ck, const uint32_t *cv, uint32_t *ok, uint32_t *ov, uint32_t shift,
-             uint32_t nbits) const
+             uint32_t nbits, uint32_t *key_max = nullptr) const
     {
         hipStream_t s = ctx->stream;
         const uint32_t vec = (reinterpret_cast<uintptr_t>(ck) & 15u) == 0;
+        uint32_t *tmax = key_max ? scratch_as<uint32_t>(ctx, kScrTileMax, ntiles) : nullptr;
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, ck, n, shift,
-                           (1u << nbits) - 1u, vec, hist);
+                           (1u << nbits) - 1u, vec, hist, tmax);
+        if (tmax)
+            hipLaunchKernelGGL(k_max2, dim3(std::min<uint32_t>(ceil_div(ntiles, kBlock * 16), 4096)),
+                               dim3(kBlock), 0, s, tmax, nullptr, ntiles, 1u, key_max);
         hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
                            csum);
         hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum, nchunks);
@@ -550,12 +576,14 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
 
 void radix_sort_pairs_late(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t n,
                            uint32_t key_bits,
-                           const std::function<void(uint32_t **kout, uint32_t **vout)> &out)
+                           const std::function<void(uint32_t **kout, uint32_t **vout)> &out,
+                           uint32_t *key_max)
 {
     std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
     uint32_t *kout = nullptr, *vout = nullptr;
     const uint32_t passes = digit_passes(key_bits);
     if (n == 0 || passes <= 1) {
+        if (key_max && n) device_max_async(ctx, kin, nullptr, n, key_max);
         out(&kout, &vout);
         radix_sort_pairs(ctx, kin, vin, kout, vout, n, key_bits);
         return;
@@ -567,7 +595,7 @@ void radix_sort_pairs_late(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin
                        static_cast<uint32_t *>(scratch(ctx, kScrSort2V, (size_t)n * 4))};
     uint32_t nbits;
     uint32_t shift = digit_shift(key_bits, 0, &nbits);
-    sp.run(kin, vin, bk[0], bv[0], shift, nbits);
+    sp.run(kin, vin, bk[0], bv[0], shift, nbits, key_max);
     out(&kout, &vout); // the caller's allocation overlaps the first pass
     int cur = 0;       // pair holding the last pass's output
     for (uint32_t p = 1; p < passes; p++) {
@@ -593,15 +621,26 @@ void vertex_map(ag_ctx *ctx, const uint32_t *sorted_keys, uint32_t n, uint32_t N
     if (n) {
         if ((reinterpret_cast<uintptr_t>(sorted_keys) & 15u) == 0)
             hipLaunchKernelGGL(k_segment_bounds4, dim3(ceil_div(ceil_div(n, 4), 256)), dim3(256), 0,
-                               s, sorted_keys, n, edges_idx, degree);
+                               s, sorted_keys, n, N, edges_idx, degree);
         else
             hipLaunchKernelGGL(k_segment_bounds, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                               sorted_keys, n, edges_idx, degree);
+                               sorted_keys, n, N, edges_idx, degree);
         hipLaunchKernelGGL(k_segment_degree, dim3(ceil_div(N, 256)), dim3(256), 0, s, edges_idx,
                            degree, N);
         AG_HIP(hipGetLastError());
     }
     if (offsets) scan_exclusive_u32(ctx, degree, offsets, N);
+}
+
+void device_max_async(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n,
+                      uint32_t *out)
+{
+    if (n == 0) return;
+    const uint32_t blocks = std::min<uint32_t>(ceil_div(n, kBlock * 16), 4096);
+    const uint32_t vec = (reinterpret_cast<uintptr_t>(a) & 15u) == 0 &&
+                         (!b || (reinterpret_cast<uintptr_t>(b) & 15u) == 0);
+    hipLaunchKernelGGL(k_max2, dim3(blocks), dim3(kBlock), 0, ctx->stream, a, b, n, vec, out);
+    AG_HIP(hipGetLastError());
 }
 
 static uint32_t device_max_impl(ag_ctx *ctx, const uint32_t *a, const uint32_t *b, uint32_t n)

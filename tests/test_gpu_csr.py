@@ -4,6 +4,7 @@ Covers the configurations' shapes (C1 uniform N=65,536 E=1,048,576; R-MAT
 scale 22 E=67,108,864 at full size), every sort mode, the inverse built from
 input order, and the edge cases the builder meets: empty lists, one edge,
 self-loops, duplicates, isolated and trailing vertices, hub vertices."""
+import ctypes as C
 import hashlib
 import json
 import os
@@ -90,6 +91,30 @@ def test_vertex_out_of_range_rejected(ctx):
     with pytest.raises(ag.AGError) as e:
         ag.Graph.build(ctx, np.array([1, 9], np.uint32), np.array([0, 0], np.uint32), 5)
     assert e.value.code == ag._lib.AG_ERR_INVALID
+    # With num_vertices given, the ids are checked after the sort (the first
+    # radix pass of each direction takes the largest key): one id past N among
+    # 1 M edges, as a source or a destination, in every sort mode and with a
+    # forward-only build, is refused with the id named, and a good build
+    # follows.
+    n, m = 100000, 1 << 20
+    rng = np.random.RandomState(3)
+    for side in ("src", "dst"):
+        for sort in range(4):
+            src = rng.randint(0, n, m).astype(np.uint32)
+            dst = rng.randint(0, n, m).astype(np.uint32)
+            (src if side == "src" else dst)[m // 2 + sort] = n + 7
+            with pytest.raises(ag.AGError) as e:
+                ag.Graph.build(ctx, src, dst, n, sort=sort)
+            assert e.value.code == ag._lib.AG_ERR_INVALID
+            assert f"vertex id {n + 7} >= num_vertices {n}" in str(e.value)
+        lib = ag._lib.load()
+        h = C.c_void_p()
+        rc = lib.ag_csr_build(ctx.h, src.ctypes.data_as(C.c_void_p), dst.ctypes.data_as(C.c_void_p),
+                              m, n, ag._lib.AG_BUILD_NO_INVERSE, C.byref(h))
+        assert rc == ag._lib.AG_ERR_INVALID
+    src = rng.randint(0, n, m).astype(np.uint32)
+    dst = rng.randint(0, n, m).astype(np.uint32)
+    _cmp(ctx, src, dst, n)
 
 
 def test_rmat_s22_full_size_bit_exact(ctx):
