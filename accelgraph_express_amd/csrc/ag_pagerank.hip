@@ -1798,13 +1798,25 @@ __global__ __launch_bounds__(kSweepBlock) void k_tab_unit_cost(const uint4 *__re
 
 // Long-sub-row items (sub-row, first position, end position) of one
 // (table, class) range of the class-sorted sub-rows.
-__global__ void k_tab_litems(const uint32_t *__restrict__ sorted, uint32_t n,
-                             const uint32_t *__restrict__ sub_start, uint4 *__restrict__ out)
+// Long-sub-row items of every (table, class) segment in one launch: segment
+// g = (first position in `sorted`, count, first item) for g < nseg, counts
+// above zero and first items ascending, so item q belongs to the last
+// segment whose first item is <= q.
+__global__ void k_tab_litems(const uint32_t *__restrict__ sorted, const uint4 *__restrict__ seg,
+                             uint32_t nseg, uint32_t total, const uint32_t *__restrict__ sub_start,
+                             uint4 *__restrict__ out)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t sid = sorted[k];
-    out[k] = make_uint4(sid, sub_start[sid], sub_start[sid + 1], 0u);
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= total) return;
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (seg[mid].z <= q) lo = mid;
+        else hi = mid;
+    }
+    const uint4 g = seg[lo];
+    const uint32_t sid = sorted[g.x + (q - g.z)];
+    out[q] = make_uint4(sid, sub_start[sid], sub_start[sid + 1], 0u);
 }
 
 // out[i] = a[idx[i]], out[n + i] = b[idx[i]], out[2n + i] = c[idx[i]]
@@ -2234,6 +2246,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     for (uint32_t t = 0; t < NT; t++)
         for (uint32_t c = 1; c <= 3; c++) nlong += ccnt[t * 8 + c];
     tp->items.alloc((size_t)std::max<uint32_t>(nlong, 1) * sizeof(uint4));
+    std::vector<uint4> lsegs; // (first in sorted_sub, count, first item) per (table, class)
     uint32_t lpos = 0;
     for (uint32_t t = 0; t < NT; t++) {
         const uint32_t tt = t << 8;
@@ -2255,10 +2268,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
                                       kSweepWaves * kLongItems};
         for (uint32_t c = 3; c >= 1; c--) {
             const uint32_t n = ccnt[t * 8 + c], f = cfirst[t * 8 + c];
-            if (n)
-                hipLaunchKernelGGL(k_tab_litems, dim3(ceil_div(n, 256)), dim3(256), 0, s,
-                                   sorted_sub + f, n, tp->sub_start.as<uint32_t>(),
-                                   tp->items.as<uint4>() + lpos);
+            if (n) lsegs.push_back(make_uint4(f, n, lpos, 0u));
             for (uint32_t k = 0; k < n; k += per_unit[c])
                 units.push_back(make_uint4(tt | c, lpos + k, std::min<uint32_t>(per_unit[c], n - k),
                                            lpos + n));
@@ -2269,6 +2279,16 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
         for (uint32_t k = tp->SB[t] + H[t]; k < tp->SB[t + 1]; k += lrows)
             units.push_back(make_uint4(tt | kUnitLane, k,
                                        std::min<uint32_t>(lrows, tp->SB[t + 1] - k), tp->SB[t + 1]));
+    }
+    if (lpos != nlong) fail(AG_ERR_INVALID, "table layout: long-item count");
+    DevBuf dsegs(std::max<size_t>(lsegs.size(), 1) * sizeof(uint4));
+    if (nlong) {
+        AG_HIP(hipMemcpyAsync(dsegs.p, lsegs.data(), lsegs.size() * sizeof(uint4),
+                              hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_tab_litems, dim3(ceil_div(nlong, 256)), dim3(256), 0, s, sorted_sub,
+                           dsegs.as<uint4>(), (uint32_t)lsegs.size(), nlong,
+                           tp->sub_start.as<uint32_t>(), tp->items.as<uint4>());
+        AG_HIP(hipGetLastError());
     }
     tp->nunits = (uint32_t)units.size();
     // the cold table's lane units close the list: consecutive kSweepBlock
