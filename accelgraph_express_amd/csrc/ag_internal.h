@@ -115,7 +115,7 @@ struct ag_ctx {
     // grow on demand and are kept until the context is destroyed, so a build
     // does not pay hipMalloc / hipFree (which waits for the device) per step.
     // The mutex serialises the host threads that use it.
-    static constexpr int kScratchSlots = 25;
+    static constexpr int kScratchSlots = 26;
     std::recursive_mutex scratch_mu;
     ag::DevBuf scratch[kScratchSlots];
 };
@@ -130,6 +130,7 @@ enum ScratchSlot : int {
     kScrBlk = 21,  // first row of each 256-slot block of an edge-slot fill
     kScrSort2K = 22, kScrSort2V = 23, // second ping-pong pair of radix_sort_pairs_late
     kScrTileMax = 24,                 // per-tile key maxima of a sort's first pass
+    kScrLay8 = 25,                    // gather-layout build: rows of the table-sorted edges
 };
 // Device buffer of at least `bytes` in `slot`; growing waits for the stream
 // first, since the old buffer may still be read by queued work.  Sizes are
@@ -208,7 +209,11 @@ void scan_exclusive_u32(ag_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t
 // Inputs are not modified; outputs may not alias inputs.
 void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                       uint32_t *vout, uint32_t n, uint32_t key_bits);
-// The same, with the output arrays supplied by `out` (which may allocate
+// The same with a second value array moving with the first.
+void radix_sort_triples(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, const uint32_t *vin2,
+                        uint32_t *kout, uint32_t *vout, uint32_t *vout2, uint32_t n,
+                        uint32_t key_bits);
+// The same as radix_sort_pairs, with the output arrays supplied by `out` (which may allocate
 // them) after the first pass is queued, so that a host-side allocation runs
 // while the device sorts.  The first pass writes to scratch.
 void radix_sort_pairs_late(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uint32_t n,

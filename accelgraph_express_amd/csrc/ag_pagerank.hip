@@ -1595,22 +1595,13 @@ __global__ __launch_bounds__(256) void k_tab_hist(const uint32_t *__restrict__ e
     if (threadIdx.x < kTabHist && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// key = table of the edge's source (nt: cold), value = edge position.
+// Table of every label-order edge (the sort key of the table layout).
 __global__ void k_tab_keys(const uint32_t *__restrict__ ecol, uint32_t E, uint32_t T, uint32_t nt,
-                           uint32_t *__restrict__ key, uint32_t *__restrict__ val)
+                           uint32_t *__restrict__ key)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     key[e] = min(ecol[e] / T, nt);
-    val[e] = e;
-}
-
-// Row of each table-sorted edge (last r with roff[r] <= e, r < nrows).
-__global__ void k_tab_rows(const uint32_t *__restrict__ perm, const uint32_t *__restrict__ erow,
-                           uint32_t E, uint32_t *__restrict__ srow)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < E) srow[i] = erow[perm[i]];
 }
 
 // erow[e] = row of edge slot e < total (rows [0, nrows), each with an
@@ -1676,9 +1667,10 @@ __global__ void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v)
         p[i] = v;
 }
 
-// Source index of every table-sorted edge into its sub-row's slot.
-__global__ void k_tab_fill_idx(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ perm,
-                               const uint32_t *__restrict__ ecol, const uint32_t *__restrict__ fscan,
+// Source index of every table-sorted edge (scol: its source label) into its
+// sub-row's slot.
+__global__ void k_tab_fill_idx(const uint32_t *__restrict__ skey, const uint32_t *__restrict__ scol,
+                               const uint32_t *__restrict__ fscan,
                                const uint32_t *__restrict__ sub_e0,
                                const uint32_t *__restrict__ sub_start, uint32_t E, uint32_t T,
                                uint32_t nt, uint32_t cold_base4, uint16_t *__restrict__ idx16,
@@ -1688,7 +1680,7 @@ __global__ void k_tab_fill_idx(const uint32_t *__restrict__ skey, const uint32_t
     if (i >= E) return;
     const uint32_t sid = fscan[i + 1] - 1u;
     const uint32_t t = skey[i];
-    const uint32_t u = ecol[perm[i]];
+    const uint32_t u = scol[i];
     const uint64_t pos = (uint64_t)sub_start[sid] * 4u + (i - sub_e0[sid]);
     if (t < nt)
         idx16[pos] = (uint16_t)(u - t * T);
@@ -2037,10 +2029,11 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     };
     // Temporaries live in the context's scratch slots (kept across builds:
     // freeing a large buffer waits for the device).  Slot use by step:
-    //   InSrc: ecol (1-7);  PairA/PairB: key/val (3), srow/flag (4),
-    //   key2/val2 (8);  InDst: fscan (4-7), sk2 (8);  Lay0/Lay1: skey/perm
-    //   (3-7);  Lay2/Lay3: sub_row/sub_tab (4-12);  Lay4: sub_e0 (4-7),
-    //   sorted_sub (8-10);  Lay5: len4 (5-9);  Lay6 is build_plan's.
+    //   InSrc: ecol (1-3);  PairA/PairB: key/erow (3), -/flag (4),
+    //   key2/val2 (8);  InDst: fscan (4-7), sk2 (8);  Lay0/Lay1: skey/scol
+    //   (3-7);  Lay8: srow (3-4);  Lay2/Lay3: sub_row/sub_tab (4-12);  Lay4:
+    //   sub_e0 (4-7), sorted_sub (8-10);  Lay5: len4 (5-9);  Lay6 is
+    //   build_plan's.
     std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
     // 1. relabelled sources of every row with in-edges, label order: the row
     // layout's CSR part holds those of rows [0, n_nonlow) already
@@ -2077,32 +2070,30 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
         tp->NT = nt + 1;
     }
     const uint32_t nt = tp->nt, NT = tp->NT;
-    // 3. stable sort of the edges by table: each table's edges stay in (row, edge) order
+    // 3. stable sort of the edges by table, each edge's row and source label
+    // moving with it: each table's edges stay in (row, edge) order
     uint32_t *skey = scratch_as<uint32_t>(ctx, kScrLay0, E);
-    uint32_t *perm = scratch_as<uint32_t>(ctx, kScrLay0 + 1, E);
+    uint32_t *scol = scratch_as<uint32_t>(ctx, kScrLay0 + 1, E);
+    uint32_t *srow = scratch_as<uint32_t>(ctx, kScrLay8, E);
     {
         uint32_t *key = scratch_as<uint32_t>(ctx, kScrPairA, E);
-        uint32_t *val = scratch_as<uint32_t>(ctx, kScrPairB, E);
-        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol, E, T, nt, key,
-                           val);
+        uint32_t *erow = scratch_as<uint32_t>(ctx, kScrPairB, E);
+        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol, E, T, nt, key);
+        // row of every label-order edge slot
+        uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(E, kFillBlock));
+        hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(nrows, 256)), dim3(256), 0, s,
+                           p->roff.as<uint32_t>(), nrows, 0u, E, brow);
+        hipLaunchKernelGGL(k_edge_rows, dim3(ceil_div(E, kFillBlock)), dim3(kFillBlock), 0, s,
+                           p->roff.as<uint32_t>(), brow, nrows, E, erow);
         AG_HIP(hipGetLastError());
-        radix_sort_pairs(ctx, key, val, skey, perm, E, bits_for(NT));
+        radix_sort_triples(ctx, key, erow, ecol, skey, srow, scol, E, bits_for(NT));
     }
     // 4. sub-rows: starts where (table, row) changes
     uint32_t *fscan = scratch_as<uint32_t>(ctx, kScrInDst, (size_t)E + 1);
     uint32_t S = 0;
     uint32_t *sub_row, *sub_tab, *sub_e0;
     {
-        uint32_t *srow = scratch_as<uint32_t>(ctx, kScrPairA, E);
         uint32_t *flag = scratch_as<uint32_t>(ctx, kScrPairB, E);
-        // row of every label-order edge slot (in flag's slot), then of
-        // every table-sorted edge (perm ascends within a table)
-        uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(E, kFillBlock));
-        hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(nrows, 256)), dim3(256), 0, s,
-                           p->roff.as<uint32_t>(), nrows, 0u, E, brow);
-        hipLaunchKernelGGL(k_edge_rows, dim3(ceil_div(E, kFillBlock)), dim3(kFillBlock), 0, s,
-                           p->roff.as<uint32_t>(), brow, nrows, E, flag);
-        hipLaunchKernelGGL(k_tab_rows, dim3(ceil_div(E, 256)), dim3(256), 0, s, perm, flag, E, srow);
         hipLaunchKernelGGL(k_tab_flags, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, srow, E, flag);
         AG_HIP(hipGetLastError());
         scan_exclusive_u32(ctx, flag, fscan, E);
@@ -2156,7 +2147,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
                        (uint64_t)tp->idx16.bytes / 2, (uint16_t)T);
     hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, tp->idx32.as<uint32_t>(),
                        (uint64_t)tp->idx32.bytes / 4, N);
-    hipLaunchKernelGGL(k_tab_fill_idx, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, perm, ecol,
+    hipLaunchKernelGGL(k_tab_fill_idx, dim3(ceil_div(E, 256)), dim3(256), 0, s, skey, scol,
                        fscan, sub_e0, tp->sub_start.as<uint32_t>(), E, T, nt, cold_base4,
                        tp->idx16.as<uint16_t>(), tp->idx32.as<uint32_t>());
     AG_HIP(hipGetLastError());

@@ -280,16 +280,18 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
-template <uint32_t NB>
+// With V2 a second value array moves with the first (vin2 -> vout2).
+template <uint32_t NB, bool V2>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
-    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
-    const uint32_t *__restrict__ hist)
+    const uint32_t *__restrict__ vin2, uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+    uint32_t *__restrict__ vout2, uint32_t n, uint32_t shift, const uint32_t *__restrict__ hist)
 {
     __shared__ uint32_t wcnt[kWaves][kRadix]; // per-wave digit counts, then in-tile positions
     __shared__ uint32_t gbase[kRadix];        // global position of the tile's first key of digit d
     __shared__ uint32_t wsum[kWaves];
     __shared__ uint32_t sk[kTile], sv[kTile];
+    __shared__ uint32_t sv2[V2 ? kTile : 1];
     constexpr uint32_t mask = (1u << NB) - 1u;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -304,13 +306,15 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     // the wave's keys from a wave-uniform base: lane offsets stay 32-bit
     const uint32_t *kw = kin + base;
     const uint32_t *vw = vin + base;
-    uint32_t k[kItems], v[kItems], r[kItems];
+    const uint32_t *vw2 = V2 ? vin2 + base : nullptr;
+    uint32_t k[kItems], v[kItems], r[kItems], v2[V2 ? kItems : 1];
     const bool full = tile0 + kTile <= n;
     if (full) {
 #pragma unroll
         for (int i = 0; i < kItems; i++) {
             k[i] = kw[i * 64 + lane];
             v[i] = vw[i * 64 + lane];
+            if constexpr (V2) v2[i] = vw2[i * 64 + lane];
         }
     } else {
 #pragma unroll
@@ -318,6 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
             const bool valid = base + (uint64_t)i * 64 + lane < n;
             k[i] = valid ? kw[i * 64 + lane] : 0u;
             v[i] = valid ? vw[i * 64 + lane] : 0u;
+            if constexpr (V2) v2[i] = valid ? vw2[i * 64 + lane] : 0u;
         }
     }
 #pragma unroll
@@ -377,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
             const uint32_t lp = wcnt[w][(k[i] >> shift) & mask] + r[i];
             sk[lp] = k[i];
             sv[lp] = v[i];
+            if constexpr (V2) sv2[lp] = v2[i];
         }
     }
     __syncthreads();
@@ -390,6 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
             const uint32_t pos = gbase[(key >> shift) & mask] + j;
             kout[pos] = key;
             vout[pos] = sv[j];
+            if constexpr (V2) vout2[pos] = sv2[j];
         }
     }
 }
@@ -539,7 +546,8 @@ struct SortPass {
     // one stable pass: (ck, cv) -> (ok, ov) by the nbits-bit digit at shift;
     // with key_max, also atomicMax(*key_max, largest key)
     void run(const uint32_t *ck, const uint32_t *cv, uint32_t *ok, uint32_t *ov, uint32_t shift,
-             uint32_t nbits, uint32_t *key_max = nullptr) const
+             uint32_t nbits, uint32_t *key_max = nullptr, const uint32_t *cv2 = nullptr,
+             uint32_t *ov2 = nullptr) const
     {
         hipStream_t s = ctx->stream;
         const uint32_t vec = (reinterpret_cast<uintptr_t>(ck) & 15u) == 0;
@@ -557,8 +565,12 @@ struct SortPass {
         switch (nbits) {
 #define AG_SCATTER(B)                                                                              \
     case B:                                                                                        \
-        hipLaunchKernelGGL(k_radix_scatter<B>, dim3(ntiles), dim3(kBlock), 0, s, ck, cv, ok, ov, n, \
-                           shift, hist);                                                           \
+        if (cv2)                                                                                   \
+            hipLaunchKernelGGL((k_radix_scatter<B, true>), dim3(ntiles), dim3(kBlock), 0, s, ck,   \
+                               cv, cv2, ok, ov, ov2, n, shift, hist);                              \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_radix_scatter<B, false>), dim3(ntiles), dim3(kBlock), 0, s, ck,  \
+                               cv, nullptr, ok, ov, nullptr, n, shift, hist);                      \
         break;
             AG_SCATTER(1) AG_SCATTER(2) AG_SCATTER(3) AG_SCATTER(4)
             AG_SCATTER(5) AG_SCATTER(6) AG_SCATTER(7) AG_SCATTER(8)
@@ -598,6 +610,42 @@ void radix_sort_pairs(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, uin
         sp.run(ck, cv, ok, ov, shift, nbits);
         ck = ok;
         cv = ov;
+    }
+}
+
+void radix_sort_triples(ag_ctx *ctx, const uint32_t *kin, const uint32_t *vin, const uint32_t *vin2,
+                        uint32_t *kout, uint32_t *vout, uint32_t *vout2, uint32_t n,
+                        uint32_t key_bits)
+{
+    hipStream_t s = ctx->stream;
+    if (n == 0) return;
+    std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
+    const uint32_t passes = digit_passes(key_bits);
+    if (passes == 0) {
+        AG_HIP(hipMemcpyAsync(kout, kin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        AG_HIP(hipMemcpyAsync(vout, vin, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        AG_HIP(hipMemcpyAsync(vout2, vin2, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+        return;
+    }
+    const SortPass sp(ctx, n);
+    uint32_t *tk = nullptr, *tv = nullptr, *tv2 = nullptr;
+    if (passes > 1) {
+        tk = static_cast<uint32_t *>(scratch(ctx, kScrSortK, (size_t)n * 4));
+        tv = static_cast<uint32_t *>(scratch(ctx, kScrSortV, (size_t)n * 4));
+        tv2 = static_cast<uint32_t *>(scratch(ctx, kScrSort2V, (size_t)n * 4));
+    }
+    const uint32_t *ck = kin, *cv = vin, *cv2 = vin2;
+    for (uint32_t p = 0; p < passes; p++) {
+        uint32_t nbits;
+        const uint32_t shift = digit_shift(key_bits, p, &nbits);
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint32_t *ok = to_out ? kout : tk;
+        uint32_t *ov = to_out ? vout : tv;
+        uint32_t *ov2 = to_out ? vout2 : tv2;
+        sp.run(ck, cv, ok, ov, shift, nbits, nullptr, cv2, ov2);
+        ck = ok;
+        cv = ov;
+        cv2 = ov2;
     }
 }
 
