@@ -1551,12 +1551,15 @@ __global__ __launch_bounds__(kFillBlock) void k_fill_csr_edges(
 }
 
 // SELL rows: one thread per row, four edges' loads in flight at a time.
+// With ecol, the row's sources also go to ecol[roff[row] ..] in row order
+// (the partitioned-table build reads them from there).
 __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint32_t *__restrict__ rdeg,
                                  const uint32_t *__restrict__ inv_idx,
                                  const uint32_t *__restrict__ inv_dst,
                                  const uint32_t *__restrict__ rank,
                                  const uint32_t *__restrict__ slice_ptr, uint32_t low_begin,
-                                 uint32_t nlow, uint32_t *__restrict__ cols)
+                                 uint32_t nlow, uint32_t *__restrict__ cols,
+                                 const uint32_t *__restrict__ roff, uint32_t *__restrict__ ecol)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nlow) return;
@@ -1565,6 +1568,7 @@ __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint3
     const uint32_t n = rdeg[row];
     const uint32_t s = inv_idx[v];
     uint32_t *p = cols + slice_ptr[t / 64] + (t % 64);
+    uint32_t *q = ecol ? ecol + roff[row] : nullptr;
     uint32_t k = 0;
     for (; k + 4 <= n; k += 4) {
         uint32_t d[4], r[4];
@@ -1574,8 +1578,16 @@ __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint3
         for (int j = 0; j < 4; j++) r[j] = rank[d[j]];
 #pragma unroll
         for (int j = 0; j < 4; j++) p[(k + j) * 64] = r[j];
+        if (q) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) q[k + j] = r[j];
+        }
     }
-    for (; k < n; k++) p[k * 64] = rank[inv_dst[s + k]];
+    for (; k < n; k++) {
+        const uint32_t r = rank[inv_dst[s + k]];
+        p[k * 64] = r;
+        if (q) q[k] = r;
+    }
 }
 
 // ---- partitioned-table layout construction ----
@@ -1833,7 +1845,7 @@ __global__ void k_gather1(const uint32_t *__restrict__ a, const uint32_t *__rest
 
 } // namespace
 
-static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn);
+static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn, bool sell_ecol);
 
 static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
 {
@@ -1953,12 +1965,17 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
                            p->rank.as<uint32_t>(), p->n_nonlow, 0u, csr_edges,
                            p->cols.as<uint32_t>());
     }
+    // partitioned tables when forced, or (auto) when one LDS table cannot
+    // hold every source; their build reads the SELL rows' sources in row
+    // order, written here beside the SELL slices (scratch slot InSrc)
+    const bool tables = tn.layout == AG_LAYOUT_TABLES || (tn.layout == AG_LAYOUT_AUTO && N > kTabEntries);
+    uint32_t *ecol_sell = tables && nlow ? scratch_as<uint32_t>(ctx, kScrInSrc, E) : nullptr;
     if (nlow)
         hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
                            p->order.as<uint32_t>(), rdeg,
                            g->inv_idx.as<uint32_t>(), g->inv_dst.as<uint32_t>(),
                            p->rank.as<uint32_t>(), p->slice_ptr.as<uint32_t>(), p->n_nonlow, nlow,
-                           p->cols.as<uint32_t>());
+                           p->cols.as<uint32_t>(), p->roff.as<uint32_t>(), ecol_sell);
     AG_HIP(hipGetLastError());
 
     // Hub chunk table (host: hub rows are few).
@@ -2000,10 +2017,7 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     p->nb_zero = ceil_div(N - p->n_nonzero, kZeroRowsPerUnit);
     p->nvblocks = p->nb_hub + p->nb_w + p->nb_h + p->nb_q + p->nb_low + p->nb_zero;
     AG_HIP(hipStreamSynchronize(s));
-    // partitioned tables when forced, or (auto) when one LDS table cannot
-    // hold every source
-    if (tn.layout == AG_LAYOUT_TABLES || (tn.layout == AG_LAYOUT_AUTO && N > kTabEntries))
-        build_tables(ctx, g, p.get(), tn);
+    if (tables) build_tables(ctx, g, p.get(), tn, ecol_sell != nullptr);
     return p.release();
 }
 
@@ -2015,7 +2029,7 @@ static constexpr uint32_t kMinTabEdges = 65536; // an LDS table needs this many 
 // dependent memory round trips whatever its length.
 static constexpr uint32_t kUnitOverhead = 4096;
 
-static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
+static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn, bool sell_ecol)
 {
     hipStream_t s = ctx->stream;
     const uint32_t N = p->N, E = p->E, nrows = p->n_nonzero, n_hub = p->n_hub;
@@ -2036,12 +2050,13 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn)
     //   build_plan's.
     std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
     // 1. relabelled sources of every row with in-edges, label order: the row
-    // layout's CSR part holds those of rows [0, n_nonlow) already
+    // layout's CSR part holds those of rows [0, n_nonlow) already, and with
+    // sell_ecol the SELL fill has written the rest here
     uint32_t *ecol = scratch_as<uint32_t>(ctx, kScrInSrc, E);
     const uint32_t e_csr = (uint32_t)std::min<uint64_t>(p->csr_edges, E);
     if (e_csr)
         AG_HIP(hipMemcpyAsync(ecol, p->cols.p, (size_t)e_csr * 4, hipMemcpyDeviceToDevice, s));
-    if (e_csr < E) {
+    if (e_csr < E && !sell_ecol) {
         uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(E - e_csr, kFillBlock));
         hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(nrows, 256)), dim3(256), 0, s,
                            p->roff.as<uint32_t>(), nrows, e_csr, E, brow);
