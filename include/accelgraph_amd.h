@@ -63,7 +63,9 @@ typedef struct ag_graph ag_graph;
 #define AG_BUILD_DEVICE_INPUT       0x400u /* src/dst are device pointers */
 
 /* Builds forward + inverse CSR on the GPU from an edge list (host pointers
- * unless AG_BUILD_DEVICE_INPUT).  num_vertices 0 means max id + 1. */
+ * unless AG_BUILD_DEVICE_INPUT).  num_vertices 0 means max id + 1.  With
+ * num_vertices given, an id >= num_vertices is found by the sort's first
+ * pass and reported (AG_ERR_INVALID, no graph) when the build ends. */
 int ag_csr_build(ag_ctx *ctx, const uint32_t *src, const uint32_t *dst, uint32_t num_edges,
                  uint32_t num_vertices, uint32_t flags, ag_graph **out);
 
