@@ -264,9 +264,10 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_apply(uint32_t *__restric
 // once, then walks them in 16 steps of 64.  In each step the lanes holding
 // equal digits are found with NB ballots; the rank of a lane is the wave's
 // running count for its digit plus the number of equal-digit lanes below it.
-// The lowest lane of each equal-digit group then advances the running count.  The tile is then reordered by digit in LDS (wave 0 < 1 < 2 <
-// 3 inside a digit, i.e. input order) and written out run by run, so that keys
-// of one digit leave as consecutive words instead of one scattered word each.
+// The lowest lane of each equal-digit group then advances the running count.
+// The tile is then reordered by digit in LDS (wave 0 < 1 < 2 < 3 inside a
+// digit, i.e. input order) and written out run by run, so that keys of one
+// digit leave as consecutive words instead of one scattered word each.
 // Workgroups are dispatched round-robin over the 8 XCDs.  Giving each XCD a
 // contiguous range of tiles puts the neighbouring runs of one digit (tile t's
 // and tile t+1's) into the same L2, where they merge into whole lines before
