@@ -604,6 +604,7 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
             f"single {t_1 * 1e3:.1f} ms")
         # the same with Aitken-type extrapolated steps, batched and single
         t_bx, st_bx, ms_bx = batched("bf16", t, ex=True)
+        t_3x, st_3x, ms_3x = batched("fp32", t, ex=True)
         t_1x, st_1x = single(t, ex=True)
         lat = [s.latency_ms for s in st_bf]
         lat_x = [s.latency_ms for s in st_bx]
@@ -616,6 +617,7 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
                           "fp32_wall": round(t_32 * 1e3, 2), "fp32_in_library": round(ms_32, 2)},
              "latency_ms_p50_bf16": round(_pct(lat, 50), 3),
              "latency_ms_p99_bf16": round(_pct(lat, 99), 3),
+             "latency_ms_p99_fp32": round(_pct([s.latency_ms for s in st_32], 99), 3),
              "iterations_mean": {"bf16": round(float(np.mean([s.iterations for s in st_bf])), 2),
                                  "fp32": round(float(np.mean([s.iterations for s in st_32])), 2),
                                  "single": round(float(np.mean([s.iterations for s in st_1])), 2)},
@@ -625,20 +627,28 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
              "fp64_error_bound": {"bf16": bounds("bf16", t), "fp32": bounds("fp32", t),
                                   "single": bounds("single", t)},
              "extrapolated": {
-                 "qps_batched_bf16": round(nq / t_bx, 2), "qps_single_fp32": round(nq / t_1x, 2),
+                 "qps_batched_bf16": round(nq / t_bx, 2), "qps_batched_fp32": round(nq / t_3x, 2),
+                 "qps_single_fp32": round(nq / t_1x, 2),
                  "speedup_vs_single_extrapolated": round(t_1x / t_bx, 2),
                  "speedup_vs_single": round(t_1 / t_bx, 2),
                  "batch_ms": {"bf16_wall": round(t_bx * 1e3, 2),
-                              "bf16_in_library": round(ms_bx, 2)},
+                              "bf16_in_library": round(ms_bx, 2),
+                              "fp32_wall": round(t_3x * 1e3, 2),
+                              "fp32_in_library": round(ms_3x, 2)},
                  "roofline_bf16": _ppr_roofline(st_bx, N, E, "bf16", ms_bx),
+                 "roofline_fp32": _ppr_roofline(st_3x, N, E, "fp32", ms_3x),
                  "latency_ms_p50_bf16": round(_pct(lat_x, 50), 3),
                  "latency_ms_p99_bf16": round(_pct(lat_x, 99), 3),
+                 "latency_ms_p99_fp32": round(_pct([s.latency_ms for s in st_3x], 99), 3),
                  "iterations_mean": {
                      "bf16": round(float(np.mean([s.iterations for s in st_bx])), 2),
+                     "fp32": round(float(np.mean([s.iterations for s in st_3x])), 2),
                      "single": round(float(np.mean([s.iterations for s in st_1x])), 2)},
                  "iterations_max": {"bf16": int(max(s.iterations for s in st_bx)),
+                                    "fp32": int(max(s.iterations for s in st_3x)),
                                     "single": int(max(s.iterations for s in st_1x))},
                  "fp64_error_bound": {"bf16": bounds("bf16", t, True),
+                                      "fp32": bounds("fp32", t, True),
                                       "single": bounds("single", t, True)}}}
         res[key] = r
         x = r["extrapolated"]
@@ -646,7 +656,8 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
             f"(p99 {r['latency_ms_p99_bf16']} ms), batched FP32 {r['qps_batched_fp32']} q/s, "
             f"single FP32 {r['qps_single_fp32']} q/s; FP64 error bounds {r['fp64_error_bound']}")
         log(f"  extrapolated: batched BF16 {x['qps_batched_bf16']} q/s (p99 "
-            f"{x['latency_ms_p99_bf16']} ms), single FP32 {x['qps_single_fp32']} q/s, "
+            f"{x['latency_ms_p99_bf16']} ms), batched FP32 {x['qps_batched_fp32']} q/s, "
+            f"single FP32 {x['qps_single_fp32']} q/s, "
             f"sweeps mean {x['iterations_mean']} max {x['iterations_max']}; "
             f"FP64 error bounds {x['fp64_error_bound']}")
     # headline: the loosest tolerance at which the single path and a batched
@@ -660,10 +671,16 @@ def ppr_leg(ag, g, N, np, src, dst, nq=64, seed=4):
         if r["fp64_error_bound"]["bf16"]["max"] <= 1e-4:
             cands.append(("bf16", r["qps_batched_bf16"], r["latency_ms_p99_bf16"],
                           r["fp64_error_bound"]["bf16"]["max"], r["roofline_bf16"]))
-        if r["extrapolated"]["fp64_error_bound"]["bf16"]["max"] <= 1e-4:
-            x = r["extrapolated"]
+        if r["fp64_error_bound"]["fp32"]["max"] <= 1e-4:
+            cands.append(("fp32", r["qps_batched_fp32"], r["latency_ms_p99_fp32"],
+                          r["fp64_error_bound"]["fp32"]["max"], r["roofline_fp32"]))
+        x = r["extrapolated"]
+        if x["fp64_error_bound"]["bf16"]["max"] <= 1e-4:
             cands.append(("bf16 extrapolated", x["qps_batched_bf16"], x["latency_ms_p99_bf16"],
                           x["fp64_error_bound"]["bf16"]["max"], x["roofline_bf16"]))
+        if x["fp64_error_bound"]["fp32"]["max"] <= 1e-4:
+            cands.append(("fp32 extrapolated", x["qps_batched_fp32"], x["latency_ms_p99_fp32"],
+                          x["fp64_error_bound"]["fp32"]["max"], x["roofline_fp32"]))
         if not cands:
             continue
         best = max(cands, key=lambda c: c[1])
