@@ -203,42 +203,69 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_sums(const uint32_t *__re
     csum[(uint64_t)blockIdx.x * kRadix + d] = s;
 }
 
-// Phase 2 (one block): exclusive prefix over chunks per digit, then over digits.
-__global__ __launch_bounds__(kRadix) void k_hist_chunk_scan(uint32_t *__restrict__ csum,
-                                                            uint32_t nchunks)
+// Phase 2 (one block of kScanThreads): exclusive prefix over chunks per
+// digit, then over digits.  Thread (part, d) sums chunks [part*64, +64) of
+// digit d (32 loads in flight at a time), the four parts' sums and the digit
+// totals are combined in LDS, and the thread reads its chunks again to write
+// their prefixes.
+constexpr int kScanParts = 4, kScanThreads = kScanParts * kRadix, kScanChunks = 64;
+static_assert(kScanParts * kScanChunks >= 256, "the chunk scan covers 256 chunks");
+__global__ __launch_bounds__(kScanThreads) void k_hist_chunk_scan(uint32_t *__restrict__ csum,
+                                                                  uint32_t nchunks)
 {
-    __shared__ uint32_t tot[kRadix];
+    __shared__ uint32_t psum[kScanParts][kRadix];
     __shared__ uint32_t ws[kRadix / 64];
-    const uint32_t d = threadIdx.x;
-    uint32_t run = 0;
-    // eight independent loads in flight per step (the loop is latency-bound)
-    uint32_t c = 0;
-    for (; c + 8 <= nchunks; c += 8) {
-        uint32_t x[8];
+    const uint32_t d = threadIdx.x % kRadix, part = threadIdx.x / kRadix;
+    const uint32_t c0 = part * kScanChunks;
+    constexpr int kHalf = kScanChunks / 2; // loads in flight per step (register budget)
+    uint32_t tot = 0;
+#pragma unroll 1
+    for (int h = 0; h < 2; h++) {
+        uint32_t x[kHalf];
+        const uint32_t cb = c0 + h * kHalf;
 #pragma unroll
-        for (int i = 0; i < 8; i++) x[i] = csum[(uint64_t)(c + i) * kRadix + d];
+        for (int i = 0; i < kHalf; i++) x[i] = cb + i < nchunks ? csum[(cb + i) * kRadix + d] : 0u;
 #pragma unroll
-        for (int i = 0; i < 8; i++) {
-            csum[(uint64_t)(c + i) * kRadix + d] = run;
+        for (int i = 0; i < kHalf; i++) tot += x[i];
+    }
+    psum[part][d] = tot;
+    __syncthreads();
+    // digit totals and their exclusive scan over digits (the first 256 threads)
+    const int lane = d & 63, w = d >> 6;
+    uint32_t dtot = 0, incl = 0;
+    if (part == 0) {
+#pragma unroll
+        for (int p = 0; p < kScanParts; p++) dtot += psum[p][d];
+        incl = wave_incl_scan(dtot, lane);
+        if (lane == 63) ws[w] = incl;
+    }
+    __syncthreads();
+    if (part == 0) {
+        uint32_t wpref = 0;
+        for (int i = 0; i < w; i++) wpref += ws[i];
+        // this digit's first output position, then each part's start
+        uint32_t start = wpref + incl - dtot;
+#pragma unroll
+        for (int p = 0; p < kScanParts; p++) {
+            const uint32_t t = psum[p][d];
+            psum[p][d] = start;
+            start += t;
+        }
+    }
+    __syncthreads();
+    uint32_t run = psum[part][d];
+#pragma unroll 1
+    for (int h = 0; h < 2; h++) {
+        uint32_t x[kHalf];
+        const uint32_t cb = c0 + h * kHalf;
+#pragma unroll
+        for (int i = 0; i < kHalf; i++) x[i] = cb + i < nchunks ? csum[(cb + i) * kRadix + d] : 0u;
+#pragma unroll
+        for (int i = 0; i < kHalf; i++) {
+            if (cb + i < nchunks) csum[(cb + i) * kRadix + d] = run;
             run += x[i];
         }
     }
-    for (; c < nchunks; c++) {
-        uint32_t x = csum[(uint64_t)c * kRadix + d];
-        csum[(uint64_t)c * kRadix + d] = run;
-        run += x;
-    }
-    tot[d] = run;
-    __syncthreads();
-    const int lane = d & 63, w = d >> 6;
-    uint32_t incl = wave_incl_scan(run, lane);
-    if (lane == 63) ws[w] = incl;
-    __syncthreads();
-    uint32_t wpref = 0;
-    for (int i = 0; i < w; i++) wpref += ws[i];
-    const uint32_t digit_base = wpref + incl - run;
-#pragma unroll 8
-    for (uint32_t k = 0; k < nchunks; k++) csum[(uint64_t)k * kRadix + d] += digit_base;
 }
 
 // Phase 3: rewrite each tile histogram entry as its global exclusive offset.
@@ -560,7 +587,7 @@ struct SortPass {
                                dim3(kBlock), 0, s, tmax, nullptr, ntiles, 1u, key_max);
         hipLaunchKernelGGL(k_hist_chunk_sums, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
                            csum);
-        hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kRadix), 0, s, csum, nchunks);
+        hipLaunchKernelGGL(k_hist_chunk_scan, dim3(1), dim3(kScanThreads), 0, s, csum, nchunks);
         hipLaunchKernelGGL(k_hist_chunk_apply, dim3(nchunks), dim3(kRadix), 0, s, hist, ntiles, tpc,
                            csum);
         switch (nbits) {
