@@ -277,11 +277,18 @@ __global__ __launch_bounds__(kRadix) void k_hist_chunk_apply(uint32_t *__restric
     const uint32_t t0 = blockIdx.x * tpc;
     const uint32_t t1 = min(ntiles, t0 + tpc);
     uint32_t run = csum[(uint64_t)blockIdx.x * kRadix + d];
-    for (uint32_t t = t0; t < t1; t++) {
-        uint64_t i = (uint64_t)t * kRadix + d;
-        uint32_t x = hist[i];
-        hist[i] = run;
-        run += x;
+    // 32 tiles' counts in flight, then their offsets written
+    constexpr uint32_t kStep = 32;
+    for (uint32_t tb = t0; tb < t1; tb += kStep) {
+        uint32_t x[kStep];
+#pragma unroll
+        for (uint32_t k = 0; k < kStep; k++)
+            x[k] = tb + k < t1 ? hist[(uint64_t)(tb + k) * kRadix + d] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kStep; k++) {
+            if (tb + k < t1) hist[(uint64_t)(tb + k) * kRadix + d] = run;
+            run += x[k];
+        }
     }
 }
 
