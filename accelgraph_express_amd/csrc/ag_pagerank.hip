@@ -1592,28 +1592,46 @@ __global__ void k_fill_sell_rows(const uint32_t *__restrict__ order, const uint3
 
 // ---- partitioned-table layout construction ----
 
-// Edges per table (sources u / T), for tables 0 .. kTabHist-2; the last bin
-// collects the rest.
+// Edges per table: a table holds the edges whose source label l has
+// l / kTabEntries == t, so its count is the sum of those labels' out-degrees,
+// for the candidate LDS tables 0 .. kTabHist-2 (hist[kTabHist-1], the rest,
+// is never read and stays as the caller cleared it).  One workgroup per
+// table, 8 loads in flight per thread, a plain store each.
 constexpr int kTabHist = 64;
-__global__ __launch_bounds__(256) void k_tab_hist(const uint32_t *__restrict__ ecol, uint32_t E,
-                                                  uint32_t T, uint32_t *__restrict__ hist)
+__global__ __launch_bounds__(1024) void k_tab_hist(const uint32_t *__restrict__ outdeg, uint32_t N,
+                                                   uint32_t *__restrict__ hist)
 {
-    __shared__ uint32_t h[kTabHist];
-    if (threadIdx.x < kTabHist) h[threadIdx.x] = 0;
+    __shared__ uint32_t ws[16];
+    const uint32_t t = blockIdx.x;
+    if (t + 1 >= (uint32_t)kTabHist) return;
+    const uint32_t l0 = (uint32_t)min<uint64_t>(N, (uint64_t)t * kTabEntries);
+    const uint32_t l1 = (uint32_t)min<uint64_t>(N, (uint64_t)l0 + kTabEntries);
+    uint32_t sum = 0;
+    for (uint32_t l = l0 + threadIdx.x; l < l1; l += 8 * 1024) {
+        uint32_t v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = l + k * 1024u < l1 ? outdeg[l + k * 1024u] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) sum += v[k];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = sum;
     __syncthreads();
-    for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < E; e += gridDim.x * 256)
-        atomicAdd(&h[min(ecol[e] / T, (uint32_t)kTabHist - 1)], 1u);
-    __syncthreads();
-    if (threadIdx.x < kTabHist && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < 16; i++) tot += ws[i];
+        hist[t] = tot;
+    }
 }
 
 // Table of every label-order edge (the sort key of the table layout).
-__global__ void k_tab_keys(const uint32_t *__restrict__ ecol, uint32_t E, uint32_t T, uint32_t nt,
+__global__ void k_tab_keys(const uint32_t *__restrict__ ecol, uint32_t E, uint32_t nt,
                            uint32_t *__restrict__ key)
 {
     const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
-    key[e] = min(ecol[e] / T, nt);
+    key[e] = min(ecol[e] / kTabEntries, nt);
 }
 
 // erow[e] = row of edge slot e < total (rows [0, nrows), each with an
@@ -2070,8 +2088,8 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn, bo
     {
         DevBuf hist(kTabHist * 4);
         AG_HIP(hipMemsetAsync(hist.p, 0, kTabHist * 4, s));
-        hipLaunchKernelGGL(k_tab_hist, dim3(std::min<uint32_t>(ceil_div(E, 256), ctx->num_cus * 8)),
-                           dim3(256), 0, s, ecol, E, T, hist.as<uint32_t>());
+        hipLaunchKernelGGL(k_tab_hist, dim3(kTabHist - 1), dim3(1024), 0, s, p->outdeg.as<uint32_t>(), N,
+                           hist.as<uint32_t>());
         AG_HIP(hipGetLastError());
         uint32_t h[kTabHist];
         dl(h, hist.p, sizeof(h));
@@ -2093,7 +2111,7 @@ static void build_tables(ag_ctx *ctx, ag_graph *g, Plan *p, const Tuning &tn, bo
     {
         uint32_t *key = scratch_as<uint32_t>(ctx, kScrPairA, E);
         uint32_t *erow = scratch_as<uint32_t>(ctx, kScrPairB, E);
-        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol, E, T, nt, key);
+        hipLaunchKernelGGL(k_tab_keys, dim3(ceil_div(E, 256)), dim3(256), 0, s, ecol, E, nt, key);
         // row of every label-order edge slot
         uint32_t *brow = scratch_as<uint32_t>(ctx, kScrBlk, ceil_div(E, kFillBlock));
         hipLaunchKernelGGL(k_block_rows, dim3(ceil_div(nrows, 256)), dim3(256), 0, s,
