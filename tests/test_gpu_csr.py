@@ -58,6 +58,12 @@ def test_edge_cases(ctx):
     assert _cmp(ctx, np.array([3], np.uint32), np.array([3], np.uint32)) == 4  # self loop
     # duplicates + isolated vertices + a trailing isolated vertex id
     _cmp(ctx, np.array([2, 2, 2, 0, 2], np.uint32), np.array([1, 1, 1, 1, 5], np.uint32), 9)
+    # one vertex: 0-bit keys (the sort is a copy), ids checked after it
+    for sort in range(4):
+        assert _cmp(ctx, np.zeros(5000, np.uint32), np.zeros(5000, np.uint32), 1, sort) == 1
+    with pytest.raises(ag.AGError) as e:
+        ag.Graph.build(ctx, np.zeros(3, np.uint32), np.array([0, 1, 0], np.uint32), 1)
+    assert e.value.code == ag._lib.AG_ERR_INVALID
     # one hub with every edge (one segment spanning many tiles)
     m = 200000
     _cmp(ctx, np.full(m, 7, np.uint32), np.arange(m, dtype=np.uint32) % 1000, 1000)
