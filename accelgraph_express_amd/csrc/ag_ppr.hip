@@ -1697,17 +1697,21 @@ uint32_t ppr_autotune(ag_ctx *ctx, ag_graph *g, uint32_t fmt, uint32_t capacity,
         const uint32_t seed = (uint32_t)(((uint64_t)q * 2654435761u) % N);
         b.admit(b.make_seeds(&seed, 1, 0.0, 1u << 30), q);
     }
-    // each width: one warm sweep, then the fastest of three timed ones (one
-    // timed sweep let run-to-run noise of a few percent decide between
-    // widths within that distance)
+    // each width, widest first: one warm sweep, then the fastest of three
+    // timed ones (one timed sweep let run-to-run noise of a few percent decide
+    // between widths within that distance).  Narrower widths re-read the
+    // index stream once per column group, so the search stops at the first
+    // width whose first timing is 1.3x the best (those narrower only slow on).
     uint32_t best = 64;
     double best_ms = 1e30;
-    for (uint32_t cpw : {4u, 8u, 16u, 32u, 64u}) {
+    for (uint32_t cpw : {64u, 32u, 16u, 8u, 4u}) {
         if (cpw < vec_width(fmt)) continue;
         b.set_cpw(cpw);
         b.sweep(1, false);
-        double ms = 1e30;
-        for (int k = 0; k < 3; k++) {
+        b.sweep(2, false);
+        double ms = b.last_sweep_ms();
+        if (ms > 1.3 * best_ms) break;
+        for (int k = 0; k < 2; k++) {
             b.sweep(2, false);
             ms = std::min(ms, b.last_sweep_ms());
         }
