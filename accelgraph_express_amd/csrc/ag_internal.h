@@ -132,15 +132,16 @@ enum ScratchSlot : int {
     kScrTileMax = 24,                 // per-tile key maxima of a sort's first pass
     kScrLay8 = 25,                    // gather-layout build: rows of the table-sorted edges
 };
-// Device buffer of at least `bytes` in `slot`; growing waits for the stream
-// first, since the old buffer may still be read by queued work.  Sizes are
-// rounded up (1 MiB, plus 1 MiB of slack) so that an array of n + 1 words
-// after one of n words does not reallocate.
+// Device buffer of at least `bytes` in `slot`; growing a slot that holds a
+// buffer waits for the stream first, since the old buffer may still be read by
+// queued work (a slot's first allocation does not wait, so it overlaps the
+// queued kernels).  Sizes are rounded up (1 MiB, plus 1 MiB of slack) so that
+// an array of n + 1 words after one of n words does not reallocate.
 inline void *scratch(ag_ctx *ctx, int slot, size_t bytes)
 {
     DevBuf &b = ctx->scratch[slot];
     if (b.bytes < bytes) {
-        AG_HIP(hipStreamSynchronize(ctx->stream));
+        if (b.p) AG_HIP(hipStreamSynchronize(ctx->stream));
         constexpr size_t kMiB = (size_t)1 << 20;
         b.alloc((bytes + kMiB - 1) / kMiB * kMiB + kMiB);
     }
