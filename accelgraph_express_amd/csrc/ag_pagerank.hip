@@ -1915,8 +1915,15 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
     // temporaries in the context's scratch slots (build_tables, run at the
     // end, leaves Lay6 / Lay7 alone)
     std::lock_guard<std::recursive_mutex> lk(ctx->scratch_mu);
-    uint32_t *keys = scratch_as<uint32_t>(ctx, kScrPairA, N);
-    uint32_t *ids = scratch_as<uint32_t>(ctx, kScrPairB, N);
+    // partitioned tables when forced, or (auto) when one LDS table cannot
+    // hold every source; their build reads the SELL rows' sources in row
+    // order, written here beside the SELL slices (scratch slot InSrc)
+    const bool tables = tn.layout == AG_LAYOUT_TABLES || (tn.layout == AG_LAYOUT_AUTO && N > kTabEntries);
+    // PairA / PairB also hold the table build's per-edge arrays: sized for
+    // them now, a first build does not regrow them (a stream wait) there
+    const size_t npair = tables ? std::max<size_t>(N, E) : N;
+    uint32_t *keys = scratch_as<uint32_t>(ctx, kScrPairA, npair);
+    uint32_t *ids = scratch_as<uint32_t>(ctx, kScrPairB, npair);
     uint32_t *skeys = scratch_as<uint32_t>(ctx, kScrLay0 + 7, N);
     uint32_t *rdeg = scratch_as<uint32_t>(ctx, kScrLay0 + 6, N);
     p->rank.alloc((size_t)N * 4);
@@ -1983,10 +1990,6 @@ static Plan *build_plan(ag_ctx *ctx, ag_graph *g)
                            p->rank.as<uint32_t>(), p->n_nonlow, 0u, csr_edges,
                            p->cols.as<uint32_t>());
     }
-    // partitioned tables when forced, or (auto) when one LDS table cannot
-    // hold every source; their build reads the SELL rows' sources in row
-    // order, written here beside the SELL slices (scratch slot InSrc)
-    const bool tables = tn.layout == AG_LAYOUT_TABLES || (tn.layout == AG_LAYOUT_AUTO && N > kTabEntries);
     uint32_t *ecol_sell = tables && nlow ? scratch_as<uint32_t>(ctx, kScrInSrc, E) : nullptr;
     if (nlow)
         hipLaunchKernelGGL(k_fill_sell_rows, dim3(ceil_div(nlow, 256)), dim3(256), 0, s,
