@@ -118,6 +118,9 @@ struct ag_ctx {
     static constexpr int kScratchSlots = 26;
     std::recursive_mutex scratch_mu;
     ag::DevBuf scratch[kScratchSlots];
+    // buffers of grown slots that queued work may still read: freed once the
+    // stream is seen idle (or with the context, after its stream is drained)
+    std::vector<ag::DevBuf> retired;
 };
 
 namespace ag {
@@ -132,18 +135,30 @@ enum ScratchSlot : int {
     kScrTileMax = 24,                 // per-tile key maxima of a sort's first pass
     kScrLay8 = 25,                    // gather-layout build: rows of the table-sorted edges
 };
-// Device buffer of at least `bytes` in `slot`; growing a slot that holds a
-// buffer waits for the stream first, since the old buffer may still be read by
-// queued work (a slot's first allocation does not wait, so it overlaps the
-// queued kernels).  Sizes are rounded up (1 MiB, plus 1 MiB of slack) so that
-// an array of n + 1 words after one of n words does not reallocate.
+// Device buffer of at least `bytes` in `slot`.  Growing a slot does not wait
+// for the stream: the old buffer, which queued work may still read, is
+// retired and freed at a later call that finds the stream idle, so the
+// allocation overlaps the queued kernels.  Sizes are rounded up (1 MiB, plus
+// 1 MiB of slack) so that an array of n + 1 words after one of n words does
+// not reallocate.
 inline void *scratch(ag_ctx *ctx, int slot, size_t bytes)
 {
+    if (!ctx->retired.empty() && hipStreamQuery(ctx->stream) == hipSuccess)
+        ctx->retired.clear();
     DevBuf &b = ctx->scratch[slot];
     if (b.bytes < bytes) {
-        if (b.p) AG_HIP(hipStreamSynchronize(ctx->stream));
+        if (b.p) ctx->retired.push_back(std::move(b));
         constexpr size_t kMiB = (size_t)1 << 20;
-        b.alloc((bytes + kMiB - 1) / kMiB * kMiB + kMiB);
+        const size_t n = (bytes + kMiB - 1) / kMiB * kMiB + kMiB;
+        try {
+            b.alloc(n);
+        } catch (const Error &) {
+            if (ctx->retired.empty()) throw;
+            // short of memory: wait for the queued work, free what it held
+            AG_HIP(hipStreamSynchronize(ctx->stream));
+            ctx->retired.clear();
+            b.alloc(n);
+        }
     }
     return b.p;
 }
